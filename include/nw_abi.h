@@ -1,0 +1,215 @@
+/*
+ * nw_abi.h -- the C-ABI of libnwamd.so.
+ *
+ * Part 1 reproduces, symbol for symbol and argument for argument, the extern
+ * "C" surface that the reference's host (simulation/particleSystem.cpp,
+ * simulation/main.cpp) links against: simulation/particleSystem.cuh:12-112,
+ * implemented upstream in simulation/particleSystem_cuda.cu:36-371.  A host
+ * built against the reference header links against this library unchanged.
+ * Every pointer is a device pointer obtained from allocateArray(); float*
+ * arrays are float4 records (pos.w = radius, vel.w = mass).
+ *
+ * Part 2 is additive (prefix nw): RNG seeding, error reporting, stream
+ * selection, the extension parameter block, kernel timing and sorting with an
+ * explicit key bound.
+ *
+ * Error convention (reference: common/helper_cuda.h:566-598): by default a
+ * failing call prints "file:line: message" and calls exit(EXIT_FAILURE).  After
+ * nwSetErrorMode(NW_ERRORS_RETURN) a failing call records the error (read with
+ * nwGetLastError / nwGetLastErrorString) and returns without launching.
+ *
+ * Threading: like the reference, the library keeps process-global state (the
+ * parameter blocks, the RNG generator, one stream) and is not re-entrant.
+ */
+#ifndef NW_ABI_H
+#define NW_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "nw_params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* opaque; the argument is accepted and ignored as in the reference
+   (simulation/particleSystem_cuda.cu:86-90) */
+struct cudaGraphicsResource;
+
+/* ======================= Part 1: reference surface ======================= */
+
+/* particleSystem.cuh:14 / particleSystem_cuda.cu:40-54.  Selects the device
+   (-device=N on the command line, else device 0), creates the library stream
+   and seeds the Philox generator with its default seed. */
+void cudaInit(int argc, char **argv);
+
+/* particleSystem.cuh:15 / particleSystem_cuda.cu:56-59.  n uniforms in (0, 1]
+   from the global counter-based generator; the generator advances. */
+void randomizeUniform(float *device, uint n);
+
+/* particleSystem.cuh:16 / particleSystem_cuda.cu:61-64.  n normals N(mean, stddev). */
+void randomizeGaussian(float *device, uint n, float mean, float stddev);
+
+/* particleSystem.cuh:18 declares int, particleSystem_cuda.cu:66 defines size_t;
+   size_t is used here (int callers convert implicitly). */
+void allocateArray(void **devPtr, size_t size);
+/* particleSystem.cuh:19 / particleSystem_cuda.cu:71-74 */
+void freeArray(void *devPtr);
+/* particleSystem.cuh:21 / particleSystem_cuda.cu:76-79: waits for all device work */
+void threadSync(void);
+
+/* particleSystem.cuh:23 / particleSystem_cuda.cu:86-90 */
+void copyArrayFromDevice(void *host, const void *device,
+                         struct cudaGraphicsResource **cuda_vbo_resource, int size);
+/* particleSystem.cuh:24 / particleSystem_cuda.cu:81-84 (offset in bytes) */
+void copyArrayToDevice(void *device, const void *host, int offset, int size);
+
+/* particleSystem.cuh:26-27 / particleSystem_cuda.cu:92-102.  The blocks are
+   copied by value into library state and travel to the kernels as kernel
+   arguments (no constant-memory upload, no host synchronisation). */
+void setParameters(SimParams *hostParams);
+void setSolventParameters(SolventParams *solventParams);
+
+/* particleSystem.cuh:29-34 / particleSystem_cuda.cu:117-133
+   (functor particles_kernel_impl.cuh:71-142) */
+void integrateFilaments(float *pos, float *vel, float *f, float *f_old,
+                        float deltaTime, uint numParticles);
+
+/* particleSystem.cuh:36-40 / particleSystem_cuda.cu:136-149
+   (functor particles_kernel_impl.cuh:145-205) */
+void integrateSolvent(float *pos, float *vel, float deltaTime,
+                      uint startIndex, uint numSolvent);
+
+/* particleSystem.cuh:42-46 / particleSystem_cuda.cu:151-169
+   (kernel particles_kernel_impl.cuh:312-342).  srd=false: MD grid of
+   SimParams; srd=true: shifted SRD grid of SolventParams. */
+#ifdef __cplusplus
+void calcHash(uint *gridParticleHash, uint *gridParticleIndex, float *pos,
+              int numParticles, bool srd = false);
+#else
+void calcHash(uint *gridParticleHash, uint *gridParticleIndex, float *pos,
+              int numParticles, nw_bool srd);
+#endif
+
+/* particleSystem.cuh:48-59 / particleSystem_cuda.cu:171-205
+   (kernel particles_kernel_impl.cuh:346-424).  cellStart[0..numCells) is set
+   to 0xffffffff first; cellEnd of empty cells is left untouched.  vel/sortedVel
+   and tangent/sortedTangent may be NULL. */
+void reorderDataAndFindCellStart(uint *cellStart, uint *cellEnd,
+                                 float *sortedPos, float *sortedVel, float *sortedTangent,
+                                 uint *gridParticleHash, uint *gridParticleIndex,
+                                 float *pos, float *vel, float *tangent,
+                                 uint numParticles, uint numCells);
+
+/* particleSystem.cuh:61-69 / particleSystem_cuda.cu:207-235
+   (kernels particles_kernel_impl.cuh:427-556, 694-744).  Adds the pair forces
+   of every bead to force[gridParticleIndex[i]]. */
+void collideFilaments(float *force, float *sortedPos, float *sortedVel, float *sortedTangent,
+                      uint *gridParticleIndex, uint *cellStart, uint *cellEnd,
+                      uint numParticles, uint numCells);
+
+/* particleSystem.cuh:71-76 / particleSystem_cuda.cu:237-256
+   (kernel particles_kernel_impl.cuh:640-691).  Draws one uniform per SRD cell
+   from the global generator (in registers; `random` is not written) and
+   rotates v - u_cell by +-alpha. */
+void collideSolvent(float *pos, float *vel, float *solventCellCOM, float *random,
+                    uint numSolventCells, uint numParticles);
+
+/* particleSystem.cuh:78-81 / particleSystem_cuda.cu:258-274
+   (kernel particles_kernel_impl.cuh:751-818).  numFilaments is a float in the
+   reference signature and stays one here. */
+void filamentForces(float *force, float *tangent, float *pos, float numFilaments);
+
+/* particleSystem.cuh:83 / particleSystem_cuda.cu:277-282.  Stable ascending
+   sort of (hash -> index) pairs, in place. */
+void sortParticles(uint *dGridParticleHash, uint *dGridParticleIndex, uint numParticles);
+
+/* particleSystem.cuh:85-90 / particleSystem_cuda.cu:284-306
+   (kernel particles_kernel_impl.cuh:820-857).  One uniform per worm from the
+   global generator (in registers; `uniform` is not written). */
+void reverseFilaments(float *force, float *forceOld, float *vel, float *pos,
+                      float *uniform, uint numFilaments);
+
+/* particleSystem.cuh:92-97 / particleSystem_cuda.cu:308-328
+   (kernel particles_kernel_impl.cuh:859-874).  f += -gamma v + xi with
+   xi ~ N(0, 2 gamma kbT) per component, drawn in registers from the global
+   generator exactly as randomizeGaussian(random, 4*numParticles, 0, sigma)
+   would have written them; `random` is not written. */
+void langevinThermostat(float *force, float *vel, float *random,
+                        float gamma, float kbT, uint numParticles);
+
+/* particleSystem.cuh:99-104 / particleSystem_cuda.cu:331-349
+   (kernel particles_kernel_impl.cuh:559-594).  The misspelling is the ABI. */
+void solventCellCentorOfMomentum(float *solventCellCOM, float *sortedVel,
+                                 uint *cellStart, uint *cellEnd,
+                                 uint *gridParticleIndex, uint numCells);
+
+/* particleSystem.cuh:106-111 / particleSystem_cuda.cu:351-369
+   (kernel particles_kernel_impl.cuh:597-637) */
+void solventIsokineticThermostat(float *vel, float *sortedVel,
+                                 uint *cellStart, uint *cellEnd,
+                                 uint *gridParticleIndex, uint numCells);
+
+/* particleSystem_cuda.cu:105-115 (inside the reference's extern "C" block).
+   computeGridSize takes C++ references upstream; pointers are the same ABI. */
+uint iDivUp(uint a, uint b);
+void computeGridSize(uint n, uint blockSize, uint *numBlocks, uint *numThreads);
+
+/* The names used for this path in the original particles sample, kept as
+   aliases: integrateSystem == integrateFilaments, collide == collideFilaments. */
+void integrateSystem(float *pos, float *vel, float *f, float *f_old,
+                     float deltaTime, uint numParticles);
+void collide(float *force, float *sortedPos, float *sortedVel, float *sortedTangent,
+             uint *gridParticleIndex, uint *cellStart, uint *cellEnd,
+             uint numParticles, uint numCells);
+
+/* ======================= Part 2: additions ======================= */
+
+enum { NW_ERRORS_EXIT = 0, NW_ERRORS_RETURN = 1 };
+
+/* Error mode (NW_ERRORS_EXIT default, like helper_cuda.h). */
+void nwSetErrorMode(int mode);
+/* 0 if no error since the last call; clears the recorded error. */
+int nwGetLastError(void);
+const char *nwGetLastErrorString(void);
+/* 1 if a HIP device is usable (initialises the library on first success). */
+int nwDeviceAvailable(void);
+
+/* Philox4x32-10 generator state: key = seed, counter offset in 128-bit blocks. */
+void nwRngSetSeed(uint64_t seed, uint64_t offset);
+void nwRngGetState(uint64_t *seed, uint64_t *offset);
+
+/* Extension parameters (physics variants); zero-filled block = reference. */
+void nwSetExtParameters(const NWExtParams *ext);
+void nwGetExtParameters(NWExtParams *ext);
+
+/* Library stream: a hipStream_t created with default flags and priority by
+   cudaInit.  nwSetStream replaces it (NULL = legacy null stream). */
+void *nwGetStream(void);
+void nwSetStream(void *stream);
+
+/* Stable (hash -> index) sort when every key is known to be < keyBound
+   (counting sort with keyBound bins).  Keys >= keyBound are still sorted
+   correctly, on a slow path, and set nwGetDeviceFlags() bit 0. */
+void nwSortParticlesBounded(uint *dGridParticleHash, uint *dGridParticleIndex,
+                            uint numParticles, uint keyBound);
+/* Stable LSD radix sort over the low `keyBits` bits of the keys (1..32);
+   keys must fit in keyBits. */
+void nwRadixSortPairs(uint *dGridParticleHash, uint *dGridParticleIndex,
+                      uint numParticles, uint keyBits);
+
+/* Device-side anomaly flags (bit 0: sort key above bound).  Synchronises. */
+uint nwGetDeviceFlags(int clear);
+
+/* Kernel timing with HIP events recorded on the library stream around every
+   launch of the named kernel family while enabled.  Names: "integrate",
+   "langevin", "filament", "hash", "sort", "reorder", "collide", "srd_com",
+   "srd_thermo", "srd_collide", "srd_stream", "reverse". */
+void nwTimerEnable(const char *kernel, int enable);
+/* Synchronises; returns total milliseconds and launch count since enable. */
+void nwTimerRead(const char *kernel, double *totalMs, uint64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

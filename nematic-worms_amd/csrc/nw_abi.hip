@@ -1,0 +1,516 @@
+// nw_abi.hip -- the extern "C" surface of libnwamd.so (declared in
+// include/nw_abi.h).  Mirrors simulation/particleSystem_cuda.cu:36-371: same
+// names, arguments, data layout and error behaviour, with the work done by the
+// HIP kernels of nw_kernels.hip / nw_sort.hip on one library stream.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "nw_abi.h"
+#include "nw_internal.hpp"
+
+namespace {
+
+using namespace nw;
+
+struct Timer {
+    bool enabled = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+    double totalMs = 0.0;
+    uint64_t launches = 0;
+};
+
+struct State {
+    bool initialized = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool ownStream = false;
+    SimParams params{};
+    SolventParams solvent{};
+    NWExtParams ext{};
+    uint64_t seed = 0;
+    uint64_t offset = 0;
+    int errorMode = NW_ERRORS_EXIT;
+    int lastError = 0;
+    std::string lastErrorString;
+    uint32_t lastHashBound = 0;   // cell count of the most recent calcHash
+    const uint32_t *lastHashPtr = nullptr;
+    SortScratch sort;
+    std::map<std::string, Timer> timers;
+};
+
+State &S() {
+    static State s;
+    return s;
+}
+
+// Records an error; in exit mode prints and terminates like helper_cuda.h's check().
+bool fail(const char *file, int line, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    State &s = S();
+    s.lastError = code ? code : -1;
+    s.lastErrorString = buf;
+    if (s.errorMode == NW_ERRORS_EXIT) {
+        fprintf(stderr, "%s:%d: %s\n", file, line, buf);
+        exit(EXIT_FAILURE);
+    }
+    return false;
+}
+
+#define NW_HIP(call)                                                                                 \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess) {                                                                      \
+            fail(__FILE__, __LINE__, int(e_), "HIP error %d (%s) in %s", int(e_), hipGetErrorString(e_), \
+                 #call);                                                                             \
+            return;                                                                                  \
+        }                                                                                            \
+    } while (0)
+
+#define NW_HIP_R(call, ret)                                                                          \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess) {                                                                      \
+            fail(__FILE__, __LINE__, int(e_), "HIP error %d (%s) in %s", int(e_), hipGetErrorString(e_), \
+                 #call);                                                                             \
+            return ret;                                                                              \
+        }                                                                                            \
+    } while (0)
+
+#define NW_REQUIRE(cond, ...)                                  \
+    do {                                                       \
+        if (!(cond)) {                                         \
+            fail(__FILE__, __LINE__, -22, __VA_ARGS__);        \
+            return;                                            \
+        }                                                      \
+    } while (0)
+
+bool ensure_init() {
+    State &s = S();
+    if (s.initialized) return true;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) {
+        fail(__FILE__, __LINE__, -19, "no HIP device available (hipGetDeviceCount: %s)", hipGetErrorString(e));
+        return false;
+    }
+    if (s.device >= count) s.device = 0;
+    if ((e = hipSetDevice(s.device)) != hipSuccess) {
+        fail(__FILE__, __LINE__, int(e), "hipSetDevice(%d): %s", s.device, hipGetErrorString(e));
+        return false;
+    }
+    if ((e = hipStreamCreate(&s.stream)) != hipSuccess) {
+        fail(__FILE__, __LINE__, int(e), "hipStreamCreate: %s", hipGetErrorString(e));
+        return false;
+    }
+    s.ownStream = true;
+    s.initialized = true;
+    return true;
+}
+
+#define NW_INIT()                  \
+    do {                           \
+        if (!ensure_init()) return; \
+    } while (0)
+
+// fast division by the worm length: q = mulhi(i, m) with m = ceil(2^32 / n)
+// is exact for every i < 2^32 / e, e = m * n - 2^32 (Granlund & Montgomery 1994).
+KP make_kp() {
+    State &s = S();
+    KP kp{};
+    kp.p = s.params;
+    kp.s = s.solvent;
+    kp.e = s.ext;
+    const uint64_t n = s.params.filamentSize;
+    if (n >= 2) {
+        const uint64_t two32 = 1ull << 32;
+        const uint64_t m = (two32 + n - 1) / n;
+        const uint64_t e = m * n - two32;
+        kp.fdivMagic = uint32_t(m);
+        kp.fdivLimit = e == 0 ? 0xffffffffu : uint32_t(std::min<uint64_t>(0xffffffffull, (two32 - 1) / e));
+        if (m >= two32) kp.fdivLimit = 0;
+    } else {
+        kp.fdivMagic = 0;
+        kp.fdivLimit = 0;
+    }
+    return kp;
+}
+
+struct TimedScope {
+    Timer *t = nullptr;
+    hipStream_t st;
+    std::pair<hipEvent_t, hipEvent_t> ev{};
+    TimedScope(const char *name, hipStream_t stream) : st(stream) {
+        auto it = S().timers.find(name);
+        if (it == S().timers.end() || !it->second.enabled) return;
+        t = &it->second;
+        if (!t->pool.empty()) {
+            ev = t->pool.back();
+            t->pool.pop_back();
+        } else {
+            (void)hipEventCreate(&ev.first);
+            (void)hipEventCreate(&ev.second);
+        }
+        (void)hipEventRecord(ev.first, st);
+    }
+    ~TimedScope() {
+        if (!t) return;
+        (void)hipEventRecord(ev.second, st);
+        t->pending.push_back(ev);
+        t->launches++;
+    }
+};
+
+uint64_t blocks_for(uint64_t n) { return (n + 3) / 4; }
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// init / RNG / memory
+// ---------------------------------------------------------------------------
+void cudaInit(int argc, char **argv) {
+    State &s = S();
+    for (int i = 1; i < argc && argv; ++i) {
+        const char *a = argv[i];
+        while (*a == '-') ++a;
+        if (strncasecmp(a, "device=", 7) == 0) s.device = atoi(a + 7);
+    }
+    if (!ensure_init()) {
+        // reference: "No CUDA Capable devices found, exiting..." + exit(EXIT_SUCCESS)
+        return;
+    }
+    s.seed = 0;
+    s.offset = 0;
+}
+
+void randomizeUniform(float *device, uint n) {
+    NW_INIT();
+    State &s = S();
+    NW_HIP(launch_randomize(device, n, 0, 0.f, 1.f, s.seed, s.offset, s.stream));
+    s.offset += blocks_for(n);
+}
+
+void randomizeGaussian(float *device, uint n, float mean, float stddev) {
+    NW_INIT();
+    State &s = S();
+    NW_HIP(launch_randomize(device, n, 1, mean, stddev, s.seed, s.offset, s.stream));
+    s.offset += blocks_for(n);
+}
+
+void allocateArray(void **devPtr, size_t size) {
+    NW_INIT();
+    NW_HIP(hipMalloc(devPtr, size));
+}
+
+void freeArray(void *devPtr) {
+    NW_INIT();
+    NW_HIP(hipFree(devPtr));
+}
+
+void threadSync(void) {
+    NW_INIT();
+    NW_HIP(hipDeviceSynchronize());
+}
+
+void copyArrayFromDevice(void *host, const void *device, struct cudaGraphicsResource **, int size) {
+    NW_INIT();
+    State &s = S();
+    if (size <= 0) return;
+    NW_HIP(hipMemcpyAsync(host, device, size_t(size), hipMemcpyDeviceToHost, s.stream));
+    NW_HIP(hipStreamSynchronize(s.stream));
+}
+
+void copyArrayToDevice(void *device, const void *host, int offset, int size) {
+    NW_INIT();
+    State &s = S();
+    if (size <= 0) return;
+    NW_HIP(hipMemcpyAsync(static_cast<char *>(device) + offset, host, size_t(size), hipMemcpyHostToDevice, s.stream));
+    NW_HIP(hipStreamSynchronize(s.stream));
+}
+
+void setParameters(SimParams *hostParams) {
+    NW_REQUIRE(hostParams != nullptr, "setParameters: NULL");
+    S().params = *hostParams;
+}
+
+void setSolventParameters(SolventParams *solventParams) {
+    NW_REQUIRE(solventParams != nullptr, "setSolventParameters: NULL");
+    S().solvent = *solventParams;
+}
+
+uint iDivUp(uint a, uint b) { return (a % b != 0) ? (a / b + 1) : (a / b); }
+
+void computeGridSize(uint n, uint blockSize, uint *numBlocks, uint *numThreads) {
+    *numThreads = n < blockSize ? n : blockSize;
+    *numBlocks = *numThreads ? iDivUp(n, *numThreads) : 0;
+}
+
+// ---------------------------------------------------------------------------
+// the time step
+// ---------------------------------------------------------------------------
+void integrateFilaments(float *pos, float *vel, float *f, float *f_old, float deltaTime, uint numParticles) {
+    NW_INIT();
+    State &s = S();
+    TimedScope ts("integrate", s.stream);
+    NW_HIP(launch_integrate_filaments(reinterpret_cast<float4 *>(pos), reinterpret_cast<float4 *>(vel),
+                                      reinterpret_cast<float4 *>(f), reinterpret_cast<float4 *>(f_old), deltaTime,
+                                      numParticles, make_kp(), s.stream));
+}
+
+void integrateSystem(float *pos, float *vel, float *f, float *f_old, float deltaTime, uint numParticles) {
+    integrateFilaments(pos, vel, f, f_old, deltaTime, numParticles);
+}
+
+void integrateSolvent(float *pos, float *vel, float deltaTime, uint startIndex, uint numSolvent) {
+    NW_INIT();
+    State &s = S();
+    TimedScope ts("srd_stream", s.stream);
+    NW_HIP(launch_integrate_solvent(reinterpret_cast<float4 *>(pos), reinterpret_cast<float4 *>(vel), deltaTime,
+                                    startIndex, numSolvent, make_kp(), s.stream));
+}
+
+void calcHash(uint *gridParticleHash, uint *gridParticleIndex, float *pos, int numParticles, bool srd) {
+    NW_INIT();
+    State &s = S();
+    NW_REQUIRE(numParticles >= 0, "calcHash: negative particle count %d", numParticles);
+    TimedScope ts("hash", s.stream);
+    NW_HIP(launch_calc_hash(gridParticleHash, gridParticleIndex, reinterpret_cast<const float4 *>(pos),
+                            uint32_t(numParticles), srd, make_kp(), s.stream));
+    s.lastHashBound = srd ? s.solvent.numCells : s.params.numCells;
+    s.lastHashPtr = gridParticleHash;
+}
+
+void sortParticles(uint *dGridParticleHash, uint *dGridParticleIndex, uint numParticles) {
+    NW_INIT();
+    State &s = S();
+    TimedScope ts("sort", s.stream);
+    if (s.lastHashPtr == dGridParticleHash && s.lastHashBound > 0) {
+        NW_HIP(counting_sort_pairs(dGridParticleHash, dGridParticleIndex, numParticles, s.lastHashBound, s.sort,
+                                   s.stream));
+    } else {
+        NW_HIP(radix_sort_pairs(dGridParticleHash, dGridParticleIndex, numParticles, 32, s.sort, s.stream));
+    }
+}
+
+void nwSortParticlesBounded(uint *dGridParticleHash, uint *dGridParticleIndex, uint numParticles, uint keyBound) {
+    NW_INIT();
+    State &s = S();
+    NW_REQUIRE(keyBound > 0, "nwSortParticlesBounded: keyBound must be > 0");
+    TimedScope ts("sort", s.stream);
+    NW_HIP(counting_sort_pairs(dGridParticleHash, dGridParticleIndex, numParticles, keyBound, s.sort, s.stream));
+}
+
+void nwRadixSortPairs(uint *dGridParticleHash, uint *dGridParticleIndex, uint numParticles, uint keyBits) {
+    NW_INIT();
+    State &s = S();
+    NW_REQUIRE(keyBits >= 1 && keyBits <= 32, "nwRadixSortPairs: keyBits %u outside 1..32", keyBits);
+    TimedScope ts("sort", s.stream);
+    NW_HIP(radix_sort_pairs(dGridParticleHash, dGridParticleIndex, numParticles, keyBits, s.sort, s.stream));
+}
+
+void reorderDataAndFindCellStart(uint *cellStart, uint *cellEnd, float *sortedPos, float *sortedVel,
+                                 float *sortedTangent, uint *gridParticleHash, uint *gridParticleIndex, float *pos,
+                                 float *vel, float *tangent, uint numParticles, uint numCells) {
+    NW_INIT();
+    State &s = S();
+    TimedScope ts("reorder", s.stream);
+    NW_HIP(launch_reorder(cellStart, cellEnd, reinterpret_cast<float4 *>(sortedPos),
+                          reinterpret_cast<float4 *>(sortedVel), reinterpret_cast<float4 *>(sortedTangent),
+                          gridParticleHash, gridParticleIndex, reinterpret_cast<const float4 *>(pos),
+                          reinterpret_cast<const float4 *>(vel), reinterpret_cast<const float4 *>(tangent),
+                          numParticles, numCells, s.stream));
+}
+
+void collideFilaments(float *force, float *sortedPos, float *sortedVel, float *sortedTangent,
+                      uint *gridParticleIndex, uint *cellStart, uint *cellEnd, uint numParticles, uint numCells) {
+    NW_INIT();
+    State &s = S();
+    const KP kp = make_kp();
+    NW_REQUIRE(kp.p.filamentSize > 0, "collideFilaments: filamentSize is 0 (call setParameters first)");
+    NW_REQUIRE(kp.p.gridSize.x > 0 && kp.p.gridSize.y > 0 && kp.p.gridSize.z > 0,
+               "collideFilaments: empty grid (call setParameters first)");
+    NW_REQUIRE(uint64_t(kp.p.gridSize.x) * kp.p.gridSize.y * kp.p.gridSize.z <= numCells,
+               "collideFilaments: grid of %u x %u x %u cells exceeds numCells %u", kp.p.gridSize.x,
+               kp.p.gridSize.y, kp.p.gridSize.z, numCells);
+    const bool needVel = kp.e.pairPotential == NW_PAIR_REF_DEM;
+    const bool needTan = kp.e.pairPotential == NW_PAIR_REF_DEM || kp.p.activity != 0.0f;
+    NW_REQUIRE(!needVel || sortedVel != nullptr, "collideFilaments: sortedVel is NULL");
+    NW_REQUIRE(!needTan || sortedTangent != nullptr, "collideFilaments: sortedTangent is NULL");
+    TimedScope ts("collide", s.stream);
+    NW_HIP(launch_collide_filaments(reinterpret_cast<float4 *>(force), reinterpret_cast<const float4 *>(sortedPos),
+                                    reinterpret_cast<const float4 *>(sortedVel),
+                                    reinterpret_cast<const float4 *>(sortedTangent), gridParticleIndex, cellStart,
+                                    cellEnd, numParticles, numCells, kp, s.stream));
+}
+
+void collide(float *force, float *sortedPos, float *sortedVel, float *sortedTangent, uint *gridParticleIndex,
+             uint *cellStart, uint *cellEnd, uint numParticles, uint numCells) {
+    collideFilaments(force, sortedPos, sortedVel, sortedTangent, gridParticleIndex, cellStart, cellEnd, numParticles,
+                     numCells);
+}
+
+void filamentForces(float *force, float *tangent, float *pos, float numFilaments) {
+    NW_INIT();
+    State &s = S();
+    NW_REQUIRE(numFilaments >= 0.0f && numFilaments < 4294967296.0f, "filamentForces: bad worm count %g",
+               double(numFilaments));
+    TimedScope ts("filament", s.stream);
+    NW_HIP(launch_filament_forces(reinterpret_cast<float4 *>(force), reinterpret_cast<float4 *>(tangent),
+                                  reinterpret_cast<const float4 *>(pos), uint32_t(numFilaments), make_kp(),
+                                  s.stream));
+}
+
+void langevinThermostat(float *force, float *vel, float *random, float gamma, float kbT, uint numParticles) {
+    (void)random;
+    NW_INIT();
+    State &s = S();
+    const float stddev = sqrtf(2.0f * gamma * kbT);
+    TimedScope ts("langevin", s.stream);
+    NW_HIP(launch_langevin(reinterpret_cast<float4 *>(force), reinterpret_cast<const float4 *>(vel), stddev,
+                           numParticles, s.seed, s.offset, make_kp(), s.stream));
+    s.offset += numParticles;  // == blocks_for(4 * numParticles)
+}
+
+void reverseFilaments(float *force, float *forceOld, float *vel, float *pos, float *uniform, uint numFilaments) {
+    (void)uniform;
+    NW_INIT();
+    State &s = S();
+    TimedScope ts("reverse", s.stream);
+    NW_HIP(launch_reverse_filaments(reinterpret_cast<float4 *>(force), reinterpret_cast<float4 *>(forceOld),
+                                    reinterpret_cast<float4 *>(vel), reinterpret_cast<float4 *>(pos), numFilaments,
+                                    s.seed, s.offset, make_kp(), s.stream));
+    s.offset += blocks_for(numFilaments);
+}
+
+void solventCellCentorOfMomentum(float *solventCellCOM, float *sortedVel, uint *cellStart, uint *cellEnd,
+                                 uint *gridParticleIndex, uint numCells) {
+    (void)gridParticleIndex;
+    NW_INIT();
+    State &s = S();
+    TimedScope ts("srd_com", s.stream);
+    NW_HIP(launch_srd_com(reinterpret_cast<float4 *>(solventCellCOM), reinterpret_cast<const float4 *>(sortedVel),
+                          cellStart, cellEnd, numCells, s.stream));
+}
+
+void solventIsokineticThermostat(float *vel, float *sortedVel, uint *cellStart, uint *cellEnd,
+                                 uint *gridParticleIndex, uint numCells) {
+    NW_INIT();
+    State &s = S();
+    TimedScope ts("srd_thermo", s.stream);
+    NW_HIP(launch_srd_isokinetic(reinterpret_cast<float4 *>(vel), reinterpret_cast<float4 *>(sortedVel), cellStart,
+                                 cellEnd, gridParticleIndex, numCells, make_kp(), s.stream));
+}
+
+void collideSolvent(float *pos, float *vel, float *solventCellCOM, float *random, uint numSolventCells,
+                    uint numParticles) {
+    (void)random;
+    NW_INIT();
+    State &s = S();
+    // cos/sin of the rotation angle once, on the host, in double precision
+    const double a = double(s.solvent.alpha);
+    const float cosA = float(std::cos(a)), sinA = float(std::sin(a));
+    TimedScope ts("srd_collide", s.stream);
+    NW_HIP(launch_collide_solvent(reinterpret_cast<const float4 *>(pos), reinterpret_cast<float4 *>(vel),
+                                  reinterpret_cast<const float4 *>(solventCellCOM), numSolventCells, numParticles,
+                                  cosA, sinA, s.seed, s.offset, make_kp(), s.stream));
+    s.offset += blocks_for(numSolventCells);
+}
+
+// ---------------------------------------------------------------------------
+// additions
+// ---------------------------------------------------------------------------
+void nwSetErrorMode(int mode) { S().errorMode = mode == NW_ERRORS_RETURN ? NW_ERRORS_RETURN : NW_ERRORS_EXIT; }
+
+int nwGetLastError(void) {
+    const int e = S().lastError;
+    S().lastError = 0;
+    return e;
+}
+
+const char *nwGetLastErrorString(void) { return S().lastErrorString.c_str(); }
+
+int nwDeviceAvailable(void) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return 0;
+    const int mode = S().errorMode;
+    S().errorMode = NW_ERRORS_RETURN;
+    const bool ok = ensure_init();
+    S().errorMode = mode;
+    return ok ? 1 : 0;
+}
+
+void nwRngSetSeed(uint64_t seed, uint64_t offset) {
+    S().seed = seed;
+    S().offset = offset;
+}
+
+void nwRngGetState(uint64_t *seed, uint64_t *offset) {
+    if (seed) *seed = S().seed;
+    if (offset) *offset = S().offset;
+}
+
+void nwSetExtParameters(const NWExtParams *ext) {
+    NW_REQUIRE(ext != nullptr, "nwSetExtParameters: NULL");
+    S().ext = *ext;
+}
+
+void nwGetExtParameters(NWExtParams *ext) {
+    if (ext) *ext = S().ext;
+}
+
+void *nwGetStream(void) { return S().stream; }
+
+void nwSetStream(void *stream) { S().stream = static_cast<hipStream_t>(stream); }
+
+uint nwGetDeviceFlags(int clear) {
+    if (!ensure_init()) return 0;
+    State &s = S();
+    uint32_t v = 0;
+    uint32_t *d = device_flags();
+    NW_HIP_R(hipStreamSynchronize(s.stream), 0);
+    NW_HIP_R(hipMemcpy(&v, d, 4, hipMemcpyDeviceToHost), 0);
+    if (clear) {
+        const uint32_t z = 0;
+        NW_HIP_R(hipMemcpy(d, &z, 4, hipMemcpyHostToDevice), 0);
+    }
+    return v;
+}
+
+void nwTimerEnable(const char *kernel, int enable) {
+    if (!kernel) return;
+    Timer &t = S().timers[kernel];
+    t.enabled = enable != 0;
+    if (enable) {
+        t.totalMs = 0.0;
+        t.launches = 0;
+    }
+}
+
+void nwTimerRead(const char *kernel, double *totalMs, uint64_t *launches) {
+    if (!kernel) return;
+    Timer &t = S().timers[kernel];
+    for (auto &ev : t.pending) {
+        (void)hipEventSynchronize(ev.second);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ev.first, ev.second);
+        t.totalMs += ms;
+        t.pool.push_back(ev);
+    }
+    t.pending.clear();
+    if (totalMs) *totalMs = t.totalMs;
+    if (launches) *launches = t.launches;
+}
+
+}  // extern "C"

@@ -1,0 +1,187 @@
+// nw_device.hpp -- device-side building blocks of the worm time step (gfx950).
+//
+// Vector arithmetic keeps the operation order of the reference's helpers
+// (common/helper_math.h: dot = x*x + y*y + z*z at :1248, length = sqrtf(dot)
+// at :1291, float3 / float divides per component at :997) so that single
+// force terms agree with the FP64 oracle to a few ulp.  The library is built
+// with -ffp-contract=off; kernels that opt into FMA contraction say so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "nw_params.h"
+
+namespace nw {
+
+// --------------------------------------------------------------------------
+// float3 arithmetic (plain struct, explicit operation order)
+// --------------------------------------------------------------------------
+struct f3 { float x, y, z; };
+
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ f3 mk(float4 a) { return {a.x, a.y, a.z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ void operator+=(f3 &a, f3 b) { a.x += b.x; a.y += b.y; a.z += b.z; }
+__device__ __forceinline__ void operator-=(f3 &a, f3 b) { a.x -= b.x; a.y -= b.y; a.z -= b.z; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float length(f3 v) { return sqrtf(dot(v, v)); }
+__device__ __forceinline__ float4 f4(f3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+
+// --------------------------------------------------------------------------
+// Kernel-argument parameter bundle.  Travels by value (272 bytes of kernarg),
+// so a launch never depends on a prior host->device upload.
+// --------------------------------------------------------------------------
+struct KP {
+    SimParams p;
+    SolventParams s;
+    NWExtParams e;
+    uint32_t fdivMagic;   // ceil(2^32 / filamentSize): q = mulhi(i, m)
+    uint32_t fdivLimit;   // mulhi quotient is exact for i < fdivLimit (0: always divide)
+};
+
+// i / filamentSize and i % filamentSize for bead ids.
+__device__ __forceinline__ void worm_of(const KP &kp, uint32_t i, uint32_t &w, uint32_t &c) {
+    const uint32_t n = kp.p.filamentSize;
+    if (i < kp.fdivLimit) {
+        w = __umulhi(i, kp.fdivMagic);
+    } else {
+        w = i / n;
+    }
+    c = i - w * n;
+}
+
+// --------------------------------------------------------------------------
+// Minimum image (particles_kernel_impl.cuh:279-301).  Mutates dr like the
+// reference's lengthPeriodic and returns |dr|.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ float length_periodic(const SimParams &p, f3 &dr) {
+    const f3 L = mk(p.boxSize.x, p.boxSize.y, p.boxSize.z);
+    const f3 h = L / 2.0f;
+    if (p.boundaryX == PERIODIC) {
+        if (dr.x > h.x) dr.x -= L.x;
+        else if (dr.x < -h.x) dr.x += L.x;
+    }
+    if (p.boundaryY == PERIODIC) {
+        if (dr.y > h.y) dr.y -= L.y;
+        else if (dr.y < -h.y) dr.y += L.y;
+    }
+    if (p.boundaryZ == PERIODIC) {
+        if (dr.z > h.z) dr.z -= L.z;
+        else if (dr.z < -h.z) dr.z += L.z;
+    }
+    return length(dr);
+}
+
+// unit(b - a) under the minimum image (particles_kernel_impl.cuh:303-309)
+__device__ __forceinline__ f3 tangent_of(const SimParams &p, f3 a, f3 b) {
+    f3 rel = b - a;
+    const float d = length_periodic(p, rel);
+    return rel / d;
+}
+
+// --------------------------------------------------------------------------
+// Grid hashing (particles_kernel_impl.cuh:208-276).  The cell index uses the
+// same FP32 sequence as the reference, floor((p - o) / c), with correctly
+// rounded division.  Wrapping uses the mathematical modulo, which equals the
+// reference's `& (G - 1)` for power-of-two G and stays correct otherwise.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ int wrap_mod(int g, uint32_t G) {
+    if ((G & (G - 1u)) == 0u) return g & int(G - 1u);
+    int m = g % int(G);
+    return m < 0 ? m + int(G) : m;
+}
+
+__device__ __forceinline__ int cell_coord(float p, float o, float c) {
+    return int(floorf((p - o) / c));
+}
+
+__device__ __forceinline__ uint32_t md_hash_wrapped(const SimParams &p, int gx, int gy, int gz) {
+    const uint32_t x = uint32_t(wrap_mod(gx, p.gridSize.x));
+    const uint32_t y = uint32_t(wrap_mod(gy, p.gridSize.y));
+    const uint32_t z = uint32_t(wrap_mod(gz, p.gridSize.z));
+    return (z * p.gridSize.y) * p.gridSize.x + y * p.gridSize.x + x;
+}
+
+// positionRelativeToBox (particles_kernel_impl.cuh:208-228)
+__device__ __forceinline__ f3 position_relative_to_box(const SimParams &p, f3 q) {
+    f3 r = mk(q.x - p.origin.x, q.y - p.origin.y, q.z - p.origin.z);
+    if (p.boundaryX == PERIODIC) {
+        if (r.x >= p.boxSize.x) r.x -= p.boxSize.x;
+        else if (r.x < 0.0f) r.x += p.boxSize.x;
+    }
+    if (p.boundaryY == PERIODIC) {
+        if (r.y >= p.boxSize.y) r.y -= p.boxSize.y;
+        else if (r.y < 0.0f) r.y += p.boxSize.y;
+    }
+    if (p.boundaryZ == PERIODIC) {
+        if (r.z >= p.boxSize.z) r.z -= p.boxSize.z;
+        else if (r.z < 0.0f) r.z += p.boxSize.z;
+    }
+    return r;
+}
+
+// SRD cell of a position (calcGridPosSRD :241-249 + calcGridHashSRD :261-276).
+// Periodic axes wrap; a non-periodic axis is clamped into [0, G) (the
+// reference leaves it unwrapped, which can index past the cell table).
+__device__ __forceinline__ uint32_t srd_hash(const SimParams &p, const SolventParams &s, f3 q) {
+    const f3 shifted = q + mk(s.randOffset.x, s.randOffset.y, s.randOffset.z);
+    const f3 r = position_relative_to_box(p, shifted);
+    int gx = int(floorf(r.x / s.cellSize.x));
+    int gy = int(floorf(r.y / s.cellSize.y));
+    int gz = int(floorf(r.z / s.cellSize.z));
+    if (p.boundaryX == PERIODIC) gx = wrap_mod(gx, s.gridSize.x);
+    else gx = min(max(gx, 0), int(s.gridSize.x) - 1);
+    if (p.boundaryY == PERIODIC) gy = wrap_mod(gy, s.gridSize.y);
+    else gy = min(max(gy, 0), int(s.gridSize.y) - 1);
+    if (p.boundaryZ == PERIODIC) gz = wrap_mod(gz, s.gridSize.z);
+    else gz = min(max(gz, 0), int(s.gridSize.z) - 1);
+    return (uint32_t(gz) * s.gridSize.y) * s.gridSize.x + uint32_t(gy) * s.gridSize.x + uint32_t(gx);
+}
+
+// --------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplies with the
+// published multipliers and Weyl key increments.
+// --------------------------------------------------------------------------
+struct u4 { uint32_t x, y, z, w; };
+
+__device__ __forceinline__ u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
+        const uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
+        c = u4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        k0 += W0;
+        k1 += W1;
+    }
+    return c;
+}
+
+// Generator block b of stream `seed`: counter = (b_lo, b_hi, 0, 0).
+__device__ __forceinline__ u4 philox_block(uint64_t seed, uint64_t b) {
+    return philox4x32_10(u4{uint32_t(b), uint32_t(b >> 32), 0u, 0u}, uint32_t(seed), uint32_t(seed >> 32));
+}
+
+// 32-bit integer -> float uniform in (0, 1]: (x >> 8) * 2^-24 + 2^-24 (exact).
+__device__ __forceinline__ float u01(uint32_t x) {
+    return float(x >> 8) * 5.9604644775390625e-8f + 5.9604644775390625e-8f;
+}
+
+// Box-Muller: two normals from two uniforms in (0, 1].
+__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float &z0, float &z1) {
+    const float u1 = u01(a), u2 = u01(b);
+    const float r = sqrtf(-2.0f * logf(u1));
+    const float th = 6.2831853071795864769f * u2;
+    float s, c;
+    sincosf(th, &s, &c);
+    z0 = r * c;
+    z1 = r * s;
+}
+
+}  // namespace nw

@@ -1,0 +1,722 @@
+// nw_kernels.hip -- the per-step kernels of the worm time step on gfx950.
+//
+// Layout: float4 records per bead/solvent particle in HBM (pos.w = radius,
+// vel.w = mass), worm-contiguous (bead = worm * n + p).  Every kernel writes
+// only slots it owns, so results are deterministic and atomic-free.
+#include "nw_internal.hpp"
+
+namespace nw {
+
+__device__ uint32_t g_flags;  // anomaly bits, read by nwGetDeviceFlags
+
+uint32_t *device_flags() {
+    void *p = nullptr;
+    (void)hipGetSymbolAddress(&p, HIP_SYMBOL(g_flags));
+    return static_cast<uint32_t *>(p);
+}
+
+// ===========================================================================
+// Integrators (particles_kernel_impl.cuh:34-205)
+// ===========================================================================
+__device__ __forceinline__ void periodic_bc(float &r, float L0, float L) {
+    if (r < L0) r += L;
+    else if (r > L + L0) r -= L;
+}
+
+__device__ __forceinline__ void damped_wall(float &r, float &v, float coef, float L0, float L, float R) {
+    if (r < L0 + R) {
+        r = L0 + R;
+        v *= coef;
+    } else if (r > (L + L0) - R) {
+        r = (L + L0) - R;
+        v *= coef;
+    }
+}
+
+// noslipWallBoundary: undo the whole step displacement, pin the axis at the
+// wall and keep only the reflected normal velocity component.
+__device__ __forceinline__ void noslip_wall(f3 &pos, int axis, f3 &vel, f3 dr, f3 N, float L0, float L, float R) {
+    float r = axis == 0 ? pos.x : pos.y;
+    bool hit = false;
+    float target = 0.0f;
+    if (r < L0 + R) {
+        hit = true;
+        target = L0 + R;
+    } else if (r > (L + L0) - R) {
+        hit = true;
+        target = (L + L0) - R;
+    }
+    if (hit) {
+        pos -= dr;
+        if (axis == 0) pos.x = target; else pos.y = target;
+        vel = mk(-vel.x * N.x, -vel.y * N.y, -vel.z * N.z);
+    }
+}
+
+__device__ __forceinline__ void apply_boundaries(const SimParams &p, f3 &pos, f3 &vel, f3 dr, float R) {
+    if (p.boundaryX == PERIODIC) periodic_bc(pos.x, p.origin.x, p.boxSize.x);
+    else if (p.boundaryX == WALL) damped_wall(pos.x, vel.x, p.boundaryDamping, p.origin.x, p.boxSize.x, R);
+    else if (p.boundaryX == WALL_NO_SLIP)
+        noslip_wall(pos, 0, vel, dr, mk(1.f, 0.f, 0.f), p.origin.x, p.boxSize.x, R);
+
+    if (p.boundaryY == PERIODIC) periodic_bc(pos.y, p.origin.y, p.boxSize.y);
+    else if (p.boundaryY == WALL) damped_wall(pos.y, vel.y, p.boundaryDamping, p.origin.y, p.boxSize.y, R);
+    else if (p.boundaryY == WALL_NO_SLIP)
+        noslip_wall(pos, 1, vel, dr, mk(0.f, 1.f, 0.f), p.origin.y, p.boxSize.y, R);
+
+    // the reference has no no-slip branch for z
+    if (p.boundaryZ == PERIODIC) periodic_bc(pos.z, p.origin.z, p.boxSize.z);
+    else if (p.boundaryZ == WALL) damped_wall(pos.z, vel.z, p.boundaryDamping, p.origin.z, p.boxSize.z, R);
+}
+
+// filament_integrator (particles_kernel_impl.cuh:71-142): 128 B/bead stream.
+__global__ __launch_bounds__(kBlock) void k_integrate_filaments(float4 *__restrict__ pos, float4 *__restrict__ vel,
+                                                                float4 *__restrict__ f, float4 *__restrict__ fold,
+                                                                float dt, uint32_t n, KP kp) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 P = pos[i], V = vel[i], F = f[i], FO = fold[i];
+    f3 r = mk(P), v = mk(V);
+    const f3 ff = mk(F), fo = mk(FO);
+    const f3 dv = 0.5f * (ff + fo) * dt;
+    v += dv;
+    const f3 dr = v * dt + 0.5f * ff * dt * dt;
+    r += dr;
+    apply_boundaries(kp.p, r, v, dr, kp.p.particleRadius);
+    pos[i] = f4(r, P.w);
+    vel[i] = f4(v, V.w);
+    f[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    fold[i] = f4(ff, FO.w);
+}
+
+// solvent_integrator (particles_kernel_impl.cuh:145-205): ballistic streaming.
+__global__ __launch_bounds__(kBlock) void k_integrate_solvent(float4 *__restrict__ pos, float4 *__restrict__ vel,
+                                                              float dt, uint32_t start, uint32_t ns, KP kp) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ns) return;
+    const uint32_t i = start + t;
+    const float4 P = pos[i], V = vel[i];
+    f3 r = mk(P), v = mk(V);
+    const f3 dr = dt * v;
+    r += dr;
+    apply_boundaries(kp.p, r, v, dr, 0.0f);
+    pos[i] = f4(r, P.w);
+    vel[i] = f4(v, V.w);
+}
+
+// ===========================================================================
+// Hash (particles_kernel_impl.cuh:312-342)
+// ===========================================================================
+__global__ __launch_bounds__(kBlock) void k_calc_hash(uint32_t *__restrict__ hash, uint32_t *__restrict__ index,
+                                                      const float4 *__restrict__ pos, uint32_t n, int srd, KP kp) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 P = pos[i];
+    const f3 q = mk(P);
+    uint32_t h;
+    if (srd) {
+        h = srd_hash(kp.p, kp.s, q);
+    } else {
+        const int gx = cell_coord(q.x, kp.p.origin.x, kp.p.cellSize.x);
+        const int gy = cell_coord(q.y, kp.p.origin.y, kp.p.cellSize.y);
+        const int gz = cell_coord(q.z, kp.p.origin.z, kp.p.cellSize.z);
+        h = md_hash_wrapped(kp.p, gx, gy, gz);
+    }
+    hash[i] = h;
+    index[i] = i;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, uint64_t n) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ===========================================================================
+// Reorder + cell bounds (particles_kernel_impl.cuh:346-424).  The previous
+// hash is read straight from memory (a cache hit for the neighbouring lane).
+// ===========================================================================
+__global__ __launch_bounds__(kBlock) void k_reorder(uint32_t *__restrict__ cellStart, uint32_t *__restrict__ cellEnd,
+                                                    float4 *__restrict__ sPos, float4 *__restrict__ sVel,
+                                                    float4 *__restrict__ sTan, const uint32_t *__restrict__ hash,
+                                                    const uint32_t *__restrict__ index,
+                                                    const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+                                                    const float4 *__restrict__ tan, uint32_t n, uint32_t numCells) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t h = hash[i];
+    const uint32_t prev = i > 0 ? hash[i - 1] : 0u;
+    if (i == 0 || h != prev) {
+        if (h < numCells) cellStart[h] = i;
+        else atomicOr(&g_flags, 2u);
+        if (i > 0 && prev < numCells) cellEnd[prev] = i;
+    }
+    if (i == n - 1) {
+        if (h < numCells) cellEnd[h] = i + 1;
+    }
+    const uint32_t s = index[i];
+    sPos[i] = pos[s];
+    if (vel != nullptr && sVel != nullptr) sVel[i] = vel[s];
+    if (tan != nullptr && sTan != nullptr) sTan[i] = tan[s];
+}
+
+// ===========================================================================
+// Pair forces (collideFilamentsKernel :694-744, collideCell :492-556,
+// collideParticles :427-462, bondHookean :466-473, extensileForce :477-487).
+//
+// The 27 reference offsets are folded per axis into distinct wrapped cells
+// with multiplicities (a grid one cell deep in z visits that layer 3 times
+// upstream: multiplicity 3).  A cell row y..z is scanned as one contiguous
+// run of the sorted arrays when its three x-cells are consecutive hashes.
+// ===========================================================================
+struct AxisCells {
+    int c[3];
+    float w[3];
+    int n;
+};
+
+__device__ __forceinline__ AxisCells axis_cells(int g, uint32_t G, bool onlyCentre) {
+    AxisCells a;
+    if (onlyCentre) {
+        a.c[0] = wrap_mod(g, G);
+        a.w[0] = 1.0f;
+        a.n = 1;
+        return a;
+    }
+    const int m = wrap_mod(g - 1, G), c = wrap_mod(g, G), p = wrap_mod(g + 1, G);
+    if (G >= 3) {
+        a.c[0] = m; a.c[1] = c; a.c[2] = p;
+        a.w[0] = a.w[1] = a.w[2] = 1.0f;
+        a.n = 3;
+    } else if (G == 2) {
+        // offsets -1 and +1 land on the same cell
+        a.c[0] = m; a.w[0] = 2.0f;
+        a.c[1] = c; a.w[1] = 1.0f;
+        a.n = 2;
+    } else {
+        a.c[0] = c; a.w[0] = 3.0f;
+        a.n = 1;
+    }
+    return a;
+}
+
+// DEM soft sphere (collideParticles)
+__device__ __forceinline__ f3 dem_force(const SimParams &p, f3 rA, f3 rB, f3 vA, f3 vB) {
+    f3 rel = rB - rA;
+    const float dist = length_periodic(p, rel);
+    const float cd = p.particleRadius + p.particleRadius;
+    f3 force = mk(0.f, 0.f, 0.f);
+    if (dist < cd) {
+        const f3 nrm = rel / dist;
+        const f3 relVel = vB - vA;
+        const f3 tanVel = relVel - (dot(relVel, nrm) * nrm);
+        force = -p.spring * (cd - dist) * nrm;
+        force += p.damping * relVel;
+        force += p.shear * tanVel;
+    }
+    return force;
+}
+
+__device__ __forceinline__ f3 bond_hookean(const SimParams &p, f3 rA, f3 rB, float k, float L) {
+    f3 rel = rB - rA;
+    const float dist = length_periodic(p, rel);
+    const f3 nrm = rel / dist;
+    return -k * (L - dist) * nrm;
+}
+
+__device__ __forceinline__ f3 extensile_force(const SimParams &p, f3 rA, f3 rB, f3 tA, f3 tB, float A) {
+    f3 rel = rB - rA;
+    const float dist = length_periodic(p, rel);
+    if (dot(tA, tB) < -0.5f) return -A * (tA - tB) / (2.0f * dist);
+    return mk(0.f, 0.f, 0.f);
+}
+
+// WCA-LJ force on A from B: F_A = -24 eps (2 s^12/r^14 - s^6/r^8) (rB - rA), r < rc
+__device__ __forceinline__ f3 wca_force(const SimParams &p, const NWExtParams &e, f3 rA, f3 rB, float rc2) {
+    f3 rel = rB - rA;
+    const float d = length_periodic(p, rel);
+    const float rr = d * d;
+    if (rr < rc2) {
+        const float inv2 = 1.0f / rr;
+        const float s2 = e.wcaSigma * e.wcaSigma * inv2;
+        const float s6 = s2 * s2 * s2;
+        const float fmag = 24.0f * e.wcaEpsilon * inv2 * s6 * (2.0f * s6 - 1.0f);
+        return -fmag * rel;
+    }
+    return mk(0.f, 0.f, 0.f);
+}
+
+template <int MODE>  // 0: reference DEM/bond/extensile, 1: WCA (+extensile)
+__global__ __launch_bounds__(kBlock) void k_collide(float4 *__restrict__ force, const float4 *__restrict__ sPos,
+                                                    const float4 *__restrict__ sVel,
+                                                    const float4 *__restrict__ sTan,
+                                                    const uint32_t *__restrict__ sIdx,
+                                                    const uint32_t *__restrict__ cellStart,
+                                                    const uint32_t *__restrict__ cellEnd, uint32_t n, KP kp) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const SimParams &p = kp.p;
+    const f3 r = mk(sPos[i]);
+    const bool useVel = MODE == 0;
+    const bool useTan = MODE == 0 || p.activity != 0.0f;
+    const f3 v = useVel ? mk(sVel[i]) : mk(0.f, 0.f, 0.f);
+    const f3 t = useTan ? mk(sTan[i]) : mk(0.f, 0.f, 0.f);
+    const uint32_t myIdx = sIdx[i];
+    uint32_t fil, ch;
+    worm_of(kp, myIdx, fil, ch);
+
+    const int gx = cell_coord(r.x, p.origin.x, p.cellSize.x);
+    const int gy = cell_coord(r.y, p.origin.y, p.cellSize.y);
+    const int gz = cell_coord(r.z, p.origin.z, p.cellSize.z);
+    const bool twoD = kp.e.stencil == NW_STENCIL_2D;
+    const AxisCells ax = axis_cells(gx, p.gridSize.x, false);
+    const AxisCells ay = axis_cells(gy, p.gridSize.y, false);
+    const AxisCells az = axis_cells(gz, p.gridSize.z, twoD);
+    const float rc2 = kp.e.wcaCutoff * kp.e.wcaCutoff;
+
+    f3 force_acc = mk(0.f, 0.f, 0.f);
+    for (int zi = 0; zi < az.n; ++zi) {
+        for (int yi = 0; yi < ay.n; ++yi) {
+            const uint32_t rowBase = (uint32_t(az.c[zi]) * p.gridSize.y + uint32_t(ay.c[yi])) * p.gridSize.x;
+            for (int xi = 0; xi < ax.n; ++xi) {
+                const float wgt = az.w[zi] * ay.w[yi] * ax.w[xi];
+                const uint32_t h = rowBase + uint32_t(ax.c[xi]);
+                const uint32_t start = cellStart[h];
+                if (start == 0xffffffffu) continue;
+                const uint32_t end = cellEnd[h];
+                f3 fc = mk(0.f, 0.f, 0.f);
+                for (uint32_t j = start; j < end; ++j) {
+                    if (j == i) continue;
+                    const f3 r2 = mk(sPos[j]);
+                    const uint32_t idx2 = sIdx[j];
+                    uint32_t fil2, ch2;
+                    worm_of(kp, idx2, fil2, ch2);
+                    const bool same = fil == fil2;
+                    if (MODE == 0) {
+                        const f3 v2 = mk(sVel[j]);
+                        if (same && ((ch + 1u == ch2) || (ch - 1u == ch2))) {
+                            fc += bond_hookean(p, r, r2, p.bondSpringK, p.bondSpringL);
+                        } else {
+                            fc += dem_force(p, r, r2, v, v2);
+                        }
+                    } else {
+                        if (!same) fc += wca_force(p, kp.e, r, r2, rc2);
+                    }
+                    if (useTan && !same) {
+                        const f3 t2 = mk(sTan[j]);
+                        fc += extensile_force(p, r, r2, t, t2, p.activity);
+                    }
+                }
+                if (wgt == 1.0f) force_acc += fc;
+                else force_acc += wgt * fc;
+            }
+        }
+    }
+    float4 F = force[myIdx];
+    F.x += force_acc.x;
+    F.y += force_acc.y;
+    F.z += force_acc.z;
+    F.w += 0.0f;
+    force[myIdx] = F;
+}
+
+// ===========================================================================
+// Bonded forces (filamentKernel :751-818).  One bead per lane; whole worms
+// are packed into each 64-lane wavefront (floor(64/n) worms, the remaining
+// lanes idle) and chain neighbours come from cross-lane shuffles instead of
+// re-reading memory.  Lane p computes the bending triple (p, p+1, p+2) and
+// the bond (p, p+1); the shares of beads p+1, p+2 are shuffled back.
+// ===========================================================================
+__device__ __forceinline__ f3 shfl_down3(f3 a, int d) {
+    return mk(__shfl_down(a.x, d, 64), __shfl_down(a.y, d, 64), __shfl_down(a.z, d, 64));
+}
+__device__ __forceinline__ f3 shfl_up3(f3 a, int d) {
+    return mk(__shfl_up(a.x, d, 64), __shfl_up(a.y, d, 64), __shfl_up(a.z, d, 64));
+}
+
+__global__ __launch_bounds__(kBlock) void k_filament_wave(float4 *__restrict__ force, float4 *__restrict__ tangent,
+                                                          const float4 *__restrict__ pos, uint32_t W, KP kp) {
+    const SimParams &p = kp.p;
+    const uint32_t n = p.filamentSize;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t wpw = 64u / n;
+    const uint32_t wl = lane / n;
+    const uint32_t pp = lane - wl * n;
+    const uint32_t worm = wave * wpw + wl;
+    const bool active = wl < wpw && worm < W;
+    const uint32_t bead = worm * n + pp;
+
+    const float4 P = active ? pos[bead] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const f3 r = mk(P);
+    const f3 rn1 = shfl_down3(r, 1);
+    const f3 rn2 = shfl_down3(r, 2);
+    const f3 rp1 = shfl_up3(r, 1);
+
+    // bending triple (pp, pp+1, pp+2)
+    f3 fi = mk(0.f, 0.f, 0.f), fj = fi, fk = fi;
+    const bool hasTriple = n >= 3 && pp + 2u < n;
+    if (hasTriple) {
+        f3 r_ij = rn1 - r;
+        f3 r_jk = rn2 - rn1;
+        const float d_ij = length_periodic(p, r_ij);
+        const float d_jk = length_periodic(p, r_jk);
+        const float d_ij_jk = dot(r_ij, r_jk);
+        const float A = p.bondBendingK / (d_ij * d_jk);
+        const f3 B = r_ij * d_ij_jk / dot(r_ij, r_ij);
+        const f3 C = r_jk * d_ij_jk / dot(r_jk, r_jk);
+        fi = -A * (r_jk - B);
+        fk = -A * (C - r_ij);
+        fj = mk(-fi.x - fk.x, -fi.y - fk.y, -fi.z - fk.z);
+    }
+    const f3 fk_from2 = shfl_up3(fk, 2);  // triple pp-2 acting on bead pp
+    const f3 fj_from1 = shfl_up3(fj, 1);  // triple pp-1 acting on bead pp
+
+    // chain bond (pp, pp+1) and drive (config physics)
+    const bool chain = kp.e.bondedMode == NW_BONDED_CHAIN;
+    f3 fb = mk(0.f, 0.f, 0.f), drive = fb;
+    if (chain && pp + 1u < n) {
+        f3 d = rn1 - r;
+        const float len = length_periodic(p, d);
+        fb = (p.bondSpringK * (len - p.bondSpringL) / len) * d;
+        drive = (kp.e.driveForce / len) * d;
+    }
+    const f3 fb_prev = shfl_up3(fb, 1);  // bond (pp-1, pp) acting on bead pp is -fb(pp-1)
+
+    if (!active) return;
+    if (n < 3 && !chain) return;
+
+    float4 F = force[bead];
+    f3 acc = mk(F.x, F.y, F.z);
+    if (pp >= 2u) acc += fk_from2;
+    if (pp >= 1u) acc += fj_from1;
+    if (hasTriple) acc += fi;
+    if (chain) {
+        if (pp >= 1u) acc -= fb_prev;
+        if (pp + 1u < n) {
+            acc += fb;
+            acc += drive;
+        }
+    }
+    force[bead] = make_float4(acc.x, acc.y, acc.z, F.w + 0.0f);
+    if (n < 3) return;
+
+    // tangents: head one-sided, interior centred, tail one-sided; with n == 3
+    // the reference's loop never reaches its tail branch
+    f3 tg;
+    bool write = true;
+    if (pp == 0u) tg = tangent_of(p, r, rn1);
+    else if (pp + 1u < n) tg = tangent_of(p, rp1, rn1);
+    else {
+        tg = tangent_of(p, rp1, r);
+        write = n > 3u;
+    }
+    if (write) tangent[bead] = f4(tg, 0.0f);
+}
+
+// Worms longer than a wavefront: one thread per worm walks the chain.
+__global__ __launch_bounds__(kBlock) void k_filament_serial(float4 *__restrict__ force, float4 *__restrict__ tangent,
+                                                            const float4 *__restrict__ pos, uint32_t W, KP kp) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W) return;
+    const SimParams &p = kp.p;
+    const uint32_t n = p.filamentSize;
+    const uint32_t head = w * n;
+    if (n >= 3) {
+        for (uint32_t q = 0; q + 2u < n; ++q) {
+            const uint32_t i = head + q;
+            const f3 ri = mk(pos[i]), rj = mk(pos[i + 1]), rk = mk(pos[i + 2]);
+            f3 r_ij = rj - ri, r_jk = rk - rj;
+            const float d_ij = length_periodic(p, r_ij);
+            const float d_jk = length_periodic(p, r_jk);
+            const float d_ij_jk = dot(r_ij, r_jk);
+            const float A = p.bondBendingK / (d_ij * d_jk);
+            const f3 B = r_ij * d_ij_jk / dot(r_ij, r_ij);
+            const f3 C = r_jk * d_ij_jk / dot(r_jk, r_jk);
+            const f3 fi = -A * (r_jk - B);
+            const f3 fk = -A * (C - r_ij);
+            const f3 fj = mk(-fi.x - fk.x, -fi.y - fk.y, -fi.z - fk.z);
+            float4 a = force[i], b = force[i + 1], c = force[i + 2];
+            force[i] = make_float4(a.x + fi.x, a.y + fi.y, a.z + fi.z, a.w + 0.0f);
+            force[i + 1] = make_float4(b.x + fj.x, b.y + fj.y, b.z + fj.z, b.w + 0.0f);
+            force[i + 2] = make_float4(c.x + fk.x, c.y + fk.y, c.z + fk.z, c.w + 0.0f);
+            tangent[i + 1] = f4(tangent_of(p, ri, rk), 0.0f);
+            if (q == 0) tangent[i] = f4(tangent_of(p, ri, rj), 0.0f);
+            else if (q == n - 3) tangent[i + 2] = f4(tangent_of(p, rj, rk), 0.0f);
+        }
+    }
+    if (kp.e.bondedMode == NW_BONDED_CHAIN && n >= 2) {
+        for (uint32_t q = 0; q + 1u < n; ++q) {
+            const uint32_t i = head + q;
+            f3 d = mk(pos[i + 1]) - mk(pos[i]);
+            const float len = length_periodic(p, d);
+            const f3 fb = (p.bondSpringK * (len - p.bondSpringL) / len) * d;
+            const f3 dv = (kp.e.driveForce / len) * d;
+            float4 a = force[i], b = force[i + 1];
+            force[i] = make_float4(a.x + fb.x + dv.x, a.y + fb.y + dv.y, a.z + fb.z + dv.z, a.w);
+            force[i + 1] = make_float4(b.x - fb.x, b.y - fb.y, b.z - fb.z, b.w);
+        }
+    }
+}
+
+// ===========================================================================
+// Langevin (langevinKernel :859-874) with the normals drawn in registers:
+// bead i uses generator block offset + i, i.e. exactly the four values that
+// randomizeGaussian(buf, 4N, 0, stddev) places in buf[4i..4i+3].
+// ===========================================================================
+__global__ __launch_bounds__(kBlock) void k_langevin(float4 *__restrict__ force, const float4 *__restrict__ vel,
+                                                     float stddev, uint32_t n, uint64_t seed, uint64_t offset,
+                                                     KP kp) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const u4 b = philox_block(seed, offset + i);
+    float z0, z1, z2, z3;
+    box_muller(b.x, b.y, z0, z1);
+    box_muller(b.z, b.w, z2, z3);
+    (void)z3;
+    const f3 xi = mk(z0 * stddev + 0.0f, z1 * stddev + 0.0f, z2 * stddev + 0.0f);
+    f3 f = mk(0.f, 0.f, 0.f);
+    f -= kp.p.gamma * mk(vel[i]);
+    f += xi;
+    float4 F = force[i];
+    force[i] = make_float4(F.x + f.x, F.y + f.y, F.z + f.z, F.w + 0.0f);
+}
+
+// randomizeUniform / randomizeGaussian: element e comes from block
+// offset + e/4, lane e%4 (normals: Box-Muller on lanes (0,1) and (2,3)).
+__global__ __launch_bounds__(kBlock) void k_randomize(float *__restrict__ out, uint32_t n, int gaussian, float mean,
+                                                      float stddev, uint64_t seed, uint64_t offset) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t e0 = uint64_t(t) * 4u;
+    if (e0 >= n) return;
+    const u4 b = philox_block(seed, offset + t);
+    float v[4];
+    if (gaussian) {
+        box_muller(b.x, b.y, v[0], v[1]);
+        box_muller(b.z, b.w, v[2], v[3]);
+        for (int k = 0; k < 4; ++k) v[k] = v[k] * stddev + mean;
+    } else {
+        v[0] = u01(b.x); v[1] = u01(b.y); v[2] = u01(b.z); v[3] = u01(b.w);
+    }
+    for (int k = 0; k < 4; ++k)
+        if (e0 + k < n) out[e0 + k] = v[k];
+}
+
+__device__ __forceinline__ float uniform_at(uint64_t seed, uint64_t offset, uint32_t e) {
+    const u4 b = philox_block(seed, offset + (e >> 2));
+    const uint32_t k = e & 3u;
+    const uint32_t x = k == 0 ? b.x : k == 1 ? b.y : k == 2 ? b.z : b.w;
+    return u01(x);
+}
+
+// ===========================================================================
+// SRD (particles_kernel_impl.cuh:559-691).  One thread per SRD cell walks its
+// sorted segment in order, so sums follow the reference's summation order.
+// ===========================================================================
+__global__ __launch_bounds__(kBlock) void k_srd_com(float4 *__restrict__ com, const float4 *__restrict__ sVel,
+                                                    const uint32_t *__restrict__ cellStart,
+                                                    const uint32_t *__restrict__ cellEnd, uint32_t numCells) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= numCells) return;
+    const uint32_t start = cellStart[c];
+    f3 m = mk(0.f, 0.f, 0.f);
+    if (start != 0xffffffffu) {
+        const uint32_t end = cellEnd[c];
+        float M = 0.0f;
+        for (uint32_t i = start; i < end; ++i) {
+            const float4 V = sVel[i];
+            m += V.w * mk(V);
+            M += V.w;
+        }
+        if (M > 0.0f) m = m * (1.0f / M);
+    }
+    com[c] = f4(m, 0.0f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_srd_isokinetic(float4 *__restrict__ vel, float4 *__restrict__ sVel,
+                                                           const uint32_t *__restrict__ cellStart,
+                                                           const uint32_t *__restrict__ cellEnd,
+                                                           const uint32_t *__restrict__ index, uint32_t numCells,
+                                                           KP kp) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= numCells) return;
+    const uint32_t start = cellStart[c];
+    if (start == 0xffffffffu) return;
+    const uint32_t end = cellEnd[c];
+    float E = 0.0f;
+    for (uint32_t i = start; i < end; ++i) {
+        const float4 V = sVel[i];
+        const f3 v = mk(V);
+        E += 0.5f * V.w * dot(v, v);
+    }
+    const float lambda = sqrtf(kp.s.kbT / E);
+    for (uint32_t i = start; i < end; ++i) {
+        const uint32_t idx = index[i];
+        const float4 V = vel[idx];
+        const float4 nv = f4(lambda * mk(V), sVel[i].w);
+        vel[idx] = nv;
+        sVel[i] = nv;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_collide_solvent(const float4 *__restrict__ pos, float4 *__restrict__ vel,
+                                                            const float4 *__restrict__ com, uint32_t numCells,
+                                                            uint32_t n, float cosA, float sinA, uint64_t seed,
+                                                            uint64_t offset, KP kp) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (!kp.e.srdCollideFilaments && i < kp.p.numParticles) return;
+    const f3 r = mk(pos[i]);
+    const float4 V = vel[i];
+    const f3 v = mk(V);
+    const uint32_t cell = srd_hash(kp.p, kp.s, r);
+    if (cell >= numCells) return;
+    const f3 u = mk(com[cell]);
+    const float uni = uniform_at(seed, offset, cell);
+    const float sgn = uni > 0.5f ? 1.0f : -1.0f;
+    const float s = sgn * sinA;
+    const f3 d = v - u;
+    const f3 rot = mk(d.x * cosA - d.y * s, d.x * s + d.y * cosA, d.z);
+    const f3 nv = u + rot;
+    vel[i] = f4(nv, V.w);
+}
+
+// ===========================================================================
+// Polarity reversal (reverseFilamentsKernel :820-857)
+// ===========================================================================
+__global__ __launch_bounds__(kBlock) void k_reverse(float4 *__restrict__ force, float4 *__restrict__ forceOld,
+                                                    float4 *__restrict__ vel, float4 *__restrict__ pos, uint32_t W,
+                                                    uint64_t seed, uint64_t offset, KP kp) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W) return;
+    if (uniform_at(seed, offset, w) > kp.p.reverseProbability) return;
+    const uint32_t n = kp.p.filamentSize;
+    const uint32_t head = w * n;
+    for (uint32_t q = 0; q < n / 2; ++q) {
+        const uint32_t i = head + q, j = head + n - 1 - q;
+        float4 t;
+        t = pos[i]; pos[i] = pos[j]; pos[j] = t;
+        t = vel[i]; vel[i] = vel[j]; vel[j] = t;
+        t = force[i]; force[i] = force[j]; force[j] = t;
+        t = forceOld[i]; forceOld[i] = forceOld[j]; forceOld[j] = t;
+    }
+}
+
+// ===========================================================================
+// launchers
+// ===========================================================================
+hipError_t launch_integrate_filaments(float4 *pos, float4 *vel, float4 *f, float4 *fold, float dt, uint32_t n,
+                                      const KP &kp, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_integrate_filaments, dim3(grid_for(n)), dim3(kBlock), 0, st, pos, vel, f, fold, dt, n, kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_integrate_solvent(float4 *pos, float4 *vel, float dt, uint32_t start, uint32_t ns,
+                                    const KP &kp, hipStream_t st) {
+    if (ns == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_integrate_solvent, dim3(grid_for(ns)), dim3(kBlock), 0, st, pos, vel, dt, start, ns, kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_calc_hash(uint32_t *hash, uint32_t *index, const float4 *pos, uint32_t n, bool srd,
+                            const KP &kp, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_calc_hash, dim3(grid_for(n)), dim3(kBlock), 0, st, hash, index, pos, n, int(srd), kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint64_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(kBlock), 0, st, p, value, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_reorder(uint32_t *cellStart, uint32_t *cellEnd, float4 *sPos, float4 *sVel, float4 *sTan,
+                          const uint32_t *hash, const uint32_t *index, const float4 *pos, const float4 *vel,
+                          const float4 *tan, uint32_t n, uint32_t numCells, hipStream_t st) {
+    hipError_t e = launch_fill_u32(cellStart, 0xffffffffu, numCells, st);
+    if (e != hipSuccess || n == 0) return e;
+    hipLaunchKernelGGL(k_reorder, dim3(grid_for(n)), dim3(kBlock), 0, st, cellStart, cellEnd, sPos, sVel, sTan, hash,
+                       index, pos, vel, tan, n, numCells);
+    return hipGetLastError();
+}
+
+hipError_t launch_collide_filaments(float4 *force, const float4 *sPos, const float4 *sVel, const float4 *sTan,
+                                    const uint32_t *sIdx, const uint32_t *cellStart, const uint32_t *cellEnd,
+                                    uint32_t n, uint32_t numCells, const KP &kp, hipStream_t st) {
+    (void)numCells;
+    if (n == 0) return hipSuccess;
+    if (kp.e.pairPotential == NW_PAIR_WCA)
+        hipLaunchKernelGGL(k_collide<1>, dim3(grid_for(n)), dim3(kBlock), 0, st, force, sPos, sVel, sTan, sIdx,
+                           cellStart, cellEnd, n, kp);
+    else
+        hipLaunchKernelGGL(k_collide<0>, dim3(grid_for(n)), dim3(kBlock), 0, st, force, sPos, sVel, sTan, sIdx,
+                           cellStart, cellEnd, n, kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_filament_forces(float4 *force, float4 *tangent, const float4 *pos, uint32_t W, const KP &kp,
+                                  hipStream_t st) {
+    const uint32_t n = kp.p.filamentSize;
+    if (W == 0 || n == 0) return hipSuccess;
+    if (n <= 64) {
+        const uint32_t wpw = 64u / n;
+        const uint64_t waves = (uint64_t(W) + wpw - 1) / wpw;
+        const uint64_t threads = waves * 64u;
+        hipLaunchKernelGGL(k_filament_wave, dim3(grid_for(threads)), dim3(kBlock), 0, st, force, tangent, pos, W, kp);
+    } else {
+        hipLaunchKernelGGL(k_filament_serial, dim3(grid_for(W)), dim3(kBlock), 0, st, force, tangent, pos, W, kp);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_langevin(float4 *force, const float4 *vel, float stddev, uint32_t n, uint64_t seed,
+                           uint64_t offset, const KP &kp, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_langevin, dim3(grid_for(n)), dim3(kBlock), 0, st, force, vel, stddev, n, seed, offset, kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_randomize(float *out, uint32_t n, int gaussian, float mean, float stddev, uint64_t seed,
+                            uint64_t offset, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = (uint64_t(n) + 3) / 4;
+    hipLaunchKernelGGL(k_randomize, dim3(grid_for(blocks)), dim3(kBlock), 0, st, out, n, gaussian, mean, stddev,
+                       seed, offset);
+    return hipGetLastError();
+}
+
+hipError_t launch_srd_com(float4 *com, const float4 *sVel, const uint32_t *cellStart, const uint32_t *cellEnd,
+                          uint32_t numCells, hipStream_t st) {
+    if (numCells == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_srd_com, dim3(grid_for(numCells)), dim3(kBlock), 0, st, com, sVel, cellStart, cellEnd,
+                       numCells);
+    return hipGetLastError();
+}
+
+hipError_t launch_srd_isokinetic(float4 *vel, float4 *sVel, const uint32_t *cellStart, const uint32_t *cellEnd,
+                                 const uint32_t *index, uint32_t numCells, const KP &kp, hipStream_t st) {
+    if (numCells == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_srd_isokinetic, dim3(grid_for(numCells)), dim3(kBlock), 0, st, vel, sVel, cellStart,
+                       cellEnd, index, numCells, kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_collide_solvent(const float4 *pos, float4 *vel, const float4 *com, uint32_t numCells, uint32_t n,
+                                  float cosA, float sinA, uint64_t seed, uint64_t offset, const KP &kp,
+                                  hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_collide_solvent, dim3(grid_for(n)), dim3(kBlock), 0, st, pos, vel, com, numCells, n, cosA,
+                       sinA, seed, offset, kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_reverse_filaments(float4 *force, float4 *forceOld, float4 *vel, float4 *pos, uint32_t W,
+                                    uint64_t seed, uint64_t offset, const KP &kp, hipStream_t st) {
+    if (W == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reverse, dim3(grid_for(W)), dim3(kBlock), 0, st, force, forceOld, vel, pos, W, seed, offset,
+                       kp);
+    return hipGetLastError();
+}
+
+}  // namespace nw
