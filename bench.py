@@ -1,0 +1,168 @@
+"""Benchmark of the worm time step on one MI355X.
+
+Workload (default "worms_srd_800k"): 40,000 worms x 20 beads = 800,000 beads in
+a 2D periodic box L = 1000 with 5,000,000 SRD solvent particles (5 per unit
+cell), WCA excluded volume, harmonic bonds k = 57 r0 = 0.8, bending kappa = 5,
+drive f_a = 1 along the bond tangent, SRD collision every 10 steps (90 deg,
+random sign per cell, grid shift in [-0.5, 0.5), beads included), dt = 1e-3.
+A step is one ParticleSystem::update() through the C-ABI; inputs are resident
+in HBM.  value = beads x steps / wall time of the timed region.
+
+Also reported: per-kernel average durations (HIP events on the library
+stream), the roofline of the dominant kernel (algorithmic bytes per launch /
+average duration vs 8 TB/s) and the oracle's FP32 OpenMP step on the host
+cores over a bounded sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "nematic-worms_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "bead-steps/s & %HBM roofline: plain vs batched; adjoint/forward cost"
+
+# Algorithmic bytes per launch of each kernel family: every distinct input
+# element read once and every output written once (float4 = 16 B, uint = 4 B).
+# N beads, NT = beads + solvent, C = MD cells, Cs = SRD cells.
+ALG_BYTES = {
+    "integrate": lambda w: 128 * w["N"],                       # R pos vel f f_old, W the same
+    "langevin": lambda w: 48 * w["N"],                         # R vel f, W f (noise in registers)
+    "filament": lambda w: 64 * w["N"],                         # R pos, RMW force, W tangent
+    "hash": lambda w: 24 * w["N"],                             # R pos, W hash index
+    "sort": lambda w: 16 * w["N"] + 8 * w["C"],                # R/W keys+values, bin counts
+    "reorder": lambda w: (40 if w["wca"] else 104) * w["N"] + 12 * w["C"],  # R hash index, gather pos (vel tan)
+    "collide": lambda w: (52 if w["wca"] else 84) * w["N"] + 8 * w["C"],  # R sPos sIdx (+sVel sTan), RMW f
+    "srd_hash": lambda w: 24 * w["NT"],
+    "srd_sort": lambda w: 16 * w["NT"] + 8 * w["Cs"],
+    "srd_reorder": lambda w: 72 * w["NT"] + 12 * w["Cs"],      # R hash index, gather pos vel
+    "srd_com": lambda w: 16 * w["NT"] + 24 * w["Cs"],          # R sVel, cell bounds, W com
+    "srd_thermo": lambda w: 68 * w["NT"] + 8 * w["Cs"],
+    "srd_collide": lambda w: 48 * w["NT"] + 16 * w["Cs"],      # R pos, RMW vel, R com
+    "srd_stream": lambda w: 64 * w["Ns"],                      # RMW pos vel of the solvent
+}
+
+
+def build_workload(name):
+    from nwamd import system
+    if name == "worms_srd_800k":
+        cfg = system.config_srd_800k()
+    elif name == "worms_dry_4k":
+        cfg = system.config_dry_4k()
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    s = system.System(cfg)
+    system.apply_worm_physics(s, kappa=5.0, gamma=2.0, kT=0.1)
+    s.reset(system.NW_INIT_LATTICE_RANDOM if cfg.srd else system.NW_INIT_RANDOM, 0.1)
+    p = s.params
+    w = dict(N=s.n, Ns=s.ns, NT=s.n + s.ns, C=p.numCells, Cs=s.solvent.numCells if cfg.srd else 0,
+             wca=cfg.ext.pairPotential == 1)
+    return s, cfg, w
+
+
+def timers(lib, names, enable):
+    for n in names:
+        lib.nwTimerEnable(n.encode(), 1 if enable else 0)
+
+
+def cpu_baseline(s, cfg, dt, steps):
+    """The oracle's FP32 step (OpenMP) on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    from nwamd import system
+    seed, offset = s.rng_state()
+    cs = orc.CpuSystem(s.params, s.solvent, cfg.ext, s.get(system.POSITION), s.get(system.VELOCITY),
+                       s.get(system.TANGENT), s.get(system.FORCE), s.get(system.FORCE_OLD), num_solvent=s.ns,
+                       srd=bool(cfg.srd), srd_gap=cfg.srdGap or 5, dt=dt, seed=seed, offset=offset,
+                       step=s.step_count, langevin_fdt=bool(cfg.langevinFdt),
+                       shift_centred=bool(cfg.srdShiftCentred))
+    cs.step(1)  # warm-up (page faults, thread pool)
+    t0 = time.perf_counter()
+    cores = cs.step(steps)
+    el = time.perf_counter() - t0
+    return dict(value=s.n * steps / el, unit="bead-steps/s", cores=int(cores), kind="port",
+                sample=f"{steps} steps of the full workload ({s.n} beads + {s.ns} solvent, "
+                       f"{(steps + 9) // 10 if cfg.srd else 0} SRD collisions), oracle FP32 OpenMP step",
+                seconds=round(el, 3))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="worms_srd_800k")
+    ap.add_argument("--dt", type=float, default=1e-3)
+    ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(replicas shard across GPUs without communication; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from nwamd import abi
+    lib = abi.load()
+    if not lib.nwDeviceAvailable():
+        raise SystemExit("bench.py: no HIP device")
+    lib.cudaInit(0, None)
+    s, cfg, w = build_workload(args.workload)
+    dt = args.dt
+    names = list(ALG_BYTES)
+
+    # warm-up, with every kernel family timed to find the dominant one
+    timers(lib, names, True)
+    s.update(dt, args.warmup)
+    s.sync()
+    per = {}
+    for n in names:
+        ms, cnt = abi.timer_read(lib, n)
+        if cnt:
+            per[n] = dict(ms_per_launch=ms / cnt, launches=cnt, ms_total=ms)
+    timers(lib, names, False)
+    dominant = max(per, key=lambda k: per[k]["ms_total"]) if per else "collide"
+
+    # timed region: only the dominant kernel carries events
+    lib.nwTimerEnable(dominant.encode(), 1)
+    s.sync()
+    t0 = time.perf_counter()
+    s.update(dt, args.steps)
+    s.sync()
+    el = time.perf_counter() - t0
+    dom_ms, dom_n = abi.timer_read(lib, dominant)
+    lib.nwTimerEnable(dominant.encode(), 0)
+
+    value = w["N"] * args.steps / el
+    alg = ALG_BYTES[dominant](w)
+    avg_s = dom_ms / max(dom_n, 1) * 1e-3
+    achieved = alg / avg_s / 1e9 if avg_s > 0 else 0.0
+    roof = dict(bound="hbm", kernel=dominant, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, alg_bytes_per_launch=int(alg),
+                avg_launch_us=round(avg_s * 1e6, 2), launches=dom_n)
+    kernels = {k: dict(avg_us=round(v["ms_per_launch"] * 1e3, 2), launches=v["launches"],
+                       gbs=round(ALG_BYTES[k](w) / (v["ms_per_launch"] * 1e-3) / 1e9, 1))
+               for k, v in per.items()}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(s, cfg, dt, args.cpu_steps)
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "bead-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "beads": w["N"], "solvent": w["Ns"], "md_cells": w["C"],
+                   "srd_cells": w["Cs"], "dt": dt, "form": "plain (ParticleSystem::update through the C-ABI)"},
+        "roofline": roof, "cpu_baseline": cpu, "kernels": kernels,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -179,6 +179,11 @@ int nwDeviceAvailable(void);
 void nwRngSetSeed(uint64_t seed, uint64_t offset);
 void nwRngGetState(uint64_t *seed, uint64_t *offset);
 
+/* Host evaluation of the generator: block b of key `seed` (the four 32-bit
+   words every device draw is built from).  For host-side draws of the driver
+   (initial conditions, the SRD grid shift). */
+void nwPhiloxBlock(uint64_t seed, uint64_t block, uint32_t out[4]);
+
 /* Extension parameters (physics variants); zero-filled block = reference. */
 void nwSetExtParameters(const NWExtParams *ext);
 void nwGetExtParameters(NWExtParams *ext);

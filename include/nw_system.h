@@ -1,0 +1,77 @@
+/*
+ * nw_system.h -- C entry points of libnwhost.so, the host driver above the
+ * C-ABI.  libnwhost.so holds ParticleSystem (a restatement of
+ * simulation/particleSystem.{h,cpp}: same constructor defaults, setters,
+ * per-step call order and .xyzv writer) and drives libnwamd.so only through
+ * include/nw_abi.h.  These functions expose it to C and Python callers.
+ */
+#ifndef NW_SYSTEM_H
+#define NW_SYSTEM_H
+
+#include <stdint.h>
+#include "nw_params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Zero fields take the reference's values (simulation/particleSystem.cpp:64-148). */
+typedef struct NWSystemConfig {
+    uint32_t numFilaments;
+    uint32_t filamentSize;
+    nw_uint3 gridSize;       /* MD cell grid */
+    float cellSize;          /* 0: 2 * particleRadius (reference) */
+    nw_float3 boxSize;       /* 0: gridSize * cellSize (reference) */
+    uint32_t srd;            /* SRD solvent on */
+    nw_uint3 srdGridSize;    /* 0: (gx/2, gy/2, max(gz/2, 1)) */
+    float srdCellSize;       /* 0: 2 * cellSize */
+    float srdDensity;        /* particles per SRD cell; 0: 10 */
+    uint32_t srdGap;         /* steps between collisions; 0: 5 */
+    float srdAlpha;          /* rotation angle (radians); 0: 2.26893 (130 deg) */
+    uint32_t srdShiftCentred;/* grid shift in [-a/2, a/2) instead of [0, a) */
+    uint32_t langevinFdt;    /* Langevin noise sqrt(2 gamma kT / dt) instead of sqrt(2 gamma kT) */
+    uint64_t seed;           /* Philox key of every draw (device and host) */
+    NWExtParams ext;         /* physics variants; zero = reference */
+    uint32_t reserved[8];
+} NWSystemConfig;
+
+/* initial conditions */
+enum {
+    NW_INIT_REFERENCE = 0,   /* lattice of straight worms, half flipped, solvent +-5 (particleSystem.cpp:583-678) */
+    NW_INIT_RANDOM = 1,      /* straight worms at uniform heads and angles, rejection-sampled (no overlap < sigma) */
+    NW_INIT_LATTICE_RANDOM = 2 /* dense: rows of straight worms with random polarity */
+};
+
+typedef struct NWSystem NWSystem;
+
+NWSystem *nwSystemCreate(const NWSystemConfig *cfg);
+void nwSystemDestroy(NWSystem *sys);
+/* SimParams / SolventParams of the system (read or overwrite; applied at the next update) */
+SimParams *nwSystemParams(NWSystem *sys);
+SolventParams *nwSystemSolventParams(NWSystem *sys);
+/* initial state; kT sets Maxwellian velocities for the random modes */
+void nwSystemReset(NWSystem *sys, int mode, float kT);
+/* `steps` calls of ParticleSystem::update(dt); no host synchronisation */
+void nwSystemUpdate(NWSystem *sys, float dt, uint32_t steps);
+void nwSystemSync(NWSystem *sys);
+/* counts */
+uint32_t nwSystemNumParticles(NWSystem *sys);
+uint32_t nwSystemNumSolvent(NWSystem *sys);
+uint32_t nwSystemNumCells(NWSystem *sys);     /* cells allocated for cellStart/cellEnd */
+uint64_t nwSystemStep(NWSystem *sys);
+/* host <-> device copies of whole arrays (float4 records):
+   0 pos (N+Ns), 1 vel (N+Ns), 2 tangent (N), 3 force (N), 4 forceOld (N) */
+void nwSystemGetArray(NWSystem *sys, int which, float *host);
+void nwSystemSetArray(NWSystem *sys, int which, const float *host);
+/* device pointers of the same arrays */
+float *nwSystemDeviceArray(NWSystem *sys, int which);
+/* RNG state (seed, generator offset) at this point of the run */
+void nwSystemRngState(NWSystem *sys, uint64_t *seed, uint64_t *offset);
+void nwSystemSetRngState(NWSystem *sys, uint64_t offset);
+/* appends one .xyzv frame (particleSystem.cpp:481-532) */
+void nwSystemWriteOutputs(NWSystem *sys, const char *fileName, int iteration, float dt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

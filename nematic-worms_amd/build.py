@@ -18,6 +18,7 @@ OUT = os.path.join(HERE, "nwamd")
 INCLUDE = os.path.join(ROOT, "include")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 
 # Device code: no FMA contraction unless a kernel opts in, IEEE-correct fp32
@@ -64,17 +65,18 @@ def build(verbose=True):
     objs = _compile_objects(LIB_SOURCES, objdir, [])
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", lib] + objs)
 
-    host_srcs = [s for s in HOST_SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    if host_srcs:
-        hobjs = _compile_objects(host_srcs, objdir, ["-I" + os.path.join(CSRC, "host")])
-        hostlib = os.path.join(OUT, "libnwhost.so")
-        _run([HIPCC, "-shared", "-fPIC", "-o", hostlib] + hobjs +
-             [f"-L{OUT}", "-lnwamd", "-Wl,-rpath,$ORIGIN"])
-        cli_srcs = [s for s in CLI_SOURCES if os.path.exists(os.path.join(CSRC, s))]
-        if cli_srcs:
-            cobjs = _compile_objects(cli_srcs, objdir, ["-I" + os.path.join(CSRC, "host")])
-            _run([HIPCC, "-o", os.path.join(OUT, "nw")] + cobjs +
-                 [f"-L{OUT}", "-lnwhost", "-lnwamd", "-Wl,-rpath,$ORIGIN"])
+    # the host driver is plain C++ above the C-ABI: no HIP in it
+    host_flags = [CXX, "-std=c++17", "-O2", "-fPIC", "-Wall", f"-I{INCLUDE}", "-I" + os.path.join(CSRC, "host")]
+    hobjs = []
+    for src in HOST_SOURCES:
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        _run(host_flags + ["-c", os.path.join(CSRC, src), "-o", obj])
+        hobjs.append(obj)
+    hostlib = os.path.join(OUT, "libnwhost.so")
+    _run([CXX, "-shared", "-o", hostlib] + hobjs + [f"-L{OUT}", "-lnwamd", "-Wl,-rpath,$ORIGIN"])
+    cobj = os.path.join(objdir, "main.cpp.o")
+    _run(host_flags + ["-c", os.path.join(CSRC, CLI_SOURCES[0]), "-o", cobj])
+    _run([CXX, "-o", os.path.join(OUT, "nw"), cobj, f"-L{OUT}", "-lnwhost", "-lnwamd", "-Wl,-rpath,$ORIGIN"])
     return lib
 
 

@@ -1,0 +1,540 @@
+// particle_system.cpp -- see particle_system.h.  Every device operation goes
+// through the C-ABI of include/nw_abi.h (libnwamd.so).
+#include "particle_system.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <stdexcept>
+
+namespace {
+constexpr uint64_t kHostKey = 0xC0FFEE1234567ull;      // host uniform stream (initial conditions)
+constexpr uint64_t kShiftKey = 0x5eed5eed5eed5eedull;  // SRD grid shift stream
+
+float u01_open(uint32_t x) { return float(x >> 8) * 5.9604644775390625e-8f; }  // [0, 1)
+
+nw_float3 f3(float x, float y, float z) { return nw_float3{x, y, z}; }
+}  // namespace
+
+// --------------------------------------------------------------------------
+// construction (particleSystem.cpp:39-111, 124-148, 172-223)
+// --------------------------------------------------------------------------
+void ParticleSystem::_defaults(uint numFilaments, uint filamentSize, nw_uint3 gridSize, bool srd) {
+    m_numFilaments = numFilaments;
+    m_filamentSize = filamentSize;
+    m_numParticles = numFilaments * filamentSize;
+    m_gridSize = gridSize;
+    m_solventEnabled = srd;
+
+    m_params.numFilaments = m_numFilaments;
+    m_params.filamentSize = m_filamentSize;
+    m_params.numParticles = m_numParticles;
+    m_params.particleRadius = 0.5f;
+    m_params.particleMass = 1.0f;
+
+    m_numGridCells = gridSize.x * gridSize.y * gridSize.z;
+    m_params.gridSize = gridSize;
+    m_params.numCells = m_numGridCells;
+    const float cellSize = m_params.particleRadius * 2.0f;
+    m_params.cellSize = f3(cellSize, cellSize, cellSize);
+    m_params.boxSize = f3(gridSize.x * cellSize, gridSize.y * cellSize, gridSize.z * cellSize);
+    m_params.origin = f3(-m_params.boxSize.x / 2.0f, -m_params.boxSize.y / 2.0f, -m_params.boxSize.z / 2.0f);
+    m_params.boundaryX = m_params.boundaryY = m_params.boundaryZ = PERIODIC;
+
+    m_params.bondSpringK = 57.f;
+    m_params.bondSpringL = m_params.particleRadius * 0.8f;
+    m_params.bondBendingK = 0.001f;
+    m_params.spring = 0.5f;
+    m_params.damping = 0.02f;
+    m_params.shear = 0.1f;
+    m_params.attraction = 0.0f;
+    m_params.boundaryDamping = -0.5f;
+    m_params.activity = 0.1f;
+    m_params.reverseProbability = 0.0f;
+    m_params.kbT = 1.0f;
+    m_params.gamma = 0.01f;
+    m_params.gravity = f3(0.f, 0.f, 0.f);
+}
+
+void ParticleSystem::_initSolventParams(const NWSystemConfig *cfg) {
+    if (!m_solventEnabled) return;
+    m_solvent.enabled = true;
+    m_solvent.alpha = (cfg && cfg->srdAlpha != 0.0f) ? cfg->srdAlpha : 2.26893f;  // 130 deg
+    nw_uint3 g{m_params.gridSize.x / 2, m_params.gridSize.y / 2, std::max(m_params.gridSize.z / 2, 1u)};
+    if (cfg && cfg->srdGridSize.x) g = cfg->srdGridSize;
+    m_solvent.gridSize = g;
+    if (cfg && cfg->srdCellSize != 0.0f) {
+        // cubic cells of the given size; an axis one cell deep spans the box
+        m_solvent.cellSize = f3(cfg->srdCellSize, cfg->srdCellSize,
+                                g.z == 1 ? m_params.boxSize.z : cfg->srdCellSize);
+    } else {
+        m_solvent.cellSize = f3(m_params.cellSize.x * 2, m_params.cellSize.y * 2, m_params.cellSize.z * 2);
+    }
+    m_solvent.numCells = g.x * g.y * g.z;
+    m_solvent.randOffset = f3(0.f, 0.f, 0.f);
+    m_solvent.particleMass = 1.0f;
+    m_solvent.kbT = m_params.kbT;
+    const float density = (cfg && cfg->srdDensity != 0.0f) ? cfg->srdDensity : 10.f;
+    m_solventUpdateGap = (cfg && cfg->srdGap) ? cfg->srdGap : 5;
+    m_numSolvent = uint(density * m_solvent.numCells);
+    m_numSolventCells = m_solvent.numCells;
+    m_solvent.numParticles = m_numSolvent;
+}
+
+ParticleSystem::ParticleSystem(uint numFilaments, uint filamentSize, nw_uint3 gridSize, bool srdSolvent) {
+    _defaults(numFilaments, filamentSize, gridSize, srdSolvent);
+    _initSolventParams(nullptr);
+    _initialize();
+}
+
+ParticleSystem::ParticleSystem(const NWSystemConfig &cfg) {
+    _defaults(cfg.numFilaments, cfg.filamentSize, cfg.gridSize, cfg.srd != 0);
+    if (cfg.cellSize != 0.0f) {
+        const float c = cfg.cellSize;
+        m_params.cellSize = f3(c, c, c);
+        m_params.boxSize = f3(m_gridSize.x * c, m_gridSize.y * c, m_gridSize.z * c);
+    }
+    if (cfg.boxSize.x != 0.0f) {
+        // box given: cells tile it exactly
+        m_params.boxSize = cfg.boxSize;
+        m_params.cellSize = f3(cfg.boxSize.x / m_gridSize.x, cfg.boxSize.y / m_gridSize.y,
+                               cfg.boxSize.z / m_gridSize.z);
+    }
+    m_params.origin = f3(-m_params.boxSize.x / 2.0f, -m_params.boxSize.y / 2.0f, -m_params.boxSize.z / 2.0f);
+    m_ext = cfg.ext;
+    m_seed = cfg.seed;
+    m_shiftCentred = cfg.srdShiftCentred != 0;
+    m_langevinFdt = cfg.langevinFdt != 0;
+    _initSolventParams(&cfg);
+    _initialize();
+}
+
+ParticleSystem::~ParticleSystem() { _finalize(); }
+
+void ParticleSystem::_initialize() {
+    const size_t N = m_numParticles, NT = size_t(m_numParticles) + m_numSolvent;
+    m_hPos.assign(NT * 4, 0.f);
+    m_hVel.assign(NT * 4, 0.f);
+    m_hForce.assign(N * 4, 0.f);
+    m_hTangent.assign(N * 4, 0.f);
+    m_numCellsAlloc = std::max(m_numGridCells, m_solventEnabled ? m_numSolventCells : 0u);
+    m_hCellStart.assign(m_numCellsAlloc, 0);
+    m_hCellEnd.assign(m_numCellsAlloc, 0);
+
+    const size_t memSize = sizeof(float) * 4 * N, memSizeN = sizeof(float) * 4 * NT;
+    // random: 4N normals, W uniforms or Cs uniforms (the reference sizes it 4N only)
+    const size_t rndFloats = std::max<size_t>({4 * N, m_numFilaments, m_solventEnabled ? m_numSolventCells : 0, 4});
+    allocateArray((void **)&m_dPos, memSizeN);
+    allocateArray((void **)&m_dVel, memSizeN);
+    allocateArray((void **)&m_dTangent, memSize);
+    allocateArray((void **)&m_dForce, memSize);
+    allocateArray((void **)&m_dForceOld, memSize);
+    allocateArray((void **)&m_dRandom, rndFloats * sizeof(float));
+    allocateArray((void **)&m_dSortedPos, memSizeN);
+    allocateArray((void **)&m_dSortedVel, memSizeN);
+    allocateArray((void **)&m_dSortedTangent, memSize);
+    allocateArray((void **)&m_dGridParticleHash, NT * sizeof(uint));
+    allocateArray((void **)&m_dGridParticleIndex, NT * sizeof(uint));
+    allocateArray((void **)&m_dCellStart, size_t(m_numCellsAlloc) * sizeof(uint));
+    allocateArray((void **)&m_dCellEnd, size_t(m_numCellsAlloc) * sizeof(uint));
+    if (m_solventEnabled) allocateArray((void **)&m_dSolventCellCOM, size_t(m_numSolventCells) * 16);
+
+    // the reference never clears force / forceOld before the first step
+    _upload(m_dForce, m_hForce, N);
+    _upload(m_dForceOld, m_hForce, N);
+    _upload(m_dTangent, m_hTangent, N);
+    std::vector<float> zeros(size_t(m_numCellsAlloc), 0.f);
+    copyArrayToDevice(m_dCellEnd, zeros.data(), 0, int(zeros.size() * sizeof(float)));
+    m_bInitialized = true;
+}
+
+void ParticleSystem::_finalize() {
+    if (!m_bInitialized) return;
+    float *fa[] = {m_dPos, m_dVel, m_dTangent, m_dForce, m_dForceOld, m_dRandom, m_dSortedPos, m_dSortedVel,
+                   m_dSortedTangent, m_dSolventCellCOM};
+    for (float *p : fa)
+        if (p) freeArray(p);
+    uint *ua[] = {m_dGridParticleHash, m_dGridParticleIndex, m_dCellStart, m_dCellEnd};
+    for (uint *p : ua)
+        if (p) freeArray(p);
+    m_bInitialized = false;
+}
+
+void ParticleSystem::_upload(float *dev, const std::vector<float> &host, size_t count4) {
+    if (count4) copyArrayToDevice(dev, host.data(), 0, int(count4 * 4 * sizeof(float)));
+}
+
+// --------------------------------------------------------------------------
+// the step (particleSystem.cpp:249-339)
+// --------------------------------------------------------------------------
+void ParticleSystem::update(float deltaTime) {
+    nwRngSetSeed(m_seed, m_rngOffset);
+    setParameters(&m_params);
+    nwSetExtParameters(&m_ext);
+
+    integrateFilaments(m_dPos, m_dVel, m_dForce, m_dForceOld, deltaTime, m_numParticles);
+
+    if (m_params.reverseProbability > 0.0f)
+        reverseFilaments(m_dForce, m_dForceOld, m_dVel, m_dPos, m_dRandom, m_numFilaments);
+
+    if (!m_solventEnabled) {
+        const float kT = m_langevinFdt ? m_params.kbT / deltaTime : m_params.kbT;
+        langevinThermostat(m_dForce, m_dVel, m_dRandom, m_params.gamma, kT, m_numParticles);
+    }
+
+    filamentForces(m_dForce, m_dTangent, m_dPos, float(m_numFilaments));
+
+    calcHash(m_dGridParticleHash, m_dGridParticleIndex, m_dPos, int(m_numParticles));
+    sortParticles(m_dGridParticleHash, m_dGridParticleIndex, m_numParticles);
+
+    // the WCA pair pass reads positions only; skip gathering what it does not read
+    const bool needVT = m_ext.pairPotential == NW_PAIR_REF_DEM || m_params.activity != 0.0f;
+    reorderDataAndFindCellStart(m_dCellStart, m_dCellEnd, m_dSortedPos, needVT ? m_dSortedVel : nullptr,
+                                needVT ? m_dSortedTangent : nullptr, m_dGridParticleHash, m_dGridParticleIndex,
+                                m_dPos, needVT ? m_dVel : nullptr, needVT ? m_dTangent : nullptr, m_numParticles,
+                                m_numGridCells);
+
+    collideFilaments(m_dForce, m_dSortedPos, m_dSortedVel, m_dSortedTangent, m_dGridParticleIndex, m_dCellStart,
+                     m_dCellEnd, m_numParticles, m_numGridCells);
+
+    if (m_solventEnabled && (m_updateCount % m_solventUpdateGap) == 0) _solventUpdate(deltaTime * m_solventUpdateGap);
+
+    m_updateCount++;
+    uint64_t seed = 0;
+    nwRngGetState(&seed, &m_rngOffset);
+}
+
+// particleSystem.cpp:341-413
+void ParticleSystem::_solventUpdate(float dt) {
+    uint32_t b[4];
+    nwPhiloxBlock(m_seed ^ kShiftKey, m_solventUpdateCount, b);
+    const float ux = u01_open(b[0]), uy = u01_open(b[1]);
+    m_solvent.randOffset.x = (m_shiftCentred ? ux - 0.5f : ux) * m_solvent.cellSize.x;
+    m_solvent.randOffset.y = (m_shiftCentred ? uy - 0.5f : uy) * m_solvent.cellSize.y;
+    m_solvent.randOffset.z = 0.0f;
+    setSolventParameters(&m_solvent);
+
+    const uint NT = m_numParticles + m_numSolvent;
+    calcHash(m_dGridParticleHash, m_dGridParticleIndex, m_dPos, int(NT), true);
+    sortParticles(m_dGridParticleHash, m_dGridParticleIndex, NT);
+    reorderDataAndFindCellStart(m_dCellStart, m_dCellEnd, m_dSortedPos, m_dSortedVel, nullptr, m_dGridParticleHash,
+                                m_dGridParticleIndex, m_dPos, m_dVel, nullptr, NT, m_numCellsAlloc);
+    if (m_ext.srdThermostat == NW_SRD_THERMO_ISOKINETIC)
+        solventIsokineticThermostat(m_dVel, m_dSortedVel, m_dCellStart, m_dCellEnd, m_dGridParticleIndex,
+                                    m_numSolventCells);
+    solventCellCentorOfMomentum(m_dSolventCellCOM, m_dSortedVel, m_dCellStart, m_dCellEnd, m_dGridParticleIndex,
+                                m_numSolventCells);
+    collideSolvent(m_dPos, m_dVel, m_dSolventCellCOM, m_dRandom, m_numSolventCells, NT);
+    integrateSolvent(m_dPos, m_dVel, dt, m_numParticles, m_numSolvent);
+    m_solventUpdateCount++;
+}
+
+// --------------------------------------------------------------------------
+// initial conditions
+// --------------------------------------------------------------------------
+float ParticleSystem::_frand() {
+    uint32_t b[4];
+    nwPhiloxBlock(m_seed ^ kHostKey, m_hostDraws / 4, b);
+    const uint32_t x = b[m_hostDraws % 4];
+    m_hostDraws++;
+    return u01_open(x);
+}
+
+// particleSystem.cpp:583-678
+void ParticleSystem::reset() {
+    const SimParams &p = m_params;
+    const float filamentLength = p.filamentSize * p.bondSpringL + 2.0f * p.particleRadius;
+    const int xdim_max = std::max(1, int(std::floor(p.boxSize.x / filamentLength)));
+    const int ydim_max = std::max(1, int(std::floor(p.boxSize.y / (2 * p.particleRadius))));
+    const int xyplane_max = xdim_max * ydim_max;
+    for (uint i = 0; i < m_numParticles; i++) {
+        const uint w = i / p.filamentSize;
+        const float x_head = (w % xdim_max) * filamentLength + p.particleRadius;
+        const float x = x_head + (i % p.filamentSize) * p.bondSpringL;
+        const float y = 2.f * p.particleRadius * ((w / xdim_max) % ydim_max + 1);
+        const float z = 2.f * p.particleRadius * (w / xyplane_max + 1);
+        float *P = &m_hPos[4 * size_t(i)], *V = &m_hVel[4 * size_t(i)], *T = &m_hTangent[4 * size_t(i)];
+        P[0] = x + p.origin.x;
+        P[1] = y + p.origin.y;
+        P[2] = z + p.origin.z;
+        P[3] = p.particleRadius;
+        V[0] = 0.1f * (_frand() * 2.0f - 1.0f);
+        V[1] = 0.1f * (_frand() * 2.0f - 1.0f);
+        V[2] = 0.1f * (_frand() * 2.0f - 1.0f);
+        V[3] = p.particleMass;
+        T[0] = 1.0f;
+        T[1] = T[2] = T[3] = 0.0f;
+    }
+    // flip half the worms
+    for (uint w = 0; w < p.numFilaments; w++) {
+        if (_frand() < 0.5f) continue;
+        for (uint q = 0; q < p.filamentSize / 2; q++) {
+            const size_t i = size_t(w) * p.filamentSize + q, j = size_t(w + 1) * p.filamentSize - 1 - q;
+            for (int c = 0; c < 4; ++c) {
+                std::swap(m_hPos[4 * i + c], m_hPos[4 * j + c]);
+                m_hTangent[4 * i + c] = -m_hTangent[4 * i + c];
+                m_hTangent[4 * j + c] = -m_hTangent[4 * j + c];
+            }
+        }
+    }
+    if (m_solventEnabled) {
+        const float v0 = 5.0f;
+        for (uint s = 0; s < m_numSolvent; s++) {
+            const size_t i = size_t(m_numParticles) + s;
+            m_hPos[4 * i + 0] = p.boxSize.x * _frand() + p.origin.x;
+            m_hPos[4 * i + 1] = p.boxSize.y * _frand() + p.origin.y;
+            m_hPos[4 * i + 2] = p.boxSize.z * _frand() + p.origin.z;
+            m_hPos[4 * i + 3] = 0.0f;
+            m_hVel[4 * i + 0] = v0 * (_frand() * 2.0f - 1.0f);
+            m_hVel[4 * i + 1] = v0 * (_frand() * 2.0f - 1.0f);
+            m_hVel[4 * i + 2] = v0 * (_frand() * 2.0f - 1.0f);
+            m_hVel[4 * i + 3] = m_solvent.particleMass;
+        }
+    }
+    const size_t NT = size_t(m_numParticles) + m_numSolvent;
+    _upload(m_dPos, m_hPos, NT);
+    _upload(m_dVel, m_hVel, NT);
+    _upload(m_dTangent, m_hTangent, m_numParticles);
+}
+
+void ParticleSystem::resetRandom(int mode, float kT) {
+    const SimParams &p = m_params;
+    const uint n = p.filamentSize, W = p.numFilaments;
+    const bool twoD = p.gridSize.z == 1;
+    const float Lx = p.boxSize.x, Ly = p.boxSize.y, Lz = p.boxSize.z;
+    const float zmid = p.origin.z + 0.5f * Lz;
+    const float bond = p.bondSpringL;
+    const float sigma = m_ext.pairPotential == NW_PAIR_WCA ? m_ext.wcaSigma : 2.0f * p.particleRadius;
+    auto wrap = [](float x, float o, float L) {
+        float r = std::fmod(x - o, L);
+        if (r < 0) r += L;
+        return o + r;
+    };
+    const float twopi = 6.28318530717958647692f;
+
+    if (mode == NW_INIT_RANDOM) {
+        // uniform heads and headings, rejected when any bead comes closer than
+        // sigma to a bead of an already placed worm (host cell grid of size >= sigma)
+        const int gx = std::max(1, int(Lx / sigma)), gy = std::max(1, int(Ly / sigma));
+        const float cx = Lx / gx, cy = Ly / gy;
+        std::vector<std::vector<uint>> grid(size_t(gx) * gy);
+        auto cellOf = [&](float x, float y, int &ix, int &iy) {
+            ix = std::min(gx - 1, std::max(0, int((x - p.origin.x) / cx)));
+            iy = std::min(gy - 1, std::max(0, int((y - p.origin.y) / cy)));
+        };
+        std::vector<float> cand(3 * n);
+        for (uint w = 0; w < W; ++w) {
+            bool placed = false;
+            for (int attempt = 0; attempt < 100000 && !placed; ++attempt) {
+                const float hx = p.origin.x + _frand() * Lx, hy = p.origin.y + _frand() * Ly;
+                const float th = _frand() * twopi;
+                bool ok = true;
+                for (uint q = 0; q < n && ok; ++q) {
+                    const float x = wrap(hx + q * bond * std::cos(th), p.origin.x, Lx);
+                    const float y = wrap(hy + q * bond * std::sin(th), p.origin.y, Ly);
+                    cand[3 * q] = x;
+                    cand[3 * q + 1] = y;
+                    cand[3 * q + 2] = twoD ? zmid : p.origin.z + _frand() * Lz;
+                    int ix, iy;
+                    cellOf(x, y, ix, iy);
+                    for (int dy = -1; dy <= 1 && ok; ++dy)
+                        for (int dx = -1; dx <= 1 && ok; ++dx) {
+                            const int jx = (ix + dx + gx) % gx, jy = (iy + dy + gy) % gy;
+                            for (uint j : grid[size_t(jy) * gx + jx]) {
+                                float ddx = m_hPos[4 * size_t(j)] - x, ddy = m_hPos[4 * size_t(j) + 1] - y;
+                                ddx -= Lx * std::round(ddx / Lx);
+                                ddy -= Ly * std::round(ddy / Ly);
+                                if (ddx * ddx + ddy * ddy < sigma * sigma) {
+                                    ok = false;
+                                    break;
+                                }
+                            }
+                        }
+                }
+                if (!ok) continue;
+                for (uint q = 0; q < n; ++q) {
+                    const size_t i = size_t(w) * n + q;
+                    m_hPos[4 * i] = cand[3 * q];
+                    m_hPos[4 * i + 1] = cand[3 * q + 1];
+                    m_hPos[4 * i + 2] = cand[3 * q + 2];
+                    m_hPos[4 * i + 3] = p.particleRadius;
+                    int ix, iy;
+                    cellOf(cand[3 * q], cand[3 * q + 1], ix, iy);
+                    grid[size_t(iy) * gx + ix].push_back(uint(i));
+                    m_hTangent[4 * i] = std::cos(th);
+                    m_hTangent[4 * i + 1] = std::sin(th);
+                    m_hTangent[4 * i + 2] = 0.f;
+                    m_hTangent[4 * i + 3] = 0.f;
+                }
+                placed = true;
+            }
+            if (!placed) throw std::runtime_error("resetRandom: could not place worm without overlap");
+        }
+    } else {
+        // rows of straight worms along x with random polarity
+        const float len = (n - 1) * bond + sigma;
+        const uint perRow = std::max(1u, uint(Lx / len));
+        const uint rows = (W + perRow - 1) / perRow;
+        const float dy = Ly / rows;
+        for (uint w = 0; w < W; ++w) {
+            const uint row = w / perRow, col = w % perRow;
+            const bool flip = _frand() < 0.5f;
+            const float x0 = p.origin.x + col * (Lx / perRow) + 0.5f * sigma;
+            const float y0 = p.origin.y + (row + 0.5f) * dy;
+            for (uint q = 0; q < n; ++q) {
+                const uint qq = flip ? n - 1 - q : q;
+                const size_t i = size_t(w) * n + q;
+                m_hPos[4 * i] = wrap(x0 + qq * bond, p.origin.x, Lx);
+                m_hPos[4 * i + 1] = y0;
+                m_hPos[4 * i + 2] = twoD ? zmid : p.origin.z + 0.5f * Lz;
+                m_hPos[4 * i + 3] = p.particleRadius;
+                m_hTangent[4 * i] = flip ? -1.f : 1.f;
+                m_hTangent[4 * i + 1] = 0.f;
+                m_hTangent[4 * i + 2] = 0.f;
+                m_hTangent[4 * i + 3] = 0.f;
+            }
+        }
+    }
+    // Maxwellian velocities at kT (unit mass), Box-Muller on host draws
+    auto normal = [&]() {
+        const float u1 = 1.0f - _frand(), u2 = _frand();
+        return std::sqrt(-2.0f * std::log(u1)) * std::cos(twopi * u2);
+    };
+    const float sv = std::sqrt(kT / p.particleMass);
+    for (uint i = 0; i < m_numParticles; ++i) {
+        float *V = &m_hVel[4 * size_t(i)];
+        V[0] = sv * normal();
+        V[1] = sv * normal();
+        V[2] = twoD ? 0.f : sv * normal();
+        V[3] = p.particleMass;
+    }
+    if (m_solventEnabled) {
+        const float ss = std::sqrt(kT / m_solvent.particleMass);
+        for (uint s = 0; s < m_numSolvent; ++s) {
+            const size_t i = size_t(m_numParticles) + s;
+            m_hPos[4 * i] = p.origin.x + _frand() * Lx;
+            m_hPos[4 * i + 1] = p.origin.y + _frand() * Ly;
+            m_hPos[4 * i + 2] = twoD ? zmid : p.origin.z + _frand() * Lz;
+            m_hPos[4 * i + 3] = 0.f;
+            m_hVel[4 * i] = ss * normal();
+            m_hVel[4 * i + 1] = ss * normal();
+            m_hVel[4 * i + 2] = twoD ? 0.f : ss * normal();
+            m_hVel[4 * i + 3] = m_solvent.particleMass;
+        }
+    }
+    const size_t NT = size_t(m_numParticles) + m_numSolvent;
+    std::fill(m_hForce.begin(), m_hForce.end(), 0.f);
+    _upload(m_dPos, m_hPos, NT);
+    _upload(m_dVel, m_hVel, NT);
+    _upload(m_dTangent, m_hTangent, m_numParticles);
+    _upload(m_dForce, m_hForce, m_numParticles);
+    _upload(m_dForceOld, m_hForce, m_numParticles);
+}
+
+// --------------------------------------------------------------------------
+// arrays, dumps, output
+// --------------------------------------------------------------------------
+float *ParticleSystem::deviceArray(ParticleArray a) {
+    switch (a) {
+        case POSITION: return m_dPos;
+        case VELOCITY: return m_dVel;
+        case TANGENT: return m_dTangent;
+        case FORCE: return m_dForce;
+        case FORCE_OLD: return m_dForceOld;
+    }
+    return nullptr;
+}
+
+float *ParticleSystem::getArray(ParticleArray a) {
+    const size_t n = (a == POSITION || a == VELOCITY) ? size_t(m_numParticles) + m_numSolvent : m_numParticles;
+    std::vector<float> &h = a == POSITION ? m_hPos : a == VELOCITY ? m_hVel : a == TANGENT ? m_hTangent : m_hForce;
+    if (h.size() < 4 * n) h.resize(4 * n);
+    if (n) copyArrayFromDevice(h.data(), deviceArray(a), nullptr, int(n * 16));
+    return h.data();
+}
+
+void ParticleSystem::setArray(ParticleArray a, const float *data, int start, int count) {
+    if (count <= 0) return;
+    copyArrayToDevice(deviceArray(a), data, start * 4 * int(sizeof(float)), count * 4 * int(sizeof(float)));
+}
+
+void ParticleSystem::dumpGrid() {
+    copyArrayFromDevice(m_hCellStart.data(), m_dCellStart, nullptr, int(sizeof(uint) * m_numGridCells));
+    copyArrayFromDevice(m_hCellEnd.data(), m_dCellEnd, nullptr, int(sizeof(uint) * m_numGridCells));
+    uint maxCellSize = 0;
+    for (uint i = 0; i < m_numGridCells; i++)
+        if (m_hCellStart[i] != 0xffffffffu) maxCellSize = std::max(maxCellSize, m_hCellEnd[i] - m_hCellStart[i]);
+    printf("maximum particles per cell = %u\n", maxCellSize);
+}
+
+void ParticleSystem::dumpSolventGrid() {
+    if (!m_solventEnabled) return;
+    copyArrayFromDevice(m_hCellStart.data(), m_dCellStart, nullptr, int(sizeof(uint) * m_numSolventCells));
+    copyArrayFromDevice(m_hCellEnd.data(), m_dCellEnd, nullptr, int(sizeof(uint) * m_numSolventCells));
+    std::vector<float> com(4 * size_t(m_numSolventCells));
+    copyArrayFromDevice(com.data(), m_dSolventCellCOM, nullptr, int(com.size() * sizeof(float)));
+    uint maxCellSize = 0;
+    for (uint i = 0; i < m_numSolventCells; i++) {
+        if (m_hCellStart[i] == 0xffffffffu) continue;
+        const uint c = m_hCellEnd[i] - m_hCellStart[i];
+        if (c > maxCellSize) {
+            printf("[%u] %u CoM = {%f %f} \n", i, c, com[4 * size_t(i)], com[4 * size_t(i) + 1]);
+            maxCellSize = c;
+        }
+    }
+    printf("maximum particles per solvent cell = %u\n", maxCellSize);
+}
+
+void ParticleSystem::dumpParticles(uint start, uint count) {
+    getArray(POSITION);
+    getArray(VELOCITY);
+    for (uint i = start; i < start + count && i < m_numParticles + m_numSolvent; i++) {
+        const float *P = &m_hPos[4 * size_t(i)], *V = &m_hVel[4 * size_t(i)];
+        printf("pos: (%.4f, %.4f, %.4f, %.4f)\n", P[0], P[1], P[2], P[3]);
+        printf("vel: (%.4f, %.4f, %.4f, %.4f)\n", V[0], V[1], V[2], V[3]);
+    }
+}
+
+// particleSystem.cpp:481-532: N, a JSON parameter line, then "A|B x y z vx vy vz"
+void ParticleSystem::writeOutputs(const std::string &fileName, int iteration, float timeDelta) {
+    const size_t N = size_t(m_numParticles) + m_numSolvent;
+    getArray(POSITION);
+    getArray(VELOCITY);
+    std::ofstream fout(fileName, std::ios::out | std::ios::app);
+    fout << N << std::endl;
+    const SimParams &p = m_params;
+    fout << "{"
+         << "\"iteration\":" << iteration << ","
+         << "\"dt\":" << timeDelta << ","
+         << "\"time\":" << iteration * timeDelta << ","
+         << "\"num_particles\":" << p.numParticles << ","
+         << "\"num_filaments\":" << p.numFilaments << ","
+         << "\"filament_size\":" << p.filamentSize << ","
+         << "\"bond_spring_coef\":" << p.bondSpringK << ","
+         << "\"bond_spring_length\":" << p.bondSpringL << ","
+         << "\"bond_bending_coef\":" << p.bondBendingK << ","
+         << "\"activity\":" << p.activity << ","
+         << "\"kbT\":" << p.kbT << ","
+         << "\"gamma\":" << p.gamma << ","
+         << "\"particle_radius\":" << p.particleRadius << ","
+         << "\"particle_mass\":" << p.particleMass << ","
+         << "\"num_cells\":" << p.numCells << ","
+         << "\"grid_size\":" << "[" << p.gridSize.x << "," << p.gridSize.y << "," << p.gridSize.z << "],"
+         << "\"box_size\":" << "[" << p.boxSize.x << "," << p.boxSize.y << "," << p.boxSize.z << "],"
+         << "\"cell_size\":" << "[" << p.cellSize.x << "," << p.cellSize.y << "," << p.cellSize.z << "],"
+         << "\"boundaries\":" << "[" << int(p.boundaryX) << "," << int(p.boundaryY) << "," << int(p.boundaryZ)
+         << "],"
+         << "\"reverse_probability\":" << p.reverseProbability << ","
+         << "\"collision_spring_coef\":" << p.spring << ","
+         << "\"collision_damping_coef\":" << p.damping << ","
+         << "\"collision_shear_coef\":" << p.shear << ","
+         << "\"collision_attraction_coef\":" << p.attraction << ","
+         << "\"boundary_damping_coef\":" << p.boundaryDamping << "}" << std::endl;
+    for (size_t i = 0; i < N; i++) {
+        const char *type = i < m_numParticles ? "A" : "B";
+        fout << type << " " << m_hPos[i * 4] << " " << m_hPos[i * 4 + 1] << " " << m_hPos[i * 4 + 2] << " "
+             << m_hVel[i * 4] << " " << m_hVel[i * 4 + 1] << " " << m_hVel[i * 4 + 2] << std::endl;
+    }
+}

@@ -1,0 +1,86 @@
+// system_capi.cpp -- C entry points of libnwhost.so (include/nw_system.h).
+#include <cstdio>
+#include <new>
+#include <stdexcept>
+
+#include "particle_system.h"
+
+struct NWSystem {
+    ParticleSystem *ps;
+};
+
+extern "C" {
+
+NWSystem *nwSystemCreate(const NWSystemConfig *cfg) {
+    if (!cfg || cfg->numFilaments == 0 || cfg->filamentSize == 0 || cfg->gridSize.x == 0 || cfg->gridSize.y == 0 ||
+        cfg->gridSize.z == 0) {
+        fprintf(stderr, "nwSystemCreate: invalid configuration\n");
+        return nullptr;
+    }
+    NWSystem *s = new (std::nothrow) NWSystem;
+    if (!s) return nullptr;
+    s->ps = new ParticleSystem(*cfg);
+    return s;
+}
+
+void nwSystemDestroy(NWSystem *sys) {
+    if (!sys) return;
+    delete sys->ps;
+    delete sys;
+}
+
+SimParams *nwSystemParams(NWSystem *sys) { return &sys->ps->params(); }
+SolventParams *nwSystemSolventParams(NWSystem *sys) { return &sys->ps->solvent(); }
+
+void nwSystemReset(NWSystem *sys, int mode, float kT) {
+    try {
+        if (mode == NW_INIT_REFERENCE) sys->ps->reset();
+        else sys->ps->resetRandom(mode, kT);
+    } catch (const std::exception &e) {
+        fprintf(stderr, "nwSystemReset: %s\n", e.what());
+    }
+}
+
+void nwSystemUpdate(NWSystem *sys, float dt, uint32_t steps) {
+    for (uint32_t i = 0; i < steps; ++i) sys->ps->update(dt);
+}
+
+void nwSystemSync(NWSystem *) { threadSync(); }
+
+uint32_t nwSystemNumParticles(NWSystem *sys) { return uint32_t(sys->ps->getNumParticles()); }
+uint32_t nwSystemNumSolvent(NWSystem *sys) { return sys->ps->getNumSolvent(); }
+uint32_t nwSystemNumCells(NWSystem *sys) { return sys->ps->getNumCellsAlloc(); }
+uint64_t nwSystemStep(NWSystem *sys) { return sys->ps->getStep(); }
+
+void nwSystemGetArray(NWSystem *sys, int which, float *host) {
+    const auto a = static_cast<ParticleSystem::ParticleArray>(which);
+    const size_t n = (a == ParticleSystem::POSITION || a == ParticleSystem::VELOCITY)
+                         ? size_t(sys->ps->getNumParticles()) + sys->ps->getNumSolvent()
+                         : size_t(sys->ps->getNumParticles());
+    if (n) copyArrayFromDevice(host, sys->ps->deviceArray(a), nullptr, int(n * 16));
+}
+
+void nwSystemSetArray(NWSystem *sys, int which, const float *host) {
+    const auto a = static_cast<ParticleSystem::ParticleArray>(which);
+    const size_t n = (a == ParticleSystem::POSITION || a == ParticleSystem::VELOCITY)
+                         ? size_t(sys->ps->getNumParticles()) + sys->ps->getNumSolvent()
+                         : size_t(sys->ps->getNumParticles());
+    if (n) copyArrayToDevice(sys->ps->deviceArray(a), host, 0, int(n * 16));
+}
+
+float *nwSystemDeviceArray(NWSystem *sys, int which) {
+    return sys->ps->deviceArray(static_cast<ParticleSystem::ParticleArray>(which));
+}
+
+void nwSystemRngState(NWSystem *sys, uint64_t *seed, uint64_t *offset) {
+    if (seed) *seed = sys->ps->seed();
+    if (offset) *offset = sys->ps->rngOffset();
+}
+
+void nwSystemSetRngState(NWSystem *sys, uint64_t offset) { sys->ps->setRngOffset(offset); }
+
+void nwSystemWriteOutputs(NWSystem *sys, const char *fileName, int iteration, float dt) {
+    sys->ps->writeOutputs(fileName, iteration, dt);
+}
+
+}  // extern "C"

@@ -41,6 +41,7 @@ struct State {
     std::string lastErrorString;
     uint32_t lastHashBound = 0;   // cell count of the most recent calcHash
     const uint32_t *lastHashPtr = nullptr;
+    bool lastHashSrd = false;     // timer family of the sort / reorder that follow
     SortScratch sort;
     std::map<std::string, Timer> timers;
 };
@@ -285,17 +286,18 @@ void calcHash(uint *gridParticleHash, uint *gridParticleIndex, float *pos, int n
     NW_INIT();
     State &s = S();
     NW_REQUIRE(numParticles >= 0, "calcHash: negative particle count %d", numParticles);
-    TimedScope ts("hash", s.stream);
+    TimedScope ts(srd ? "srd_hash" : "hash", s.stream);
     NW_HIP(launch_calc_hash(gridParticleHash, gridParticleIndex, reinterpret_cast<const float4 *>(pos),
                             uint32_t(numParticles), srd, make_kp(), s.stream));
     s.lastHashBound = srd ? s.solvent.numCells : s.params.numCells;
     s.lastHashPtr = gridParticleHash;
+    s.lastHashSrd = srd;
 }
 
 void sortParticles(uint *dGridParticleHash, uint *dGridParticleIndex, uint numParticles) {
     NW_INIT();
     State &s = S();
-    TimedScope ts("sort", s.stream);
+    TimedScope ts(s.lastHashSrd ? "srd_sort" : "sort", s.stream);
     if (s.lastHashPtr == dGridParticleHash && s.lastHashBound > 0) {
         NW_HIP(counting_sort_pairs(dGridParticleHash, dGridParticleIndex, numParticles, s.lastHashBound, s.sort,
                                    s.stream));
@@ -325,7 +327,7 @@ void reorderDataAndFindCellStart(uint *cellStart, uint *cellEnd, float *sortedPo
                                  float *vel, float *tangent, uint numParticles, uint numCells) {
     NW_INIT();
     State &s = S();
-    TimedScope ts("reorder", s.stream);
+    TimedScope ts(s.lastHashSrd ? "srd_reorder" : "reorder", s.stream);
     NW_HIP(launch_reorder(cellStart, cellEnd, reinterpret_cast<float4 *>(sortedPos),
                           reinterpret_cast<float4 *>(sortedVel), reinterpret_cast<float4 *>(sortedTangent),
                           gridParticleHash, gridParticleIndex, reinterpret_cast<const float4 *>(pos),
@@ -459,6 +461,14 @@ void nwRngSetSeed(uint64_t seed, uint64_t offset) {
 void nwRngGetState(uint64_t *seed, uint64_t *offset) {
     if (seed) *seed = S().seed;
     if (offset) *offset = S().offset;
+}
+
+void nwPhiloxBlock(uint64_t seed, uint64_t block, uint32_t out[4]) {
+    const u4 b = philox_block(seed, block);
+    out[0] = b.x;
+    out[1] = b.y;
+    out[2] = b.z;
+    out[3] = b.w;
 }
 
 void nwSetExtParameters(const NWExtParams *ext) {

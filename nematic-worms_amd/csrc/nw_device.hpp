@@ -55,31 +55,32 @@ __device__ __forceinline__ void worm_of(const KP &kp, uint32_t i, uint32_t &w, u
 }
 
 // --------------------------------------------------------------------------
-// Minimum image (particles_kernel_impl.cuh:279-301).  Mutates dr like the
-// reference's lengthPeriodic and returns |dr|.
-// --------------------------------------------------------------------------
-__device__ __forceinline__ float length_periodic(const SimParams &p, f3 &dr) {
-    const f3 L = mk(p.boxSize.x, p.boxSize.y, p.boxSize.z);
-    const f3 h = L / 2.0f;
-    if (p.boundaryX == PERIODIC) {
-        if (dr.x > h.x) dr.x -= L.x;
-        else if (dr.x < -h.x) dr.x += L.x;
+// Minimum image (lengthPeriodic, particles_kernel_impl.cuh:279-301).
+// Minimum-image difference b - a.  Same branch decisions as lengthPeriodic
+// (wrap when |b - a| > L/2), but a wrapped component is evaluated against the
+// box faces, (b - L/2) - (a + L/2), whose two subtractions are exact for
+// positions near opposite faces; its rounding error then scales with the
+// separation instead of with L (FP32 b - a - L loses ~ulp(L) = 6e-5 at L = 1000).
+__device__ __forceinline__ float mi_axis(float b, float a, float L, bool periodic) {
+    float d = b - a;
+    if (periodic) {
+        const float h = L / 2.0f;
+        if (d > h) d = (b - h) - (a + h);
+        else if (d < -h) d = (b + h) - (a - h);
     }
-    if (p.boundaryY == PERIODIC) {
-        if (dr.y > h.y) dr.y -= L.y;
-        else if (dr.y < -h.y) dr.y += L.y;
-    }
-    if (p.boundaryZ == PERIODIC) {
-        if (dr.z > h.z) dr.z -= L.z;
-        else if (dr.z < -h.z) dr.z += L.z;
-    }
-    return length(dr);
+    return d;
+}
+
+__device__ __forceinline__ f3 min_image(const SimParams &p, f3 a, f3 b) {
+    return mk(mi_axis(b.x, a.x, p.boxSize.x, p.boundaryX == PERIODIC),
+              mi_axis(b.y, a.y, p.boxSize.y, p.boundaryY == PERIODIC),
+              mi_axis(b.z, a.z, p.boxSize.z, p.boundaryZ == PERIODIC));
 }
 
 // unit(b - a) under the minimum image (particles_kernel_impl.cuh:303-309)
 __device__ __forceinline__ f3 tangent_of(const SimParams &p, f3 a, f3 b) {
-    f3 rel = b - a;
-    const float d = length_periodic(p, rel);
+    const f3 rel = min_image(p, a, b);
+    const float d = length(rel);
     return rel / d;
 }
 
@@ -149,13 +150,17 @@ __device__ __forceinline__ uint32_t srd_hash(const SimParams &p, const SolventPa
 // --------------------------------------------------------------------------
 struct u4 { uint32_t x, y, z, w; };
 
-__device__ __forceinline__ u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
+__host__ __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b) {
+    return uint32_t((uint64_t(a) * uint64_t(b)) >> 32);
+}
+
+__host__ __device__ __forceinline__ u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
     const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
     const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        const uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
-        const uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
+        const uint32_t hi0 = mulhi32(M0, c.x), lo0 = M0 * c.x;
+        const uint32_t hi1 = mulhi32(M1, c.z), lo1 = M1 * c.z;
         c = u4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
         k0 += W0;
         k1 += W1;
@@ -164,7 +169,7 @@ __device__ __forceinline__ u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
 }
 
 // Generator block b of stream `seed`: counter = (b_lo, b_hi, 0, 0).
-__device__ __forceinline__ u4 philox_block(uint64_t seed, uint64_t b) {
+__host__ __device__ __forceinline__ u4 philox_block(uint64_t seed, uint64_t b) {
     return philox4x32_10(u4{uint32_t(b), uint32_t(b >> 32), 0u, 0u}, uint32_t(seed), uint32_t(seed >> 32));
 }
 

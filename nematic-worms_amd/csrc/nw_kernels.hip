@@ -201,8 +201,8 @@ __device__ __forceinline__ AxisCells axis_cells(int g, uint32_t G, bool onlyCent
 
 // DEM soft sphere (collideParticles)
 __device__ __forceinline__ f3 dem_force(const SimParams &p, f3 rA, f3 rB, f3 vA, f3 vB) {
-    f3 rel = rB - rA;
-    const float dist = length_periodic(p, rel);
+    const f3 rel = min_image(p, rA, rB);
+    const float dist = length(rel);
     const float cd = p.particleRadius + p.particleRadius;
     f3 force = mk(0.f, 0.f, 0.f);
     if (dist < cd) {
@@ -217,23 +217,23 @@ __device__ __forceinline__ f3 dem_force(const SimParams &p, f3 rA, f3 rB, f3 vA,
 }
 
 __device__ __forceinline__ f3 bond_hookean(const SimParams &p, f3 rA, f3 rB, float k, float L) {
-    f3 rel = rB - rA;
-    const float dist = length_periodic(p, rel);
+    const f3 rel = min_image(p, rA, rB);
+    const float dist = length(rel);
     const f3 nrm = rel / dist;
     return -k * (L - dist) * nrm;
 }
 
 __device__ __forceinline__ f3 extensile_force(const SimParams &p, f3 rA, f3 rB, f3 tA, f3 tB, float A) {
-    f3 rel = rB - rA;
-    const float dist = length_periodic(p, rel);
+    const f3 rel = min_image(p, rA, rB);
+    const float dist = length(rel);
     if (dot(tA, tB) < -0.5f) return -A * (tA - tB) / (2.0f * dist);
     return mk(0.f, 0.f, 0.f);
 }
 
 // WCA-LJ force on A from B: F_A = -24 eps (2 s^12/r^14 - s^6/r^8) (rB - rA), r < rc
 __device__ __forceinline__ f3 wca_force(const SimParams &p, const NWExtParams &e, f3 rA, f3 rB, float rc2) {
-    f3 rel = rB - rA;
-    const float d = length_periodic(p, rel);
+    const f3 rel = min_image(p, rA, rB);
+    const float d = length(rel);
     const float rr = d * d;
     if (rr < rc2) {
         const float inv2 = 1.0f / rr;
@@ -356,10 +356,10 @@ __global__ __launch_bounds__(kBlock) void k_filament_wave(float4 *__restrict__ f
     f3 fi = mk(0.f, 0.f, 0.f), fj = fi, fk = fi;
     const bool hasTriple = n >= 3 && pp + 2u < n;
     if (hasTriple) {
-        f3 r_ij = rn1 - r;
-        f3 r_jk = rn2 - rn1;
-        const float d_ij = length_periodic(p, r_ij);
-        const float d_jk = length_periodic(p, r_jk);
+        const f3 r_ij = min_image(p, r, rn1);
+        const f3 r_jk = min_image(p, rn1, rn2);
+        const float d_ij = length(r_ij);
+        const float d_jk = length(r_jk);
         const float d_ij_jk = dot(r_ij, r_jk);
         const float A = p.bondBendingK / (d_ij * d_jk);
         const f3 B = r_ij * d_ij_jk / dot(r_ij, r_ij);
@@ -375,8 +375,8 @@ __global__ __launch_bounds__(kBlock) void k_filament_wave(float4 *__restrict__ f
     const bool chain = kp.e.bondedMode == NW_BONDED_CHAIN;
     f3 fb = mk(0.f, 0.f, 0.f), drive = fb;
     if (chain && pp + 1u < n) {
-        f3 d = rn1 - r;
-        const float len = length_periodic(p, d);
+        const f3 d = min_image(p, r, rn1);
+        const float len = length(d);
         fb = (p.bondSpringK * (len - p.bondSpringL) / len) * d;
         drive = (kp.e.driveForce / len) * d;
     }
@@ -425,9 +425,9 @@ __global__ __launch_bounds__(kBlock) void k_filament_serial(float4 *__restrict__
         for (uint32_t q = 0; q + 2u < n; ++q) {
             const uint32_t i = head + q;
             const f3 ri = mk(pos[i]), rj = mk(pos[i + 1]), rk = mk(pos[i + 2]);
-            f3 r_ij = rj - ri, r_jk = rk - rj;
-            const float d_ij = length_periodic(p, r_ij);
-            const float d_jk = length_periodic(p, r_jk);
+            const f3 r_ij = min_image(p, ri, rj), r_jk = min_image(p, rj, rk);
+            const float d_ij = length(r_ij);
+            const float d_jk = length(r_jk);
             const float d_ij_jk = dot(r_ij, r_jk);
             const float A = p.bondBendingK / (d_ij * d_jk);
             const f3 B = r_ij * d_ij_jk / dot(r_ij, r_ij);
@@ -447,8 +447,8 @@ __global__ __launch_bounds__(kBlock) void k_filament_serial(float4 *__restrict__
     if (kp.e.bondedMode == NW_BONDED_CHAIN && n >= 2) {
         for (uint32_t q = 0; q + 1u < n; ++q) {
             const uint32_t i = head + q;
-            f3 d = mk(pos[i + 1]) - mk(pos[i]);
-            const float len = length_periodic(p, d);
+            const f3 d = min_image(p, mk(pos[i]), mk(pos[i + 1]));
+            const float len = length(d);
             const f3 fb = (p.bondSpringK * (len - p.bondSpringL) / len) * d;
             const f3 dv = (kp.e.driveForce / len) * d;
             float4 a = force[i], b = force[i + 1];

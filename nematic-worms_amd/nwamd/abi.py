@@ -108,6 +108,7 @@ PROTOTYPES = {
     "nwDeviceAvailable": (c_int, []),
     "nwRngSetSeed": (None, [c_u64, c_u64]),
     "nwRngGetState": (None, [ctypes.POINTER(c_u64), ctypes.POINTER(c_u64)]),
+    "nwPhiloxBlock": (None, [c_u64, c_u64, ctypes.POINTER(ctypes.c_uint32)]),
     "nwSetExtParameters": (None, [ctypes.POINTER(NWExtParams)]),
     "nwGetExtParameters": (None, [ctypes.POINTER(NWExtParams)]),
     "nwGetStream": (c_void_p, []),

@@ -266,7 +266,7 @@ int orc_system_step_f32(OrcSystem *sys, uint32_t steps) {
         if (sys->srdEnabled && (sys->step % sys->srdGap) == 0) {
             const float cell[3] = {sys->s.cellSize.x, sys->s.cellSize.y, sys->s.cellSize.z};
             float sh[3];
-            orc_srd_shift(sys->seed, sys->step / sys->srdGap, sys->e.srdCollideFilaments != 0, cell, sh);
+            orc_srd_shift(sys->seed, sys->step / sys->srdGap, (int)sys->srdShiftCentred, cell, sh);
             sys->s.randOffset.x = sh[0];
             sys->s.randOffset.y = sh[1];
             sys->s.randOffset.z = sh[2];

@@ -113,7 +113,7 @@ typedef struct OrcSystem {
     SimParams p;
     SolventParams s;
     NWExtParams e;
-    uint32_t numSolvent, srdGap, srdEnabled, langevinFdt;
+    uint32_t numSolvent, srdGap, srdEnabled, langevinFdt, srdShiftCentred;
     float dt;
     uint64_t seed, offset;
     uint64_t step;

@@ -28,20 +28,28 @@ static inline R F(dot)(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 static inline R F(sqrt_)(R x) { return (R)sqrt((double)x); }
 static inline R F(len)(V3 a) { return F(sqrt_)(F(dot)(a, a)); }
 
-/* lengthPeriodic (particles_kernel_impl.cuh:279-301): minimum image in place */
-static inline R F(len_periodic)(const SimParams *p, V3 *dr) {
-    const R Lx = (R)p->boxSize.x, Ly = (R)p->boxSize.y, Lz = (R)p->boxSize.z;
-    const R hx = Lx / (R)2, hy = Ly / (R)2, hz = Lz / (R)2;
-    if (p->boundaryX == PERIODIC) { if (dr->x > hx) dr->x -= Lx; else if (dr->x < -hx) dr->x += Lx; }
-    if (p->boundaryY == PERIODIC) { if (dr->y > hy) dr->y -= Ly; else if (dr->y < -hy) dr->y += Ly; }
-    if (p->boundaryZ == PERIODIC) { if (dr->z > hz) dr->z -= Lz; else if (dr->z < -hz) dr->z += Lz; }
-    return F(len)(*dr);
+/* lengthPeriodic (particles_kernel_impl.cuh:279-301): minimum image b - a.
+   A component is wrapped when |b - a| > L/2 as upstream; the wrapped value is
+   formed as (b - L/2) - (a + L/2) (in double this equals b - a - L exactly). */
+static inline R F(mi_axis)(R b, R a, R L, int periodic) {
+    R d = b - a;
+    if (periodic) {
+        const R h = L / (R)2;
+        if (d > h) d = (b - h) - (a + h);
+        else if (d < -h) d = (b + h) - (a - h);
+    }
+    return d;
+}
+static inline V3 F(min_image)(const SimParams *p, V3 a, V3 b) {
+    return F(mk)(F(mi_axis)(b.x, a.x, (R)p->boxSize.x, p->boundaryX == PERIODIC),
+                 F(mi_axis)(b.y, a.y, (R)p->boxSize.y, p->boundaryY == PERIODIC),
+                 F(mi_axis)(b.z, a.z, (R)p->boxSize.z, p->boundaryZ == PERIODIC));
 }
 
 /* getTangent (particles_kernel_impl.cuh:303-309) */
 static inline V3 F(tangent)(const SimParams *p, V3 a, V3 b) {
-    V3 rel = F(sub)(b, a);
-    const R d = F(len_periodic)(p, &rel);
+    const V3 rel = F(min_image)(p, a, b);
+    const R d = F(len)(rel);
     return F(divs)(rel, d);
 }
 
@@ -121,8 +129,8 @@ void F(orc_integrate_solvent)(const SimParams *p, const float *pos, const float 
 
 /* ---- pair terms (particles_kernel_impl.cuh:427-487; archived LJ) ---- */
 static inline V3 F(dem)(const SimParams *p, V3 rA, V3 rB, V3 vA, V3 vB) {
-    V3 rel = F(sub)(rB, rA);
-    const R dist = F(len_periodic)(p, &rel);
+    const V3 rel = F(min_image)(p, rA, rB);
+    const R dist = F(len)(rel);
     const R cd = (R)p->particleRadius + (R)p->particleRadius;
     V3 force = F(mk)(0, 0, 0);
     if (dist < cd) {
@@ -136,21 +144,21 @@ static inline V3 F(dem)(const SimParams *p, V3 rA, V3 rB, V3 vA, V3 vB) {
     return force;
 }
 static inline V3 F(bond)(const SimParams *p, V3 rA, V3 rB, R k, R L) {
-    V3 rel = F(sub)(rB, rA);
-    const R dist = F(len_periodic)(p, &rel);
+    const V3 rel = F(min_image)(p, rA, rB);
+    const R dist = F(len)(rel);
     const V3 nrm = F(divs)(rel, dist);
     return F(scl)(-k * (L - dist), nrm);
 }
 static inline V3 F(extensile)(const SimParams *p, V3 rA, V3 rB, V3 tA, V3 tB, R A) {
-    V3 rel = F(sub)(rB, rA);
-    const R dist = F(len_periodic)(p, &rel);
+    const V3 rel = F(min_image)(p, rA, rB);
+    const R dist = F(len)(rel);
     if (F(dot)(tA, tB) < (R)-0.5) return F(divs)(F(scl)(-A, F(sub)(tA, tB)), (R)2 * dist);
     return F(mk)(0, 0, 0);
 }
 /* WCA-LJ: F_A = -24 eps (2 s^12 / r^14 - s^6 / r^8) (rB - rA) for r^2 < rc^2 */
 static inline V3 F(wca)(const SimParams *p, const NWExtParams *e, V3 rA, V3 rB) {
-    V3 rel = F(sub)(rB, rA);
-    const R d = F(len_periodic)(p, &rel);
+    const V3 rel = F(min_image)(p, rA, rB);
+    const R d = F(len)(rel);
     const R rr = d * d;
     const R rc = (R)e->wcaCutoff;
     if (rr < rc * rc) {
@@ -242,9 +250,9 @@ void F(orc_filament_forces)(const SimParams *p, const NWExtParams *e, const floa
             for (uint32_t q = 0; q + 2u < n; ++q) {
                 const uint32_t i = head + q;
                 const V3 ri = F(ld)(pos, i), rj = F(ld)(pos, i + 1), rk = F(ld)(pos, i + 2);
-                V3 r_ij = F(sub)(rj, ri), r_jk = F(sub)(rk, rj);
-                const R d_ij = F(len_periodic)(p, &r_ij);
-                const R d_jk = F(len_periodic)(p, &r_jk);
+                const V3 r_ij = F(min_image)(p, ri, rj), r_jk = F(min_image)(p, rj, rk);
+                const R d_ij = F(len)(r_ij);
+                const R d_jk = F(len)(r_jk);
                 const R d_ij_jk = F(dot)(r_ij, r_jk);
                 const R A = kb / (d_ij * d_jk);
                 const V3 B = F(divs)(F(scl)(d_ij_jk, r_ij), F(dot)(r_ij, r_ij));
@@ -264,8 +272,8 @@ void F(orc_filament_forces)(const SimParams *p, const NWExtParams *e, const floa
         if (e && e->bondedMode == NW_BONDED_CHAIN && n >= 2) {
             for (uint32_t q = 0; q + 1u < n; ++q) {
                 const uint32_t i = head + q;
-                V3 d = F(sub)(F(ld)(pos, i + 1), F(ld)(pos, i));
-                const R len = F(len_periodic)(p, &d);
+                const V3 d = F(min_image)(p, F(ld)(pos, i), F(ld)(pos, i + 1));
+                const R len = F(len)(d);
                 /* archived InterForceKernel: f_i += K (r - L) / r * dr, f_{i+1} -= the same */
                 const V3 fb = F(scl)((R)p->bondSpringK * (len - (R)p->bondSpringL) / len, d);
                 /* archived DriveForceKernel: f_i += f_a * unit(r_{i+1} - r_i), i < n-1 */

@@ -206,3 +206,53 @@ def srd_shift(seed, k, centred, cell):
     c = np.ascontiguousarray(cell, np.float32)
     lib().orc_srd_shift(seed, k, int(centred), ptr(c), ptr(out))
     return out
+
+
+# ---- whole-step CPU system (the CPU baseline) --------------------------------
+class OrcSystemC(ctypes.Structure):
+    _fields_ = [
+        ("p", SimParams), ("s", SolventParams), ("e", NWExtParams),
+        ("numSolvent", u32), ("srdGap", u32), ("srdEnabled", u32), ("langevinFdt", u32), ("srdShiftCentred", u32),
+        ("dt", f32), ("seed", u64), ("offset", u64), ("step", u64),
+        ("pos", P), ("vel", P), ("force", P), ("forceOld", P), ("tangent", P),
+        ("sPos", P), ("sVel", P), ("sTan", P), ("com", P), ("rnd", P),
+        ("hash", P), ("index", P), ("cellStart", P), ("cellEnd", P),
+        ("numCellsAlloc", u32),
+    ]
+
+
+class CpuSystem:
+    """FP32 OpenMP restatement of ParticleSystem::update (simulation/particleSystem.cpp:249-413)."""
+
+    def __init__(self, p, s, e, pos, vel, tangent, force, force_old, *, num_solvent, srd, srd_gap, dt,
+                 seed, offset, step=0, langevin_fdt=False, shift_centred=False):
+        N = p.numParticles
+        NT = N + (num_solvent if srd else 0)
+        Cs = s.numCells if srd else 0
+        self.arrays = dict(
+            pos=np.ascontiguousarray(pos, np.float32).copy(), vel=np.ascontiguousarray(vel, np.float32).copy(),
+            force=np.ascontiguousarray(force, np.float32).copy(),
+            forceOld=np.ascontiguousarray(force_old, np.float32).copy(),
+            tangent=np.ascontiguousarray(tangent, np.float32).copy(),
+            sPos=np.empty((NT, 4), np.float32), sVel=np.empty((NT, 4), np.float32), sTan=np.empty((N, 4), np.float32),
+            com=np.empty((max(Cs, 1), 4), np.float32),
+            rnd=np.empty(max(4 * N, p.numFilaments, Cs, 4), np.float32),
+            hash=np.empty(NT, np.uint32), index=np.empty(NT, np.uint32))
+        ncells = max(p.numCells, Cs)
+        self.arrays["cellStart"] = np.empty(ncells, np.uint32)
+        self.arrays["cellEnd"] = np.zeros(ncells, np.uint32)
+        c = OrcSystemC()
+        c.p, c.s, c.e = p, s, e
+        c.numSolvent, c.srdGap, c.srdEnabled = num_solvent, srd_gap, int(srd)
+        c.langevinFdt, c.srdShiftCentred = int(langevin_fdt), int(shift_centred)
+        c.dt, c.seed, c.offset, c.step = dt, seed, offset, step
+        for k, a in self.arrays.items():
+            setattr(c, k, a.ctypes.data)
+        c.numCellsAlloc = ncells
+        self.c = c
+
+    def step(self, k=1):
+        return lib().orc_system_step_f32(ctypes.byref(self.c), k)
+
+    def get(self, name):
+        return self.arrays[name]

@@ -1,0 +1,147 @@
+"""The host driver (ParticleSystem, simulation/particleSystem.cpp:249-413)
+through the C-ABI on the GPU, against the oracle's whole FP32 step started
+from the same state and generator offset."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import orc
+from nwamd import abi, system
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def make_cpu(s, cfg, dt):
+    seed, offset = s.rng_state()
+    return orc.CpuSystem(s.params, s.solvent, cfg.ext, s.get(system.POSITION), s.get(system.VELOCITY),
+                         s.get(system.TANGENT), s.get(system.FORCE), s.get(system.FORCE_OLD), num_solvent=s.ns,
+                         srd=bool(cfg.srd), srd_gap=cfg.srdGap or 5, dt=dt, seed=seed, offset=offset,
+                         step=s.step_count, langevin_fdt=bool(cfg.langevinFdt),
+                         shift_centred=bool(cfg.srdShiftCentred))
+
+
+def periodic_diff(a, b, L):
+    d = (a - b).astype(np.float64)
+    return d - L * np.round(d / L)
+
+
+def compare(s, cpu, atol_pos, atol_vel):
+    L = np.array([s.params.boxSize.x, s.params.boxSize.y, s.params.boxSize.z])
+    gp, gv = s.get(system.POSITION), s.get(system.VELOCITY)
+    cp, cv = cpu.get("pos"), cpu.get("vel")
+    dp = np.abs(periodic_diff(gp[:, :3], cp[:, :3], L)).max()
+    dv = np.abs(gv[:, :3].astype(np.float64) - cv[:, :3]).max()
+    assert dp < atol_pos, f"positions differ by {dp}"
+    assert dv < atol_vel, f"velocities differ by {dv}"
+    return dp, dv
+
+
+@pytest.fixture(scope="module")
+def lib():
+    L = abi.load()
+    L.nwSetErrorMode(abi.NW_ERRORS_RETURN)
+    assert L.nwDeviceAvailable()
+    L.cudaInit(0, None)
+    return L
+
+
+def test_reference_defaults_match_oracle(lib):
+    """Reference constructor defaults (DEM + bonds + extensile, 27-cell stencil,
+    reference Langevin noise), 3D power-of-two grid."""
+    cfg = system.NWSystemConfig()
+    cfg.numFilaments, cfg.filamentSize = 300, 8
+    cfg.gridSize = abi.UInt3(32, 32, 4)
+    cfg.seed = 11
+    s = system.System(cfg)
+    s.params.bondBendingK = 0.5
+    s.params.activity = 0.2
+    s.reset(system.NW_INIT_REFERENCE)
+    cpu = make_cpu(s, cfg, 0.002)
+    for _ in range(3):
+        s.update(0.002, 5)
+        cpu.step(5)
+        compare(s, cpu, 1e-4, 1e-3)
+    assert s.step_count == 15
+
+
+def test_reference_2d_grid_with_srd(lib):
+    """Reference defaults with the SRD solvent (isokinetic thermostat, solvent-only
+    rotation, shift in [0, a)) on a 2D grid: collisions at steps 0, 5, 10."""
+    cfg = system.NWSystemConfig()
+    cfg.numFilaments, cfg.filamentSize = 64, 10
+    cfg.gridSize = abi.UInt3(32, 32, 1)
+    cfg.srd = 1
+    cfg.seed = 5
+    s = system.System(cfg)
+    s.reset(system.NW_INIT_REFERENCE)
+    assert s.ns == 10 * 16 * 16
+    cpu = make_cpu(s, cfg, 0.002)
+    s.update(0.002, 12)
+    cpu.step(12)
+    compare(s, cpu, 1e-4, 2e-3)
+
+
+def test_worm_srd_workload_small_matches_oracle(lib):
+    """The benchmark physics (WCA, chain bonds, bending, drive, SRD with beads,
+    90 deg, centred shift) on a 100 x 100 box with a non-power-of-two grid."""
+    cfg = system.config_srd_800k()
+    cfg.numFilaments, cfg.filamentSize = 400, 20
+    cfg.gridSize = abi.UInt3(89, 89, 1)
+    cfg.boxSize = abi.Float3(100.0, 100.0, 100.0 / 89)
+    cfg.srdGridSize = abi.UInt3(100, 100, 1)
+    s = system.System(cfg)
+    system.apply_worm_physics(s, kappa=5.0, gamma=2.0, kT=0.1)
+    s.reset(system.NW_INIT_LATTICE_RANDOM, 0.1)
+    cpu = make_cpu(s, cfg, 1e-3)
+    s.update(1e-3, 21)
+    cpu.step(21)
+    compare(s, cpu, 1e-4, 1e-3)
+    # per-cell momentum conservation makes the total momentum drift only by forces
+    v = s.get(system.VELOCITY)
+    assert np.all(np.isfinite(v))
+
+
+def test_dry_workload_langevin_fdt(lib):
+    cfg = system.config_dry_4k()
+    s = system.System(cfg)
+    system.apply_worm_physics(s, kappa=5.0, gamma=2.0, kT=0.1)
+    s.reset(system.NW_INIT_RANDOM, 0.1)
+    cpu = make_cpu(s, cfg, 1e-3)
+    s.update(1e-3, 10)
+    cpu.step(10)
+    compare(s, cpu, 1e-4, 1e-3)
+
+
+def test_xyzv_output(lib, tmp_path):
+    cfg = system.NWSystemConfig()
+    cfg.numFilaments, cfg.filamentSize = 10, 5
+    cfg.gridSize = abi.UInt3(16, 16, 1)
+    s = system.System(cfg)
+    s.reset(system.NW_INIT_REFERENCE)
+    f = str(tmp_path / "out.xyzv")
+    s.write_outputs(f, 0, 0.002)
+    s.update(0.002, 2)
+    s.write_outputs(f, 2, 0.002)
+    lines = open(f).read().splitlines()
+    assert lines[0] == "50"
+    assert lines[1].startswith('{"iteration":0,"dt":0.002,')
+    assert lines[2].startswith("A ") and len(lines[2].split()) == 7
+    assert lines[52] == "50" and '"iteration":2' in lines[53]
+    pos = s.get(system.POSITION)
+    last = np.array([[float(x) for x in ln.split()[1:4]] for ln in lines[54:104]])
+    assert np.allclose(last, pos[:, :3], atol=1e-4)
+
+
+def test_cli_runs(tmp_path):
+    exe = os.path.join(ROOT, "nematic-worms_amd", "nwamd", "nw")
+    out = str(tmp_path / "cli.xyzv")
+    r = subprocess.run([exe, "-iterations=20", "-numFilaments=50", "-filamentSize=10", "-gridX=32", "-gridY=32",
+                        "-printRate=10", f"-o={out}"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Throughput" in r.stdout
+    frames = [ln for ln in open(out).read().splitlines() if ln == "500"]
+    assert len(frames) == 3
