@@ -31,6 +31,7 @@ METRIC = "bead-steps/s & %HBM roofline: plain vs batched; adjoint/forward cost"
 # element read once and every output written once (float4 = 16 B, uint = 4 B).
 # N beads, NT = beads + solvent, C = MD cells, Cs = SRD cells.
 ALG_BYTES = {
+    # per-call path (the reference's entry points)
     "integrate": lambda w: 128 * w["N"],                       # R pos vel f f_old, W the same
     "langevin": lambda w: 48 * w["N"],                         # R vel f, W f (noise in registers)
     "filament": lambda w: 64 * w["N"],                         # R pos, RMW force, W tangent
@@ -45,10 +46,25 @@ ALG_BYTES = {
     "srd_thermo": lambda w: 68 * w["NT"] + 8 * w["Cs"],
     "srd_collide": lambda w: 48 * w["NT"] + 16 * w["Cs"],      # R pos, RMW vel, R com
     "srd_stream": lambda w: 64 * w["Ns"],                      # RMW pos vel of the solvent
+    # fused path (nwFusedMdStep / nwFusedSrdStep)
+    "md_front": lambda w: 152 * w["N"] + 8 * w["C"],           # R pos vel f f_old, W + tangent, W hash slot, bins
+    "md_scan": lambda w: 8 * w["C"],                           # R bins, W offsets
+    "md_scatter": lambda w: 12 * w["N"] + 4 * w["C"],          # R hash slot, W ids, R offsets
+    "md_cells": lambda w: (40 if w["wca"] else 104) * w["N"] + 8 * w["C"],  # ids, gather pos (vel tan), bins
+    "md_collide": lambda w: (52 if w["wca"] else 84) * w["N"] + 4 * w["C"],  # R sPos sIdx (+sVel sTan), RMW f
+    "srd_front": lambda w: 24 * w["NT"] + 8 * w["Cs"],         # R pos, W hash slot, bins
+    "srd_scan": lambda w: 8 * w["Cs"],
+    "srd_scatter": lambda w: 12 * w["NT"] + 4 * w["Cs"],
+    "srd_cells": lambda w: 24 * w["NT"] + 24 * w["Cs"],        # ids, gather vel, W com, bins
+    "srd_rotate": lambda w: 48 * w["NT"] + 16 * w["Ns"] + 16 * w["Cs"],  # R pos, RMW vel, W solvent pos, R com
 }
+# families that are one kernel (the roofline is quoted for a single kernel)
+SINGLE_KERNEL = {"integrate", "langevin", "filament", "hash", "collide", "srd_hash", "srd_com", "srd_thermo",
+                 "srd_collide", "srd_stream", "md_front", "md_scatter", "md_cells", "md_collide", "srd_front",
+                 "srd_scatter", "srd_cells", "srd_rotate"}
 
 
-def build_workload(name):
+def build_workload(name, per_call=False):
     from nwamd import system
     if name == "worms_srd_800k":
         cfg = system.config_srd_800k()
@@ -56,6 +72,7 @@ def build_workload(name):
         cfg = system.config_dry_4k()
     else:
         raise SystemExit(f"unknown workload {name}")
+    cfg.perCallSequence = 1 if per_call else 0
     s = system.System(cfg)
     system.apply_worm_physics(s, kappa=5.0, gamma=2.0, kT=0.1)
     s.reset(system.NW_INIT_LATTICE_RANDOM if cfg.srd else system.NW_INIT_RANDOM, 0.1)
@@ -100,6 +117,8 @@ def main():
     ap.add_argument("--dt", type=float, default=1e-3)
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--per-call-steps", type=int, default=100,
+                    help="steps of the reference per-call sequence timed beside (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -125,7 +144,8 @@ def main():
         if cnt:
             per[n] = dict(ms_per_launch=ms / cnt, launches=cnt, ms_total=ms)
     timers(lib, names, False)
-    dominant = max(per, key=lambda k: per[k]["ms_total"]) if per else "collide"
+    single = [k for k in per if k in SINGLE_KERNEL]
+    dominant = max(single, key=lambda k: per[k]["ms_total"]) if single else "md_collide"
 
     # timed region: only the dominant kernel carries events
     lib.nwTimerEnable(dominant.encode(), 1)
@@ -151,6 +171,21 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(s, cfg, dt, args.cpu_steps)
+    s.close()
+
+    per_call = None
+    if args.per_call_steps > 0:
+        s2, _, _ = build_workload(args.workload, per_call=True)
+        s2.update(dt, args.warmup)
+        s2.sync()
+        t0 = time.perf_counter()
+        s2.update(dt, args.per_call_steps)
+        s2.sync()
+        el2 = time.perf_counter() - t0
+        per_call = dict(value=round(w["N"] * args.per_call_steps / el2, 1), unit="bead-steps/s",
+                        steps=args.per_call_steps, ms_per_step=round(el2 / args.per_call_steps * 1e3, 4),
+                        form="the reference's per-call C-ABI sequence (ParticleSystem::update, 10+ launches/step)")
+        s2.close()
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "bead-steps/s", "n_gpus": 1,
@@ -158,8 +193,9 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": args.workload, "beads": w["N"], "solvent": w["Ns"], "md_cells": w["C"],
-                   "srd_cells": w["Cs"], "dt": dt, "form": "plain (ParticleSystem::update through the C-ABI)"},
-        "roofline": roof, "cpu_baseline": cpu, "kernels": kernels,
+                   "srd_cells": w["Cs"], "dt": dt,
+                   "form": "plain: one replica, ParticleSystem::update through the fused C-ABI entry points"},
+        "roofline": roof, "cpu_baseline": cpu, "per_call_sequence": per_call, "kernels": kernels,
     }
     print(json.dumps(out), flush=True)
 

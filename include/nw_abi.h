@@ -203,6 +203,34 @@ void nwSortParticlesBounded(uint *dGridParticleHash, uint *dGridParticleIndex,
 void nwRadixSortPairs(uint *dGridParticleHash, uint *dGridParticleIndex,
                       uint numParticles, uint keyBits);
 
+/* Fused MD step: the computation of integrateFilaments, reverseFilaments (when
+   SimParams.reverseProbability > 0), langevinThermostat (when `langevin`),
+   filamentForces, calcHash, sortParticles, reorderDataAndFindCellStart and
+   collideFilaments -- in the order ParticleSystem::update issues them
+   (simulation/particleSystem.cpp:257-330) -- in five launches, for worms of
+   1..64 beads.  No counterpart in the reference.  Draws advance the generator
+   exactly as the separate calls do.  On return gridParticleIndex holds the
+   bead ids in stable cell order, sortedPos (and sortedVel / sortedTangent,
+   which may be NULL when the WCA pair term without activity does not read
+   them) the gathered records, gridParticleHash the cell of every bead in bead
+   order; the reference's cellStart/cellEnd tables are not produced. */
+void nwFusedMdStep(float *pos, float *vel, float *force, float *forceOld, float *tangent,
+                   uint *gridParticleHash, uint *gridParticleIndex,
+                   float *sortedPos, float *sortedVel, float *sortedTangent,
+                   float deltaTime, float langevinGamma, float langevinKbT, int langevin,
+                   uint numFilaments);
+
+/* Fused SRD collision step: calcHash(srd) + sortParticles +
+   [solventIsokineticThermostat] + solventCellCentorOfMomentum + collideSolvent
+   + integrateSolvent (simulation/particleSystem.cpp:352-410) in five launches
+   over the numParticles beads followed by numSolvent solvent particles, with
+   the SolventParams of the last setSolventParameters.  No counterpart in the
+   reference.  solventCellCOM receives (centre-of-mass velocity, rotation sign)
+   per SRD cell. */
+void nwFusedSrdStep(float *pos, float *vel, float *solventCellCOM, uint *gridParticleHash,
+                    uint *gridParticleIndex, float deltaTime, uint numParticles, uint numSolvent,
+                    int isokinetic);
+
 /* Device-side anomaly flags (bit 0: sort key above bound).  Synchronises. */
 uint nwGetDeviceFlags(int clear);
 

@@ -32,7 +32,9 @@ typedef struct NWSystemConfig {
     uint32_t langevinFdt;    /* Langevin noise sqrt(2 gamma kT / dt) instead of sqrt(2 gamma kT) */
     uint64_t seed;           /* Philox key of every draw (device and host) */
     NWExtParams ext;         /* physics variants; zero = reference */
-    uint32_t reserved[8];
+    uint32_t perCallSequence;/* 1: issue the reference's per-call C-ABI sequence every step;
+                                0: the fused entry points (same results, fewer launches) */
+    uint32_t reserved[7];
 } NWSystemConfig;
 
 /* initial conditions */

@@ -107,6 +107,7 @@ ParticleSystem::ParticleSystem(const NWSystemConfig &cfg) {
     m_seed = cfg.seed;
     m_shiftCentred = cfg.srdShiftCentred != 0;
     m_langevinFdt = cfg.langevinFdt != 0;
+    m_perCall = cfg.perCallSequence != 0;
     _initSolventParams(&cfg);
     _initialize();
 }
@@ -170,6 +171,10 @@ void ParticleSystem::_upload(float *dev, const std::vector<float> &host, size_t 
 // the step (particleSystem.cpp:249-339)
 // --------------------------------------------------------------------------
 void ParticleSystem::update(float deltaTime) {
+    if (fusedActive()) {
+        _updateFused(deltaTime);
+        return;
+    }
     nwRngSetSeed(m_seed, m_rngOffset);
     setParameters(&m_params);
     nwSetExtParameters(&m_ext);
@@ -206,8 +211,32 @@ void ParticleSystem::update(float deltaTime) {
     nwRngGetState(&seed, &m_rngOffset);
 }
 
-// particleSystem.cpp:341-413
-void ParticleSystem::_solventUpdate(float dt) {
+// The same step through the fused entry points: one call for the MD part,
+// one per SRD collision.
+void ParticleSystem::_updateFused(float deltaTime) {
+    nwRngSetSeed(m_seed, m_rngOffset);
+    setParameters(&m_params);
+    nwSetExtParameters(&m_ext);
+    const float kT = m_langevinFdt ? m_params.kbT / deltaTime : m_params.kbT;
+    const bool needVel = m_ext.pairPotential == NW_PAIR_REF_DEM;
+    const bool needTan = needVel || m_params.activity != 0.0f;
+    nwFusedMdStep(m_dPos, m_dVel, m_dForce, m_dForceOld, m_dTangent, m_dGridParticleHash, m_dGridParticleIndex,
+                  m_dSortedPos, needVel ? m_dSortedVel : nullptr, needTan ? m_dSortedTangent : nullptr, deltaTime,
+                  m_params.gamma, kT, m_solventEnabled ? 0 : 1, m_numFilaments);
+    if (m_solventEnabled && (m_updateCount % m_solventUpdateGap) == 0) {
+        _srdShift();
+        nwFusedSrdStep(m_dPos, m_dVel, m_dSolventCellCOM, m_dGridParticleHash, m_dGridParticleIndex,
+                       deltaTime * m_solventUpdateGap, m_numParticles, m_numSolvent,
+                       m_ext.srdThermostat == NW_SRD_THERMO_ISOKINETIC ? 1 : 0);
+        m_solventUpdateCount++;
+    }
+    m_updateCount++;
+    uint64_t seed = 0;
+    nwRngGetState(&seed, &m_rngOffset);
+}
+
+// SRD grid shift of this collision (particleSystem.cpp:345-349)
+void ParticleSystem::_srdShift() {
     uint32_t b[4];
     nwPhiloxBlock(m_seed ^ kShiftKey, m_solventUpdateCount, b);
     const float ux = u01_open(b[0]), uy = u01_open(b[1]);
@@ -215,6 +244,11 @@ void ParticleSystem::_solventUpdate(float dt) {
     m_solvent.randOffset.y = (m_shiftCentred ? uy - 0.5f : uy) * m_solvent.cellSize.y;
     m_solvent.randOffset.z = 0.0f;
     setSolventParameters(&m_solvent);
+}
+
+// particleSystem.cpp:341-413
+void ParticleSystem::_solventUpdate(float dt) {
+    _srdShift();
 
     const uint NT = m_numParticles + m_numSolvent;
     calcHash(m_dGridParticleHash, m_dGridParticleIndex, m_dPos, int(NT), true);

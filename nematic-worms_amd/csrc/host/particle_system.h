@@ -71,11 +71,15 @@ public:
     uint64_t rngOffset() const { return m_rngOffset; }
     void setRngOffset(uint64_t o) { m_rngOffset = o; }
     uint64_t seed() const { return m_seed; }
+    void setPerCallSequence(bool on) { m_perCall = on; }
+    bool fusedActive() const { return !m_perCall && m_filamentSize >= 1 && m_filamentSize <= 64; }
 
 private:
     void _defaults(uint numFilaments, uint filamentSize, nw_uint3 gridSize, bool srd);
     void _initSolventParams(const NWSystemConfig *cfg);
     void _solventUpdate(float dt);
+    void _updateFused(float deltaTime);
+    void _srdShift();
     void _initialize();
     void _finalize();
     float _frand();  // host uniform in [0, 1] (stands in for rand()/RAND_MAX)
@@ -105,6 +109,7 @@ private:
     uint m_numSolvent = 0, m_numSolventCells = 0, m_solventUpdateCount = 0, m_solventUpdateGap = 5;
     bool m_shiftCentred = false;
     bool m_langevinFdt = false;
+    bool m_perCall = false;     // reference per-call sequence instead of the fused entry points
 
     uint64_t m_seed = 0;
     uint64_t m_rngOffset = 0;   // device generator offset owned by this system

@@ -13,6 +13,7 @@
 
 #include "nw_abi.h"
 #include "nw_internal.hpp"
+#include "nw_fused.hpp"
 
 namespace {
 
@@ -43,6 +44,7 @@ struct State {
     const uint32_t *lastHashPtr = nullptr;
     bool lastHashSrd = false;     // timer family of the sort / reorder that follow
     SortScratch sort;
+    FusedScratch fused;
     std::map<std::string, Timer> timers;
 };
 
@@ -428,6 +430,115 @@ void collideSolvent(float *pos, float *vel, float *solventCellCOM, float *random
                                   reinterpret_cast<const float4 *>(solventCellCOM), numSolventCells, numParticles,
                                   cosA, sinA, s.seed, s.offset, make_kp(), s.stream));
     s.offset += blocks_for(numSolventCells);
+}
+
+// ---------------------------------------------------------------------------
+// fused step
+// ---------------------------------------------------------------------------
+void nwFusedMdStep(float *pos, float *vel, float *force, float *forceOld, float *tangent, uint *gridParticleHash,
+                   uint *gridParticleIndex, float *sortedPos, float *sortedVel, float *sortedTangent,
+                   float deltaTime, float langevinGamma, float langevinKbT, int langevin, uint numFilaments) {
+    NW_INIT();
+    State &s = S();
+    const KP kp = make_kp();
+    const uint32_t n = kp.p.filamentSize;
+    NW_REQUIRE(n >= 1 && n <= 64, "nwFusedMdStep: worms of %u beads (1..64 supported)", n);
+    const uint64_t cells = uint64_t(kp.p.gridSize.x) * kp.p.gridSize.y * kp.p.gridSize.z;
+    NW_REQUIRE(cells > 0 && cells <= kp.p.numCells, "nwFusedMdStep: grid does not fit numCells");
+    const uint64_t N = uint64_t(numFilaments) * n;
+    NW_REQUIRE(N < (1ull << 32), "nwFusedMdStep: too many beads");
+    const bool needVel = kp.e.pairPotential == NW_PAIR_REF_DEM;
+    const bool needTan = needVel || kp.p.activity != 0.0f;
+    NW_REQUIRE(!needVel || sortedVel != nullptr, "nwFusedMdStep: sortedVel is NULL");
+    NW_REQUIRE(!needTan || sortedTangent != nullptr, "nwFusedMdStep: sortedTangent is NULL");
+    if (N == 0) return;
+    const uint32_t bins = uint32_t(cells) + 1;
+    NW_HIP(ensure_fused_scratch(s.fused, N, bins, s.stream));
+
+    MdFrontArgs a;
+    a.pos = reinterpret_cast<float4 *>(pos);
+    a.vel = reinterpret_cast<float4 *>(vel);
+    a.force = reinterpret_cast<float4 *>(force);
+    a.forceOld = reinterpret_cast<float4 *>(forceOld);
+    a.tangent = reinterpret_cast<float4 *>(tangent);
+    a.hash = gridParticleHash;
+    a.dt = deltaTime;
+    a.W = numFilaments;
+    a.seed = s.seed;
+    a.offRev = s.offset;
+    if (kp.p.reverseProbability > 0.0f) s.offset += blocks_for(numFilaments);
+    a.offLgv = s.offset;
+    a.langevin = langevin ? 1 : 0;
+    a.lgvStd = sqrtf(2.0f * langevinGamma * langevinKbT);
+    if (langevin) s.offset += N;
+    {
+        TimedScope ts("md_front", s.stream);
+        NW_HIP(launch_md_front(a, kp, s.fused, s.stream));
+    }
+    {
+        TimedScope ts("md_scan", s.stream);
+        NW_HIP(launch_scan_bins(s.fused, bins, s.stream));
+    }
+    {
+        TimedScope ts("md_scatter", s.stream);
+        NW_HIP(launch_scatter_ix(gridParticleHash, s.fused, uint32_t(N), gridParticleIndex, s.stream));
+    }
+    {
+        TimedScope ts("md_cells", s.stream);
+        NW_HIP(launch_md_cells(s.fused, bins, gridParticleIndex, a.pos, needVel ? a.vel : nullptr,
+                               needTan ? a.tangent : nullptr, reinterpret_cast<float4 *>(sortedPos),
+                               needVel ? reinterpret_cast<float4 *>(sortedVel) : nullptr,
+                               needTan ? reinterpret_cast<float4 *>(sortedTangent) : nullptr, s.stream));
+    }
+    {
+        TimedScope ts("md_collide", s.stream);
+        NW_HIP(launch_collide_csr(a.force, reinterpret_cast<const float4 *>(sortedPos),
+                                  reinterpret_cast<const float4 *>(sortedVel),
+                                  reinterpret_cast<const float4 *>(sortedTangent), gridParticleIndex, s.fused,
+                                  uint32_t(N), kp, s.stream));
+    }
+    s.lastHashPtr = nullptr;
+}
+
+void nwFusedSrdStep(float *pos, float *vel, float *solventCellCOM, uint *gridParticleHash, uint *gridParticleIndex,
+                    float deltaTime, uint numParticles, uint numSolvent, int isokinetic) {
+    NW_INIT();
+    State &s = S();
+    const KP kp = make_kp();
+    const uint64_t NT = uint64_t(numParticles) + numSolvent;
+    NW_REQUIRE(NT < (1ull << 32), "nwFusedSrdStep: too many particles");
+    const uint64_t cells = uint64_t(kp.s.gridSize.x) * kp.s.gridSize.y * kp.s.gridSize.z;
+    NW_REQUIRE(cells > 0 && cells == kp.s.numCells, "nwFusedSrdStep: SRD grid does not match numCells");
+    const uint32_t bins = uint32_t(cells) + 1;
+    NW_HIP(ensure_fused_scratch(s.fused, NT, bins, s.stream));
+    const uint64_t off = s.offset;
+    s.offset += blocks_for(cells);
+    const double alpha = double(kp.s.alpha);
+    const float cosA = float(std::cos(alpha)), sinA = float(std::sin(alpha));
+    float4 *P = reinterpret_cast<float4 *>(pos), *V = reinterpret_cast<float4 *>(vel);
+    {
+        TimedScope ts("srd_front", s.stream);
+        NW_HIP(launch_srd_front(P, uint32_t(NT), gridParticleHash, s.fused, kp, s.stream));
+    }
+    {
+        TimedScope ts("srd_scan", s.stream);
+        NW_HIP(launch_scan_bins(s.fused, bins, s.stream));
+    }
+    {
+        TimedScope ts("srd_scatter", s.stream);
+        NW_HIP(launch_scatter_ix(gridParticleHash, s.fused, uint32_t(NT), gridParticleIndex, s.stream));
+    }
+    {
+        TimedScope ts("srd_cells", s.stream);
+        NW_HIP(launch_srd_cells(s.fused, bins, gridParticleIndex, V, reinterpret_cast<float4 *>(solventCellCOM),
+                                isokinetic ? 1 : 0, s.seed, off, kp, s.stream));
+    }
+    {
+        TimedScope ts("srd_rotate", s.stream);
+        NW_HIP(launch_srd_rotate(P, V, reinterpret_cast<const float4 *>(solventCellCOM), numParticles,
+                                 uint32_t(NT), deltaTime, cosA, sinA, kp, s.stream));
+    }
+    s.lastHashPtr = nullptr;
 }
 
 // ---------------------------------------------------------------------------

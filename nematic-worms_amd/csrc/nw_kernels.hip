@@ -4,6 +4,7 @@
 // vel.w = mass), worm-contiguous (bead = worm * n + p).  Every kernel writes
 // only slots it owns, so results are deterministic and atomic-free.
 #include "nw_internal.hpp"
+#include "nw_physics.hpp"
 
 namespace nw {
 
@@ -18,57 +19,6 @@ uint32_t *device_flags() {
 // ===========================================================================
 // Integrators (particles_kernel_impl.cuh:34-205)
 // ===========================================================================
-__device__ __forceinline__ void periodic_bc(float &r, float L0, float L) {
-    if (r < L0) r += L;
-    else if (r > L + L0) r -= L;
-}
-
-__device__ __forceinline__ void damped_wall(float &r, float &v, float coef, float L0, float L, float R) {
-    if (r < L0 + R) {
-        r = L0 + R;
-        v *= coef;
-    } else if (r > (L + L0) - R) {
-        r = (L + L0) - R;
-        v *= coef;
-    }
-}
-
-// noslipWallBoundary: undo the whole step displacement, pin the axis at the
-// wall and keep only the reflected normal velocity component.
-__device__ __forceinline__ void noslip_wall(f3 &pos, int axis, f3 &vel, f3 dr, f3 N, float L0, float L, float R) {
-    float r = axis == 0 ? pos.x : pos.y;
-    bool hit = false;
-    float target = 0.0f;
-    if (r < L0 + R) {
-        hit = true;
-        target = L0 + R;
-    } else if (r > (L + L0) - R) {
-        hit = true;
-        target = (L + L0) - R;
-    }
-    if (hit) {
-        pos -= dr;
-        if (axis == 0) pos.x = target; else pos.y = target;
-        vel = mk(-vel.x * N.x, -vel.y * N.y, -vel.z * N.z);
-    }
-}
-
-__device__ __forceinline__ void apply_boundaries(const SimParams &p, f3 &pos, f3 &vel, f3 dr, float R) {
-    if (p.boundaryX == PERIODIC) periodic_bc(pos.x, p.origin.x, p.boxSize.x);
-    else if (p.boundaryX == WALL) damped_wall(pos.x, vel.x, p.boundaryDamping, p.origin.x, p.boxSize.x, R);
-    else if (p.boundaryX == WALL_NO_SLIP)
-        noslip_wall(pos, 0, vel, dr, mk(1.f, 0.f, 0.f), p.origin.x, p.boxSize.x, R);
-
-    if (p.boundaryY == PERIODIC) periodic_bc(pos.y, p.origin.y, p.boxSize.y);
-    else if (p.boundaryY == WALL) damped_wall(pos.y, vel.y, p.boundaryDamping, p.origin.y, p.boxSize.y, R);
-    else if (p.boundaryY == WALL_NO_SLIP)
-        noslip_wall(pos, 1, vel, dr, mk(0.f, 1.f, 0.f), p.origin.y, p.boxSize.y, R);
-
-    // the reference has no no-slip branch for z
-    if (p.boundaryZ == PERIODIC) periodic_bc(pos.z, p.origin.z, p.boxSize.z);
-    else if (p.boundaryZ == WALL) damped_wall(pos.z, vel.z, p.boundaryDamping, p.origin.z, p.boxSize.z, R);
-}
-
 // filament_integrator (particles_kernel_impl.cuh:71-142): 128 B/bead stream.
 __global__ __launch_bounds__(kBlock) void k_integrate_filaments(float4 *__restrict__ pos, float4 *__restrict__ vel,
                                                                 float4 *__restrict__ f, float4 *__restrict__ fold,
@@ -168,83 +118,6 @@ __global__ __launch_bounds__(kBlock) void k_reorder(uint32_t *__restrict__ cellS
 // upstream: multiplicity 3).  A cell row y..z is scanned as one contiguous
 // run of the sorted arrays when its three x-cells are consecutive hashes.
 // ===========================================================================
-struct AxisCells {
-    int c[3];
-    float w[3];
-    int n;
-};
-
-__device__ __forceinline__ AxisCells axis_cells(int g, uint32_t G, bool onlyCentre) {
-    AxisCells a;
-    if (onlyCentre) {
-        a.c[0] = wrap_mod(g, G);
-        a.w[0] = 1.0f;
-        a.n = 1;
-        return a;
-    }
-    const int m = wrap_mod(g - 1, G), c = wrap_mod(g, G), p = wrap_mod(g + 1, G);
-    if (G >= 3) {
-        a.c[0] = m; a.c[1] = c; a.c[2] = p;
-        a.w[0] = a.w[1] = a.w[2] = 1.0f;
-        a.n = 3;
-    } else if (G == 2) {
-        // offsets -1 and +1 land on the same cell
-        a.c[0] = m; a.w[0] = 2.0f;
-        a.c[1] = c; a.w[1] = 1.0f;
-        a.n = 2;
-    } else {
-        a.c[0] = c; a.w[0] = 3.0f;
-        a.n = 1;
-    }
-    return a;
-}
-
-// DEM soft sphere (collideParticles)
-__device__ __forceinline__ f3 dem_force(const SimParams &p, f3 rA, f3 rB, f3 vA, f3 vB) {
-    const f3 rel = min_image(p, rA, rB);
-    const float dist = length(rel);
-    const float cd = p.particleRadius + p.particleRadius;
-    f3 force = mk(0.f, 0.f, 0.f);
-    if (dist < cd) {
-        const f3 nrm = rel / dist;
-        const f3 relVel = vB - vA;
-        const f3 tanVel = relVel - (dot(relVel, nrm) * nrm);
-        force = -p.spring * (cd - dist) * nrm;
-        force += p.damping * relVel;
-        force += p.shear * tanVel;
-    }
-    return force;
-}
-
-__device__ __forceinline__ f3 bond_hookean(const SimParams &p, f3 rA, f3 rB, float k, float L) {
-    const f3 rel = min_image(p, rA, rB);
-    const float dist = length(rel);
-    const f3 nrm = rel / dist;
-    return -k * (L - dist) * nrm;
-}
-
-__device__ __forceinline__ f3 extensile_force(const SimParams &p, f3 rA, f3 rB, f3 tA, f3 tB, float A) {
-    const f3 rel = min_image(p, rA, rB);
-    const float dist = length(rel);
-    if (dot(tA, tB) < -0.5f) return -A * (tA - tB) / (2.0f * dist);
-    return mk(0.f, 0.f, 0.f);
-}
-
-// WCA-LJ force on A from B: F_A = -24 eps (2 s^12/r^14 - s^6/r^8) (rB - rA), r < rc
-__device__ __forceinline__ f3 wca_force(const SimParams &p, const NWExtParams &e, f3 rA, f3 rB, float rc2) {
-    const f3 rel = min_image(p, rA, rB);
-    const float d = length(rel);
-    const float rr = d * d;
-    if (rr < rc2) {
-        const float inv2 = 1.0f / rr;
-        const float s2 = e.wcaSigma * e.wcaSigma * inv2;
-        const float s6 = s2 * s2 * s2;
-        const float fmag = 24.0f * e.wcaEpsilon * inv2 * s6 * (2.0f * s6 - 1.0f);
-        return -fmag * rel;
-    }
-    return mk(0.f, 0.f, 0.f);
-}
-
 template <int MODE>  // 0: reference DEM/bond/extensile, 1: WCA (+extensile)
 __global__ __launch_bounds__(kBlock) void k_collide(float4 *__restrict__ force, const float4 *__restrict__ sPos,
                                                     const float4 *__restrict__ sVel,
@@ -326,91 +199,18 @@ __global__ __launch_bounds__(kBlock) void k_collide(float4 *__restrict__ force, 
 // re-reading memory.  Lane p computes the bending triple (p, p+1, p+2) and
 // the bond (p, p+1); the shares of beads p+1, p+2 are shuffled back.
 // ===========================================================================
-__device__ __forceinline__ f3 shfl_down3(f3 a, int d) {
-    return mk(__shfl_down(a.x, d, 64), __shfl_down(a.y, d, 64), __shfl_down(a.z, d, 64));
-}
-__device__ __forceinline__ f3 shfl_up3(f3 a, int d) {
-    return mk(__shfl_up(a.x, d, 64), __shfl_up(a.y, d, 64), __shfl_up(a.z, d, 64));
-}
-
 __global__ __launch_bounds__(kBlock) void k_filament_wave(float4 *__restrict__ force, float4 *__restrict__ tangent,
                                                           const float4 *__restrict__ pos, uint32_t W, KP kp) {
-    const SimParams &p = kp.p;
-    const uint32_t n = p.filamentSize;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t wpw = 64u / n;
-    const uint32_t wl = lane / n;
-    const uint32_t pp = lane - wl * n;
-    const uint32_t worm = wave * wpw + wl;
-    const bool active = wl < wpw && worm < W;
-    const uint32_t bead = worm * n + pp;
-
-    const float4 P = active ? pos[bead] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const f3 r = mk(P);
-    const f3 rn1 = shfl_down3(r, 1);
-    const f3 rn2 = shfl_down3(r, 2);
-    const f3 rp1 = shfl_up3(r, 1);
-
-    // bending triple (pp, pp+1, pp+2)
-    f3 fi = mk(0.f, 0.f, 0.f), fj = fi, fk = fi;
-    const bool hasTriple = n >= 3 && pp + 2u < n;
-    if (hasTriple) {
-        const f3 r_ij = min_image(p, r, rn1);
-        const f3 r_jk = min_image(p, rn1, rn2);
-        const float d_ij = length(r_ij);
-        const float d_jk = length(r_jk);
-        const float d_ij_jk = dot(r_ij, r_jk);
-        const float A = p.bondBendingK / (d_ij * d_jk);
-        const f3 B = r_ij * d_ij_jk / dot(r_ij, r_ij);
-        const f3 C = r_jk * d_ij_jk / dot(r_jk, r_jk);
-        fi = -A * (r_jk - B);
-        fk = -A * (C - r_ij);
-        fj = mk(-fi.x - fk.x, -fi.y - fk.y, -fi.z - fk.z);
-    }
-    const f3 fk_from2 = shfl_up3(fk, 2);  // triple pp-2 acting on bead pp
-    const f3 fj_from1 = shfl_up3(fj, 1);  // triple pp-1 acting on bead pp
-
-    // chain bond (pp, pp+1) and drive (config physics)
-    const bool chain = kp.e.bondedMode == NW_BONDED_CHAIN;
-    f3 fb = mk(0.f, 0.f, 0.f), drive = fb;
-    if (chain && pp + 1u < n) {
-        const f3 d = min_image(p, r, rn1);
-        const float len = length(d);
-        fb = (p.bondSpringK * (len - p.bondSpringL) / len) * d;
-        drive = (kp.e.driveForce / len) * d;
-    }
-    const f3 fb_prev = shfl_up3(fb, 1);  // bond (pp-1, pp) acting on bead pp is -fb(pp-1)
-
-    if (!active) return;
-    if (n < 3 && !chain) return;
-
-    float4 F = force[bead];
-    f3 acc = mk(F.x, F.y, F.z);
-    if (pp >= 2u) acc += fk_from2;
-    if (pp >= 1u) acc += fj_from1;
-    if (hasTriple) acc += fi;
-    if (chain) {
-        if (pp >= 1u) acc -= fb_prev;
-        if (pp + 1u < n) {
-            acc += fb;
-            acc += drive;
-        }
-    }
-    force[bead] = make_float4(acc.x, acc.y, acc.z, F.w + 0.0f);
-    if (n < 3) return;
-
-    // tangents: head one-sided, interior centred, tail one-sided; with n == 3
-    // the reference's loop never reaches its tail branch
-    f3 tg;
-    bool write = true;
-    if (pp == 0u) tg = tangent_of(p, r, rn1);
-    else if (pp + 1u < n) tg = tangent_of(p, rp1, rn1);
-    else {
-        tg = tangent_of(p, rp1, r);
-        write = n > 3u;
-    }
-    if (write) tangent[bead] = f4(tg, 0.0f);
+    const uint32_t n = kp.p.filamentSize;
+    const WaveWorm m = wave_worm(n, W);
+    const float4 P = m.active ? pos[m.bead] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 F = m.active ? force[m.bead] : make_float4(0.f, 0.f, 0.f, 0.f);
+    f3 acc = mk(F.x, F.y, F.z), tg;
+    const bool writeTan = bonded_lane(kp, mk(P), m.pp, acc, tg);
+    if (!m.active) return;
+    if (n < 3 && kp.e.bondedMode != NW_BONDED_CHAIN) return;
+    force[m.bead] = make_float4(acc.x, acc.y, acc.z, F.w + 0.0f);
+    if (writeTan) tangent[m.bead] = f4(tg, 0.0f);
 }
 
 // Worms longer than a wavefront: one thread per worm walks the chain.
@@ -499,13 +299,6 @@ __global__ __launch_bounds__(kBlock) void k_randomize(float *__restrict__ out, u
     }
     for (int k = 0; k < 4; ++k)
         if (e0 + k < n) out[e0 + k] = v[k];
-}
-
-__device__ __forceinline__ float uniform_at(uint64_t seed, uint64_t offset, uint32_t e) {
-    const u4 b = philox_block(seed, offset + (e >> 2));
-    const uint32_t k = e & 3u;
-    const uint32_t x = k == 0 ? b.x : k == 1 ? b.y : k == 2 ? b.z : b.w;
-    return u01(x);
 }
 
 // ===========================================================================

@@ -115,6 +115,8 @@ PROTOTYPES = {
     "nwSetStream": (None, [c_void_p]),
     "nwSortParticlesBounded": (None, [P, P, c_uint, c_uint]),
     "nwRadixSortPairs": (None, [P, P, c_uint, c_uint]),
+    "nwFusedMdStep": (None, [P, P, P, P, P, P, P, P, P, P, c_float, c_float, c_float, c_int, c_uint]),
+    "nwFusedSrdStep": (None, [P, P, P, P, P, c_float, c_uint, c_uint, c_int]),
     "nwGetDeviceFlags": (c_uint, [c_int]),
     "nwTimerEnable": (None, [ctypes.c_char_p, c_int]),
     "nwTimerRead": (None, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_u64)]),

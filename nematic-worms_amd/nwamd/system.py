@@ -22,7 +22,7 @@ class NWSystemConfig(ctypes.Structure):
         ("srdGridSize", abi.UInt3), ("srdCellSize", ctypes.c_float), ("srdDensity", ctypes.c_float),
         ("srdGap", ctypes.c_uint32), ("srdAlpha", ctypes.c_float), ("srdShiftCentred", ctypes.c_uint32),
         ("langevinFdt", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("ext", abi.NWExtParams),
-        ("reserved", ctypes.c_uint32 * 8),
+        ("perCallSequence", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 7),
     ]
 
 

@@ -1,0 +1,181 @@
+"""The fused entry points (nwFusedMdStep, nwFusedSrdStep) against the
+reference's per-call sequence on the GPU and against the oracle.
+
+The per-call path is itself checked against the oracle (test_gpu_parity.py);
+the fused path evaluates the same expressions in the same order, so positions,
+velocities, old forces, tangents, the stable cell order and every SRD output
+must agree bit for bit.  Pair forces are summed per row of three cells instead
+of per cell, so they agree to the 1e-5 bar."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import orc
+from nwamd import abi
+from nwamd.abi import DeviceArray
+from nwtest import ext_params, force_tol_check, random_forces, random_worms, sim_params, solvent_params
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(lib, a):
+    return DeviceArray.from_numpy(lib, a)
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def per_call_md(lib, p, e, st, dt, langevin, gamma, kT):
+    """ParticleSystem::update's MD part through the reference entry points."""
+    N, W, C = p.numParticles, p.numFilaments, p.numCells
+    lib.integrateFilaments(st["pos"].p, st["vel"].p, st["f"].p, st["fo"].p, dt, N)
+    if p.reverseProbability > 0:
+        lib.reverseFilaments(st["f"].p, st["fo"].p, st["vel"].p, st["pos"].p, None, W)
+    if langevin:
+        lib.langevinThermostat(st["f"].p, st["vel"].p, None, gamma, kT, N)
+    lib.filamentForces(st["f"].p, st["tan"].p, st["pos"].p, float(W))
+    lib.calcHash(st["h"].p, st["i"].p, st["pos"].p, N, False)
+    lib.sortParticles(st["h"].p, st["i"].p, N)
+    lib.reorderDataAndFindCellStart(st["cs"].p, st["ce"].p, st["sp"].p, st["sv"].p, st["st"].p, st["h"].p,
+                                    st["i"].p, st["pos"].p, st["vel"].p, st["tan"].p, N, C)
+    lib.collideFilaments(st["f"].p, st["sp"].p, st["sv"].p, st["st"].p, st["i"].p, st["cs"].p, st["ce"].p, N, C)
+
+
+def state(lib, p, pos, vel, tan, f, fo, extra=0):
+    N, C = p.numParticles, p.numCells
+    d = dict(pos=dev(lib, pos), vel=dev(lib, vel), tan=dev(lib, tan), f=dev(lib, f), fo=dev(lib, fo))
+    for k in ("h", "i"):
+        d[k] = DeviceArray(lib, 4 * (N + extra))
+    d["cs"], d["ce"] = DeviceArray(lib, 4 * C), DeviceArray(lib, 4 * C)
+    for k in ("sp", "sv", "st"):
+        d[k] = DeviceArray(lib, 16 * (N + extra))
+    return d
+
+
+def down(d, name, n, dtype=np.float32, w=4):
+    return d[name].download(dtype, (n, w) if w > 1 else (n,))
+
+
+@pytest.mark.parametrize("mode,grid,n,rev,lgv", [
+    ("ref", (16, 16, 4), 8, 0.0, True),
+    ("ref2d", (32, 32, 1), 10, 0.3, True),
+    ("wca", (89, 89, 1), 20, 0.2, True),
+    ("wca", (89, 89, 1), 20, 0.0, False),
+    ("wca3d", (12, 12, 6), 30, 0.1, True),
+    ("wca", (40, 40, 1), 3, 0.5, False),
+])
+def test_fused_md_step_matches_per_call(nwlib, mode, grid, n, rev, lgv):
+    rng = np.random.default_rng([len(mode), grid[0], grid[2], n, int(rev * 10), int(lgv)])
+    twod = grid[2] == 1
+    cell = 1.0 if mode.startswith("ref") else 1.1236
+    W = max(4, int(grid[0] * grid[1] * grid[2] * (0.1 if mode.startswith("ref") else 0.06)))
+    p = sim_params(grid=grid, cell=cell, W=W, n=n, bondSpringL=0.8 if mode != "ref" else 0.4,
+                   reverseProbability=rev, bondBendingK=3.0, activity=0.2 if mode == "ref" else 0.0)
+    if mode.startswith("wca"):
+        e = ext_params(pairPotential=abi.NW_PAIR_WCA, bondedMode=abi.NW_BONDED_CHAIN, driveForce=1.0,
+                       stencil=abi.NW_STENCIL_2D if twod else abi.NW_STENCIL_REF27)
+    else:
+        e = ext_params(stencil=abi.NW_STENCIL_REF27)
+    pos, vel, tan = random_worms(p, rng, twod=twod)
+    f, fo = random_forces(rng, W * n, 0.5), random_forces(rng, W * n, 0.5)
+    f[:, 3] = 0
+    lib = nwlib
+    lib.setParameters(ctypes.byref(p))
+    lib.nwSetExtParameters(ctypes.byref(e))
+    dt, gamma, kT = 0.002, 0.8, 0.4
+    a = state(lib, p, pos, vel, tan, f, fo)
+    b = state(lib, p, pos, vel, tan, f, fo)
+    lib.nwRngSetSeed(21, 1000)
+    per_call_md(lib, p, e, a, dt, lgv, gamma, kT)
+    sa, oa = ctypes.c_uint64(), ctypes.c_uint64()
+    lib.nwRngGetState(ctypes.byref(sa), ctypes.byref(oa))
+    lib.nwRngSetSeed(21, 1000)
+    lib.nwFusedMdStep(b["pos"].p, b["vel"].p, b["f"].p, b["fo"].p, b["tan"].p, b["h"].p, b["i"].p, b["sp"].p,
+                      b["sv"].p, b["st"].p, dt, gamma, kT, int(lgv), W)
+    sb, ob = ctypes.c_uint64(), ctypes.c_uint64()
+    lib.nwRngGetState(ctypes.byref(sb), ctypes.byref(ob))
+    assert oa.value == ob.value
+    N = W * n
+    for k in ("pos", "vel", "fo", "tan"):
+        assert np.array_equal(bits(down(a, k, N)), bits(down(b, k, N))), k
+    assert np.array_equal(down(a, "i", N, np.uint32, 1), down(b, "i", N, np.uint32, 1))
+    assert np.array_equal(bits(down(a, "sp", N)), bits(down(b, "sp", N)))
+    # pair forces: the oracle on the per-call path's sorted state
+    fa, fb = down(a, "f", N), down(b, "f", N)
+    force_tol_check(fb, fa.astype(np.float64), rtol=1e-5, name=f"fused vs per-call ({mode})")
+
+
+@pytest.mark.parametrize("beads,iso", [(1, 0), (0, 1), (1, 1)])
+def test_fused_srd_step_matches_per_call(nwlib, beads, iso):
+    rng = np.random.default_rng(7 + beads + 2 * iso)
+    lib = nwlib
+    p = sim_params(grid=(64, 64, 1), cell=1.0, W=100, n=10)
+    pos, vel, _ = random_worms(p, rng, twod=True)
+    Ns, N = 30000, p.numParticles
+    sp = np.zeros((Ns, 4), np.float32)
+    sp[:, :2] = (rng.random((Ns, 2)) * 64 - 32).astype(np.float32)
+    sp[:, 2] = p.origin.z + 0.5
+    sv = np.zeros((Ns, 4), np.float32)
+    sv[:, :2] = rng.normal(0, 1, (Ns, 2))
+    sv[:, 3] = 1.0
+    P, V = np.concatenate([pos, sp]), np.concatenate([vel, sv])
+    s = solvent_params(p, (50, 50, 1), (64 / 50, 64 / 50, 1.0), alpha=np.pi / 2, kbT=0.3,
+                       offset=(-0.3, 0.41, 0.0), ns=Ns)
+    e = ext_params(srdCollideFilaments=beads)
+    lib.setParameters(ctypes.byref(p))
+    lib.setSolventParameters(ctypes.byref(s))
+    lib.nwSetExtParameters(ctypes.byref(e))
+    NT, Cs, dt = N + Ns, s.numCells, 0.05
+    res = []
+    for fused in (False, True):
+        dP, dV = dev(lib, P), dev(lib, V)
+        h, i = DeviceArray(lib, 4 * NT), DeviceArray(lib, 4 * NT)
+        com = DeviceArray(lib, 16 * Cs)
+        lib.nwRngSetSeed(3, 77)
+        if fused:
+            lib.nwFusedSrdStep(dP.p, dV.p, com.p, h.p, i.p, dt, N, Ns, iso)
+        else:
+            cs, ce = DeviceArray(lib, 4 * Cs), DeviceArray(lib, 4 * Cs)
+            sP, sV = DeviceArray(lib, 16 * NT), DeviceArray(lib, 16 * NT)
+            lib.calcHash(h.p, i.p, dP.p, NT, True)
+            lib.sortParticles(h.p, i.p, NT)
+            lib.reorderDataAndFindCellStart(cs.p, ce.p, sP.p, sV.p, None, h.p, i.p, dP.p, dV.p, None, NT, Cs)
+            if iso:
+                lib.solventIsokineticThermostat(dV.p, sV.p, cs.p, ce.p, i.p, Cs)
+            lib.solventCellCentorOfMomentum(com.p, sV.p, cs.p, ce.p, i.p, Cs)
+            lib.collideSolvent(dP.p, dV.p, com.p, None, Cs, NT)
+            lib.integrateSolvent(dP.p, dV.p, dt, N, Ns)
+        res.append((dP.download(np.float32, (NT, 4)), dV.download(np.float32, (NT, 4)),
+                    com.download(np.float32, (Cs, 4)), i.download(np.uint32, (NT,))))
+    (pa, va, ca, ia), (pb, vb, cb, ib) = res
+    assert np.array_equal(bits(pa), bits(pb))
+    assert np.array_equal(bits(va), bits(vb))
+    assert np.array_equal(bits(ca[:, :3]), bits(cb[:, :3]))
+    assert np.array_equal(ia, ib)  # the stable cell order
+    uni = orc.random_uniform(3, 77, Cs)
+    assert np.array_equal(cb[:, 3], np.where(uni > 0.5, 1.0, -1.0).astype(np.float32))
+
+
+def test_fused_step_repeats_bitwise(nwlib):
+    """Atomic histogram slots are arbitrary, the results are not: two runs agree bit for bit."""
+    rng = np.random.default_rng(99)
+    lib = nwlib
+    p = sim_params(grid=(64, 64, 1), cell=1.1236, W=800, n=20, bondSpringL=0.8, activity=0.0)
+    e = ext_params(pairPotential=abi.NW_PAIR_WCA, bondedMode=abi.NW_BONDED_CHAIN, driveForce=1.0,
+                   stencil=abi.NW_STENCIL_2D)
+    pos, vel, tan = random_worms(p, rng, twod=True)
+    f = np.zeros_like(pos)
+    lib.setParameters(ctypes.byref(p))
+    lib.nwSetExtParameters(ctypes.byref(e))
+    out = []
+    for _ in range(2):
+        b = state(lib, p, pos, vel, tan, f, f)
+        lib.nwRngSetSeed(5, 0)
+        for _ in range(5):
+            lib.nwFusedMdStep(b["pos"].p, b["vel"].p, b["f"].p, b["fo"].p, b["tan"].p, b["h"].p, b["i"].p,
+                              b["sp"].p, None, None, 0.001, 2.0, 100.0, 1, p.numFilaments)
+        out.append([down(b, k, p.numParticles) for k in ("pos", "vel", "f")])
+    for x, y in zip(*out):
+        assert np.array_equal(bits(x), bits(y))

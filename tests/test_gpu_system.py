@@ -49,13 +49,15 @@ def lib():
     return L
 
 
-def test_reference_defaults_match_oracle(lib):
+@pytest.mark.parametrize("per_call", [0, 1])
+def test_reference_defaults_match_oracle(lib, per_call):
     """Reference constructor defaults (DEM + bonds + extensile, 27-cell stencil,
     reference Langevin noise), 3D power-of-two grid."""
     cfg = system.NWSystemConfig()
     cfg.numFilaments, cfg.filamentSize = 300, 8
     cfg.gridSize = abi.UInt3(32, 32, 4)
     cfg.seed = 11
+    cfg.perCallSequence = per_call
     s = system.System(cfg)
     s.params.bondBendingK = 0.5
     s.params.activity = 0.2
@@ -68,7 +70,8 @@ def test_reference_defaults_match_oracle(lib):
     assert s.step_count == 15
 
 
-def test_reference_2d_grid_with_srd(lib):
+@pytest.mark.parametrize("per_call", [0, 1])
+def test_reference_2d_grid_with_srd(lib, per_call):
     """Reference defaults with the SRD solvent (isokinetic thermostat, solvent-only
     rotation, shift in [0, a)) on a 2D grid: collisions at steps 0, 5, 10."""
     cfg = system.NWSystemConfig()
@@ -76,6 +79,7 @@ def test_reference_2d_grid_with_srd(lib):
     cfg.gridSize = abi.UInt3(32, 32, 1)
     cfg.srd = 1
     cfg.seed = 5
+    cfg.perCallSequence = per_call
     s = system.System(cfg)
     s.reset(system.NW_INIT_REFERENCE)
     assert s.ns == 10 * 16 * 16
@@ -85,7 +89,8 @@ def test_reference_2d_grid_with_srd(lib):
     compare(s, cpu, 1e-4, 2e-3)
 
 
-def test_worm_srd_workload_small_matches_oracle(lib):
+@pytest.mark.parametrize("per_call", [0, 1])
+def test_worm_srd_workload_small_matches_oracle(lib, per_call):
     """The benchmark physics (WCA, chain bonds, bending, drive, SRD with beads,
     90 deg, centred shift) on a 100 x 100 box with a non-power-of-two grid."""
     cfg = system.config_srd_800k()
@@ -93,6 +98,7 @@ def test_worm_srd_workload_small_matches_oracle(lib):
     cfg.gridSize = abi.UInt3(89, 89, 1)
     cfg.boxSize = abi.Float3(100.0, 100.0, 100.0 / 89)
     cfg.srdGridSize = abi.UInt3(100, 100, 1)
+    cfg.perCallSequence = per_call
     s = system.System(cfg)
     system.apply_worm_physics(s, kappa=5.0, gamma=2.0, kT=0.1)
     s.reset(system.NW_INIT_LATTICE_RANDOM, 0.1)
@@ -105,8 +111,10 @@ def test_worm_srd_workload_small_matches_oracle(lib):
     assert np.all(np.isfinite(v))
 
 
-def test_dry_workload_langevin_fdt(lib):
+@pytest.mark.parametrize("per_call", [0, 1])
+def test_dry_workload_langevin_fdt(lib, per_call):
     cfg = system.config_dry_4k()
+    cfg.perCallSequence = per_call
     s = system.System(cfg)
     system.apply_worm_physics(s, kappa=5.0, gamma=2.0, kT=0.1)
     s.reset(system.NW_INIT_RANDOM, 0.1)
