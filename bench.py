@@ -46,22 +46,36 @@ ALG_BYTES = {
     "srd_thermo": lambda w: 68 * w["NT"] + 8 * w["Cs"],
     "srd_collide": lambda w: 48 * w["NT"] + 16 * w["Cs"],      # R pos, RMW vel, R com
     "srd_stream": lambda w: 64 * w["Ns"],                      # RMW pos vel of the solvent
-    # fused path (nwFusedMdStep / nwFusedSrdStep)
-    "md_front": lambda w: 152 * w["N"] + 8 * w["C"],           # R pos vel f f_old, W + tangent, W hash slot, bins
-    "md_scan": lambda w: 8 * w["C"],                           # R bins, W offsets
-    "md_scatter": lambda w: 12 * w["N"] + 4 * w["C"],          # R hash slot, W ids, R offsets
-    "md_cells": lambda w: (40 if w["wca"] else 104) * w["N"] + 8 * w["C"],  # ids, gather pos (vel tan), bins
-    "md_collide": lambda w: (52 if w["wca"] else 84) * w["N"] + 4 * w["C"],  # R sPos sIdx (+sVel sTan), RMW f
-    "srd_front": lambda w: 24 * w["NT"] + 8 * w["Cs"],         # R pos, W hash slot, bins
-    "srd_scan": lambda w: 8 * w["Cs"],
-    "srd_scatter": lambda w: 12 * w["NT"] + 4 * w["Cs"],
-    "srd_cells": lambda w: 24 * w["NT"] + 24 * w["Cs"],        # ids, gather vel, W com, bins
-    "srd_rotate": lambda w: 48 * w["NT"] + 16 * w["Ns"] + 16 * w["Cs"],  # R pos, RMW vel, W solvent pos, R com
+    # fused path (nwFusedMdStep / nwFusedSrdStep); Pm / Ps = bytes of the
+    # sort's tile x bucket prefix matrix (written by the tile pass, read once)
+    "md_front": lambda w: 156 * w["N"] + w["Pm"],              # R pos vel f f_old, W + tangent, W cell, W pair
+    "md_sort": lambda w: 48 * w["N"] + 4 * w["C"] + w["Pm"],   # R pairs, W ids cells, gather pos, W offsets
+    "md_collide": lambda w: (56 if w["wca"] else 88) * w["N"] + 4 * w["C"],  # R sPos sIdx sCell (+sVel sTan), RMW f
+    "srdf_tiles": lambda w: 28 * w["NT"] + w["Ps"],            # R pos, W cell, W pair
+    "srdf_cells": lambda w: 28 * w["NT"] + 20 * w["Cs"] + w["Ps"],  # R pairs, W ids, gather vel, W offsets com
+    "srdf_rotate": lambda w: 36 * w["NT"] + 32 * w["Ns"] + 16 * w["Cs"],  # R cell, RMW vel (+pos), R com
 }
+
+
+def bucket_plan(n, cells, tile):
+    """The fused sort's plan (make_bucket_plan): buckets of 2^S cells with at
+    most ~1536 members on average; returns (buckets, tiles)."""
+    per = n / cells
+    S = 0
+    while S < 11 and (1 << (S + 1)) * per <= 1536.0:
+        S += 1
+    B = (cells + (1 << S) - 1) >> S
+    if tile == 0:
+        want, tile = n * (B + 1) / float(1 << 21), 4096
+        while tile < (1 << 17) and tile < want:
+            tile <<= 1
+    return B, (n + tile - 1) // tile
+
+
 # families that are one kernel (the roofline is quoted for a single kernel)
-SINGLE_KERNEL = {"integrate", "langevin", "filament", "hash", "collide", "srd_hash", "srd_com", "srd_thermo",
-                 "srd_collide", "srd_stream", "md_front", "md_scatter", "md_cells", "md_collide", "srd_front",
-                 "srd_scatter", "srd_cells", "srd_rotate"}
+SINGLE_KERNEL = {"integrate", "langevin", "filament", "hash", "collide", "srd_com", "srd_thermo",
+                 "srd_hash", "srd_collide", "srd_stream", "md_front", "md_sort", "md_collide", "srdf_tiles",
+                 "srdf_cells", "srdf_rotate"}
 
 
 def build_workload(name, per_call=False):
@@ -79,6 +93,11 @@ def build_workload(name, per_call=False):
     p = s.params
     w = dict(N=s.n, Ns=s.ns, NT=s.n + s.ns, C=p.numCells, Cs=s.solvent.numCells if cfg.srd else 0,
              wca=cfg.ext.pairPotential == 1)
+    n = p.filamentSize
+    B, tiles = bucket_plan(w["N"], w["C"], 16 * (64 // n) * n)
+    w["Pm"] = 4 * tiles * (B + 1)
+    B, tiles = bucket_plan(w["NT"], w["Cs"], 0) if cfg.srd else (0, 0)
+    w["Ps"] = 4 * tiles * (B + 1)
     return s, cfg, w
 
 
@@ -116,6 +135,8 @@ def main():
     ap.add_argument("--workload", default="worms_srd_800k")
     ap.add_argument("--dt", type=float, default=1e-3)
     ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--profile-steps", type=int, default=20,
+                    help="untimed steps after the warm-up that rank the kernel families")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--per-call-steps", type=int, default=100,
                     help="steps of the reference per-call sequence timed beside (0: skip)")
@@ -134,9 +155,12 @@ def main():
     dt = args.dt
     names = list(ALG_BYTES)
 
-    # warm-up, with every kernel family timed to find the dominant one
-    timers(lib, names, True)
+    # untimed warm-up, then (also untimed) every kernel family timed over a
+    # few steps to find the dominant one
     s.update(dt, args.warmup)
+    s.sync()
+    timers(lib, names, True)
+    s.update(dt, args.profile_steps)
     s.sync()
     per = {}
     for n in names:

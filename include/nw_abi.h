@@ -207,13 +207,15 @@ void nwRadixSortPairs(uint *dGridParticleHash, uint *dGridParticleIndex,
    SimParams.reverseProbability > 0), langevinThermostat (when `langevin`),
    filamentForces, calcHash, sortParticles, reorderDataAndFindCellStart and
    collideFilaments -- in the order ParticleSystem::update issues them
-   (simulation/particleSystem.cpp:257-330) -- in five launches, for worms of
+   (simulation/particleSystem.cpp:257-330) -- in three launches, for worms of
    1..64 beads.  No counterpart in the reference.  Draws advance the generator
    exactly as the separate calls do.  On return gridParticleIndex holds the
    bead ids in stable cell order, sortedPos (and sortedVel / sortedTangent,
    which may be NULL when the WCA pair term without activity does not read
-   them) the gathered records, gridParticleHash the cell of every bead in bead
-   order; the reference's cellStart/cellEnd tables are not produced. */
+   them) the gathered records, gridParticleHash (may be NULL) the cell of
+   every bead in bead order; the reference's cellStart/cellEnd tables are not
+   produced.  Fails (NW error -22) when the grid has more than ~33M cells or
+   the beads x cells product exceeds the sort's tile x bucket table. */
 void nwFusedMdStep(float *pos, float *vel, float *force, float *forceOld, float *tangent,
                    uint *gridParticleHash, uint *gridParticleIndex,
                    float *sortedPos, float *sortedVel, float *sortedTangent,
@@ -222,22 +224,32 @@ void nwFusedMdStep(float *pos, float *vel, float *force, float *forceOld, float 
 
 /* Fused SRD collision step: calcHash(srd) + sortParticles +
    [solventIsokineticThermostat] + solventCellCentorOfMomentum + collideSolvent
-   + integrateSolvent (simulation/particleSystem.cpp:352-410) in five launches
+   + integrateSolvent (simulation/particleSystem.cpp:352-410) in three launches
    over the numParticles beads followed by numSolvent solvent particles, with
    the SolventParams of the last setSolventParameters.  No counterpart in the
    reference.  solventCellCOM receives (centre-of-mass velocity, rotation sign)
-   per SRD cell. */
+   per SRD cell, gridParticleIndex the particle ids in stable cell order,
+   gridParticleHash (may be NULL) the SRD cell of every particle in particle
+   order. */
 void nwFusedSrdStep(float *pos, float *vel, float *solventCellCOM, uint *gridParticleHash,
                     uint *gridParticleIndex, float deltaTime, uint numParticles, uint numSolvent,
                     int isokinetic);
+
+/* 1 when the fused steps' cell sort can plan numParticles particles over
+   numCells cells (filamentSize 1..64: the MD step's tiles of whole worms;
+   0: the SRD step), else 0.  No device work. */
+int nwFusedPlanFits(uint numParticles, uint numCells, uint filamentSize);
 
 /* Device-side anomaly flags (bit 0: sort key above bound).  Synchronises. */
 uint nwGetDeviceFlags(int clear);
 
 /* Kernel timing with HIP events recorded on the library stream around every
-   launch of the named kernel family while enabled.  Names: "integrate",
-   "langevin", "filament", "hash", "sort", "reorder", "collide", "srd_com",
-   "srd_thermo", "srd_collide", "srd_stream", "reverse". */
+   launch of the named kernel family while enabled.  Names of the per-call
+   entry points: "integrate", "langevin", "filament", "hash", "sort",
+   "reorder", "collide", "srd_hash", "srd_sort", "srd_reorder", "srd_com",
+   "srd_thermo", "srd_collide", "srd_stream", "reverse"; of the fused steps:
+   "md_front", "md_sort", "md_collide", "srdf_tiles", "srdf_cells",
+   "srdf_rotate". */
 void nwTimerEnable(const char *kernel, int enable);
 /* Synchronises; returns total milliseconds and launch count since enable. */
 void nwTimerRead(const char *kernel, double *totalMs, uint64_t *launches);

@@ -125,8 +125,16 @@ typedef struct NWExtParams {
     float wcaSigma;                  /* LJ sigma */
     float wcaCutoff;                 /* cutoff radius; WCA uses 2^(1/6) sigma */
     float driveForce;                /* f_a along unit(r[p+1]-r[p]) for p < n-1 */
-    unsigned int reserved[7];
+    unsigned int collideKernel;      /* fused pair pass: NW_COLLIDE_AUTO / _GLOBAL / _LDS */
+    unsigned int reserved[6];
 } NWExtParams;
+
+/* pair-force kernel of the fused step (identical results; different data paths) */
+enum {
+    NW_COLLIDE_AUTO = 0,   /* LDS tiles when the stencil has one z layer, else global */
+    NW_COLLIDE_GLOBAL = 1, /* every candidate read through the cache hierarchy */
+    NW_COLLIDE_LDS = 2     /* neighbour rows staged in LDS per 256-bead tile */
+};
 
 #ifdef __cplusplus
 }

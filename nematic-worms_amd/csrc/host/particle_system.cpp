@@ -170,6 +170,13 @@ void ParticleSystem::_upload(float *dev, const std::vector<float> &host, size_t 
 // --------------------------------------------------------------------------
 // the step (particleSystem.cpp:249-339)
 // --------------------------------------------------------------------------
+bool ParticleSystem::fusedActive() const {
+    if (m_perCall || m_filamentSize < 1 || m_filamentSize > 64) return false;
+    const uint mdCells = m_params.gridSize.x * m_params.gridSize.y * m_params.gridSize.z;
+    if (!nwFusedPlanFits(m_numParticles, mdCells, m_filamentSize)) return false;
+    return !m_solventEnabled || nwFusedPlanFits(m_numParticles + m_numSolvent, m_numSolventCells, 0);
+}
+
 void ParticleSystem::update(float deltaTime) {
     if (fusedActive()) {
         _updateFused(deltaTime);

@@ -72,7 +72,8 @@ public:
     void setRngOffset(uint64_t o) { m_rngOffset = o; }
     uint64_t seed() const { return m_seed; }
     void setPerCallSequence(bool on) { m_perCall = on; }
-    bool fusedActive() const { return !m_perCall && m_filamentSize >= 1 && m_filamentSize <= 64; }
+    // the fused entry points when their sort plans fit, else the per-call sequence
+    bool fusedActive() const;
 
 private:
     void _defaults(uint numFilaments, uint filamentSize, nw_uint3 gridSize, bool srd);

@@ -13,7 +13,7 @@
 
 #include "nw_abi.h"
 #include "nw_internal.hpp"
-#include "nw_fused.hpp"
+#include "nw_bucket.hpp"
 
 namespace {
 
@@ -44,7 +44,7 @@ struct State {
     const uint32_t *lastHashPtr = nullptr;
     bool lastHashSrd = false;     // timer family of the sort / reorder that follow
     SortScratch sort;
-    FusedScratch fused;
+    BucketScratch bucket;
     std::map<std::string, Timer> timers;
 };
 
@@ -438,6 +438,7 @@ void collideSolvent(float *pos, float *vel, float *solventCellCOM, float *random
 void nwFusedMdStep(float *pos, float *vel, float *force, float *forceOld, float *tangent, uint *gridParticleHash,
                    uint *gridParticleIndex, float *sortedPos, float *sortedVel, float *sortedTangent,
                    float deltaTime, float langevinGamma, float langevinKbT, int langevin, uint numFilaments) {
+    (void)gridParticleHash;
     NW_INIT();
     State &s = S();
     const KP kp = make_kp();
@@ -446,14 +447,15 @@ void nwFusedMdStep(float *pos, float *vel, float *force, float *forceOld, float 
     const uint64_t cells = uint64_t(kp.p.gridSize.x) * kp.p.gridSize.y * kp.p.gridSize.z;
     NW_REQUIRE(cells > 0 && cells <= kp.p.numCells, "nwFusedMdStep: grid does not fit numCells");
     const uint64_t N = uint64_t(numFilaments) * n;
-    NW_REQUIRE(N < (1ull << 32), "nwFusedMdStep: too many beads");
+    NW_REQUIRE(N < (1ull << 31), "nwFusedMdStep: too many beads");
     const bool needVel = kp.e.pairPotential == NW_PAIR_REF_DEM;
     const bool needTan = needVel || kp.p.activity != 0.0f;
     NW_REQUIRE(!needVel || sortedVel != nullptr, "nwFusedMdStep: sortedVel is NULL");
     NW_REQUIRE(!needTan || sortedTangent != nullptr, "nwFusedMdStep: sortedTangent is NULL");
     if (N == 0) return;
-    const uint32_t bins = uint32_t(cells) + 1;
-    NW_HIP(ensure_fused_scratch(s.fused, N, bins, s.stream));
+    const BucketPlan bp = make_bucket_plan(N, cells, md_tile_items(n));
+    NW_REQUIRE(!*bucket_plan_problem(bp), "nwFusedMdStep: %s", bucket_plan_problem(bp));
+    NW_HIP(ensure_bucket_scratch(s.bucket, bp));
 
     MdFrontArgs a;
     a.pos = reinterpret_cast<float4 *>(pos);
@@ -462,6 +464,9 @@ void nwFusedMdStep(float *pos, float *vel, float *force, float *forceOld, float 
     a.forceOld = reinterpret_cast<float4 *>(forceOld);
     a.tangent = reinterpret_cast<float4 *>(tangent);
     a.hash = gridParticleHash;
+    a.pairs = s.bucket.pairs;
+    a.P = s.bucket.P;
+    a.bp = bp;
     a.dt = deltaTime;
     a.W = numFilaments;
     a.seed = s.seed;
@@ -473,29 +478,21 @@ void nwFusedMdStep(float *pos, float *vel, float *force, float *forceOld, float 
     if (langevin) s.offset += N;
     {
         TimedScope ts("md_front", s.stream);
-        NW_HIP(launch_md_front(a, kp, s.fused, s.stream));
+        NW_HIP(launch_md_front(a, kp, s.stream));
     }
     {
-        TimedScope ts("md_scan", s.stream);
-        NW_HIP(launch_scan_bins(s.fused, bins, s.stream));
-    }
-    {
-        TimedScope ts("md_scatter", s.stream);
-        NW_HIP(launch_scatter_ix(gridParticleHash, s.fused, uint32_t(N), gridParticleIndex, s.stream));
-    }
-    {
-        TimedScope ts("md_cells", s.stream);
-        NW_HIP(launch_md_cells(s.fused, bins, gridParticleIndex, a.pos, needVel ? a.vel : nullptr,
+        TimedScope ts("md_sort", s.stream);
+        NW_HIP(launch_md_cells(bp, s.bucket, gridParticleIndex, a.pos, needVel ? a.vel : nullptr,
                                needTan ? a.tangent : nullptr, reinterpret_cast<float4 *>(sortedPos),
                                needVel ? reinterpret_cast<float4 *>(sortedVel) : nullptr,
                                needTan ? reinterpret_cast<float4 *>(sortedTangent) : nullptr, s.stream));
     }
     {
         TimedScope ts("md_collide", s.stream);
-        NW_HIP(launch_collide_csr(a.force, reinterpret_cast<const float4 *>(sortedPos),
+        NW_HIP(launch_collide_off(a.force, reinterpret_cast<const float4 *>(sortedPos),
                                   reinterpret_cast<const float4 *>(sortedVel),
-                                  reinterpret_cast<const float4 *>(sortedTangent), gridParticleIndex, s.fused,
-                                  uint32_t(N), kp, s.stream));
+                                  reinterpret_cast<const float4 *>(sortedTangent), gridParticleIndex,
+                                  s.bucket.off, s.bucket.sortedCell, uint32_t(N), kp, s.stream));
     }
     s.lastHashPtr = nullptr;
 }
@@ -506,37 +503,34 @@ void nwFusedSrdStep(float *pos, float *vel, float *solventCellCOM, uint *gridPar
     State &s = S();
     const KP kp = make_kp();
     const uint64_t NT = uint64_t(numParticles) + numSolvent;
-    NW_REQUIRE(NT < (1ull << 32), "nwFusedSrdStep: too many particles");
+    NW_REQUIRE(NT < (1ull << 31), "nwFusedSrdStep: too many particles");
     const uint64_t cells = uint64_t(kp.s.gridSize.x) * kp.s.gridSize.y * kp.s.gridSize.z;
     NW_REQUIRE(cells > 0 && cells == kp.s.numCells, "nwFusedSrdStep: SRD grid does not match numCells");
-    const uint32_t bins = uint32_t(cells) + 1;
-    NW_HIP(ensure_fused_scratch(s.fused, NT, bins, s.stream));
+    NW_REQUIRE(gridParticleIndex != nullptr && solventCellCOM != nullptr, "nwFusedSrdStep: NULL output buffer");
+    if (NT == 0) return;
+    const BucketPlan bp = make_bucket_plan(NT, cells, 0);
+    NW_REQUIRE(!*bucket_plan_problem(bp), "nwFusedSrdStep: %s", bucket_plan_problem(bp));
+    NW_HIP(ensure_bucket_scratch(s.bucket, bp));
     const uint64_t off = s.offset;
     s.offset += blocks_for(cells);
     const double alpha = double(kp.s.alpha);
     const float cosA = float(std::cos(alpha)), sinA = float(std::sin(alpha));
     float4 *P = reinterpret_cast<float4 *>(pos), *V = reinterpret_cast<float4 *>(vel);
+    float4 *com = reinterpret_cast<float4 *>(solventCellCOM);
+    uint32_t *hash = gridParticleHash ? gridParticleHash : s.bucket.hash;
+    const int iso = isokinetic ? 1 : 0;
     {
-        TimedScope ts("srd_front", s.stream);
-        NW_HIP(launch_srd_front(P, uint32_t(NT), gridParticleHash, s.fused, kp, s.stream));
+        TimedScope ts("srdf_tiles", s.stream);
+        NW_HIP(launch_srd_tiles(P, hash, bp, s.bucket, kp, s.stream));
     }
     {
-        TimedScope ts("srd_scan", s.stream);
-        NW_HIP(launch_scan_bins(s.fused, bins, s.stream));
+        TimedScope ts("srdf_cells", s.stream);
+        NW_HIP(launch_srd_cells(bp, s.bucket, gridParticleIndex, V, com, iso, s.seed, off, kp, s.stream));
     }
     {
-        TimedScope ts("srd_scatter", s.stream);
-        NW_HIP(launch_scatter_ix(gridParticleHash, s.fused, uint32_t(NT), gridParticleIndex, s.stream));
-    }
-    {
-        TimedScope ts("srd_cells", s.stream);
-        NW_HIP(launch_srd_cells(s.fused, bins, gridParticleIndex, V, reinterpret_cast<float4 *>(solventCellCOM),
-                                isokinetic ? 1 : 0, s.seed, off, kp, s.stream));
-    }
-    {
-        TimedScope ts("srd_rotate", s.stream);
-        NW_HIP(launch_srd_rotate(P, V, reinterpret_cast<const float4 *>(solventCellCOM), numParticles,
-                                 uint32_t(NT), deltaTime, cosA, sinA, kp, s.stream));
+        TimedScope ts("srdf_rotate", s.stream);
+        NW_HIP(launch_srd_rotate(hash, com, s.bucket.lam, P, V, uint32_t(NT), numParticles, deltaTime, cosA, sinA,
+                                 iso, kp, s.stream));
     }
     s.lastHashPtr = nullptr;
 }
@@ -594,6 +588,13 @@ void nwGetExtParameters(NWExtParams *ext) {
 void *nwGetStream(void) { return S().stream; }
 
 void nwSetStream(void *stream) { S().stream = static_cast<hipStream_t>(stream); }
+
+int nwFusedPlanFits(uint numParticles, uint numCells, uint filamentSize) {
+    if (filamentSize > 64 || numCells == 0) return 0;
+    const BucketPlan bp =
+        make_bucket_plan(numParticles, numCells, filamentSize ? md_tile_items(filamentSize) : 0u);
+    return *bucket_plan_problem(bp) ? 0 : 1;
+}
 
 uint nwGetDeviceFlags(int clear) {
     if (!ensure_init()) return 0;

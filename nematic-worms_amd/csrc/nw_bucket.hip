@@ -1,0 +1,452 @@
+// nw_bucket.hip -- atomic-free two-level cell sort of the fused steps and the
+// SRD collision over it (gfx950).
+//
+// Device-scope atomics serialise far from the CUs on MI355X (a few tens of
+// G/s measured), so a histogram with one global counter per cell costs more
+// than the data it sorts, and a global scan over a tile x bucket matrix costs
+// launches.  Here every counter lives in LDS and no global scan is needed:
+//   level 1   (k_srd_tiles, and the MD front kernel in nw_fused.hip) a tile
+//             of consecutive particles counts its members per bucket in LDS,
+//             scans the counts and writes the prefix as one row of P, then
+//             its (cell, id) pairs grouped by bucket into its pair region
+//   level 2   (k_bucket_cells) a workgroup per bucket, buckets mapped to
+//             XCDs in contiguous runs so neighbours share L2 lines of P: the
+//             column sum of P at the bucket is its first sorted slot, the
+//             column differences are its segments in the tiles' regions; the
+//             members are counted per cell in LDS, placed, each cell sorted
+//             by id (the order of a stable sort by cell), written out with
+//             the CSR offsets; MD gathers the sorted records, SRD sums every
+//             cell in sorted order ([isokinetic] + centre of momentum)
+//   rotate    (k_srd_rotate) particle order, coalesced: every particle
+//             rotates about its cell's momentum, solvent streams
+// Results are deterministic and equal to a stable sort by cell.
+#include "nw_bucket.hpp"
+#include "nw_physics.hpp"
+
+namespace nw {
+
+constexpr int kTileBlock = 1024;
+
+// ---------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------
+BucketPlan make_bucket_plan(uint64_t n, uint64_t cells, uint32_t tileItems) {
+    BucketPlan b{};
+    b.n = uint32_t(n);
+    b.cells = uint32_t(cells);
+    // at most ~1536 members per bucket on average (the LDS stage holds 2048),
+    // at most 2048 cells per bucket
+    const double perCell = cells ? double(n) / double(cells) : 1.0;
+    int S = 0;
+    while (S < 11 && double(1u << (S + 1)) * perCell <= 1536.0) ++S;
+    b.shift = uint32_t(S);
+    b.buckets = uint32_t((cells + (1u << S) - 1) >> S);
+    if (tileItems == 0) {
+        // P stays near 2^21 entries: tiles of n (buckets + 1) / 2^21, power of two
+        const double want = double(n) * double(b.buckets + 1) / double(1u << 21);
+        uint32_t T = 4096;
+        while (T < (1u << 17) && double(T) < want) T <<= 1;
+        tileItems = T;
+    }
+    b.tileItems = tileItems;
+    b.tiles = uint32_t((n + tileItems - 1) / tileItems);
+    return b;
+}
+
+const char *bucket_plan_problem(const BucketPlan &bp) {
+    if (bp.buckets > kMaxBuckets) return "grid has too many cells for the fused sort";
+    if (bp.tiles > kMaxTiles) return "too many particles for the fused sort";
+    if (uint64_t(bp.tiles) * (bp.buckets + 1) > kMaxMatrix) return "particles x cells too large for the fused sort";
+    return "";
+}
+
+// level-2 LDS layout (bytes), shared by host (size) and device (carving)
+struct CellsLayout {
+    uint32_t wsum, cnt, cur, lid, lcell, segStart, segBase, tmpId, tmpCell, lv, total;
+};
+
+__host__ __device__ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
+
+__host__ __device__ inline CellsLayout cells_layout(const BucketPlan &bp, bool srd) {
+    CellsLayout L;
+    const uint32_t nc = 1u << bp.shift;
+    uint32_t o = 0;
+    L.wsum = o; o += align16(4u * (kCellsBlock / 64 + 4));
+    L.cnt = o;  o += align16(4u * (nc + 1));
+    L.cur = o;  o += align16(4u * nc);
+    L.lid = o;  o += align16(4u * kBucketCap);
+    L.lcell = o; if (!srd) o += align16(2u * kBucketCap);
+    // region A (segment tables, staged pairs) -- SRD velocities alias it
+    const uint32_t a0 = o;
+    L.segStart = o; o += align16(4u * (bp.tiles + 1));
+    L.segBase = o;  o += align16(4u * bp.tiles);
+    L.tmpId = o;    o += align16(4u * kBucketCap);
+    L.tmpCell = o;  o += align16(2u * kBucketCap);
+    L.lv = a0;
+    if (srd && o < a0 + 16u * kBucketCap) o = a0 + 16u * kBucketCap;
+    L.total = o;
+    return L;
+}
+
+// blocks i = x + 8k of a launch are spread round-robin over the 8 XCDs; map
+// them so every XCD takes one contiguous run of buckets (a bijection)
+__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t i, uint32_t G) {
+    const uint32_t q = G >> 3, r = G & 7u;
+    const uint32_t x = i & 7u, k = i >> 3;
+    return x < r ? x * (q + 1u) + k : r * (q + 1u) + (x - r) * q + k;
+}
+
+// ---------------------------------------------------------------------------
+// level 1 (SRD): cell of every particle, tile histogram, P row, pairs
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kTileBlock) void k_srd_tiles(const float4 *__restrict__ pos, uint32_t *__restrict__ hash,
+                                                          uint2 *__restrict__ pairs, uint32_t *__restrict__ P,
+                                                          BucketPlan bp, KP kp) {
+    extern __shared__ uint32_t lh[];
+    uint32_t *wsum = lh + bp.buckets + 1;
+    const uint32_t B1 = bp.buckets + 1;
+    for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) lh[b] = 0u;
+    __syncthreads();
+    const uint32_t t = blockIdx.x;
+    const uint32_t i0 = t * bp.tileItems;
+    const uint32_t i1 = min(i0 + bp.tileItems, bp.n);
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const uint32_t h = srd_hash(kp.p, kp.s, mk(pos[i]));
+        hash[i] = h;
+        atomicAdd(&lh[h >> bp.shift], 1u);
+    }
+    __syncthreads();
+    block_excl_scan(lh, bp.buckets, wsum);
+    uint32_t *row = P + uint64_t(t) * B1;
+    for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) row[b] = lh[b];
+    __syncthreads();
+    // second pass over the same particles (each thread re-reads its own hash)
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const uint32_t h = hash[i];
+        const uint32_t slot = atomicAdd(&lh[h >> bp.shift], 1u);
+        pairs[i0 + slot] = make_uint2(h, i);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// level 2
+// ---------------------------------------------------------------------------
+struct CellsArgs {
+    const uint2 *pairs;
+    const uint32_t *P;
+    uint32_t *ids, *off, *cellOf;
+    // MD gathers (sPos NULL: none)
+    const float4 *pos, *vel, *tan;
+    float4 *sPos, *sVel, *sTan;
+    // SRD cell sums
+    const float4 *srdVel;
+    float4 *com;
+    float *lam;
+    int iso;
+    float kbT;
+    uint64_t seed, offset;
+};
+
+// index of the largest t in [0, tiles) with segStart[t] <= k
+__device__ __forceinline__ uint32_t seg_of(const uint32_t *segStart, uint32_t tiles, uint32_t k) {
+    uint32_t lo = 0, hi = tiles;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (segStart[mid] <= k) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <bool SRD>
+__global__ __launch_bounds__(kCellsBlock) void k_bucket_cells(CellsArgs o, BucketPlan bp) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const CellsLayout L = cells_layout(bp, SRD);
+    uint32_t *wsum = reinterpret_cast<uint32_t *>(smem + L.wsum);
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(smem + L.cnt);
+    uint32_t *cur = reinterpret_cast<uint32_t *>(smem + L.cur);
+    uint32_t *lid = reinterpret_cast<uint32_t *>(smem + L.lid);
+    uint16_t *lcell = reinterpret_cast<uint16_t *>(smem + L.lcell);
+    uint32_t *segStart = reinterpret_cast<uint32_t *>(smem + L.segStart);
+    uint32_t *segBase = reinterpret_cast<uint32_t *>(smem + L.segBase);
+    uint32_t *tmpId = reinterpret_cast<uint32_t *>(smem + L.tmpId);
+    uint16_t *tmpCell = reinterpret_cast<uint16_t *>(smem + L.tmpCell);
+    float4 *lv = reinterpret_cast<float4 *>(smem + L.lv);
+
+    const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
+    const uint32_t B1 = bp.buckets + 1;
+    const uint32_t tid = threadIdx.x, bd = blockDim.x;
+
+    // segments of this bucket in every tile; first sorted slot
+    uint32_t part = 0;
+    for (uint32_t t = tid; t < bp.tiles; t += bd) {
+        const uint32_t *row = o.P + uint64_t(t) * B1 + b;
+        const uint32_t lo = row[0], hi = row[1];
+        segBase[t] = lo;
+        segStart[t] = hi - lo;
+        part += lo;
+    }
+    for (int d = 32; d >= 1; d >>= 1) part += __shfl_xor(part, d, 64);
+    if ((tid & 63u) == 0u) wsum[kCellsBlock / 64 + (tid >> 6)] = part;  // upper half of wsum
+    __syncthreads();
+    uint32_t sb = 0;
+    for (uint32_t w = 0; w < kCellsBlock / 64; ++w) sb += wsum[kCellsBlock / 64 + w];
+    block_excl_scan(segStart, bp.tiles, wsum);
+    const uint32_t m = segStart[bp.tiles];
+    const uint32_t cell0 = b << bp.shift;
+    const uint32_t nc = min(1u << bp.shift, bp.cells - cell0);
+    for (uint32_t c = tid; c < nc; c += bd) cnt[c] = 0u;
+    __syncthreads();
+
+    const bool staged = m <= kBucketCap;
+    const uint64_t T = bp.tileItems;
+    for (uint32_t k = tid; k < m; k += bd) {
+        const uint32_t t = seg_of(segStart, bp.tiles, k);
+        const uint2 q = o.pairs[uint64_t(t) * T + segBase[t] + (k - segStart[t])];
+        const uint32_t lc = q.x - cell0;
+        if (staged) {
+            tmpId[k] = q.y;
+            tmpCell[k] = uint16_t(lc);
+        }
+        atomicAdd(&cnt[lc], 1u);
+    }
+    __syncthreads();
+    block_excl_scan(cnt, nc, wsum);
+    for (uint32_t c = tid; c < nc; c += bd) cur[c] = cnt[c];
+    __syncthreads();
+
+    uint32_t *out = o.ids + sb;
+    if (staged) {
+        for (uint32_t k = tid; k < m; k += bd) {
+            const uint32_t lc = tmpCell[k];
+            const uint32_t slot = atomicAdd(&cur[lc], 1u);
+            lid[slot] = tmpId[k];
+            if (!SRD) lcell[slot] = uint16_t(lc);
+        }
+    } else {
+        for (uint32_t k = tid; k < m; k += bd) {
+            const uint32_t t = seg_of(segStart, bp.tiles, k);
+            const uint2 q = o.pairs[uint64_t(t) * T + segBase[t] + (k - segStart[t])];
+            out[atomicAdd(&cur[q.x - cell0], 1u)] = q.y;
+        }
+    }
+    __syncthreads();
+    // every cell in ascending id order; CSR offsets
+    uint32_t *arr = staged ? lid : out;
+    for (uint32_t c = tid; c < nc; c += bd) {
+        const uint32_t a = cnt[c], z = cnt[c + 1];
+        for (uint32_t u = a + 1; u < z; ++u) {
+            const uint32_t x = arr[u];
+            uint32_t w = u;
+            while (w > a && arr[w - 1] > x) {
+                arr[w] = arr[w - 1];
+                --w;
+            }
+            arr[w] = x;
+        }
+        o.off[cell0 + c] = sb + a;
+    }
+    if (tid == 0 && cell0 + nc == bp.cells) o.off[bp.cells] = bp.n;
+    __syncthreads();
+
+    if (!SRD) {
+        for (uint32_t k = tid; k < m; k += bd) {
+            const uint32_t id = arr[k];
+            if (staged) out[k] = id;
+            o.cellOf[sb + k] = cell0 + (staged ? uint32_t(lcell[k]) : seg_of(cnt, nc, k));
+            if (o.sPos) {
+                o.sPos[sb + k] = o.pos[id];
+                if (o.sVel) o.sVel[sb + k] = o.vel[id];
+                if (o.sTan) o.sTan[sb + k] = o.tan[id];
+            }
+        }
+        return;
+    }
+
+    // SRD: stage the members' velocities (region A is dead now)
+    if (staged) {
+        for (uint32_t k = tid; k < m; k += bd) {
+            const uint32_t id = lid[k];
+            out[k] = id;
+            lv[k] = o.srdVel[id];
+        }
+        __syncthreads();
+    }
+    // one thread per cell, sums in sorted order: the arithmetic of
+    // isokineticThermostatSRD and cellCenterOfMomentumSRD
+    // (particles_kernel_impl.cuh:559-637)
+    for (uint32_t c = tid; c < nc; c += bd) {
+        const uint32_t a = cnt[c], z = cnt[c + 1];
+        f3 mom = mk(0.f, 0.f, 0.f);
+        float lambda = 1.0f;
+        if (a != z) {
+            if (o.iso) {
+                float E = 0.0f;
+                for (uint32_t u = a; u < z; ++u) {
+                    const float4 V = staged ? lv[u] : o.srdVel[out[u]];
+                    const f3 v = mk(V);
+                    E += 0.5f * V.w * dot(v, v);
+                }
+                lambda = sqrtf(o.kbT / E);
+            }
+            float M = 0.0f;
+            for (uint32_t u = a; u < z; ++u) {
+                float4 V = staged ? lv[u] : o.srdVel[out[u]];
+                if (o.iso) V = f4(lambda * mk(V), V.w);
+                mom += V.w * mk(V);
+                M += V.w;
+            }
+            if (M > 0.0f) mom = mom * (1.0f / M);
+        }
+        const uint32_t cell = cell0 + c;
+        const float sgn = uniform_at(o.seed, o.offset, cell) > 0.5f ? 1.0f : -1.0f;
+        o.com[cell] = f4(mom, sgn);
+        if (o.iso) o.lam[cell] = lambda;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SRD rotation and streaming in particle order (collideSolventKernel
+// :651-691, solvent_integrator :145-205); the isokinetic factor of the cell
+// scales the velocity first, as isokineticThermostatSRD writes it back
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_srd_rotate(const uint32_t *__restrict__ hash,
+                                                       const float4 *__restrict__ com, const float *__restrict__ lam,
+                                                       float4 *__restrict__ pos, float4 *__restrict__ vel,
+                                                       uint32_t NT, uint32_t N, float dt, float cosA, float sinA,
+                                                       int iso, KP kp) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NT) return;
+    const bool solvent = i >= N;
+    const bool rotBeads = kp.e.srdCollideFilaments != 0;
+    if (!solvent && !rotBeads && !iso) return;
+    const uint32_t c = hash[i];
+    float4 V = vel[i];
+    if (iso) {
+        const float l = lam[c];
+        V = f4(l * mk(V), V.w);
+    }
+    if (!solvent && !rotBeads) {
+        vel[i] = V;
+        return;
+    }
+    const float4 C = com[c];
+    const f3 u = mk(C);
+    const float sn = C.w * sinA;
+    const f3 d = mk(V) - u;
+    const f3 rot = mk(d.x * cosA - d.y * sn, d.x * sn + d.y * cosA, d.z);
+    f3 v = u + rot;
+    if (solvent) {
+        const float4 Pp = pos[i];
+        f3 r = mk(Pp);
+        const f3 dr = dt * v;
+        r += dr;
+        apply_boundaries(kp.p, r, v, dr, 0.0f);
+        pos[i] = f4(r, Pp.w);
+    }
+    vel[i] = f4(v, V.w);
+}
+
+// ---------------------------------------------------------------------------
+// scratch and launchers
+// ---------------------------------------------------------------------------
+template <typename T>
+static hipError_t regrow(T *&p, uint64_t &cap, uint64_t need, uint64_t elem) {
+    if (need <= cap) return hipSuccess;
+    const uint64_t c = need + need / 8 + 1024;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    const hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), c * elem);
+    if (e == hipSuccess) cap = c;
+    return e;
+}
+
+hipError_t ensure_bucket_scratch(BucketScratch &s, const BucketPlan &bp) {
+    hipError_t e;
+    if (bp.n > s.cap_n) {
+        const uint64_t c = uint64_t(bp.n) + bp.n / 8 + 1024;
+        if (s.hash) (void)hipFree(s.hash);
+        if (s.sortedCell) (void)hipFree(s.sortedCell);
+        s.hash = s.sortedCell = nullptr;
+        s.cap_n = 0;
+        if ((e = hipMalloc(&s.hash, c * 4)) != hipSuccess) return e;
+        if ((e = hipMalloc(&s.sortedCell, c * 4)) != hipSuccess) return e;
+        s.cap_n = c;
+    }
+    if ((e = regrow(s.pairs, s.cap_pairs, uint64_t(bp.tiles) * bp.tileItems, 8)) != hipSuccess) return e;
+    if ((e = regrow(s.P, s.cap_mat, uint64_t(bp.tiles) * (bp.buckets + 1), 4)) != hipSuccess) return e;
+    if (uint64_t(bp.cells) + 2 > s.cap_cells) {
+        const uint64_t c = uint64_t(bp.cells) + bp.cells / 8 + 1024;
+        if (s.off) (void)hipFree(s.off);
+        if (s.lam) (void)hipFree(s.lam);
+        s.off = nullptr;
+        s.lam = nullptr;
+        s.cap_cells = 0;
+        if ((e = hipMalloc(&s.off, (c + 2) * 4)) != hipSuccess) return e;
+        if ((e = hipMalloc(&s.lam, c * 4)) != hipSuccess) return e;
+        s.cap_cells = c;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_srd_tiles(const float4 *pos, uint32_t *hash, const BucketPlan &bp, BucketScratch &s,
+                            const KP &kp, hipStream_t st) {
+    if (bp.n == 0) return hipSuccess;
+    const size_t lds = (size_t(bp.buckets) + 1 + kTileBlock / 64) * sizeof(uint32_t);
+    hipLaunchKernelGGL(k_srd_tiles, dim3(bp.tiles), dim3(kTileBlock), lds, st, pos, hash, s.pairs, s.P, bp, kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_md_cells(const BucketPlan &bp, BucketScratch &s, uint32_t *ids, const float4 *pos,
+                           const float4 *vel, const float4 *tan, float4 *sPos, float4 *sVel, float4 *sTan,
+                           hipStream_t st) {
+    if (bp.n == 0) return hipSuccess;
+    CellsArgs o{};
+    o.pairs = s.pairs;
+    o.P = s.P;
+    o.ids = ids;
+    o.off = s.off;
+    o.cellOf = s.sortedCell;
+    o.pos = pos;
+    o.vel = vel;
+    o.tan = tan;
+    o.sPos = sPos;
+    o.sVel = sVel;
+    o.sTan = sTan;
+    const CellsLayout L = cells_layout(bp, false);
+    hipLaunchKernelGGL(k_bucket_cells<false>, dim3(bp.buckets), dim3(kCellsBlock), L.total, st, o, bp);
+    return hipGetLastError();
+}
+
+hipError_t launch_srd_cells(const BucketPlan &bp, BucketScratch &s, uint32_t *ids, const float4 *vel, float4 *com,
+                            int iso, uint64_t seed, uint64_t offset, const KP &kp, hipStream_t st) {
+    if (bp.n == 0) return hipSuccess;
+    CellsArgs o{};
+    o.pairs = s.pairs;
+    o.P = s.P;
+    o.ids = ids;
+    o.off = s.off;
+    o.cellOf = s.sortedCell;
+    o.srdVel = vel;
+    o.com = com;
+    o.lam = s.lam;
+    o.iso = iso;
+    o.kbT = kp.s.kbT;
+    o.seed = seed;
+    o.offset = offset;
+    const CellsLayout L = cells_layout(bp, true);
+    hipLaunchKernelGGL(k_bucket_cells<true>, dim3(bp.buckets), dim3(kCellsBlock), L.total, st, o, bp);
+    return hipGetLastError();
+}
+
+hipError_t launch_srd_rotate(const uint32_t *hash, const float4 *com, const float *lam, float4 *pos, float4 *vel,
+                             uint32_t NT, uint32_t N, float dt, float cosA, float sinA, int iso, const KP &kp,
+                             hipStream_t st) {
+    if (NT == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_srd_rotate, dim3(grid_for(NT)), dim3(kBlock), 0, st, hash, com, lam, pos, vel, NT, N, dt,
+                       cosA, sinA, iso, kp);
+    return hipGetLastError();
+}
+
+}  // namespace nw

@@ -1,0 +1,124 @@
+// nw_bucket.hpp -- two-level cell sort, sorted SRD collision (nw_bucket.hip)
+// and the fused MD kernels (nw_fused.hip).
+#pragma once
+
+#include "nw_internal.hpp"
+
+namespace nw {
+
+// Level 1 cuts the particles into tiles of tileItems consecutive ids; every
+// tile counts its members per bucket (2^shift consecutive cells) and writes
+// one row of the prefix matrix P[tile][bucket] (buckets + 1 entries: the
+// exclusive prefix of its counts, the tile total last) plus its (cell, id)
+// pairs grouped by bucket into its own region of the pair array.  Level 2
+// runs one workgroup per bucket: the column sum of P[.][bucket] is the
+// bucket's first output slot, the column differences locate its members.
+struct BucketPlan {
+    uint32_t n;          // particles
+    uint32_t cells;      // cells of the grid
+    uint32_t shift;      // log2(cells per bucket), <= 11
+    uint32_t buckets;    // ceil(cells / 2^shift)
+    uint32_t tileItems;  // particles per level-1 tile
+    uint32_t tiles;      // ceil(n / tileItems)
+};
+
+constexpr uint32_t kBucketCap = 2048;     // bucket members staged in LDS (more: global-memory path)
+constexpr uint32_t kMaxTiles = 4096;      // level-2 segment tables in LDS
+constexpr uint32_t kMaxBuckets = 16383;   // level-1 LDS histogram
+constexpr uint64_t kMaxMatrix = 1ull << 24;  // entries of P
+constexpr int kCellsBlock = 256;          // level-2 workgroup
+
+// tileItems == 0: chosen for the particle count (the SRD tiles); the MD
+// tiles are the fused front kernel's workgroups
+BucketPlan make_bucket_plan(uint64_t n, uint64_t cells, uint32_t tileItems);
+// empty string when the plan fits the limits above, else the reason
+const char *bucket_plan_problem(const BucketPlan &bp);
+
+struct BucketScratch {
+    uint32_t *hash = nullptr;        // cell per particle, particle order
+    uint2 *pairs = nullptr;          // (cell, id), per tile grouped by bucket
+    uint32_t *P = nullptr;           // tiles x (buckets + 1)
+    uint32_t *sortedCell = nullptr;  // cell of each sorted slot
+    uint32_t *off = nullptr;         // CSR cell offsets (cells + 1)
+    float *lam = nullptr;            // isokinetic factor per SRD cell
+    uint64_t cap_n = 0, cap_pairs = 0, cap_mat = 0, cap_cells = 0;
+};
+
+hipError_t ensure_bucket_scratch(BucketScratch &s, const BucketPlan &bp);
+
+// beads per fused-front workgroup (the MD tile): 16 waves of whole worms
+inline uint32_t md_tile_items(uint32_t filamentSize) { return 16u * (64u / filamentSize) * filamentSize; }
+
+struct MdFrontArgs {
+    float4 *pos, *vel, *force, *forceOld, *tangent;
+    uint32_t *hash;  // may be NULL
+    uint2 *pairs;
+    uint32_t *P;
+    BucketPlan bp;
+    float dt, lgvStd;
+    int langevin;
+    uint32_t W;
+    uint64_t seed, offRev, offLgv;
+};
+
+// integrate + reversal + Langevin + bonded + tangent + MD cell + level 1
+hipError_t launch_md_front(const MdFrontArgs &a, const KP &kp, hipStream_t st);
+// level 2 for MD: ids in stable cell order, CSR offsets, cell of every
+// slot, gathered records (sVel / sTan may be NULL)
+hipError_t launch_md_cells(const BucketPlan &bp, BucketScratch &s, uint32_t *ids, const float4 *pos,
+                           const float4 *vel, const float4 *tan, float4 *sPos, float4 *sVel, float4 *sTan,
+                           hipStream_t st);
+// SRD: cell of every particle + level 1
+hipError_t launch_srd_tiles(const float4 *pos, uint32_t *hash, const BucketPlan &bp, BucketScratch &s,
+                            const KP &kp, hipStream_t st);
+// SRD level 2: ids in stable cell order, per cell (centre-of-mass velocity,
+// rotation sign) into com and the isokinetic factor into s.lam
+hipError_t launch_srd_cells(const BucketPlan &bp, BucketScratch &s, uint32_t *ids, const float4 *vel, float4 *com,
+                            int iso, uint64_t seed, uint64_t offset, const KP &kp, hipStream_t st);
+// SRD rotation of every particle about its cell's momentum, solvent streaming
+hipError_t launch_srd_rotate(const uint32_t *hash, const float4 *com, const float *lam, float4 *pos, float4 *vel,
+                             uint32_t NT, uint32_t N, float dt, float cosA, float sinA, int iso, const KP &kp,
+                             hipStream_t st);
+// pair pass over CSR offsets; sortedCell may be NULL (cells recomputed from positions)
+hipError_t launch_collide_off(float4 *force, const float4 *sPos, const float4 *sVel, const float4 *sTan,
+                              const uint32_t *sIdx, const uint32_t *off, const uint32_t *sortedCell, uint32_t n,
+                              const KP &kp, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// block-level helpers shared by the level-1 kernels (whole workgroup calls)
+// ---------------------------------------------------------------------------
+// in-place exclusive scan of a[0..m) over the workgroup, a[m] = total;
+// wsum holds blockDim/64 words; ends with a barrier
+__device__ __forceinline__ void block_excl_scan(uint32_t *a, uint32_t m, uint32_t *wsum) {
+    const uint32_t T = blockDim.x;
+    const uint32_t per = (m + T - 1) / T;
+    const uint32_t b0 = threadIdx.x * per;
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < per; ++k)
+        if (b0 + k < m) s += a[b0 + k];
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    uint32_t incl = s;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(incl, d, 64);
+        if (lane >= uint32_t(d)) incl += t;
+    }
+    if (lane == 63u) wsum[wid] = incl;
+    __syncthreads();
+    uint32_t run = incl - s;
+    for (uint32_t w = 0; w < wid; ++w) run += wsum[w];
+    for (uint32_t k = 0; k < per; ++k) {
+        if (b0 + k < m) {
+            const uint32_t v = a[b0 + k];
+            a[b0 + k] = run;
+            run += v;
+        }
+    }
+    if (threadIdx.x == T - 1) {
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < T / 64; ++w) tot += wsum[w];
+        a[m] = tot;
+    }
+    __syncthreads();
+}
+
+}  // namespace nw

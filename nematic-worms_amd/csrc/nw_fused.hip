@@ -1,51 +1,67 @@
-// nw_fused.hip -- the worm step as few, fat kernels (gfx950).
+// nw_fused.hip -- the fused MD step kernels (gfx950).
 //
-// The reference issues ~10 launches per MD step and ~9 per SRD collision,
-// several of them tiny; on MI355X each dispatch costs a few microseconds of
-// fixed time, and the reorder gathers re-read what the sort just touched.
-// This file computes the same step (bit for bit where the per-call path is
-// bit-exact) in:
-//   MD :  k_md_front   integrate + reversal + Langevin + bonded + tangent +
-//                      cell hash + histogram (whole worms per wavefront)
-//         scan         cell offsets (CSR) from the histogram
-//         k_scatter_ix bead ids into their cells (arbitrary order in a cell)
-//         k_md_cells   per cell: stable order restored, positions gathered,
-//                      histogram bin cleared for the next step
-//         k_collide_csr pair forces; a row of three cells is one run of beads
-//   SRD:  k_srd_front  shifted SRD hash + histogram of beads and solvent
-//         scan, k_scatter_ix
-//         k_srd_cells  per cell: stable order, isokinetic rescale, centre of
-//                      momentum in sorted order, the cell's rotation sign
-//         k_srd_rotate rotation about the cell momentum + solvent streaming
-#include "nw_fused.hpp"
+// The reference issues ~10 launches per MD step, several of them tiny; on
+// MI355X each dispatch costs microseconds of fixed time, and the reorder
+// gathers re-read what the sort just touched.  The fused step is
+//   k_md_front      integrate + reversal + Langevin + bonded + tangent + cell
+//                   hash (whole worms per wavefront, chain neighbours by
+//                   cross-lane shuffles)
+//   cell sort       nw_bucket.hip (stable, atomic-free, gathers the records)
+//   k_collide_lds   pair forces, neighbour rows staged in LDS (k_collide_csr
+//                   without LDS for stencils over several z layers)
+// computing the per-call path's expressions in the same order.
+#include "nw_bucket.hpp"
 #include "nw_physics.hpp"
 
 namespace nw {
 
+// cell coordinates of sorted bead i: from the sort's cell ids when given
+// (already wrapped into the grid), else from the position
+__device__ __forceinline__ void bead_cell(const KP &kp, const uint32_t *__restrict__ sCell, uint32_t i, f3 r, int &gx,
+                                          int &gy, int &gz) {
+    const SimParams &p = kp.p;
+    if (sCell) {
+        const uint32_t c = sCell[i];
+        const uint32_t Gx = p.gridSize.x, layer = p.gridSize.x * p.gridSize.y;
+        const uint32_t z = c / layer, rem = c - z * layer;
+        const uint32_t y = rem / Gx;
+        gx = int(rem - y * Gx);
+        gy = int(y);
+        gz = int(z);
+    } else {
+        gx = wrap_mod(cell_coord(r.x, p.origin.x, p.cellSize.x), p.gridSize.x);
+        gy = wrap_mod(cell_coord(r.y, p.origin.y, p.cellSize.y), p.gridSize.y);
+        gz = wrap_mod(cell_coord(r.z, p.origin.z, p.cellSize.z), p.gridSize.z);
+    }
+}
+
 // ===========================================================================
 // MD front
 // ===========================================================================
-__global__ __launch_bounds__(kBlock) void k_md_front(float4 *__restrict__ pos, float4 *__restrict__ vel,
-                                                     float4 *__restrict__ force, float4 *__restrict__ forceOld,
-                                                     float4 *__restrict__ tangent, uint32_t *__restrict__ hash,
-                                                     uint32_t *__restrict__ slot, uint32_t *__restrict__ counts,
-                                                     float dt, float lgvStd, int langevin, uint32_t W, uint64_t seed,
-                                                     uint64_t offRev, uint64_t offLgv, KP kp) {
+constexpr int kFrontBlock = 1024;  // one level-1 tile of the cell sort
+
+__global__ __launch_bounds__(kFrontBlock) void k_md_front(MdFrontArgs a, KP kp) {
+    extern __shared__ uint32_t lh[];  // bucket histogram (buckets + 1), wave sums
     const SimParams &p = kp.p;
     const uint32_t n = p.filamentSize;
-    const WaveWorm m = wave_worm(n, W);
+    const BucketPlan &bp = a.bp;
+    const uint32_t B1 = bp.buckets + 1;
+    uint32_t *wsum = lh + B1;
+    for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) lh[b] = 0u;
+
+    const WaveWorm m = wave_worm(n, a.W);
     const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 P = m.active ? pos[m.bead] : zero4;
-    const float4 V = m.active ? vel[m.bead] : zero4;
-    const float4 F = m.active ? force[m.bead] : zero4;
-    const float4 FO = m.active ? forceOld[m.bead] : zero4;
+    const float4 P = m.active ? a.pos[m.bead] : zero4;
+    const float4 V = m.active ? a.vel[m.bead] : zero4;
+    const float4 F = m.active ? a.force[m.bead] : zero4;
+    const float4 FO = m.active ? a.forceOld[m.bead] : zero4;
 
     // velocity Verlet (filament_integrator, particles_kernel_impl.cuh:71-142)
     f3 r = mk(P), v = mk(V);
     const f3 ff = mk(F), fo = mk(FO);
-    const f3 dv = 0.5f * (ff + fo) * dt;
+    const f3 dv = 0.5f * (ff + fo) * a.dt;
     v += dv;
-    const f3 dr = v * dt + 0.5f * ff * dt * dt;
+    const f3 dr = v * a.dt + 0.5f * ff * a.dt * a.dt;
     r += dr;
     apply_boundaries(p, r, v, dr, p.particleRadius);
     f3 fold = ff;
@@ -54,7 +70,7 @@ __global__ __launch_bounds__(kBlock) void k_md_front(float4 *__restrict__ pos, f
     // polarity reversal (reverseFilamentsKernel :820-857): bead p <-> n-1-p
     // of the worm, taken from the mirror lane; the force is zero here
     if (p.reverseProbability > 0.0f) {
-        const bool rev = m.active && !(uniform_at(seed, offRev, m.worm) > p.reverseProbability);
+        const bool rev = m.active && !(uniform_at(a.seed, a.offRev, m.worm) > p.reverseProbability);
         const int src = rev ? int(m.wl * n + (n - 1u - m.pp)) : int(m.lane);
         r = mk(__shfl(r.x, src, 64), __shfl(r.y, src, 64), __shfl(r.z, src, 64));
         v = mk(__shfl(v.x, src, 64), __shfl(v.y, src, 64), __shfl(v.z, src, 64));
@@ -66,13 +82,13 @@ __global__ __launch_bounds__(kBlock) void k_md_front(float4 *__restrict__ pos, f
 
     // Langevin (langevinKernel :859-874), the force starts from zero
     f3 acc = mk(0.f, 0.f, 0.f);
-    if (langevin) {
-        const u4 b = philox_block(seed, offLgv + m.bead);
+    if (a.langevin) {
+        const u4 b = philox_block(a.seed, a.offLgv + m.bead);
         float z0, z1, z2, z3;
         box_muller(b.x, b.y, z0, z1);
         box_muller(b.z, b.w, z2, z3);
         (void)z3;
-        const f3 xi = mk(z0 * lgvStd + 0.0f, z1 * lgvStd + 0.0f, z2 * lgvStd + 0.0f);
+        const f3 xi = mk(z0 * a.lgvStd + 0.0f, z1 * a.lgvStd + 0.0f, z2 * a.lgvStd + 0.0f);
         f3 f = mk(0.f, 0.f, 0.f);
         f -= p.gamma * v;
         f += xi;
@@ -82,69 +98,27 @@ __global__ __launch_bounds__(kBlock) void k_md_front(float4 *__restrict__ pos, f
     // bonded forces and tangents on the new positions
     f3 tg;
     const bool writeTan = bonded_lane(kp, r, m.pp, acc, tg);
-    if (!m.active) return;
 
-    pos[m.bead] = f4(r, pw);
-    vel[m.bead] = f4(v, vw);
-    force[m.bead] = f4(acc, 0.0f);
-    forceOld[m.bead] = f4(fold, fow);
-    if (writeTan) tangent[m.bead] = f4(tg, 0.0f);
-
-    // cell of the new position and its histogram slot
-    const uint32_t h = md_hash_wrapped(p, cell_coord(r.x, p.origin.x, p.cellSize.x),
-                                       cell_coord(r.y, p.origin.y, p.cellSize.y),
-                                       cell_coord(r.z, p.origin.z, p.cellSize.z));
-    hash[m.bead] = h;
-    slot[m.bead] = atomicAdd(&counts[h], 1u);
-}
-
-// ===========================================================================
-// shared: scatter ids into their bins
-// ===========================================================================
-__global__ __launch_bounds__(kBlock) void k_scatter_ix(const uint32_t *__restrict__ hash,
-                                                       const uint32_t *__restrict__ slot,
-                                                       const uint32_t *__restrict__ offsets, uint32_t n,
-                                                       uint32_t *__restrict__ out) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    out[offsets[hash[i]] + slot[i]] = i;
-}
-
-// ascending insertion sort of ids[s, e) (a cell's members: a handful)
-__device__ __forceinline__ void sort_segment(uint32_t *__restrict__ ids, uint32_t s, uint32_t e) {
-    for (uint32_t a = s + 1; a < e; ++a) {
-        const uint32_t x = ids[a];
-        uint32_t c = a;
-        while (c > s) {
-            const uint32_t y = ids[c - 1];
-            if (y <= x) break;
-            ids[c] = y;
-            --c;
-        }
-        ids[c] = x;
+    uint32_t h = 0, rank = 0;
+    __syncthreads();  // histogram cleared
+    if (m.active) {
+        a.pos[m.bead] = f4(r, pw);
+        a.vel[m.bead] = f4(v, vw);
+        a.force[m.bead] = f4(acc, 0.0f);
+        a.forceOld[m.bead] = f4(fold, fow);
+        if (writeTan) a.tangent[m.bead] = f4(tg, 0.0f);
+        // cell of the new position (calcHashD)
+        h = md_hash_wrapped(p, cell_coord(r.x, p.origin.x, p.cellSize.x), cell_coord(r.y, p.origin.y, p.cellSize.y),
+                            cell_coord(r.z, p.origin.z, p.cellSize.z));
+        if (a.hash) a.hash[m.bead] = h;
+        rank = atomicAdd(&lh[h >> bp.shift], 1u);
     }
-}
-
-// ===========================================================================
-// MD cells: stable order + gather + clear histogram
-// ===========================================================================
-__global__ __launch_bounds__(kBlock) void k_md_cells(const uint32_t *__restrict__ offsets,
-                                                     uint32_t *__restrict__ counts, uint32_t bins,
-                                                     uint32_t *__restrict__ ids, const float4 *__restrict__ pos,
-                                                     const float4 *__restrict__ vel, const float4 *__restrict__ tan,
-                                                     float4 *__restrict__ sPos, float4 *__restrict__ sVel,
-                                                     float4 *__restrict__ sTan) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= bins) return;
-    counts[c] = 0u;
-    const uint32_t s = offsets[c], e = offsets[c + 1];
-    sort_segment(ids, s, e);
-    for (uint32_t j = s; j < e; ++j) {
-        const uint32_t b = ids[j];
-        sPos[j] = pos[b];
-        if (sVel) sVel[j] = vel[b];
-        if (sTan) sTan[j] = tan[b];
-    }
+    __syncthreads();
+    // level 1 of the cell sort: this workgroup's beads are tile blockIdx.x
+    block_excl_scan(lh, bp.buckets, wsum);
+    uint32_t *row = a.P + uint64_t(blockIdx.x) * B1;
+    for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) row[b] = lh[b];
+    if (m.active) a.pairs[uint64_t(blockIdx.x) * bp.tileItems + lh[h >> bp.shift] + rank] = make_uint2(h, m.bead);
 }
 
 // ===========================================================================
@@ -189,7 +163,8 @@ __global__ __launch_bounds__(kBlock) void k_collide_csr(float4 *__restrict__ for
                                                         const float4 *__restrict__ sVel,
                                                         const float4 *__restrict__ sTan,
                                                         const uint32_t *__restrict__ sIdx,
-                                                        const uint32_t *__restrict__ off, uint32_t n, KP kp) {
+                                                        const uint32_t *__restrict__ off,
+                                                        const uint32_t *__restrict__ sCell, uint32_t n, KP kp) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const SimParams &p = kp.p;
@@ -203,13 +178,12 @@ __global__ __launch_bounds__(kBlock) void k_collide_csr(float4 *__restrict__ for
     const uint32_t me = sIdx[i];
     worm_of(kp, me, c.fil, c.ch);
 
-    const int gx = cell_coord(c.r.x, p.origin.x, p.cellSize.x);
-    const int gy = cell_coord(c.r.y, p.origin.y, p.cellSize.y);
-    const int gz = cell_coord(c.r.z, p.origin.z, p.cellSize.z);
+    int gx, gy, gz;
+    bead_cell(kp, sCell, i, c.r, gx, gy, gz);
     const uint32_t Gx = p.gridSize.x;
     const AxisCells ay = axis_cells(gy, p.gridSize.y, false);
     const AxisCells az = axis_cells(gz, p.gridSize.z, kp.e.stencil == NW_STENCIL_2D);
-    const int cx = wrap_mod(gx, Gx);
+    const int cx = gx;
 
     f3 acc = mk(0.f, 0.f, 0.f);
     for (int zi = 0; zi < az.n; ++zi) {
@@ -253,179 +227,193 @@ __global__ __launch_bounds__(kBlock) void k_collide_csr(float4 *__restrict__ for
 }
 
 // ===========================================================================
-// SRD
+// pair forces with LDS tiles (stencils of one z layer, gridSize.x, .y >= 3)
+//
+// A workgroup owns 256 consecutive sorted beads whose cells span the hash
+// interval [h0, h1].  Every neighbour cell of those beads that does not wrap
+// lies in one of three hash intervals [h0 - 1 + dy Gx, h1 + 1 + dy Gx],
+// dy = -1, 0, 1, i.e. in three contiguous runs of the sorted arrays.  The
+// workgroup copies the runs into LDS with coalesced loads (positions and worm
+// ids, plus velocities / tangents when the pair terms read them), then every
+// candidate run that lies inside a staged run is read from LDS; the rest
+// (x or y wrap, a run longer than the tile) is read from global memory.  The
+// candidate order and arithmetic are those of k_collide_csr: bitwise equal.
 // ===========================================================================
-__global__ __launch_bounds__(kBlock) void k_srd_front(const float4 *__restrict__ pos, uint32_t n,
-                                                      uint32_t *__restrict__ hash, uint32_t *__restrict__ slot,
-                                                      uint32_t *__restrict__ counts, KP kp) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+constexpr int kTileCap = 384;  // staged beads per row run
+
+template <int MODE, bool TAN>
+__global__ __launch_bounds__(kBlock) void k_collide_lds(float4 *__restrict__ force, const float4 *__restrict__ sPos,
+                                                        const float4 *__restrict__ sVel,
+                                                        const float4 *__restrict__ sTan,
+                                                        const uint32_t *__restrict__ sIdx,
+                                                        const uint32_t *__restrict__ off,
+                                                        const uint32_t *__restrict__ sCell, uint32_t n, KP kp) {
+    __shared__ float4 lpos[3 * kTileCap];
+    __shared__ uint32_t lfil[3 * kTileCap];
+    __shared__ uint32_t lch[MODE == 0 ? 3 * kTileCap : 1];
+    __shared__ float4 lvel[MODE == 0 ? 3 * kTileCap : 1];
+    __shared__ float4 ltan[TAN ? 3 * kTileCap : 1];
+    __shared__ uint32_t rs[3], rl[3];
+
+    const SimParams &p = kp.p;
+    const uint32_t Gx = p.gridSize.x;
+    const uint32_t C = p.gridSize.x * p.gridSize.y * p.gridSize.z;
+    const uint32_t i0 = blockIdx.x * blockDim.x;
+    const uint32_t i = i0 + threadIdx.x;
+
+    // hash interval of the tile (sorted: first and last bead)
+    if (threadIdx.x < 3) {
+        const uint32_t last = min(i0 + blockDim.x, n) - 1u;
+        const f3 a = mk(sPos[i0]), b = mk(sPos[last]);
+        const uint32_t h0 = md_hash_wrapped(p, cell_coord(a.x, p.origin.x, p.cellSize.x),
+                                            cell_coord(a.y, p.origin.y, p.cellSize.y),
+                                            cell_coord(a.z, p.origin.z, p.cellSize.z));
+        const uint32_t h1 = md_hash_wrapped(p, cell_coord(b.x, p.origin.x, p.cellSize.x),
+                                            cell_coord(b.y, p.origin.y, p.cellSize.y),
+                                            cell_coord(b.z, p.origin.z, p.cellSize.z));
+        const int dy = int(threadIdx.x) - 1;
+        const int64_t lo = int64_t(h0) - 1 + int64_t(dy) * Gx;
+        const int64_t hi = int64_t(h1) + 1 + int64_t(dy) * Gx;  // inclusive
+        const uint32_t clo = uint32_t(lo < 0 ? 0 : (lo >= int64_t(C) ? C : lo));
+        const uint32_t chi = uint32_t(hi + 1 < 0 ? 0 : (hi + 1 > int64_t(C) ? C : hi + 1));
+        const uint32_t s = off[clo], e = off[chi];
+        rs[threadIdx.x] = s;
+        rl[threadIdx.x] = (e > s && e - s <= uint32_t(kTileCap)) ? e - s : 0u;
+    }
+    __syncthreads();
+    for (int r = 0; r < 3; ++r) {
+        const uint32_t s = rs[r], l = rl[r];
+        for (uint32_t k = threadIdx.x; k < l; k += blockDim.x) {
+            const uint32_t j = s + k;
+            lpos[r * kTileCap + k] = sPos[j];
+            uint32_t fil, ch;
+            worm_of(kp, sIdx[j], fil, ch);
+            lfil[r * kTileCap + k] = fil;
+            if (MODE == 0) {
+                lch[r * kTileCap + k] = ch;
+                lvel[r * kTileCap + k] = sVel[j];
+            }
+            if (TAN) ltan[r * kTileCap + k] = sTan[j];
+        }
+    }
+    __syncthreads();
     if (i >= n) return;
-    const uint32_t h = srd_hash(kp.p, kp.s, mk(pos[i]));
-    hash[i] = h;
-    slot[i] = atomicAdd(&counts[h], 1u);
-}
 
-// per SRD cell: members in stable order, optional isokinetic rescale
-// (isokineticThermostatSRD :604-637), centre of momentum in sorted order
-// (cellCenterOfMomentumSRD :559-594), rotation sign in com.w
-__global__ __launch_bounds__(kBlock) void k_srd_cells(const uint32_t *__restrict__ offsets,
-                                                      uint32_t *__restrict__ counts, uint32_t bins,
-                                                      uint32_t *__restrict__ ids, float4 *__restrict__ vel,
-                                                      float4 *__restrict__ com, int isokinetic, uint64_t seed,
-                                                      uint64_t offset, KP kp) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= bins) return;
-    counts[c] = 0u;
-    if (c + 1u == bins) return;  // the overflow bin: always empty here
-    const uint32_t s = offsets[c], e = offsets[c + 1];
-    sort_segment(ids, s, e);
-    f3 m = mk(0.f, 0.f, 0.f);
-    if (s != e) {
-        float lambda = 1.0f;
-        if (isokinetic) {
-            float E = 0.0f;
-            for (uint32_t j = s; j < e; ++j) {
-                const float4 V = vel[ids[j]];
-                const f3 v = mk(V);
-                E += 0.5f * V.w * dot(v, v);
-            }
-            lambda = sqrtf(kp.s.kbT / E);
-        }
-        float M = 0.0f;
-        for (uint32_t j = s; j < e; ++j) {
-            const uint32_t b = ids[j];
-            float4 V = vel[b];
-            if (isokinetic) {
-                V = f4(lambda * mk(V), V.w);
-                vel[b] = V;
-            }
-            m += V.w * mk(V);
-            M += V.w;
-        }
-        if (M > 0.0f) m = m * (1.0f / M);
-    }
-    const float sgn = uniform_at(seed, offset, c) > 0.5f ? 1.0f : -1.0f;
-    com[c] = f4(m, sgn);
-}
+    const f3 r0 = mk(sPos[i]);
+    const f3 v0 = MODE == 0 ? mk(sVel[i]) : mk(0.f, 0.f, 0.f);
+    const f3 t0 = TAN ? mk(sTan[i]) : mk(0.f, 0.f, 0.f);
+    const float rc2 = kp.e.wcaCutoff * kp.e.wcaCutoff;
+    const uint32_t me = sIdx[i];
+    uint32_t fil, ch;
+    worm_of(kp, me, fil, ch);
 
-// collideSolventKernel (:651-691) for every particle that rotates, then
-// solvent_integrator (:145-205) for the solvent
-__global__ __launch_bounds__(kBlock) void k_srd_rotate(float4 *__restrict__ pos, float4 *__restrict__ vel,
-                                                       const float4 *__restrict__ com, uint32_t N, uint32_t nt,
-                                                       float dt, float cosA, float sinA, KP kp) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nt) return;
-    const bool solvent = i >= N;
-    const bool rotate = solvent || kp.e.srdCollideFilaments;
-    if (!rotate) return;
-    const float4 P = pos[i];
-    const float4 V = vel[i];
-    f3 r = mk(P), v = mk(V);
-    const float4 C = com[srd_hash(kp.p, kp.s, r)];
-    const f3 u = mk(C);
-    const float s = C.w * sinA;
-    const f3 d = v - u;
-    const f3 rot = mk(d.x * cosA - d.y * s, d.x * s + d.y * cosA, d.z);
-    v = u + rot;
-    if (solvent) {
-        const f3 dr = dt * v;
-        r += dr;
-        apply_boundaries(kp.p, r, v, dr, 0.0f);
-        pos[i] = f4(r, P.w);
+    // one candidate, from LDS or global; terms added to fc in scan_run's order
+    auto pair = [&](f3 &fc, const f3 r2, uint32_t fil2, uint32_t ch2, const f3 v2, const f3 t2) {
+        const bool same = fil == fil2;
+        if (MODE == 0) {
+            if (same && ((ch + 1u == ch2) || (ch - 1u == ch2)))
+                fc += bond_hookean(p, r0, r2, p.bondSpringK, p.bondSpringL);
+            else
+                fc += dem_force(p, r0, r2, v0, v2);
+        } else if (!same) {
+            fc += wca_force(p, kp.e, r0, r2, rc2);
+        }
+        if (TAN && !same) fc += extensile_force(p, r0, r2, t0, t2, p.activity);
+    };
+    auto run = [&](uint32_t a, uint32_t b) {
+        f3 fc = mk(0.f, 0.f, 0.f);
+        if (a >= b) return fc;
+        int src = -1;
+        for (int r = 0; r < 3; ++r)
+            if (rl[r] && a >= rs[r] && b <= rs[r] + rl[r]) src = r;
+        if (src >= 0) {
+            const uint32_t base = uint32_t(src) * kTileCap - rs[src];
+            for (uint32_t j = a; j < b; ++j) {
+                if (j == i) continue;
+                const uint32_t q = base + j;
+                pair(fc, mk(lpos[q]), lfil[q], MODE == 0 ? lch[q] : 0u,
+                     MODE == 0 ? mk(lvel[q]) : mk(0.f, 0.f, 0.f), TAN ? mk(ltan[q]) : mk(0.f, 0.f, 0.f));
+            }
+        } else {
+            for (uint32_t j = a; j < b; ++j) {
+                if (j == i) continue;
+                uint32_t fil2, ch2;
+                worm_of(kp, sIdx[j], fil2, ch2);
+                pair(fc, mk(sPos[j]), fil2, ch2, MODE == 0 ? mk(sVel[j]) : mk(0.f, 0.f, 0.f),
+                     TAN ? mk(sTan[j]) : mk(0.f, 0.f, 0.f));
+            }
+        }
+        return fc;
+    };
+
+    int gx, gy, gz;
+    bead_cell(kp, sCell, i, r0, gx, gy, gz);
+    const AxisCells ay = axis_cells(gy, p.gridSize.y, false);
+    const AxisCells az = axis_cells(gz, p.gridSize.z, kp.e.stencil == NW_STENCIL_2D);
+    const int cx = gx;
+    f3 acc = mk(0.f, 0.f, 0.f);
+    for (int yi = 0; yi < ay.n; ++yi) {
+        const float wzy = az.w[0] * ay.w[yi];
+        const uint32_t base = (uint32_t(az.c[0]) * p.gridSize.y + uint32_t(ay.c[yi])) * Gx;
+        uint32_t a0, a1, b0 = 0, b1 = 0;
+        if (cx >= 1 && cx + 1 < int(Gx)) {
+            a0 = off[base + cx - 1];
+            a1 = off[base + cx + 2];
+        } else if (cx == 0) {
+            a0 = off[base + Gx - 1];
+            a1 = off[base + Gx];
+            b0 = off[base];
+            b1 = off[base + 2];
+        } else {
+            a0 = off[base + Gx - 2];
+            a1 = off[base + Gx];
+            b0 = off[base];
+            b1 = off[base + 1];
+        }
+        f3 fr = run(a0, a1);
+        fr += run(b0, b1);
+        if (wzy == 1.0f) acc += fr;
+        else acc += wzy * fr;
     }
-    vel[i] = f4(v, V.w);
+    float4 Fo = force[me];
+    force[me] = make_float4(Fo.x + acc.x, Fo.y + acc.y, Fo.z + acc.z, Fo.w + 0.0f);
 }
 
 // ===========================================================================
 // launchers
 // ===========================================================================
-hipError_t ensure_fused_scratch(FusedScratch &f, uint64_t n, uint64_t bins, hipStream_t st) {
-    hipError_t e;
-    if (n > f.cap_n) {
-        const uint64_t cap = n + n / 8 + 1024;
-        if (f.slot) { (void)hipFree(f.slot); (void)hipFree(f.hashTmp); }
-        f.slot = f.hashTmp = nullptr;
-        if ((e = hipMalloc(&f.slot, cap * 4)) != hipSuccess) return e;
-        if ((e = hipMalloc(&f.hashTmp, cap * 4)) != hipSuccess) return e;
-        f.cap_n = cap;
-    }
-    if (bins > f.cap_bins) {
-        const uint64_t cap = bins + bins / 8 + 1024;
-        if (f.counts) { (void)hipFree(f.counts); (void)hipFree(f.offsets); (void)hipFree(f.scan); }
-        f.counts = f.offsets = f.scan = nullptr;
-        if ((e = hipMalloc(&f.counts, cap * 4)) != hipSuccess) return e;
-        if ((e = hipMalloc(&f.offsets, (cap + 1) * 4)) != hipSuccess) return e;
-        if ((e = hipMalloc(&f.scan, scan_scratch_words(cap) * 4)) != hipSuccess) return e;
-        f.cap_bins = cap;
-        f.dirty = true;
-    }
-    if (f.dirty) {
-        // the bins are kept at zero by the consumers; clear once after (re)allocation
-        if ((e = launch_fill_u32(f.counts, 0u, f.cap_bins, st)) != hipSuccess) return e;
-        f.dirty = false;
-    }
-    return hipSuccess;
-}
-
-hipError_t launch_md_front(const MdFrontArgs &a, const KP &kp, FusedScratch &f, hipStream_t st) {
+hipError_t launch_md_front(const MdFrontArgs &a, const KP &kp, hipStream_t st) {
     const uint32_t n = kp.p.filamentSize;
     const uint32_t wpw = 64u / n;
     const uint64_t threads = ((uint64_t(a.W) + wpw - 1) / wpw) * 64u;
-    hipLaunchKernelGGL(k_md_front, dim3(grid_for(threads)), dim3(kBlock), 0, st, a.pos, a.vel, a.force, a.forceOld,
-                       a.tangent, a.hash, f.slot, f.counts, a.dt, a.lgvStd, a.langevin, a.W, a.seed, a.offRev,
-                       a.offLgv, kp);
+    const size_t lds = (size_t(a.bp.buckets) + 1 + kFrontBlock / 64) * sizeof(uint32_t);
+    hipLaunchKernelGGL(k_md_front, dim3(grid_for(threads, kFrontBlock)), dim3(kFrontBlock), lds, st, a, kp);
     return hipGetLastError();
 }
 
-hipError_t launch_scan_bins(FusedScratch &f, uint64_t bins, hipStream_t st) {
-    return exclusive_scan_u32(f.counts, f.offsets, bins, f.scan, st);
-}
-
-hipError_t launch_scatter_ix(const uint32_t *hash, FusedScratch &f, uint32_t n, uint32_t *out, hipStream_t st) {
+hipError_t launch_collide_off(float4 *force, const float4 *sPos, const float4 *sVel, const float4 *sTan,
+                              const uint32_t *sIdx, const uint32_t *off, const uint32_t *sCell, uint32_t n,
+                              const KP &kp, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scatter_ix, dim3(grid_for(n)), dim3(kBlock), 0, st, hash, f.slot, f.offsets, n, out);
-    return hipGetLastError();
-}
-
-hipError_t launch_md_cells(FusedScratch &f, uint32_t bins, uint32_t *ids, const float4 *pos, const float4 *vel,
-                           const float4 *tan, float4 *sPos, float4 *sVel, float4 *sTan, hipStream_t st) {
-    hipLaunchKernelGGL(k_md_cells, dim3(grid_for(bins)), dim3(kBlock), 0, st, f.offsets, f.counts, bins, ids, pos,
-                       vel, tan, sPos, sVel, sTan);
-    return hipGetLastError();
-}
-
-hipError_t launch_collide_csr(float4 *force, const float4 *sPos, const float4 *sVel, const float4 *sTan,
-                              const uint32_t *sIdx, const FusedScratch &f, uint32_t n, const KP &kp,
-                              hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    if (kp.e.pairPotential == NW_PAIR_WCA)
-        hipLaunchKernelGGL(k_collide_csr<1>, dim3(grid_for(n)), dim3(kBlock), 0, st, force, sPos, sVel, sTan, sIdx,
-                           f.offsets, n, kp);
-    else
-        hipLaunchKernelGGL(k_collide_csr<0>, dim3(grid_for(n)), dim3(kBlock), 0, st, force, sPos, sVel, sTan, sIdx,
-                           f.offsets, n, kp);
-    return hipGetLastError();
-}
-
-hipError_t launch_srd_front(const float4 *pos, uint32_t nt, uint32_t *hash, FusedScratch &f, const KP &kp,
-                            hipStream_t st) {
-    if (nt == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_srd_front, dim3(grid_for(nt)), dim3(kBlock), 0, st, pos, nt, hash, f.slot, f.counts, kp);
-    return hipGetLastError();
-}
-
-hipError_t launch_srd_cells(FusedScratch &f, uint32_t bins, uint32_t *ids, float4 *vel, float4 *com, int iso,
-                            uint64_t seed, uint64_t offset, const KP &kp, hipStream_t st) {
-    hipLaunchKernelGGL(k_srd_cells, dim3(grid_for(bins)), dim3(kBlock), 0, st, f.offsets, f.counts, bins, ids, vel,
-                       com, iso, seed, offset, kp);
-    return hipGetLastError();
-}
-
-hipError_t launch_srd_rotate(float4 *pos, float4 *vel, const float4 *com, uint32_t N, uint32_t nt, float dt,
-                             float cosA, float sinA, const KP &kp, hipStream_t st) {
-    if (nt == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_srd_rotate, dim3(grid_for(nt)), dim3(kBlock), 0, st, pos, vel, com, N, nt, dt, cosA, sinA,
-                       kp);
+    const bool oneLayer = kp.e.stencil == NW_STENCIL_2D || kp.p.gridSize.z == 1;
+    const bool ldsOk = oneLayer && kp.p.gridSize.x >= 3;
+    const bool lds = ldsOk && kp.e.collideKernel != NW_COLLIDE_GLOBAL;
+    const bool wca = kp.e.pairPotential == NW_PAIR_WCA;
+    const bool tan = !wca || kp.p.activity != 0.0f;
+    const dim3 g(grid_for(n)), b(kBlock);
+    if (lds) {
+        if (wca && !tan)
+            hipLaunchKernelGGL((k_collide_lds<1, false>), g, b, 0, st, force, sPos, sVel, sTan, sIdx, off, sCell, n,
+                               kp);
+        else if (wca)
+            hipLaunchKernelGGL((k_collide_lds<1, true>), g, b, 0, st, force, sPos, sVel, sTan, sIdx, off, sCell, n, kp);
+        else
+            hipLaunchKernelGGL((k_collide_lds<0, true>), g, b, 0, st, force, sPos, sVel, sTan, sIdx, off, sCell, n, kp);
+    } else if (wca) {
+        hipLaunchKernelGGL(k_collide_csr<1>, g, b, 0, st, force, sPos, sVel, sTan, sIdx, off, sCell, n, kp);
+    } else {
+        hipLaunchKernelGGL(k_collide_csr<0>, g, b, 0, st, force, sPos, sVel, sTan, sIdx, off, sCell, n, kp);
+    }
     return hipGetLastError();
 }
 

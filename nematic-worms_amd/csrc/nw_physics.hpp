@@ -129,10 +129,10 @@ __device__ __forceinline__ f3 extensile_force(const SimParams &p, f3 rA, f3 rB, 
 // WCA-LJ force on A from B: F_A = -24 eps (2 s^12/r^14 - s^6/r^8) (rB - rA), r < rc
 __device__ __forceinline__ f3 wca_force(const SimParams &p, const NWExtParams &e, f3 rA, f3 rB, float rc2) {
     const f3 rel = min_image(p, rA, rB);
-    const float d = length(rel);
-    const float rr = d * d;
+    const float rr = dot(rel, rel);
     if (rr < rc2) {
-        const float inv2 = 1.0f / rr;
+        // hardware reciprocal (1 ulp): the force is a product of powers of 1/r^2
+        const float inv2 = __builtin_amdgcn_rcpf(rr);
         const float s2 = e.wcaSigma * e.wcaSigma * inv2;
         const float s6 = s2 * s2 * s2;
         const float fmag = 24.0f * e.wcaEpsilon * inv2 * s6 * (2.0f * s6 - 1.0f);

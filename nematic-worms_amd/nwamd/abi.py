@@ -65,8 +65,11 @@ class NWExtParams(ctypes.Structure):
         ("pairPotential", c_uint), ("bondedMode", c_uint), ("stencil", c_uint),
         ("srdCollideFilaments", c_uint), ("srdThermostat", c_uint),
         ("wcaEpsilon", c_float), ("wcaSigma", c_float), ("wcaCutoff", c_float), ("driveForce", c_float),
-        ("reserved", c_uint * 7),
+        ("collideKernel", c_uint), ("reserved", c_uint * 6),
     ]
+
+
+NW_COLLIDE_AUTO, NW_COLLIDE_GLOBAL, NW_COLLIDE_LDS = 0, 1, 2
 
 
 assert ctypes.sizeof(SimParams) == 144
@@ -117,6 +120,7 @@ PROTOTYPES = {
     "nwRadixSortPairs": (None, [P, P, c_uint, c_uint]),
     "nwFusedMdStep": (None, [P, P, P, P, P, P, P, P, P, P, c_float, c_float, c_float, c_int, c_uint]),
     "nwFusedSrdStep": (None, [P, P, P, P, P, c_float, c_uint, c_uint, c_int]),
+    "nwFusedPlanFits": (c_int, [c_uint, c_uint, c_uint]),
     "nwGetDeviceFlags": (c_uint, [c_int]),
     "nwTimerEnable": (None, [ctypes.c_char_p, c_int]),
     "nwTimerRead": (None, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_u64)]),

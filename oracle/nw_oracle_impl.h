@@ -158,8 +158,7 @@ static inline V3 F(extensile)(const SimParams *p, V3 rA, V3 rB, V3 tA, V3 tB, R 
 /* WCA-LJ: F_A = -24 eps (2 s^12 / r^14 - s^6 / r^8) (rB - rA) for r^2 < rc^2 */
 static inline V3 F(wca)(const SimParams *p, const NWExtParams *e, V3 rA, V3 rB) {
     const V3 rel = F(min_image)(p, rA, rB);
-    const R d = F(len)(rel);
-    const R rr = d * d;
+    const R rr = F(dot)(rel, rel);
     const R rc = (R)e->wcaCutoff;
     if (rr < rc * rc) {
         const R inv2 = (R)1 / rr;

@@ -101,3 +101,16 @@ def test_no_device_errors_are_reported_not_fatal():
     L.allocateArray(ctypes.byref(ptr), 64)
     assert L.nwGetLastError() != 0
     assert b"device" in L.nwGetLastErrorString()
+
+
+def test_fused_plan_limits():
+    """nwFusedPlanFits does no device work: the benchmark's systems fit, a
+    grid beyond the level-1 histogram and worms over 64 beads do not."""
+    L = abi.load()
+    assert L.nwFusedPlanFits(800000, 890 * 890, 20) == 1
+    assert L.nwFusedPlanFits(5800000, 1000 * 1000, 0) == 1
+    assert L.nwFusedPlanFits(0, 16, 10) == 1
+    assert L.nwFusedPlanFits(1000, 16, 65) == 0
+    assert L.nwFusedPlanFits(1000, 0, 10) == 0
+    assert L.nwFusedPlanFits(1000, 16384 * 2048 + 1, 10) == 0
+    assert L.nwFusedPlanFits(2 ** 30, 2 ** 24, 0) == 0

@@ -81,7 +81,35 @@ def test_fused_md_step_matches_per_call(nwlib, mode, grid, n, rev, lgv):
     pos, vel, tan = random_worms(p, rng, twod=twod)
     f, fo = random_forces(rng, W * n, 0.5), random_forces(rng, W * n, 0.5)
     f[:, 3] = 0
-    lib = nwlib
+    compare_md_step(nwlib, p, e, pos, vel, tan, f, fo, lgv, mode)
+
+
+@pytest.mark.parametrize("mode", ["ref2d", "wca"])
+def test_fused_md_step_crowded_bucket(nwlib, mode):
+    """More beads in one bucket of 1024 cells than the sort stages in LDS
+    (2048): that bucket takes the global-memory path."""
+    rng = np.random.default_rng([5, len(mode)])
+    grid, n = (64, 64, 1), 10
+    cell = 1.0 if mode == "ref2d" else 1.1236
+    W = 300 if mode == "ref2d" else 220
+    p = sim_params(grid=grid, cell=cell, W=W, n=n, bondSpringL=0.4 if mode == "ref2d" else 0.8,
+                   bondBendingK=3.0, activity=0.0)
+    if mode == "wca":
+        e = ext_params(pairPotential=abi.NW_PAIR_WCA, bondedMode=abi.NW_BONDED_CHAIN, driveForce=1.0,
+                       stencil=abi.NW_STENCIL_2D)
+    else:
+        e = ext_params(stencil=abi.NW_STENCIL_REF27)
+    pos, vel, tan = random_worms(p, rng, twod=True)
+    # squeeze every worm into cell rows 1..14 (the first bucket holds rows 0..15)
+    y0 = p.origin.y
+    pos[:, 1] = (y0 + cell * (1.0 + (pos[:, 1] - y0) * (13.5 / (64 * cell)))).astype(np.float32)
+    f, fo = random_forces(rng, W * n, 0.5), random_forces(rng, W * n, 0.5)
+    f[:, 3] = 0
+    compare_md_step(nwlib, p, e, pos, vel, tan, f, fo, False, mode)
+
+
+def compare_md_step(lib, p, e, pos, vel, tan, f, fo, lgv, mode):
+    W, n = p.numFilaments, p.filamentSize
     lib.setParameters(ctypes.byref(p))
     lib.nwSetExtParameters(ctypes.byref(e))
     dt, gamma, kT = 0.002, 0.8, 0.4
@@ -102,20 +130,26 @@ def test_fused_md_step_matches_per_call(nwlib, mode, grid, n, rev, lgv):
         assert np.array_equal(bits(down(a, k, N)), bits(down(b, k, N))), k
     assert np.array_equal(down(a, "i", N, np.uint32, 1), down(b, "i", N, np.uint32, 1))
     assert np.array_equal(bits(down(a, "sp", N)), bits(down(b, "sp", N)))
+    # the fused step leaves the cell of every bead in bead order
+    ha, hb, ib = down(a, "h", N, np.uint32, 1), down(b, "h", N, np.uint32, 1), down(b, "i", N, np.uint32, 1)
+    assert np.array_equal(hb[ib], ha)
     # pair forces: the oracle on the per-call path's sorted state
     fa, fb = down(a, "f", N), down(b, "f", N)
     force_tol_check(fb, fa.astype(np.float64), rtol=1e-5, name=f"fused vs per-call ({mode})")
 
 
-@pytest.mark.parametrize("beads,iso", [(1, 0), (0, 1), (1, 1)])
-def test_fused_srd_step_matches_per_call(nwlib, beads, iso):
-    rng = np.random.default_rng(7 + beads + 2 * iso)
+@pytest.mark.parametrize("beads,iso,crowd", [(1, 0, 0), (0, 1, 0), (1, 1, 0), (1, 1, 1)])
+def test_fused_srd_step_matches_per_call(nwlib, beads, iso, crowd):
+    """crowd: the solvent fills an 8 x 8 patch, so one bucket of cells holds
+    far more members than the sort stages in LDS (the global-memory path)."""
+    rng = np.random.default_rng(7 + beads + 2 * iso + 4 * crowd)
     lib = nwlib
     p = sim_params(grid=(64, 64, 1), cell=1.0, W=100, n=10)
     pos, vel, _ = random_worms(p, rng, twod=True)
     Ns, N = 30000, p.numParticles
     sp = np.zeros((Ns, 4), np.float32)
-    sp[:, :2] = (rng.random((Ns, 2)) * 64 - 32).astype(np.float32)
+    span = 8.0 if crowd else 64.0
+    sp[:, :2] = (rng.random((Ns, 2)) * span - 32).astype(np.float32)
     sp[:, 2] = p.origin.z + 0.5
     sv = np.zeros((Ns, 4), np.float32)
     sv[:, :2] = rng.normal(0, 1, (Ns, 2))
@@ -148,8 +182,9 @@ def test_fused_srd_step_matches_per_call(nwlib, beads, iso):
             lib.collideSolvent(dP.p, dV.p, com.p, None, Cs, NT)
             lib.integrateSolvent(dP.p, dV.p, dt, N, Ns)
         res.append((dP.download(np.float32, (NT, 4)), dV.download(np.float32, (NT, 4)),
-                    com.download(np.float32, (Cs, 4)), i.download(np.uint32, (NT,))))
-    (pa, va, ca, ia), (pb, vb, cb, ib) = res
+                    com.download(np.float32, (Cs, 4)), i.download(np.uint32, (NT,)), h.download(np.uint32, (NT,))))
+    (pa, va, ca, ia, ha), (pb, vb, cb, ib, hb) = res
+    assert np.array_equal(hb[ib], ha)  # fused: cells in particle order; per call: sorted
     assert np.array_equal(bits(pa), bits(pb))
     assert np.array_equal(bits(va), bits(vb))
     assert np.array_equal(bits(ca[:, :3]), bits(cb[:, :3]))
@@ -178,4 +213,33 @@ def test_fused_step_repeats_bitwise(nwlib):
                               b["sp"].p, None, None, 0.001, 2.0, 100.0, 1, p.numFilaments)
         out.append([down(b, k, p.numParticles) for k in ("pos", "vel", "f")])
     for x, y in zip(*out):
+        assert np.array_equal(bits(x), bits(y))
+
+
+@pytest.mark.parametrize("mode,activity", [("wca", 0.0), ("wca", 0.3), ("ref", 0.2)])
+def test_lds_and_global_pair_kernels_agree_bitwise(nwlib, mode, activity):
+    rng = np.random.default_rng([7, int(activity * 10), len(mode)])
+    lib = nwlib
+    grid = (120, 97, 1)
+    cell = 1.1236 if mode == "wca" else 1.0
+    p = sim_params(grid=grid, cell=cell, W=1200, n=10, bondSpringL=0.8 if mode == "wca" else 0.4,
+                   activity=activity)
+    pos, vel, tan = random_worms(p, rng, twod=True)
+    f = np.zeros_like(pos)
+    res = []
+    for kern in (abi.NW_COLLIDE_GLOBAL, abi.NW_COLLIDE_LDS):
+        if mode == "wca":
+            e = ext_params(pairPotential=abi.NW_PAIR_WCA, bondedMode=abi.NW_BONDED_CHAIN, driveForce=1.0,
+                           stencil=abi.NW_STENCIL_2D, collideKernel=kern)
+        else:
+            e = ext_params(stencil=abi.NW_STENCIL_REF27, collideKernel=kern)
+        lib.setParameters(ctypes.byref(p))
+        lib.nwSetExtParameters(ctypes.byref(e))
+        b = state(lib, p, pos, vel, tan, f, f)
+        lib.nwRngSetSeed(1, 0)
+        for _ in range(3):
+            lib.nwFusedMdStep(b["pos"].p, b["vel"].p, b["f"].p, b["fo"].p, b["tan"].p, b["h"].p, b["i"].p,
+                              b["sp"].p, b["sv"].p, b["st"].p, 0.002, 1.0, 0.5, 1, p.numFilaments)
+        res.append([down(b, k, p.numParticles) for k in ("pos", "vel", "f")])
+    for x, y in zip(*res):
         assert np.array_equal(bits(x), bits(y))
