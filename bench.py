@@ -135,6 +135,9 @@ def main():
     ap.add_argument("--workload", default="worms_srd_800k")
     ap.add_argument("--dt", type=float, default=1e-3)
     ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="at the end, one elementwise kernel reading and writing this many MiB "
+                         "(a known-size reference for HBM byte counters)")
     ap.add_argument("--profile-steps", type=int, default=20,
                     help="untimed steps after the warm-up that rank the kernel families")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -222,6 +225,15 @@ def main():
         "roofline": roof, "cpu_baseline": cpu, "per_call_sequence": per_call, "kernels": kernels,
     }
     print(json.dumps(out), flush=True)
+    if args.copy_probe_mb > 0:
+        import torch
+        a = torch.ones(args.copy_probe_mb * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        torch.neg(a, out=b)
+        torch.cuda.synchronize()
+        print(f"copy probe: neg over {a.numel()} floats, {a.numel() * 4} B read and {a.numel() * 4} B written",
+              flush=True)
 
 
 if __name__ == "__main__":

@@ -77,6 +77,17 @@ __device__ __forceinline__ f3 min_image(const SimParams &p, f3 a, f3 b) {
               mi_axis(b.z, a.z, p.boxSize.z, p.boundaryZ == PERIODIC));
 }
 
+// The same value for the pair loops: pairs closer than half the box on every
+// axis (all but a sliver near the periodic faces) take the plain difference,
+// which is what mi_axis returns for them; the wrap tests run only when some
+// lane holds a longer component.
+__device__ __forceinline__ f3 min_image_pair(const SimParams &p, f3 a, f3 b) {
+    const f3 d = mk(b.x - a.x, b.y - a.y, b.z - a.z);
+    if (!(fabsf(d.x) > 0.5f * p.boxSize.x || fabsf(d.y) > 0.5f * p.boxSize.y || fabsf(d.z) > 0.5f * p.boxSize.z))
+        return d;
+    return min_image(p, a, b);
+}
+
 // unit(b - a) under the minimum image (particles_kernel_impl.cuh:303-309)
 __device__ __forceinline__ f3 tangent_of(const SimParams &p, f3 a, f3 b) {
     const f3 rel = min_image(p, a, b);

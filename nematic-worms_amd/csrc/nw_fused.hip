@@ -5,11 +5,13 @@
 // gathers re-read what the sort just touched.  The fused step is
 //   k_md_front      integrate + reversal + Langevin + bonded + tangent + cell
 //                   hash (whole worms per wavefront, chain neighbours by
-//                   cross-lane shuffles)
+//                   cross-lane shuffles) + the tile pass of the cell sort
 //   cell sort       nw_bucket.hip (stable, atomic-free, gathers the records)
-//   k_collide_lds   pair forces, neighbour rows staged in LDS (k_collide_csr
-//                   without LDS for stencils over several z layers)
-// computing the per-call path's expressions in the same order.
+//   k_collide_rows  pair forces, neighbour rows staged in LDS, one flattened
+//                   candidate list per bead (k_collide_csr without LDS for
+//                   stencils over several z layers)
+// computing the per-call path's expressions in the same order, except that
+// pair terms are accumulated one by one rather than per cell.
 #include "nw_bucket.hpp"
 #include "nw_physics.hpp"
 
@@ -123,42 +125,133 @@ __global__ __launch_bounds__(kFrontBlock) void k_md_front(MdFrontArgs a, KP kp) 
 
 // ===========================================================================
 // pair forces over CSR cell offsets
+//
+// Candidates are visited in stencil order (z rows, y rows, x cells, each
+// cell's beads in sorted order) and every term goes straight into the bead's
+// accumulator, multiplied by the stencil weight of its row (a cell counted
+// k times by the reference's folded stencil contributes k x term; weight 1
+// leaves the term unchanged).  The global and the LDS kernel below follow this
+// order exactly and agree bit for bit.
 // ===========================================================================
-template <int MODE>
-struct PairCtx {
+template <int MODE, bool TAN>
+struct Owner {
     f3 r, v, t;
-    uint32_t i, fil, ch;
-    bool useTan;
-    float rc2;
+    uint32_t fil, ch;
 };
 
-template <int MODE>
-__device__ __forceinline__ f3 scan_run(const KP &kp, const PairCtx<MODE> &c, uint32_t lo, uint32_t hi,
-                                       const float4 *__restrict__ sPos, const float4 *__restrict__ sVel,
-                                       const float4 *__restrict__ sTan, const uint32_t *__restrict__ sIdx) {
+// one candidate's terms (collideParticles / bond / WCA, then the extensile
+// term), skipped for the bead itself and, in WCA mode, for its own worm
+template <int MODE, bool TAN, bool UNIT = false>
+__device__ __forceinline__ void add_pair(const KP &kp, const Owner<MODE, TAN> &o, float rc2, float w, f3 &acc,
+                                         f3 r2, uint32_t fil2, uint32_t ch2, f3 v2, f3 t2) {
     const SimParams &p = kp.p;
-    f3 fc = mk(0.f, 0.f, 0.f);
-    for (uint32_t j = lo; j < hi; ++j) {
-        if (j == c.i) continue;
-        const f3 r2 = mk(sPos[j]);
-        uint32_t fil2, ch2;
-        worm_of(kp, sIdx[j], fil2, ch2);
-        const bool same = c.fil == fil2;
-        if (MODE == 0) {
-            const f3 v2 = mk(sVel[j]);
-            if (same && ((c.ch + 1u == ch2) || (c.ch - 1u == ch2)))
-                fc += bond_hookean(p, c.r, r2, p.bondSpringK, p.bondSpringL);
-            else
-                fc += dem_force(p, c.r, r2, c.v, v2);
-        } else if (!same) {
-            fc += wca_force(p, kp.e, c.r, r2, c.rc2);
-        }
-        if (c.useTan && !same) fc += extensile_force(p, c.r, r2, c.t, mk(sTan[j]), p.activity);
+    const bool same = o.fil == fil2;
+    if (MODE == 0) {
+        if (same && ch2 == o.ch) return;
+        f3 t;
+        if (same && ((o.ch + 1u == ch2) || (o.ch - 1u == ch2)))
+            t = bond_hookean(p, o.r, r2, p.bondSpringK, p.bondSpringL);
+        else
+            t = dem_force(p, o.r, r2, o.v, v2);
+        acc += UNIT ? t : w * t;
+    } else {
+        if (same) return;
+        const f3 t = wca_force(p, kp.e, o.r, r2, rc2);
+        acc += UNIT ? t : w * t;
     }
-    return fc;
+    if (TAN && !same) {
+        const f3 t = extensile_force(p, o.r, r2, o.t, t2, p.activity);
+        acc += UNIT ? t : w * t;
+    }
 }
 
-template <int MODE>
+template <int MODE, bool TAN>
+__device__ __forceinline__ void add_run_global(const KP &kp, const Owner<MODE, TAN> &o, float rc2, float w, f3 &acc,
+                                               uint32_t a, uint32_t b, const float4 *__restrict__ sPos,
+                                               const float4 *__restrict__ sVel, const float4 *__restrict__ sTan,
+                                               const uint32_t *__restrict__ sIdx) {
+    for (uint32_t j = a; j < b; ++j) {
+        uint32_t fil2, ch2;
+        worm_of(kp, sIdx[j], fil2, ch2);
+        add_pair<MODE, TAN>(kp, o, rc2, w, acc, mk(sPos[j]), fil2, ch2, MODE == 0 ? mk(sVel[j]) : mk(0.f, 0.f, 0.f),
+                            TAN ? mk(sTan[j]) : mk(0.f, 0.f, 0.f));
+    }
+}
+
+// sorted-slot runs of the three x cells around cx in the row starting at
+// cell `base`: one run, or two across the x wrap (Gx >= 3)
+__device__ __forceinline__ void row_runs(const uint32_t *__restrict__ off, uint32_t base, int cx, uint32_t Gx,
+                                         uint32_t &a0, uint32_t &a1, uint32_t &b0, uint32_t &b1) {
+    const bool mid = cx >= 1 && cx + 1 < int(Gx);
+    const bool left = cx == 0;
+    const uint32_t ia0 = mid ? base + uint32_t(cx) - 1u : (left ? base + Gx - 1u : base + Gx - 2u);
+    const uint32_t ia1 = mid ? base + uint32_t(cx) + 2u : base + Gx;
+    a0 = off[ia0];
+    a1 = off[ia1];
+    b0 = 0u;
+    b1 = 0u;
+    if (!mid) {
+        b0 = off[base];
+        b1 = off[left ? base + 2u : base + 1u];
+    }
+}
+
+// every candidate of the owner from global memory, in stencil order
+template <int MODE, bool TAN>
+__device__ __forceinline__ void add_all_global(const KP &kp, const Owner<MODE, TAN> &o, float rc2, f3 &acc, int gx,
+                                               int gy, int gz, const uint32_t *__restrict__ off,
+                                               const float4 *__restrict__ sPos, const float4 *__restrict__ sVel,
+                                               const float4 *__restrict__ sTan, const uint32_t *__restrict__ sIdx) {
+    const SimParams &p = kp.p;
+    const uint32_t Gx = p.gridSize.x;
+    const AxisCells ay = axis_cells(gy, p.gridSize.y, false);
+    const AxisCells az = axis_cells(gz, p.gridSize.z, kp.e.stencil == NW_STENCIL_2D);
+#pragma unroll
+    for (int zi = 0; zi < 3; ++zi) {
+        if (zi >= az.n) break;
+#pragma unroll
+        for (int yi = 0; yi < 3; ++yi) {
+            if (yi >= ay.n) break;
+            const float wzy = az.w[zi] * ay.w[yi];
+            const uint32_t base = (uint32_t(az.c[zi]) * p.gridSize.y + uint32_t(ay.c[yi])) * Gx;
+            if (Gx >= 3) {
+                uint32_t a0, a1, b0, b1;
+                row_runs(off, base, gx, Gx, a0, a1, b0, b1);
+                add_run_global<MODE, TAN>(kp, o, rc2, wzy, acc, a0, a1, sPos, sVel, sTan, sIdx);
+                add_run_global<MODE, TAN>(kp, o, rc2, wzy, acc, b0, b1, sPos, sVel, sTan, sIdx);
+            } else {
+                const AxisCells ax = axis_cells(gx, Gx, false);
+#pragma unroll
+                for (int xi = 0; xi < 3; ++xi) {
+                    if (xi >= ax.n) break;
+                    const uint32_t h = base + uint32_t(ax.c[xi]);
+                    add_run_global<MODE, TAN>(kp, o, rc2, wzy * ax.w[xi], acc, off[h], off[h + 1], sPos, sVel,
+                                              sTan, sIdx);
+                }
+            }
+        }
+    }
+}
+
+template <int MODE, bool TAN>
+__device__ __forceinline__ Owner<MODE, TAN> load_owner(const KP &kp, uint32_t i, uint32_t me,
+                                                       const float4 *__restrict__ sPos,
+                                                       const float4 *__restrict__ sVel,
+                                                       const float4 *__restrict__ sTan) {
+    Owner<MODE, TAN> o;
+    o.r = mk(sPos[i]);
+    o.v = MODE == 0 ? mk(sVel[i]) : mk(0.f, 0.f, 0.f);
+    o.t = TAN ? mk(sTan[i]) : mk(0.f, 0.f, 0.f);
+    worm_of(kp, me, o.fil, o.ch);
+    return o;
+}
+
+__device__ __forceinline__ void add_force(float4 *__restrict__ force, uint32_t me, f3 acc) {
+    const float4 Fo = force[me];
+    force[me] = make_float4(Fo.x + acc.x, Fo.y + acc.y, Fo.z + acc.z, Fo.w + 0.0f);
+}
+
+template <int MODE, bool TAN>
 __global__ __launch_bounds__(kBlock) void k_collide_csr(float4 *__restrict__ force, const float4 *__restrict__ sPos,
                                                         const float4 *__restrict__ sVel,
                                                         const float4 *__restrict__ sTan,
@@ -167,216 +260,141 @@ __global__ __launch_bounds__(kBlock) void k_collide_csr(float4 *__restrict__ for
                                                         const uint32_t *__restrict__ sCell, uint32_t n, KP kp) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const SimParams &p = kp.p;
-    PairCtx<MODE> c;
-    c.i = i;
-    c.r = mk(sPos[i]);
-    c.useTan = MODE == 0 || p.activity != 0.0f;
-    c.v = MODE == 0 ? mk(sVel[i]) : mk(0.f, 0.f, 0.f);
-    c.t = c.useTan ? mk(sTan[i]) : mk(0.f, 0.f, 0.f);
-    c.rc2 = kp.e.wcaCutoff * kp.e.wcaCutoff;
     const uint32_t me = sIdx[i];
-    worm_of(kp, me, c.fil, c.ch);
-
+    const Owner<MODE, TAN> o = load_owner<MODE, TAN>(kp, i, me, sPos, sVel, sTan);
     int gx, gy, gz;
-    bead_cell(kp, sCell, i, c.r, gx, gy, gz);
-    const uint32_t Gx = p.gridSize.x;
-    const AxisCells ay = axis_cells(gy, p.gridSize.y, false);
-    const AxisCells az = axis_cells(gz, p.gridSize.z, kp.e.stencil == NW_STENCIL_2D);
-    const int cx = gx;
-
+    bead_cell(kp, sCell, i, o.r, gx, gy, gz);
     f3 acc = mk(0.f, 0.f, 0.f);
-    for (int zi = 0; zi < az.n; ++zi) {
-        for (int yi = 0; yi < ay.n; ++yi) {
-            const float wzy = az.w[zi] * ay.w[yi];
-            const uint32_t base = (uint32_t(az.c[zi]) * p.gridSize.y + uint32_t(ay.c[yi])) * Gx;
-            f3 fr = mk(0.f, 0.f, 0.f);
-            if (Gx >= 3) {
-                // cells cx-1, cx, cx+1 in this order: one run, or two across the x wrap
-                uint32_t a0, a1, b0 = 0, b1 = 0;
-                if (cx >= 1 && cx + 1 < int(Gx)) {
-                    a0 = off[base + cx - 1];
-                    a1 = off[base + cx + 2];
-                } else if (cx == 0) {
-                    a0 = off[base + Gx - 1];
-                    a1 = off[base + Gx];
-                    b0 = off[base];
-                    b1 = off[base + 2];
-                } else {
-                    a0 = off[base + Gx - 2];
-                    a1 = off[base + Gx];
-                    b0 = off[base];
-                    b1 = off[base + 1];
-                }
-                fr = scan_run<MODE>(kp, c, a0, a1, sPos, sVel, sTan, sIdx);
-                fr += scan_run<MODE>(kp, c, b0, b1, sPos, sVel, sTan, sIdx);
-            } else {
-                const AxisCells ax = axis_cells(gx, Gx, false);
-                for (int xi = 0; xi < ax.n; ++xi) {
-                    const uint32_t h = base + uint32_t(ax.c[xi]);
-                    const f3 fc = scan_run<MODE>(kp, c, off[h], off[h + 1], sPos, sVel, sTan, sIdx);
-                    fr += ax.w[xi] * fc;
-                }
-            }
-            if (wzy == 1.0f) acc += fr;
-            else acc += wzy * fr;
-        }
-    }
-    float4 Fo = force[me];
-    force[me] = make_float4(Fo.x + acc.x, Fo.y + acc.y, Fo.z + acc.z, Fo.w + 0.0f);
+    add_all_global<MODE, TAN>(kp, o, kp.e.wcaCutoff * kp.e.wcaCutoff, acc, gx, gy, gz, off, sPos, sVel, sTan, sIdx);
+    add_force(force, me, acc);
 }
 
 // ===========================================================================
-// pair forces with LDS tiles (stencils of one z layer, gridSize.x, .y >= 3)
+// pair forces with LDS row runs (stencils of one z layer, gridSize.x >= 3,
+// cells of the sorted beads given)
 //
 // A workgroup owns 256 consecutive sorted beads whose cells span the hash
 // interval [h0, h1].  Every neighbour cell of those beads that does not wrap
 // lies in one of three hash intervals [h0 - 1 + dy Gx, h1 + 1 + dy Gx],
-// dy = -1, 0, 1, i.e. in three contiguous runs of the sorted arrays.  The
-// workgroup copies the runs into LDS with coalesced loads (positions and worm
-// ids, plus velocities / tangents when the pair terms read them), then every
-// candidate run that lies inside a staged run is read from LDS; the rest
-// (x or y wrap, a run longer than the tile) is read from global memory.  The
-// candidate order and arithmetic are those of k_collide_csr: bitwise equal.
+// dy = -1, 0, 1, i.e. in three contiguous runs of the sorted arrays, staged in
+// LDS with coalesced loads (position + worm id in one float4; velocity +
+// chain index, tangent when the terms read them).  A bead whose three row
+// runs all lie in staged runs walks them as one flattened candidate list, so
+// the lanes of a wavefront stay busy until the longest list ends rather than
+// the longest run of every row; any other bead (x or y wrap, an overlong
+// run) takes the global-memory walk.  Same order and arithmetic as
+// k_collide_csr.
 // ===========================================================================
-constexpr int kTileCap = 384;  // staged beads per row run
+constexpr int kRunCap = 384;  // staged beads per row run
 
-template <int MODE, bool TAN>
-__global__ __launch_bounds__(kBlock) void k_collide_lds(float4 *__restrict__ force, const float4 *__restrict__ sPos,
-                                                        const float4 *__restrict__ sVel,
-                                                        const float4 *__restrict__ sTan,
-                                                        const uint32_t *__restrict__ sIdx,
-                                                        const uint32_t *__restrict__ off,
-                                                        const uint32_t *__restrict__ sCell, uint32_t n, KP kp) {
-    __shared__ float4 lpos[3 * kTileCap];
-    __shared__ uint32_t lfil[3 * kTileCap];
-    __shared__ uint32_t lch[MODE == 0 ? 3 * kTileCap : 1];
-    __shared__ float4 lvel[MODE == 0 ? 3 * kTileCap : 1];
-    __shared__ float4 ltan[TAN ? 3 * kTileCap : 1];
+template <int MODE, bool TAN, bool UNIT>
+__global__ __launch_bounds__(kBlock) void k_collide_rows(float4 *__restrict__ force, const float4 *__restrict__ sPos,
+                                                         const float4 *__restrict__ sVel,
+                                                         const float4 *__restrict__ sTan,
+                                                         const uint32_t *__restrict__ sIdx,
+                                                         const uint32_t *__restrict__ off,
+                                                         const uint32_t *__restrict__ sCell, uint32_t n, KP kp) {
+    __shared__ float4 lpos[3 * kRunCap];
+    __shared__ float4 lvel[MODE == 0 ? 3 * kRunCap : 1];
+    __shared__ float4 ltan[TAN ? 3 * kRunCap : 1];
     __shared__ uint32_t rs[3], rl[3];
 
     const SimParams &p = kp.p;
-    const uint32_t Gx = p.gridSize.x;
-    const uint32_t C = p.gridSize.x * p.gridSize.y * p.gridSize.z;
+    const uint32_t Gx = p.gridSize.x, Gy = p.gridSize.y;
+    const uint32_t layer = Gx * Gy;
+    const uint32_t C = layer * p.gridSize.z;
     const uint32_t i0 = blockIdx.x * blockDim.x;
     const uint32_t i = i0 + threadIdx.x;
 
-    // hash interval of the tile (sorted: first and last bead)
     if (threadIdx.x < 3) {
         const uint32_t last = min(i0 + blockDim.x, n) - 1u;
-        const f3 a = mk(sPos[i0]), b = mk(sPos[last]);
-        const uint32_t h0 = md_hash_wrapped(p, cell_coord(a.x, p.origin.x, p.cellSize.x),
-                                            cell_coord(a.y, p.origin.y, p.cellSize.y),
-                                            cell_coord(a.z, p.origin.z, p.cellSize.z));
-        const uint32_t h1 = md_hash_wrapped(p, cell_coord(b.x, p.origin.x, p.cellSize.x),
-                                            cell_coord(b.y, p.origin.y, p.cellSize.y),
-                                            cell_coord(b.z, p.origin.z, p.cellSize.z));
+        const uint32_t h0 = sCell[i0], h1 = sCell[last];
         const int dy = int(threadIdx.x) - 1;
         const int64_t lo = int64_t(h0) - 1 + int64_t(dy) * Gx;
-        const int64_t hi = int64_t(h1) + 1 + int64_t(dy) * Gx;  // inclusive
-        const uint32_t clo = uint32_t(lo < 0 ? 0 : (lo >= int64_t(C) ? C : lo));
-        const uint32_t chi = uint32_t(hi + 1 < 0 ? 0 : (hi + 1 > int64_t(C) ? C : hi + 1));
+        const int64_t hi = int64_t(h1) + 2 + int64_t(dy) * Gx;  // exclusive
+        const uint32_t clo = uint32_t(lo < 0 ? 0 : (lo > int64_t(C) ? C : lo));
+        const uint32_t chi = uint32_t(hi < 0 ? 0 : (hi > int64_t(C) ? C : hi));
         const uint32_t s = off[clo], e = off[chi];
         rs[threadIdx.x] = s;
-        rl[threadIdx.x] = (e > s && e - s <= uint32_t(kTileCap)) ? e - s : 0u;
+        rl[threadIdx.x] = (e > s && e - s <= uint32_t(kRunCap)) ? e - s : 0u;
     }
     __syncthreads();
+#pragma unroll
     for (int r = 0; r < 3; ++r) {
         const uint32_t s = rs[r], l = rl[r];
         for (uint32_t k = threadIdx.x; k < l; k += blockDim.x) {
             const uint32_t j = s + k;
-            lpos[r * kTileCap + k] = sPos[j];
+            const float4 P = sPos[j];
             uint32_t fil, ch;
             worm_of(kp, sIdx[j], fil, ch);
-            lfil[r * kTileCap + k] = fil;
+            lpos[r * kRunCap + k] = make_float4(P.x, P.y, P.z, __uint_as_float(fil));
             if (MODE == 0) {
-                lch[r * kTileCap + k] = ch;
-                lvel[r * kTileCap + k] = sVel[j];
+                const float4 V = sVel[j];
+                lvel[r * kRunCap + k] = make_float4(V.x, V.y, V.z, __uint_as_float(ch));
             }
-            if (TAN) ltan[r * kTileCap + k] = sTan[j];
+            if (TAN) ltan[r * kRunCap + k] = sTan[j];
         }
     }
     __syncthreads();
     if (i >= n) return;
 
-    const f3 r0 = mk(sPos[i]);
-    const f3 v0 = MODE == 0 ? mk(sVel[i]) : mk(0.f, 0.f, 0.f);
-    const f3 t0 = TAN ? mk(sTan[i]) : mk(0.f, 0.f, 0.f);
-    const float rc2 = kp.e.wcaCutoff * kp.e.wcaCutoff;
     const uint32_t me = sIdx[i];
-    uint32_t fil, ch;
-    worm_of(kp, me, fil, ch);
+    const Owner<MODE, TAN> o = load_owner<MODE, TAN>(kp, i, me, sPos, sVel, sTan);
+    const float rc2 = kp.e.wcaCutoff * kp.e.wcaCutoff;
+    const uint32_t c = sCell[i];
+    const uint32_t gz = c / layer, rem = c - gz * layer;
+    const uint32_t gyu = rem / Gx;
+    const int gx = int(rem - gyu * Gx), gy = int(gyu);
+    const AxisCells ay = axis_cells(gy, Gy, false);
+    const float wz = kp.e.stencil == NW_STENCIL_2D ? 1.0f : axis_cells(int(gz), p.gridSize.z, false).w[0];
 
-    // one candidate, from LDS or global; terms added to fc in scan_run's order
-    auto pair = [&](f3 &fc, const f3 r2, uint32_t fil2, uint32_t ch2, const f3 v2, const f3 t2) {
-        const bool same = fil == fil2;
-        if (MODE == 0) {
-            if (same && ((ch + 1u == ch2) || (ch - 1u == ch2)))
-                fc += bond_hookean(p, r0, r2, p.bondSpringK, p.bondSpringL);
-            else
-                fc += dem_force(p, r0, r2, v0, v2);
-        } else if (!same) {
-            fc += wca_force(p, kp.e, r0, r2, rc2);
-        }
-        if (TAN && !same) fc += extensile_force(p, r0, r2, t0, t2, p.activity);
-    };
-    auto run = [&](uint32_t a, uint32_t b) {
-        f3 fc = mk(0.f, 0.f, 0.f);
-        if (a >= b) return fc;
-        int src = -1;
-        for (int r = 0; r < 3; ++r)
-            if (rl[r] && a >= rs[r] && b <= rs[r] + rl[r]) src = r;
-        if (src >= 0) {
-            const uint32_t base = uint32_t(src) * kTileCap - rs[src];
-            for (uint32_t j = a; j < b; ++j) {
-                if (j == i) continue;
-                const uint32_t q = base + j;
-                pair(fc, mk(lpos[q]), lfil[q], MODE == 0 ? lch[q] : 0u,
-                     MODE == 0 ? mk(lvel[q]) : mk(0.f, 0.f, 0.f), TAN ? mk(ltan[q]) : mk(0.f, 0.f, 0.f));
+    // LDS start and length of every row's run; any row outside the stage
+    // sends the bead down the global walk
+    uint32_t q[3] = {0u, 0u, 0u}, L[3] = {0u, 0u, 0u};
+    float w[3] = {0.f, 0.f, 0.f};
+    bool fast = gx >= 1 && gx + 1 < int(Gx);
+#pragma unroll
+    for (int yi = 0; yi < 3; ++yi) {
+        if (yi < ay.n && fast) {
+            const uint32_t base = (gz * Gy + uint32_t(ay.c[yi])) * Gx;
+            const uint32_t a = off[base + gx - 1], b = off[base + gx + 2];
+            bool hit = a == b;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                if (!hit && rl[r] && a >= rs[r] && b <= rs[r] + rl[r]) {
+                    q[yi] = uint32_t(r) * kRunCap + (a - rs[r]);
+                    hit = true;
+                }
             }
-        } else {
-            for (uint32_t j = a; j < b; ++j) {
-                if (j == i) continue;
-                uint32_t fil2, ch2;
-                worm_of(kp, sIdx[j], fil2, ch2);
-                pair(fc, mk(sPos[j]), fil2, ch2, MODE == 0 ? mk(sVel[j]) : mk(0.f, 0.f, 0.f),
-                     TAN ? mk(sTan[j]) : mk(0.f, 0.f, 0.f));
-            }
+            L[yi] = b - a;
+            w[yi] = wz * ay.w[yi];
+            fast = hit;
         }
-        return fc;
-    };
-
-    int gx, gy, gz;
-    bead_cell(kp, sCell, i, r0, gx, gy, gz);
-    const AxisCells ay = axis_cells(gy, p.gridSize.y, false);
-    const AxisCells az = axis_cells(gz, p.gridSize.z, kp.e.stencil == NW_STENCIL_2D);
-    const int cx = gx;
-    f3 acc = mk(0.f, 0.f, 0.f);
-    for (int yi = 0; yi < ay.n; ++yi) {
-        const float wzy = az.w[0] * ay.w[yi];
-        const uint32_t base = (uint32_t(az.c[0]) * p.gridSize.y + uint32_t(ay.c[yi])) * Gx;
-        uint32_t a0, a1, b0 = 0, b1 = 0;
-        if (cx >= 1 && cx + 1 < int(Gx)) {
-            a0 = off[base + cx - 1];
-            a1 = off[base + cx + 2];
-        } else if (cx == 0) {
-            a0 = off[base + Gx - 1];
-            a1 = off[base + Gx];
-            b0 = off[base];
-            b1 = off[base + 2];
-        } else {
-            a0 = off[base + Gx - 2];
-            a1 = off[base + Gx];
-            b0 = off[base];
-            b1 = off[base + 1];
-        }
-        f3 fr = run(a0, a1);
-        fr += run(b0, b1);
-        if (wzy == 1.0f) acc += fr;
-        else acc += wzy * fr;
     }
-    float4 Fo = force[me];
-    force[me] = make_float4(Fo.x + acc.x, Fo.y + acc.y, Fo.z + acc.z, Fo.w + 0.0f);
+
+    f3 acc = mk(0.f, 0.f, 0.f);
+    if (fast) {
+        const uint32_t L01 = L[0] + L[1], K = L01 + L[2];
+        const uint32_t d0 = q[0], d1 = q[1] - L[0], d2 = q[2] - L01;  // slot = t + d (mod 2^32)
+        for (uint32_t t = 0; t < K; ++t) {
+            const bool in0 = t < L[0], in1 = t < L01;
+            const uint32_t s = t + (in0 ? d0 : (in1 ? d1 : d2));
+            const float wt = UNIT ? 1.0f : (in0 ? w[0] : (in1 ? w[1] : w[2]));
+            const float4 P = lpos[s];
+            f3 v2 = mk(0.f, 0.f, 0.f);
+            uint32_t ch2 = 0u;
+            if (MODE == 0) {
+                const float4 V = lvel[s];
+                v2 = mk(V);
+                ch2 = __float_as_uint(V.w);
+            }
+            add_pair<MODE, TAN, UNIT>(kp, o, rc2, wt, acc, mk(P), __float_as_uint(P.w), ch2, v2,
+                                      TAN ? mk(ltan[s]) : mk(0.f, 0.f, 0.f));
+        }
+    } else {
+        add_all_global<MODE, TAN>(kp, o, rc2, acc, gx, gy, int(gz), off, sPos, sVel, sTan, sIdx);
+    }
+    add_force(force, me, acc);
 }
 
 // ===========================================================================
@@ -396,24 +414,30 @@ hipError_t launch_collide_off(float4 *force, const float4 *sPos, const float4 *s
                               const KP &kp, hipStream_t st) {
     if (n == 0) return hipSuccess;
     const bool oneLayer = kp.e.stencil == NW_STENCIL_2D || kp.p.gridSize.z == 1;
-    const bool ldsOk = oneLayer && kp.p.gridSize.x >= 3;
+    const bool ldsOk = oneLayer && kp.p.gridSize.x >= 3 && sCell != nullptr;
     const bool lds = ldsOk && kp.e.collideKernel != NW_COLLIDE_GLOBAL;
     const bool wca = kp.e.pairPotential == NW_PAIR_WCA;
     const bool tan = !wca || kp.p.activity != 0.0f;
     const dim3 g(grid_for(n)), b(kBlock);
+#define NW_COLLIDE(K, ...) hipLaunchKernelGGL((K<__VA_ARGS__>), g, b, 0, st, force, sPos, sVel, sTan, sIdx, off, sCell, n, kp)
     if (lds) {
-        if (wca && !tan)
-            hipLaunchKernelGGL((k_collide_lds<1, false>), g, b, 0, st, force, sPos, sVel, sTan, sIdx, off, sCell, n,
-                               kp);
-        else if (wca)
-            hipLaunchKernelGGL((k_collide_lds<1, true>), g, b, 0, st, force, sPos, sVel, sTan, sIdx, off, sCell, n, kp);
-        else
-            hipLaunchKernelGGL((k_collide_lds<0, true>), g, b, 0, st, force, sPos, sVel, sTan, sIdx, off, sCell, n, kp);
-    } else if (wca) {
-        hipLaunchKernelGGL(k_collide_csr<1>, g, b, 0, st, force, sPos, sVel, sTan, sIdx, off, sCell, n, kp);
+        // every row weight is 1 for the one-layer stencil over >= 3 rows
+        const bool unit = kp.e.stencil == NW_STENCIL_2D && kp.p.gridSize.y >= 3;
+        if (unit) {
+            if (wca && !tan) NW_COLLIDE(k_collide_rows, 1, false, true);
+            else if (wca) NW_COLLIDE(k_collide_rows, 1, true, true);
+            else NW_COLLIDE(k_collide_rows, 0, true, true);
+        } else {
+            if (wca && !tan) NW_COLLIDE(k_collide_rows, 1, false, false);
+            else if (wca) NW_COLLIDE(k_collide_rows, 1, true, false);
+            else NW_COLLIDE(k_collide_rows, 0, true, false);
+        }
     } else {
-        hipLaunchKernelGGL(k_collide_csr<0>, g, b, 0, st, force, sPos, sVel, sTan, sIdx, off, sCell, n, kp);
+        if (wca && !tan) NW_COLLIDE(k_collide_csr, 1, false);
+        else if (wca) NW_COLLIDE(k_collide_csr, 1, true);
+        else NW_COLLIDE(k_collide_csr, 0, true);
     }
+#undef NW_COLLIDE
     return hipGetLastError();
 }
 

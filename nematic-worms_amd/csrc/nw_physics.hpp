@@ -97,7 +97,7 @@ __device__ __forceinline__ AxisCells axis_cells(int g, uint32_t G, bool onlyCent
 
 // DEM soft sphere (collideParticles)
 __device__ __forceinline__ f3 dem_force(const SimParams &p, f3 rA, f3 rB, f3 vA, f3 vB) {
-    const f3 rel = min_image(p, rA, rB);
+    const f3 rel = min_image_pair(p, rA, rB);
     const float dist = length(rel);
     const float cd = p.particleRadius + p.particleRadius;
     f3 force = mk(0.f, 0.f, 0.f);
@@ -113,14 +113,14 @@ __device__ __forceinline__ f3 dem_force(const SimParams &p, f3 rA, f3 rB, f3 vA,
 }
 
 __device__ __forceinline__ f3 bond_hookean(const SimParams &p, f3 rA, f3 rB, float k, float L) {
-    const f3 rel = min_image(p, rA, rB);
+    const f3 rel = min_image_pair(p, rA, rB);
     const float dist = length(rel);
     const f3 nrm = rel / dist;
     return -k * (L - dist) * nrm;
 }
 
 __device__ __forceinline__ f3 extensile_force(const SimParams &p, f3 rA, f3 rB, f3 tA, f3 tB, float A) {
-    const f3 rel = min_image(p, rA, rB);
+    const f3 rel = min_image_pair(p, rA, rB);
     const float dist = length(rel);
     if (dot(tA, tB) < -0.5f) return -A * (tA - tB) / (2.0f * dist);
     return mk(0.f, 0.f, 0.f);
@@ -128,7 +128,7 @@ __device__ __forceinline__ f3 extensile_force(const SimParams &p, f3 rA, f3 rB, 
 
 // WCA-LJ force on A from B: F_A = -24 eps (2 s^12/r^14 - s^6/r^8) (rB - rA), r < rc
 __device__ __forceinline__ f3 wca_force(const SimParams &p, const NWExtParams &e, f3 rA, f3 rB, float rc2) {
-    const f3 rel = min_image(p, rA, rB);
+    const f3 rel = min_image_pair(p, rA, rB);
     const float rr = dot(rel, rel);
     if (rr < rc2) {
         // hardware reciprocal (1 ulp): the force is a product of powers of 1/r^2
