@@ -78,6 +78,25 @@ SINGLE_KERNEL = {"integrate", "langevin", "filament", "hash", "collide", "srd_co
                  "srdf_cells", "srdf_rotate"}
 
 
+# kernel behind each single-kernel family of the fused path (the names that
+# rocprofv3 reports), for the HBM traffic measured by its counters
+FAMILY_KERNEL = {"md_front": "k_md_front", "md_sort": "k_bucket_cells<false>",
+                 "md_collide": "k_collide_rows<1, false, true>", "srdf_tiles": "k_srd_tiles",
+                 "srdf_cells": "k_bucket_cells<true>", "srdf_rotate": "k_srd_rotate"}
+
+
+def measured_traffic(workload, family):
+    """HBM bytes per dispatch of the family's kernel from the committed
+    counter summary of this workload (profiles/<workload>_traffic.json,
+    written by python -m nwamd.profsum traffic), or None."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    kern = FAMILY_KERNEL.get(family)
+    if not kern or not os.path.exists(path):
+        return None
+    b = json.load(open(path)).get("bytes_per_dispatch", {}).get(kern)
+    return int(b) if b else None
+
+
 def build_workload(name, per_call=False):
     from nwamd import system
     if name == "worms_srd_800k":
@@ -174,8 +193,10 @@ def main():
     single = [k for k in per if k in SINGLE_KERNEL]
     dominant = max(single, key=lambda k: per[k]["ms_total"]) if single else "md_collide"
 
-    # timed region: only the dominant kernel carries events
-    lib.nwTimerEnable(dominant.encode(), 1)
+    # timed region: only the dominant kernel carries events, on every 10th
+    # launch when it runs every step (an event pair stalls the stream ~5 us)
+    stride = 10 if per.get(dominant, {}).get("launches", 0) >= args.profile_steps else 1
+    lib.nwTimerEnable(dominant.encode(), stride)
     s.sync()
     t0 = time.perf_counter()
     s.update(dt, args.steps)
@@ -189,8 +210,9 @@ def main():
     avg_s = dom_ms / max(dom_n, 1) * 1e-3
     achieved = alg / avg_s / 1e9 if avg_s > 0 else 0.0
     roof = dict(bound="hbm", kernel=dominant, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, alg_bytes_per_launch=int(alg),
-                avg_launch_us=round(avg_s * 1e6, 2), launches=dom_n)
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(args.workload, dominant),
+                alg_bytes_per_launch=int(alg),
+                avg_launch_us=round(avg_s * 1e6, 2), launches_timed=dom_n, timed_every=stride)
     kernels = {k: dict(avg_us=round(v["ms_per_launch"] * 1e3, 2), launches=v["launches"],
                        gbs=round(ALG_BYTES[k](w) / (v["ms_per_launch"] * 1e-3) / 1e9, 1))
                for k, v in per.items()}

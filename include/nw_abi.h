@@ -243,8 +243,10 @@ int nwFusedPlanFits(uint numParticles, uint numCells, uint filamentSize);
 /* Device-side anomaly flags (bit 0: sort key above bound).  Synchronises. */
 uint nwGetDeviceFlags(int clear);
 
-/* Kernel timing with HIP events recorded on the library stream around every
-   launch of the named kernel family while enabled.  Names of the per-call
+/* Kernel timing with HIP events recorded on the library stream around the
+   launches of the named kernel family while enabled: enable = 0 stops,
+   enable = k >= 1 (re)starts timing every k-th launch (an event pair costs a
+   few microseconds of stream time; sampling keeps that out of most steps).  Names of the per-call
    entry points: "integrate", "langevin", "filament", "hash", "sort",
    "reorder", "collide", "srd_hash", "srd_sort", "srd_reorder", "srd_com",
    "srd_thermo", "srd_collide", "srd_stream", "reverse"; of the fused steps:

@@ -21,6 +21,7 @@ using namespace nw;
 
 struct Timer {
     bool enabled = false;
+    uint32_t stride = 1, seen = 0;  // time every stride-th launch
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
     double totalMs = 0.0;
@@ -156,6 +157,7 @@ struct TimedScope {
     TimedScope(const char *name, hipStream_t stream) : st(stream) {
         auto it = S().timers.find(name);
         if (it == S().timers.end() || !it->second.enabled) return;
+        if (it->second.seen++ % it->second.stride != 0) return;
         t = &it->second;
         if (!t->pool.empty()) {
             ev = t->pool.back();
@@ -613,8 +615,10 @@ uint nwGetDeviceFlags(int clear) {
 void nwTimerEnable(const char *kernel, int enable) {
     if (!kernel) return;
     Timer &t = S().timers[kernel];
-    t.enabled = enable != 0;
-    if (enable) {
+    t.enabled = enable > 0;
+    if (enable > 0) {
+        t.stride = uint32_t(enable);
+        t.seen = 0;
         t.totalMs = 0.0;
         t.launches = 0;
     }
