@@ -240,13 +240,44 @@ void nwFusedSrdStep(float *pos, float *vel, float *solventCellCOM, uint *gridPar
    0: the SRD step), else 0.  No device work. */
 int nwFusedPlanFits(uint numParticles, uint numCells, uint filamentSize);
 
+/* Order parameters of a worm configuration (the reference's analysis,
+   analysis/analyze_correlations.py:13-38, analysis/afa/utils.py:35-49, on
+   the device).  Bead tangent t = unit(r[p+1] - r[p-1]), one-sided at the worm
+   ends, bond tangent b = unit(r[p+1] - r[p]), both under the minimum image.
+   No counterpart in the reference's C-ABI. */
+#define NW_OBS_SUMS 19
+typedef struct NWObservables {
+    double nematicS;    /* largest eigenvalue of <1.5 t t - 0.5 I> over bead tangents */
+    double polarP;      /* |<t>| */
+    double bondS;       /* largest eigenvalue of <1.5 b b - 0.5 I> over bond tangents */
+    double polarSpeed;  /* <v . t>: mean speed along the tangent */
+    double meanV2;      /* <|v|^2> */
+    double beads;       /* beads with a tangent (worms of >= 2 beads) */
+    double bonds;
+    /* FP64 sums: t t (xx xy xz yy yz zz), t (x y z), beads, b b (6), bonds,
+       v . t, |v|^2 */
+    double sums[NW_OBS_SUMS];
+} NWObservables;
+
+/* Observables of numFilaments worms of SimParams.filamentSize beads (pos,
+   vel: float4 per bead, worm-contiguous) with the box of the last
+   setParameters.  Synchronises the library stream.  Deterministic: a fixed
+   reduction order. */
+void nwObservables(const float *pos, const float *vel, uint numFilaments, NWObservables *out);
+
+/* Largest eigenvalue of the symmetric 3x3 matrix (a11 a12 a13 a22 a23 a33),
+   FP64, closed form (the trigonometric solution of the characteristic
+   cubic). */
+double nwSymEig3Max(const double a[6]);
+
 /* Device-side anomaly flags (bit 0: sort key above bound).  Synchronises. */
 uint nwGetDeviceFlags(int clear);
 
 /* Kernel timing with HIP events recorded on the library stream around the
    launches of the named kernel family while enabled: enable = 0 stops,
    enable = k >= 1 (re)starts timing every k-th launch (an event pair costs a
-   few microseconds of stream time; sampling keeps that out of most steps).  Names of the per-call
+   few microseconds of stream time; sampling keeps that out of most steps).
+   Names of the per-call
    entry points: "integrate", "langevin", "filament", "hash", "sort",
    "reorder", "collide", "srd_hash", "srd_sort", "srd_reorder", "srd_com",
    "srd_thermo", "srd_collide", "srd_stream", "reverse"; of the fused steps:

@@ -10,6 +10,7 @@
 
 #include <stdint.h>
 #include "nw_params.h"
+#include "nw_abi.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -70,6 +71,8 @@ float *nwSystemDeviceArray(NWSystem *sys, int which);
 /* RNG state (seed, generator offset) at this point of the run */
 void nwSystemRngState(NWSystem *sys, uint64_t *seed, uint64_t *offset);
 void nwSystemSetRngState(NWSystem *sys, uint64_t offset);
+/* order parameters of the current worm configuration (nwObservables) */
+void nwSystemObservables(NWSystem *sys, NWObservables *out);
 /* appends one .xyzv frame (particleSystem.cpp:481-532) */
 void nwSystemWriteOutputs(NWSystem *sys, const char *fileName, int iteration, float dt);
 

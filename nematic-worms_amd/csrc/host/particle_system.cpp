@@ -170,6 +170,11 @@ void ParticleSystem::_upload(float *dev, const std::vector<float> &host, size_t 
 // --------------------------------------------------------------------------
 // the step (particleSystem.cpp:249-339)
 // --------------------------------------------------------------------------
+void ParticleSystem::observables(NWObservables *out) {
+    setParameters(&m_params);
+    nwObservables(m_dPos, m_dVel, m_numFilaments, out);
+}
+
 bool ParticleSystem::fusedActive() const {
     if (m_perCall || m_filamentSize < 1 || m_filamentSize > 64) return false;
     const uint mdCells = m_params.gridSize.x * m_params.gridSize.y * m_params.gridSize.z;

@@ -41,6 +41,8 @@ public:
     void dumpSolventGrid();
     void dumpParticles(uint start, uint count);
     void writeOutputs(const std::string &fileName, int iteration, float timeDelta);
+    // nematic / polar order and speed along the tangent of the worms (on the device)
+    void observables(NWObservables *out);
 
     // setters / getters (particleSystem.h:53-138)
     void setDamping(float x) { m_params.gamma = x; }

@@ -79,6 +79,8 @@ void nwSystemRngState(NWSystem *sys, uint64_t *seed, uint64_t *offset) {
 
 void nwSystemSetRngState(NWSystem *sys, uint64_t offset) { sys->ps->setRngOffset(offset); }
 
+void nwSystemObservables(NWSystem *sys, NWObservables *out) { sys->ps->observables(out); }
+
 void nwSystemWriteOutputs(NWSystem *sys, const char *fileName, int iteration, float dt) {
     sys->ps->writeOutputs(fileName, iteration, dt);
 }

@@ -14,6 +14,7 @@
 #include "nw_abi.h"
 #include "nw_internal.hpp"
 #include "nw_bucket.hpp"
+#include "nw_observe.hpp"
 
 namespace {
 
@@ -46,6 +47,8 @@ struct State {
     bool lastHashSrd = false;     // timer family of the sort / reorder that follow
     SortScratch sort;
     BucketScratch bucket;
+    double *obsPartial = nullptr;  // observe_rows x kObsSums, then kObsSums totals
+    uint64_t obsCap = 0;
     std::map<std::string, Timer> timers;
 };
 
@@ -590,6 +593,66 @@ void nwGetExtParameters(NWExtParams *ext) {
 void *nwGetStream(void) { return S().stream; }
 
 void nwSetStream(void *stream) { S().stream = static_cast<hipStream_t>(stream); }
+
+double nwSymEig3Max(const double a[6]) {
+    const double a11 = a[0], a12 = a[1], a13 = a[2], a22 = a[3], a23 = a[4], a33 = a[5];
+    const double p1 = a12 * a12 + a13 * a13 + a23 * a23;
+    if (p1 == 0.0) return std::fmax(a11, std::fmax(a22, a33));
+    const double q = (a11 + a22 + a33) / 3.0;
+    const double p2 = (a11 - q) * (a11 - q) + (a22 - q) * (a22 - q) + (a33 - q) * (a33 - q) + 2.0 * p1;
+    const double p = std::sqrt(p2 / 6.0);
+    const double b11 = (a11 - q) / p, b22 = (a22 - q) / p, b33 = (a33 - q) / p;
+    const double b12 = a12 / p, b13 = a13 / p, b23 = a23 / p;
+    const double det = b11 * (b22 * b33 - b23 * b23) - b12 * (b12 * b33 - b23 * b13) + b13 * (b12 * b23 - b22 * b13);
+    const double r = std::fmin(1.0, std::fmax(-1.0, det / 2.0));
+    return q + 2.0 * p * std::cos(std::acos(r) / 3.0);
+}
+
+void nwObservables(const float *pos, const float *vel, uint numFilaments, NWObservables *out) {
+    NW_INIT();
+    NW_REQUIRE(out != nullptr && pos != nullptr && vel != nullptr, "nwObservables: NULL argument");
+    State &s = S();
+    const KP kp = make_kp();
+    const uint32_t n = kp.p.filamentSize;
+    NW_REQUIRE(n >= 1 && n <= 64, "nwObservables: worms of %u beads (1..64 supported)", n);
+    std::memset(out, 0, sizeof(*out));
+    const uint64_t rows = observe_rows(numFilaments, n);
+    if (rows == 0) return;
+    const uint64_t need = (rows + 1) * kObsSums;
+    if (need > s.obsCap) {
+        if (s.obsPartial) (void)hipFree(s.obsPartial);
+        s.obsPartial = nullptr;
+        s.obsCap = 0;
+        NW_HIP(hipMalloc(&s.obsPartial, need * sizeof(double)));
+        s.obsCap = need;
+    }
+    double *total = s.obsPartial + rows * kObsSums;
+    NW_HIP(launch_observe(reinterpret_cast<const float4 *>(pos), reinterpret_cast<const float4 *>(vel), numFilaments,
+                          s.obsPartial, total, kp, s.stream));
+    double sums[kObsSums];
+    NW_HIP(hipMemcpyAsync(sums, total, sizeof(sums), hipMemcpyDeviceToHost, s.stream));
+    NW_HIP(hipStreamSynchronize(s.stream));
+    static_assert(kObsSums == NW_OBS_SUMS, "NWObservables.sums layout");
+    std::memcpy(out->sums, sums, sizeof(sums));
+    const double nb = sums[kObsBeads], nbo = sums[kObsBonds];
+    out->beads = nb;
+    out->bonds = nbo;
+    if (nb > 0) {
+        const double Q[6] = {1.5 * sums[kObsTxx] / nb - 0.5, 1.5 * sums[kObsTxy] / nb, 1.5 * sums[kObsTxz] / nb,
+                             1.5 * sums[kObsTyy] / nb - 0.5, 1.5 * sums[kObsTyz] / nb, 1.5 * sums[kObsTzz] / nb - 0.5};
+        out->nematicS = nwSymEig3Max(Q);
+        const double mx = sums[kObsTx] / nb, my = sums[kObsTy] / nb, mz = sums[kObsTz] / nb;
+        out->polarP = std::sqrt(mx * mx + my * my + mz * mz);
+        out->polarSpeed = sums[kObsVdotT] / nb;
+        out->meanV2 = sums[kObsV2] / nb;
+    }
+    if (nbo > 0) {
+        const double Q[6] = {1.5 * sums[kObsBxx] / nbo - 0.5, 1.5 * sums[kObsBxy] / nbo, 1.5 * sums[kObsBxz] / nbo,
+                             1.5 * sums[kObsByy] / nbo - 0.5, 1.5 * sums[kObsByz] / nbo,
+                             1.5 * sums[kObsBzz] / nbo - 0.5};
+        out->bondS = nwSymEig3Max(Q);
+    }
+}
 
 int nwFusedPlanFits(uint numParticles, uint numCells, uint filamentSize) {
     if (filamentSize > 64 || numCells == 0) return 0;

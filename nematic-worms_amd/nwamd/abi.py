@@ -76,6 +76,14 @@ assert ctypes.sizeof(SimParams) == 144
 assert ctypes.sizeof(SolventParams) == 60
 assert ctypes.sizeof(NWExtParams) == 64
 
+NW_OBS_SUMS = 19
+
+
+class NWObservables(ctypes.Structure):
+    _fields_ = [("nematicS", ctypes.c_double), ("polarP", ctypes.c_double), ("bondS", ctypes.c_double),
+                ("polarSpeed", ctypes.c_double), ("meanV2", ctypes.c_double), ("beads", ctypes.c_double),
+                ("bonds", ctypes.c_double), ("sums", ctypes.c_double * NW_OBS_SUMS)]
+
 P = c_void_p
 # name -> (restype, argtypes); the reference surface first, then additions.
 PROTOTYPES = {
@@ -121,6 +129,8 @@ PROTOTYPES = {
     "nwFusedMdStep": (None, [P, P, P, P, P, P, P, P, P, P, c_float, c_float, c_float, c_int, c_uint]),
     "nwFusedSrdStep": (None, [P, P, P, P, P, c_float, c_uint, c_uint, c_int]),
     "nwFusedPlanFits": (c_int, [c_uint, c_uint, c_uint]),
+    "nwObservables": (None, [ctypes.c_void_p, ctypes.c_void_p, c_uint, ctypes.POINTER(NWObservables)]),
+    "nwSymEig3Max": (ctypes.c_double, [ctypes.POINTER(ctypes.c_double)]),
     "nwGetDeviceFlags": (c_uint, [c_int]),
     "nwTimerEnable": (None, [ctypes.c_char_p, c_int]),
     "nwTimerRead": (None, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_u64)]),

@@ -55,6 +55,7 @@ def load():
             "nwSystemRngState": (None, [P, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
             "nwSystemSetRngState": (None, [P, u64]),
             "nwSystemWriteOutputs": (None, [P, ctypes.c_char_p, ctypes.c_int, f]),
+            "nwSystemObservables": (None, [P, ctypes.POINTER(abi.NWObservables)]),
         }
         for name, (res, args) in protos.items():
             fn = getattr(L, name)
@@ -124,6 +125,13 @@ class System:
         s, o = ctypes.c_uint64(), ctypes.c_uint64()
         self.lib.nwSystemRngState(self.h, ctypes.byref(s), ctypes.byref(o))
         return s.value, o.value
+
+    def observables(self):
+        """nwObservables of the current configuration as a dict."""
+        o = abi.NWObservables()
+        self.lib.nwSystemObservables(self.h, ctypes.byref(o))
+        return dict(nematicS=o.nematicS, polarP=o.polarP, bondS=o.bondS, polarSpeed=o.polarSpeed,
+                    meanV2=o.meanV2, beads=o.beads, bonds=o.bonds, sums=list(o.sums))
 
     def write_outputs(self, path, iteration, dt):
         self.lib.nwSystemWriteOutputs(self.h, path.encode(), iteration, dt)

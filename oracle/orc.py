@@ -208,6 +208,68 @@ def srd_shift(seed, k, centred, cell):
     return out
 
 
+# ---- order parameters ------------------------------------------------------
+def _mi_axis32(b, a, L, periodic):
+    """mi_axis in FP32 (nw_oracle_impl.h), vectorised."""
+    b, a = np.asarray(b, np.float32), np.asarray(a, np.float32)
+    d = (b - a).astype(np.float32)
+    if periodic:
+        h = np.float32(L) / np.float32(2.0)
+        up = ((b - h).astype(np.float32) - (a + h).astype(np.float32)).astype(np.float32)
+        dn = ((b + h).astype(np.float32) - (a - h).astype(np.float32)).astype(np.float32)
+        d = np.where(d > h, up, np.where(d < -h, dn, d))
+    return d
+
+
+def _min_image32(p, a, b):
+    per = (p.boundaryX == 0, p.boundaryY == 0, p.boundaryZ == 0)
+    L = (p.boxSize.x, p.boxSize.y, p.boxSize.z)
+    return np.stack([_mi_axis32(b[:, k], a[:, k], L[k], per[k]) for k in range(3)], axis=1)
+
+
+OBS_SUMS = 19
+
+
+def observables(p, pos, vel, W):
+    """The sums of nwObservables (same tangent definitions: the reference's
+    analysis, analysis/afa/utils.py:35-49 and analyze_correlations.py:13-38,
+    with the minimum image) in FP64 from the FP32 minimum-image differences,
+    and S, P from numpy's symmetric eigensolver."""
+    n = p.filamentSize
+    r = np.asarray(pos, np.float32)[:W * n, :3].reshape(W, n, 3)
+    v = np.asarray(vel, np.float32)[:W * n, :3].reshape(W, n, 3).astype(np.float64)
+    out = np.zeros(OBS_SUMS)
+    res = dict(sums=out, nematicS=0.0, polarP=0.0, bondS=0.0, polarSpeed=0.0, meanV2=0.0)
+    if n < 2 or W == 0:
+        return res
+    lo = np.concatenate([r[:, :1], r[:, :-1]], axis=1)    # r[p-1], r[0] at the head
+    hi = np.concatenate([r[:, 1:], r[:, -1:]], axis=1)    # r[p+1], r[n-1] at the tail
+    d = _min_image32(p, lo.reshape(-1, 3), hi.reshape(-1, 3)).astype(np.float64)
+    t = d / np.sqrt((d * d).sum(1))[:, None]
+    e = _min_image32(p, r[:, :-1].reshape(-1, 3), r[:, 1:].reshape(-1, 3)).astype(np.float64)
+    b = e / np.sqrt((e * e).sum(1))[:, None]
+    iu = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    out[0:6] = [(t[:, i] * t[:, j]).sum() for i, j in iu]
+    out[6:9] = t.sum(0)
+    out[9] = len(t)
+    out[10:16] = [(b[:, i] * b[:, j]).sum() for i, j in iu]
+    out[16] = len(b)
+    vv = v.reshape(-1, 3)
+    out[17] = (vv * t).sum()
+    out[18] = (vv * vv).sum()
+
+    def smax(s6, cnt):
+        Q = np.empty((3, 3))
+        for k, (i, j) in enumerate(iu):
+            Q[i, j] = Q[j, i] = 1.5 * s6[k] / cnt
+        Q -= 0.5 * np.eye(3)
+        return float(np.linalg.eigvalsh(Q)[-1])
+
+    res.update(nematicS=smax(out[0:6], out[9]), polarP=float(np.linalg.norm(out[6:9] / out[9])),
+               bondS=smax(out[10:16], out[16]), polarSpeed=out[17] / out[9], meanV2=out[18] / out[9])
+    return res
+
+
 # ---- whole-step CPU system (the CPU baseline) --------------------------------
 class OrcSystemC(ctypes.Structure):
     _fields_ = [

@@ -9,6 +9,7 @@ invariants (Newton's third law, momentum conservation, time reversibility,
 equipartition) and Philox known-answer vectors from an independent
 implementation (tests/golden/philox_kat.json).
 """
+import ctypes
 import json
 import math
 import os
@@ -18,6 +19,7 @@ import pytest
 
 import orc
 from nwamd import abi
+import nwtest
 from nwtest import ext_params, random_worms, sim_params, solvent_params
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -555,3 +557,65 @@ def test_reversal_reverses_selected_worms():
             assert np.array_equal(seg_out, seg_in[::-1] if uni[w] <= 0.4 else seg_in)
     twice = orc.reverse_filaments(p, uni, W, *out)
     assert all(np.array_equal(x, y) for x, y in zip(twice, arrs))
+
+
+# ---- order parameters (orc.observables) --------------------------------------
+def _straight_worms(p, angles, centres, bond=0.8):
+    """Straight worms along the given in-plane angles, centred at `centres`."""
+    n = p.filamentSize
+    pos = np.zeros((len(angles) * n, 4), np.float32)
+    for w, (th, c) in enumerate(zip(angles, centres)):
+        u = np.array([np.cos(th), np.sin(th), 0.0])
+        for q in range(n):
+            pos[w * n + q, :3] = c + (q - (n - 1) / 2) * bond * u
+    pos[:, :3] = nwtest.wrap_into_box(p, pos[:, :3])
+    return pos
+
+
+def test_observables_aligned_and_antialigned_worms():
+    p = sim_params(grid=(40, 40, 1), cell=1.0, W=8, n=6)
+    rng = np.random.default_rng(3)
+    centres = np.c_[rng.random((8, 2)) * 40 - 20, np.zeros(8)]
+    vel = np.zeros((48, 4), np.float32)
+    pos = _straight_worms(p, [0.3] * 8, centres)
+    o = orc.observables(p, pos, vel, 8)
+    assert o["nematicS"] == pytest.approx(1.0, abs=1e-6) and o["polarP"] == pytest.approx(1.0, abs=1e-6)
+    assert o["bondS"] == pytest.approx(1.0, abs=1e-6)
+    # half the worms reversed: nematic order stays, polar order vanishes
+    pos = _straight_worms(p, [0.3, 0.3 + np.pi] * 4, centres)
+    o = orc.observables(p, pos, vel, 8)  # FP32 positions: tangents to ~1e-7
+    assert o["nematicS"] == pytest.approx(1.0, abs=1e-6) and o["polarP"] == pytest.approx(0.0, abs=1e-6)
+
+
+def test_observables_isotropic_plane_and_periodic_worms():
+    """M straight worms at angles k pi / M: <cos 2 theta> = <sin 2 theta> = 0, so
+    the 3D Q tensor of in-plane tangents has eigenvalues 1/4, 1/4, -1/2; a
+    worm across the periodic x face keeps its tangent under the minimum image."""
+    M = 12
+    p = sim_params(grid=(30, 30, 1), cell=1.0, W=M, n=5)
+    centres = np.zeros((M, 3))
+    centres[:, 0] = np.linspace(-12, 12, M)
+    pos = _straight_worms(p, np.arange(M) * np.pi / M, centres)
+    o = orc.observables(p, pos, np.zeros((M * 5, 4), np.float32), M)
+    assert o["nematicS"] == pytest.approx(0.25, abs=1e-6)
+    q = sim_params(grid=(30, 30, 1), cell=1.0, W=1, n=5)
+    pos = _straight_worms(q, [0.0], np.array([[14.9, 0.0, 0.0]]))
+    assert pos[0, 0] > 13.0 and pos[-1, 0] < -13.0  # the worm wraps
+    vel = np.zeros((5, 4), np.float32)
+    vel[:, 0] = 2.0
+    o = orc.observables(q, pos, vel, 1)
+    assert o["nematicS"] == pytest.approx(1.0, abs=1e-6) and o["polarP"] == pytest.approx(1.0, abs=1e-6)
+    assert o["polarSpeed"] == pytest.approx(2.0, abs=1e-6) and o["meanV2"] == pytest.approx(4.0)
+
+
+def test_library_sym_eig3_matches_numpy():
+    """nwSymEig3Max (host code in libnwamd.so, no device) against LAPACK."""
+    L = abi.load()
+    rng = np.random.default_rng(11)
+    for k in range(200):
+        A = rng.normal(size=(3, 3))
+        A = A + A.T
+        if k % 4 == 0:
+            A[0, 1] = A[1, 0] = A[0, 2] = A[2, 0] = A[1, 2] = A[2, 1] = 0.0
+        a6 = (ctypes.c_double * 6)(A[0, 0], A[0, 1], A[0, 2], A[1, 1], A[1, 2], A[2, 2])
+        assert L.nwSymEig3Max(a6) == pytest.approx(np.linalg.eigvalsh(A)[-1], abs=1e-12)
