@@ -214,8 +214,9 @@ void nwRadixSortPairs(uint *dGridParticleHash, uint *dGridParticleIndex,
    which may be NULL when the WCA pair term without activity does not read
    them) the gathered records, gridParticleHash (may be NULL) the cell of
    every bead in bead order; the reference's cellStart/cellEnd tables are not
-   produced.  Fails (NW error -22) when the grid has more than ~33M cells or
-   the beads x cells product exceeds the sort's tile x bucket table. */
+   produced.  Fails (NW error -22) when the grid has more than ~67M cells or
+   the beads x cells product exceeds the sort's tile x bucket table
+   (nwFusedPlanFits). */
 void nwFusedMdStep(float *pos, float *vel, float *force, float *forceOld, float *tangent,
                    uint *gridParticleHash, uint *gridParticleIndex,
                    float *sortedPos, float *sortedVel, float *sortedTangent,
@@ -239,6 +240,40 @@ void nwFusedSrdStep(float *pos, float *vel, float *solventCellCOM, uint *gridPar
    numCells cells (filamentSize 1..64: the MD step's tiles of whole worms;
    0: the SRD step), else 0.  No device work. */
 int nwFusedPlanFits(uint numParticles, uint numCells, uint filamentSize);
+
+/* ------------------------------------------------------------------------
+   Batched replicas: many independent worm systems of uneven size stepped by
+   one launch of each fused kernel per step (no counterpart in the
+   reference).  Replica r's beads occupy [beadOffset[r], beadOffset[r+1]) of
+   the concatenated float4 arrays, its cells [cellOffset[r], ...) of one
+   concatenated cell space; each keeps its own SimParams, NWExtParams and
+   Philox stream, so a replica's trajectory is the one nwFusedMdStep gives it
+   alone (same seed and offset), bit for bit.
+   ------------------------------------------------------------------------ */
+typedef struct NWReplicaParams {
+    SimParams params;     /* box, grid (one layer, gridSize.x >= 3), worms, force constants */
+    NWExtParams ext;      /* pair potential and stencil must agree over the batch */
+    uint64_t seed;        /* the replica's Philox key */
+    uint64_t offset;      /* its generator offset before the first batched step */
+    float langevinGamma;  /* noise sigma = sqrt(2 gamma kT), as nwFusedMdStep */
+    float langevinKbT;
+    int langevin;
+    uint32_t reserved[5];
+} NWReplicaParams;
+
+/* Replaces the batch (one per process) and resets its step count. */
+void nwBatchSetReplicas(const NWReplicaParams *reps, uint numReplicas);
+/* numReplicas + 1 bead and cell offsets of the current batch. */
+void nwBatchLayout(uint *beadOffset, uint *cellOffset);
+/* `steps` fused MD steps of every replica over the concatenated arrays
+   (sortedVel / sortedTangent may be NULL as in nwFusedMdStep).  useGraph:
+   after one direct step, the step is captured once as a hipGraph (per
+   argument set) and replayed. */
+void nwBatchMdStep(float *pos, float *vel, float *force, float *forceOld, float *tangent,
+                   uint *gridParticleIndex, float *sortedPos, float *sortedVel, float *sortedTangent,
+                   float deltaTime, uint steps, int useGraph);
+/* Generator offset of one replica after the batched steps so far. */
+void nwBatchRngState(uint replica, uint64_t *offset);
 
 /* Order parameters of a worm configuration (the reference's analysis,
    analysis/analyze_correlations.py:13-38, analysis/afa/utils.py:35-49, on
@@ -281,8 +316,8 @@ uint nwGetDeviceFlags(int clear);
    entry points: "integrate", "langevin", "filament", "hash", "sort",
    "reorder", "collide", "srd_hash", "srd_sort", "srd_reorder", "srd_com",
    "srd_thermo", "srd_collide", "srd_stream", "reverse"; of the fused steps:
-   "md_front", "md_sort", "md_collide", "srdf_tiles", "srdf_cells",
-   "srdf_rotate". */
+   "md_front", "md_tiles" (systems too large for the front's tiles),
+   "md_sort", "md_collide", "srdf_tiles", "srdf_cells", "srdf_rotate". */
 void nwTimerEnable(const char *kernel, int enable);
 /* Synchronises; returns total milliseconds and launch count since enable. */
 void nwTimerRead(const char *kernel, double *totalMs, uint64_t *launches);

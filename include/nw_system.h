@@ -71,6 +71,53 @@ float *nwSystemDeviceArray(NWSystem *sys, int which);
 /* RNG state (seed, generator offset) at this point of the run */
 void nwSystemRngState(NWSystem *sys, uint64_t *seed, uint64_t *offset);
 void nwSystemSetRngState(NWSystem *sys, uint64_t offset);
+/* ------------------------------------------------------------------------
+   Replica sets: many independent 2D periodic worm systems of uneven size
+   (WCA + chain bonds + bending + drive + Langevin), stepped together by the
+   batched fused step (nwBatchMdStep) or one replica per launch sequence.
+   ------------------------------------------------------------------------ */
+typedef struct NWReplicaSpec {
+    uint32_t numFilaments, filamentSize;
+    float boxSize;      /* side of the square periodic box */
+    float driveForce;   /* f_a along the bond tangent */
+    float bendingK;     /* kappa */
+    float activity;     /* extensile pair strength (0: none) */
+    uint64_t seed;      /* initial conditions and Philox noise */
+    uint32_t reserved[4];
+} NWReplicaSpec;
+
+typedef struct NWReplicaPhysics {
+    float bondK, bondL;           /* harmonic chain bonds */
+    float wcaEpsilon, wcaSigma;   /* WCA between beads of different worms */
+    float gamma, kT;              /* Langevin */
+    float cellMin;                /* smallest MD cell side (>= 2^(1/6) sigma) */
+    int langevinFdt;              /* noise sqrt(2 gamma kT / dt) instead of sqrt(2 gamma kT) */
+    uint32_t reserved[8];
+} NWReplicaPhysics;
+
+enum {
+    NW_REPLICAS_GRAPH = 0,          /* batched step, captured once as a hipGraph and replayed */
+    NW_REPLICAS_LAUNCHES = 1,       /* batched step, launched kernel by kernel */
+    NW_REPLICAS_ONE_PER_LAUNCH = 2  /* every replica through its own nwFusedMdStep per step */
+};
+
+typedef struct NWReplicaSet NWReplicaSet;
+NWReplicaSet *nwReplicaSetCreate(const NWReplicaSpec *specs, uint32_t numReplicas, const NWReplicaPhysics *phys);
+void nwReplicaSetDestroy(NWReplicaSet *set);
+void nwReplicaSetReset(NWReplicaSet *set, int mode, float kT);  /* NW_INIT_RANDOM / NW_INIT_LATTICE_RANDOM */
+void nwReplicaSetUpdate(NWReplicaSet *set, float dt, uint32_t steps, int form);
+void nwReplicaSetSync(NWReplicaSet *set);
+uint32_t nwReplicaSetNumReplicas(NWReplicaSet *set);
+/* beads of one replica, or of all (replica < 0) */
+uint64_t nwReplicaSetNumParticles(NWReplicaSet *set, int replica);
+/* float4 records of one replica (which: 0 pos, 1 vel, 2 tangent, 3 force, 4 forceOld) */
+void nwReplicaSetGetArray(NWReplicaSet *set, int which, int replica, float *host);
+void nwReplicaSetSetArray(NWReplicaSet *set, int which, int replica, const float *host);
+/* the replica's SimParams / NWExtParams and generator state */
+void nwReplicaSetParams(NWReplicaSet *set, int replica, SimParams *p, NWExtParams *e);
+void nwReplicaSetRngState(NWReplicaSet *set, int replica, uint64_t *seed, uint64_t *offset);
+void nwReplicaSetObservables(NWReplicaSet *set, int replica, NWObservables *out);
+
 /* order parameters of the current worm configuration (nwObservables) */
 void nwSystemObservables(NWSystem *sys, NWObservables *out);
 /* appends one .xyzv frame (particleSystem.cpp:481-532) */

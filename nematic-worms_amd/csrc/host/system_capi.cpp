@@ -4,9 +4,14 @@
 #include <stdexcept>
 
 #include "particle_system.h"
+#include "replica_set.h"
 
 struct NWSystem {
     ParticleSystem *ps;
+};
+
+struct NWReplicaSet {
+    ReplicaSet *rs;
 };
 
 extern "C" {
@@ -83,6 +88,61 @@ void nwSystemObservables(NWSystem *sys, NWObservables *out) { sys->ps->observabl
 
 void nwSystemWriteOutputs(NWSystem *sys, const char *fileName, int iteration, float dt) {
     sys->ps->writeOutputs(fileName, iteration, dt);
+}
+
+// ---- replica sets ----------------------------------------------------------
+NWReplicaSet *nwReplicaSetCreate(const NWReplicaSpec *specs, uint32_t numReplicas, const NWReplicaPhysics *phys) {
+    if ((numReplicas && !specs) || !phys) {
+        fprintf(stderr, "nwReplicaSetCreate: NULL argument\n");
+        return nullptr;
+    }
+    try {
+        NWReplicaSet *s = new NWReplicaSet;
+        s->rs = new ReplicaSet(specs, numReplicas, *phys);
+        return s;
+    } catch (const std::exception &e) {
+        fprintf(stderr, "nwReplicaSetCreate: %s\n", e.what());
+        return nullptr;
+    }
+}
+
+void nwReplicaSetDestroy(NWReplicaSet *set) {
+    if (!set) return;
+    delete set->rs;
+    delete set;
+}
+
+void nwReplicaSetReset(NWReplicaSet *set, int mode, float kT) {
+    try {
+        set->rs->reset(mode, kT);
+    } catch (const std::exception &e) {
+        fprintf(stderr, "nwReplicaSetReset: %s\n", e.what());
+    }
+}
+
+void nwReplicaSetUpdate(NWReplicaSet *set, float dt, uint32_t steps, int form) { set->rs->update(dt, steps, form); }
+void nwReplicaSetSync(NWReplicaSet *set) { set->rs->sync(); }
+uint32_t nwReplicaSetNumReplicas(NWReplicaSet *set) { return set->rs->numReplicas(); }
+uint64_t nwReplicaSetNumParticles(NWReplicaSet *set, int replica) { return set->rs->numParticles(replica); }
+
+void nwReplicaSetGetArray(NWReplicaSet *set, int which, int replica, float *host) {
+    set->rs->getArray(which, replica, host);
+}
+
+void nwReplicaSetSetArray(NWReplicaSet *set, int which, int replica, const float *host) {
+    set->rs->setArray(which, replica, host);
+}
+
+void nwReplicaSetParams(NWReplicaSet *set, int replica, SimParams *p, NWExtParams *e) {
+    set->rs->params(replica, p, e);
+}
+
+void nwReplicaSetRngState(NWReplicaSet *set, int replica, uint64_t *seed, uint64_t *offset) {
+    set->rs->rngState(replica, seed, offset);
+}
+
+void nwReplicaSetObservables(NWReplicaSet *set, int replica, NWObservables *out) {
+    set->rs->observables(replica, out);
 }
 
 }  // extern "C"

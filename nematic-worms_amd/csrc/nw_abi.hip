@@ -49,6 +49,21 @@ struct State {
     BucketScratch bucket;
     double *obsPartial = nullptr;  // observe_rows x kObsSums, then kObsSums totals
     uint64_t obsCap = 0;
+    // batched replicas
+    struct Batch {
+        std::vector<BatchRep> reps;
+        std::vector<uint32_t> beadOff, cellOff;
+        BatchRep *dev = nullptr;
+        uint64_t *stepCtr = nullptr;
+        uint32_t R = 0, N = 0, cells = 0, waves = 0, blocks = 0;
+        int mode = 1;
+        bool tan = false, unit = true;
+        uint64_t steps = 0;
+        BucketScratch scratch;
+        hipGraphExec_t exec = nullptr;
+        hipGraph_t graph = nullptr;
+        std::vector<uintptr_t> key;  // arguments the graph was captured with
+    } batch;
     std::map<std::string, Timer> timers;
 };
 
@@ -132,13 +147,12 @@ bool ensure_init() {
 
 // fast division by the worm length: q = mulhi(i, m) with m = ceil(2^32 / n)
 // is exact for every i < 2^32 / e, e = m * n - 2^32 (Granlund & Montgomery 1994).
-KP make_kp() {
-    State &s = S();
+KP make_kp_for(const SimParams &p, const SolventParams &sp, const NWExtParams &ext) {
     KP kp{};
-    kp.p = s.params;
-    kp.s = s.solvent;
-    kp.e = s.ext;
-    const uint64_t n = s.params.filamentSize;
+    kp.p = p;
+    kp.s = sp;
+    kp.e = ext;
+    const uint64_t n = p.filamentSize;
     if (n >= 2) {
         const uint64_t two32 = 1ull << 32;
         const uint64_t m = (two32 + n - 1) / n;
@@ -152,6 +166,8 @@ KP make_kp() {
     }
     return kp;
 }
+
+KP make_kp() { return make_kp_for(S().params, S().solvent, S().ext); }
 
 struct TimedScope {
     Timer *t = nullptr;
@@ -458,7 +474,11 @@ void nwFusedMdStep(float *pos, float *vel, float *force, float *forceOld, float 
     NW_REQUIRE(!needVel || sortedVel != nullptr, "nwFusedMdStep: sortedVel is NULL");
     NW_REQUIRE(!needTan || sortedTangent != nullptr, "nwFusedMdStep: sortedTangent is NULL");
     if (N == 0) return;
-    const BucketPlan bp = make_bucket_plan(N, cells, md_tile_items(n));
+    // the front's workgroups are the sort's tiles while the tile x bucket
+    // table stays small; larger systems take a tile pass of their own
+    BucketPlan bp = make_bucket_plan(N, cells, md_tile_items(n));
+    const bool frontTiles = !*bucket_plan_problem(bp);
+    if (!frontTiles) bp = make_bucket_plan(N, cells, 0);
     NW_REQUIRE(!*bucket_plan_problem(bp), "nwFusedMdStep: %s", bucket_plan_problem(bp));
     NW_HIP(ensure_bucket_scratch(s.bucket, bp));
 
@@ -468,8 +488,8 @@ void nwFusedMdStep(float *pos, float *vel, float *force, float *forceOld, float 
     a.force = reinterpret_cast<float4 *>(force);
     a.forceOld = reinterpret_cast<float4 *>(forceOld);
     a.tangent = reinterpret_cast<float4 *>(tangent);
-    a.hash = gridParticleHash;
-    a.pairs = s.bucket.pairs;
+    a.hash = gridParticleHash ? gridParticleHash : (frontTiles ? nullptr : s.bucket.hash);
+    a.pairs = frontTiles ? s.bucket.pairs : nullptr;
     a.P = s.bucket.P;
     a.bp = bp;
     a.dt = deltaTime;
@@ -484,6 +504,10 @@ void nwFusedMdStep(float *pos, float *vel, float *force, float *forceOld, float 
     {
         TimedScope ts("md_front", s.stream);
         NW_HIP(launch_md_front(a, kp, s.stream));
+    }
+    if (!frontTiles) {
+        TimedScope ts("md_tiles", s.stream);
+        NW_HIP(launch_hash_tiles(a.hash, bp, s.bucket, s.stream));
     }
     {
         TimedScope ts("md_sort", s.stream);
@@ -654,11 +678,178 @@ void nwObservables(const float *pos, const float *vel, uint numFilaments, NWObse
     }
 }
 
+// ---------------------------------------------------------------------------
+// batched replicas
+// ---------------------------------------------------------------------------
+static void batch_drop_graph(State::Batch &b) {
+    if (b.exec) (void)hipGraphExecDestroy(b.exec);
+    if (b.graph) (void)hipGraphDestroy(b.graph);
+    b.exec = nullptr;
+    b.graph = nullptr;
+    b.key.clear();
+}
+
+void nwBatchSetReplicas(const NWReplicaParams *reps, uint numReplicas) {
+    NW_INIT();
+    State &s = S();
+    State::Batch &b = s.batch;
+    NW_REQUIRE(numReplicas == 0 || reps != nullptr, "nwBatchSetReplicas: NULL replicas");
+    batch_drop_graph(b);
+    b.reps.assign(numReplicas, BatchRep{});
+    b.beadOff.assign(numReplicas + 1, 0u);
+    b.cellOff.assign(numReplicas + 1, 0u);
+    uint64_t N = 0, C = 0, waves = 0, blocks = 0;
+    for (uint r = 0; r < numReplicas; ++r) {
+        const NWReplicaParams &q = reps[r];
+        const SimParams &p = q.params;
+        const uint32_t n = p.filamentSize, W = p.numFilaments;
+        NW_REQUIRE(n >= 1 && n <= 64, "nwBatchSetReplicas: replica %u has worms of %u beads (1..64)", r, n);
+        const bool oneLayer = q.ext.stencil == NW_STENCIL_2D || p.gridSize.z == 1;
+        NW_REQUIRE(oneLayer && p.gridSize.x >= 3 && p.gridSize.y >= 1,
+                   "nwBatchSetReplicas: replica %u needs a one-layer stencil and gridSize.x >= 3", r);
+        NW_REQUIRE(q.ext.pairPotential == reps[0].ext.pairPotential,
+                   "nwBatchSetReplicas: replica %u: one pair potential per batch", r);
+        const uint64_t cells = uint64_t(p.gridSize.x) * p.gridSize.y * p.gridSize.z;
+        BatchRep &d = b.reps[r];
+        d.kp = make_kp_for(p, SolventParams{}, q.ext);
+        d.kp.p.numParticles = W * n;
+        d.beadOffset = uint32_t(N);
+        d.cellOffset = uint32_t(C);
+        d.waveOffset = uint32_t(waves);
+        d.blockOffset = uint32_t(blocks);
+        d.seed = q.seed;
+        d.rngBase = q.offset;
+        d.revBlocks = p.reverseProbability > 0.0f ? uint32_t(blocks_for(W)) : 0u;
+        d.rngStride = d.revBlocks + (q.langevin ? uint64_t(W) * n : 0u);
+        d.lgvStd = sqrtf(2.0f * q.langevinGamma * q.langevinKbT);
+        d.langevin = q.langevin ? 1 : 0;
+        d.numBeads = W * n;
+        N += uint64_t(W) * n;
+        C += cells;
+        waves += (uint64_t(W) + 64u / n - 1) / (64u / n);
+        blocks += (uint64_t(W) * n + kBlock - 1) / kBlock;
+        NW_REQUIRE(N < (1ull << 31) && C < (1ull << 32) - 2, "nwBatchSetReplicas: batch too large");
+        b.beadOff[r + 1] = uint32_t(N);
+        b.cellOff[r + 1] = uint32_t(C);
+        const bool wca = q.ext.pairPotential == NW_PAIR_WCA;
+        if (!wca || p.activity != 0.0f) b.tan = true;
+        if (!(q.ext.stencil == NW_STENCIL_2D && p.gridSize.y >= 3)) b.unit = false;
+    }
+    b.R = numReplicas;
+    b.N = uint32_t(N);
+    b.cells = uint32_t(C);
+    b.waves = uint32_t(waves);
+    b.blocks = uint32_t(blocks);
+    b.mode = numReplicas && reps[0].ext.pairPotential == NW_PAIR_WCA ? 1 : 0;
+    if (b.mode == 0) b.tan = true;
+    if (numReplicas == 0) b.tan = false, b.unit = true;
+    b.steps = 0;
+    if (b.dev) (void)hipFree(b.dev);
+    b.dev = nullptr;
+    if (numReplicas) {
+        NW_HIP(hipMalloc(&b.dev, sizeof(BatchRep) * numReplicas));
+        NW_HIP(hipMemcpy(b.dev, b.reps.data(), sizeof(BatchRep) * numReplicas, hipMemcpyHostToDevice));
+    }
+    if (!b.stepCtr) NW_HIP(hipMalloc(&b.stepCtr, sizeof(uint64_t)));
+    NW_HIP(hipMemset(b.stepCtr, 0, sizeof(uint64_t)));
+    if (N) {
+        const BucketPlan bp = make_bucket_plan(N, C, 0);
+        NW_REQUIRE(!*bucket_plan_problem(bp), "nwBatchSetReplicas: %s", bucket_plan_problem(bp));
+        NW_HIP(ensure_bucket_scratch(b.scratch, bp));
+    }
+}
+
+void nwBatchLayout(uint *beadOffset, uint *cellOffset) {
+    const State::Batch &b = S().batch;
+    for (uint r = 0; r <= b.R; ++r) {
+        if (beadOffset) beadOffset[r] = b.beadOff.empty() ? 0u : b.beadOff[r];
+        if (cellOffset) cellOffset[r] = b.cellOff.empty() ? 0u : b.cellOff[r];
+    }
+}
+
+void nwBatchRngState(uint replica, uint64_t *offset) {
+    const State::Batch &b = S().batch;
+    NW_REQUIRE(replica < b.R, "nwBatchRngState: replica %u of %u", replica, b.R);
+    if (offset) *offset = b.reps[replica].rngBase + b.steps * b.reps[replica].rngStride;
+}
+
+// the launches of one batched step
+static hipError_t batch_step(State::Batch &b, const BatchArrays &a, const float4 *vel, const float4 *tan,
+                             float4 *sVel, float4 *sTan, float dt, hipStream_t st) {
+    const BucketPlan bp = make_bucket_plan(b.N, b.cells, 0);
+    hipError_t e;
+    if ((e = launch_md_front_batch(a, b.dev, b.R, b.waves, b.stepCtr, dt, st)) != hipSuccess) return e;
+    if ((e = launch_hash_tiles(a.hash, bp, b.scratch, st)) != hipSuccess) return e;
+    if ((e = launch_md_cells(bp, b.scratch, a.ids, a.pos, vel, tan, a.sPos, sVel, sTan, st)) != hipSuccess) return e;
+    return launch_collide_batch(a, b.dev, b.R, b.blocks, b.mode, b.tan, b.unit, b.stepCtr, st);
+}
+
+void nwBatchMdStep(float *pos, float *vel, float *force, float *forceOld, float *tangent, uint *gridParticleIndex,
+                   float *sortedPos, float *sortedVel, float *sortedTangent, float deltaTime, uint steps,
+                   int useGraph) {
+    NW_INIT();
+    State &s = S();
+    State::Batch &b = s.batch;
+    if (b.N == 0 || steps == 0) return;
+    NW_REQUIRE(pos && vel && force && forceOld && tangent && gridParticleIndex && sortedPos,
+               "nwBatchMdStep: NULL array");
+    NW_REQUIRE(b.mode == 1 || sortedVel != nullptr, "nwBatchMdStep: sortedVel is NULL");
+    NW_REQUIRE(!b.tan || sortedTangent != nullptr, "nwBatchMdStep: sortedTangent is NULL");
+    BatchArrays a;
+    a.pos = reinterpret_cast<float4 *>(pos);
+    a.vel = reinterpret_cast<float4 *>(vel);
+    a.force = reinterpret_cast<float4 *>(force);
+    a.forceOld = reinterpret_cast<float4 *>(forceOld);
+    a.tangent = reinterpret_cast<float4 *>(tangent);
+    a.hash = b.scratch.hash;
+    a.ids = gridParticleIndex;
+    a.off = b.scratch.off;
+    a.sCell = b.scratch.sortedCell;
+    a.sPos = reinterpret_cast<float4 *>(sortedPos);
+    a.sVel = b.mode == 0 ? reinterpret_cast<float4 *>(sortedVel) : nullptr;
+    a.sTan = b.tan ? reinterpret_cast<float4 *>(sortedTangent) : nullptr;
+    const float4 *gv = b.mode == 0 ? a.vel : nullptr;
+    const float4 *gt = b.tan ? a.tangent : nullptr;
+    uint32_t dtBits;
+    std::memcpy(&dtBits, &deltaTime, 4);
+    const std::vector<uintptr_t> key = {uintptr_t(pos), uintptr_t(vel), uintptr_t(force), uintptr_t(forceOld),
+                                        uintptr_t(tangent), uintptr_t(gridParticleIndex), uintptr_t(sortedPos),
+                                        uintptr_t(sortedVel), uintptr_t(sortedTangent), uintptr_t(dtBits)};
+    uint done = 0;
+    if (useGraph && (!b.exec || b.key != key)) {
+        // one direct step first (scratch and kernel attributes are set up
+        // outside the capture), then capture a step
+        batch_drop_graph(b);
+        {
+            TimedScope ts("batch_step", s.stream);
+            NW_HIP(batch_step(b, a, gv, gt, a.sVel, a.sTan, deltaTime, s.stream));
+        }
+        ++done;
+        ++b.steps;
+        if (done == steps) return;
+        NW_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t e1 = batch_step(b, a, gv, gt, a.sVel, a.sTan, deltaTime, s.stream);
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(s.stream, &g);
+        NW_HIP(e1);
+        NW_HIP(e2);
+        b.graph = g;
+        NW_HIP(hipGraphInstantiate(&b.exec, g, nullptr, nullptr, 0));
+        b.key = key;
+    }
+    for (; done < steps; ++done) {
+        TimedScope ts("batch_step", s.stream);
+        if (useGraph) NW_HIP(hipGraphLaunch(b.exec, s.stream));
+        else NW_HIP(batch_step(b, a, gv, gt, a.sVel, a.sTan, deltaTime, s.stream));
+        ++b.steps;
+    }
+}
+
 int nwFusedPlanFits(uint numParticles, uint numCells, uint filamentSize) {
     if (filamentSize > 64 || numCells == 0) return 0;
-    const BucketPlan bp =
-        make_bucket_plan(numParticles, numCells, filamentSize ? md_tile_items(filamentSize) : 0u);
-    return *bucket_plan_problem(bp) ? 0 : 1;
+    if (filamentSize && !*bucket_plan_problem(make_bucket_plan(numParticles, numCells, md_tile_items(filamentSize))))
+        return 1;
+    return *bucket_plan_problem(make_bucket_plan(numParticles, numCells, 0)) ? 0 : 1;
 }
 
 uint nwGetDeviceFlags(int clear) {

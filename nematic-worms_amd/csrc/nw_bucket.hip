@@ -5,7 +5,7 @@
 // G/s measured), so a histogram with one global counter per cell costs more
 // than the data it sorts, and a global scan over a tile x bucket matrix costs
 // launches.  Here every counter lives in LDS and no global scan is needed:
-//   level 1   (k_srd_tiles, and the MD front kernel in nw_fused.hip) a tile
+//   level 1   (k_tiles, or the MD front kernel in nw_fused.hip) a tile
 //             of consecutive particles counts its members per bucket in LDS,
 //             scans the counts and writes the prefix as one row of P, then
 //             its (cell, id) pairs grouped by bucket into its pair region
@@ -97,11 +97,14 @@ __device__ __forceinline__ uint32_t xcd_contiguous(uint32_t i, uint32_t G) {
 }
 
 // ---------------------------------------------------------------------------
-// level 1 (SRD): cell of every particle, tile histogram, P row, pairs
+// level 1 as a kernel of its own: the cell of every particle (SRD: from the
+// position, written to hash; otherwise read from hash), tile histogram, P
+// row, pairs
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kTileBlock) void k_srd_tiles(const float4 *__restrict__ pos, uint32_t *__restrict__ hash,
-                                                          uint2 *__restrict__ pairs, uint32_t *__restrict__ P,
-                                                          BucketPlan bp, KP kp) {
+template <bool FROM_POS>
+__global__ __launch_bounds__(kTileBlock) void k_tiles(const float4 *__restrict__ pos, uint32_t *__restrict__ hash,
+                                                      uint2 *__restrict__ pairs, uint32_t *__restrict__ P,
+                                                      BucketPlan bp, KP kp) {
     extern __shared__ uint32_t lh[];
     uint32_t *wsum = lh + bp.buckets + 1;
     const uint32_t B1 = bp.buckets + 1;
@@ -111,8 +114,13 @@ __global__ __launch_bounds__(kTileBlock) void k_srd_tiles(const float4 *__restri
     const uint32_t i0 = t * bp.tileItems;
     const uint32_t i1 = min(i0 + bp.tileItems, bp.n);
     for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        const uint32_t h = srd_hash(kp.p, kp.s, mk(pos[i]));
-        hash[i] = h;
+        uint32_t h;
+        if (FROM_POS) {
+            h = srd_hash(kp.p, kp.s, mk(pos[i]));
+            hash[i] = h;
+        } else {
+            h = hash[i];
+        }
         atomicAdd(&lh[h >> bp.shift], 1u);
     }
     __syncthreads();
@@ -120,7 +128,7 @@ __global__ __launch_bounds__(kTileBlock) void k_srd_tiles(const float4 *__restri
     uint32_t *row = P + uint64_t(t) * B1;
     for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) row[b] = lh[b];
     __syncthreads();
-    // second pass over the same particles (each thread re-reads its own hash)
+    // second pass over the same particles (each thread re-reads its own cells)
     for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         const uint32_t h = hash[i];
         const uint32_t slot = atomicAdd(&lh[h >> bp.shift], 1u);
@@ -390,12 +398,35 @@ hipError_t ensure_bucket_scratch(BucketScratch &s, const BucketPlan &bp) {
     return hipSuccess;
 }
 
-hipError_t launch_srd_tiles(const float4 *pos, uint32_t *hash, const BucketPlan &bp, BucketScratch &s,
-                            const KP &kp, hipStream_t st) {
+static hipError_t launch_tiles(bool fromPos, const float4 *pos, uint32_t *hash, const BucketPlan &bp,
+                               BucketScratch &s, const KP &kp, hipStream_t st) {
     if (bp.n == 0) return hipSuccess;
     const size_t lds = (size_t(bp.buckets) + 1 + kTileBlock / 64) * sizeof(uint32_t);
-    hipLaunchKernelGGL(k_srd_tiles, dim3(bp.tiles), dim3(kTileBlock), lds, st, pos, hash, s.pairs, s.P, bp, kp);
+    const void *fn = fromPos ? reinterpret_cast<const void *>(&k_tiles<true>)
+                             : reinterpret_cast<const void *>(&k_tiles<false>);
+    // LDS beyond 64 KiB is granted per kernel, once (outside any graph capture:
+    // the first launch of a size runs directly)
+    static size_t granted[2] = {65536, 65536};
+    if (lds > granted[fromPos ? 1 : 0]) {
+        const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        if (e != hipSuccess) return e;
+        granted[fromPos ? 1 : 0] = lds;
+    }
+    if (fromPos)
+        hipLaunchKernelGGL(k_tiles<true>, dim3(bp.tiles), dim3(kTileBlock), lds, st, pos, hash, s.pairs, s.P, bp, kp);
+    else
+        hipLaunchKernelGGL(k_tiles<false>, dim3(bp.tiles), dim3(kTileBlock), lds, st, pos, hash, s.pairs, s.P, bp,
+                           kp);
     return hipGetLastError();
+}
+
+hipError_t launch_srd_tiles(const float4 *pos, uint32_t *hash, const BucketPlan &bp, BucketScratch &s,
+                            const KP &kp, hipStream_t st) {
+    return launch_tiles(true, pos, hash, bp, s, kp, st);
+}
+
+hipError_t launch_hash_tiles(const uint32_t *hash, const BucketPlan &bp, BucketScratch &s, hipStream_t st) {
+    return launch_tiles(false, nullptr, const_cast<uint32_t *>(hash), bp, s, KP{}, st);
 }
 
 hipError_t launch_md_cells(const BucketPlan &bp, BucketScratch &s, uint32_t *ids, const float4 *pos,

@@ -24,7 +24,7 @@ struct BucketPlan {
 
 constexpr uint32_t kBucketCap = 2048;     // bucket members staged in LDS (more: global-memory path)
 constexpr uint32_t kMaxTiles = 4096;      // level-2 segment tables in LDS
-constexpr uint32_t kMaxBuckets = 16383;   // level-1 LDS histogram
+constexpr uint32_t kMaxBuckets = 32767;   // level-1 LDS histogram (128 KiB of the 160 KiB)
 constexpr uint64_t kMaxMatrix = 1ull << 24;  // entries of P
 constexpr int kCellsBlock = 256;          // level-2 workgroup
 
@@ -51,8 +51,8 @@ inline uint32_t md_tile_items(uint32_t filamentSize) { return 16u * (64u / filam
 
 struct MdFrontArgs {
     float4 *pos, *vel, *force, *forceOld, *tangent;
-    uint32_t *hash;  // may be NULL
-    uint2 *pairs;
+    uint32_t *hash;  // may be NULL when the front is the tile pass
+    uint2 *pairs;    // NULL: no tile pass in the front (cells to hash)
     uint32_t *P;
     BucketPlan bp;
     float dt, lgvStd;
@@ -61,8 +61,18 @@ struct MdFrontArgs {
     uint64_t seed, offRev, offLgv;
 };
 
-// integrate + reversal + Langevin + bonded + tangent + MD cell + level 1
+// what one lane of the fused front produces for its bead
+struct FrontLane {
+    f3 r, v, acc, fold, tg;
+    float pw, vw, fow;
+    bool writeTan;
+};
+
+// integrate + reversal + Langevin + bonded + tangent + MD cell (+ level 1
+// when a.pairs is given: the workgroups of 1024 threads are the tiles)
 hipError_t launch_md_front(const MdFrontArgs &a, const KP &kp, hipStream_t st);
+// level 1 over precomputed cells (any tile size)
+hipError_t launch_hash_tiles(const uint32_t *hash, const BucketPlan &bp, BucketScratch &s, hipStream_t st);
 // level 2 for MD: ids in stable cell order, CSR offsets, cell of every
 // slot, gathered records (sVel / sTan may be NULL)
 hipError_t launch_md_cells(const BucketPlan &bp, BucketScratch &s, uint32_t *ids, const float4 *pos,
@@ -83,6 +93,39 @@ hipError_t launch_srd_rotate(const uint32_t *hash, const float4 *com, const floa
 hipError_t launch_collide_off(float4 *force, const float4 *sPos, const float4 *sVel, const float4 *sTan,
                               const uint32_t *sIdx, const uint32_t *off, const uint32_t *sortedCell, uint32_t n,
                               const KP &kp, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// batched replicas (the fused MD step over many independent systems)
+// ---------------------------------------------------------------------------
+struct BatchRep {
+    KP kp;                 // the replica's parameters (box, grid, worms, force constants)
+    uint32_t beadOffset;   // first bead in the concatenated arrays
+    uint32_t cellOffset;   // first cell in the concatenated cell space
+    uint32_t waveOffset;   // first wavefront of the front kernel
+    uint32_t blockOffset;  // first workgroup of the pair kernel
+    uint64_t seed;         // Philox key
+    uint64_t rngBase;      // generator offset at step 0 of the batch
+    uint64_t rngStride;    // blocks drawn per step (reversal + Langevin)
+    uint32_t revBlocks;    // reversal blocks per step (0 without reversal)
+    float lgvStd;          // sqrt(2 gamma kT) of the Langevin noise
+    int32_t langevin;
+    uint32_t numBeads;     // numFilaments x filamentSize
+};
+
+struct BatchArrays {
+    float4 *pos, *vel, *force, *forceOld, *tangent;
+    uint32_t *hash;   // global cell of every bead
+    uint32_t *ids;    // bead ids in cell order
+    uint32_t *off;    // CSR offsets over the concatenated cells
+    uint32_t *sCell;  // cell of every sorted slot
+    float4 *sPos, *sVel, *sTan;
+};
+
+hipError_t launch_md_front_batch(const BatchArrays &a, const BatchRep *reps, uint32_t R, uint32_t waves,
+                                 const uint64_t *stepCtr, float dt, hipStream_t st);
+// the last kernel of a batched step: advances *stepCtr
+hipError_t launch_collide_batch(const BatchArrays &a, const BatchRep *reps, uint32_t R, uint32_t blocks, int mode,
+                                bool tan, bool unit, uint64_t *stepCtr, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // block-level helpers shared by the level-1 kernels (whole workgroup calls)

@@ -40,30 +40,29 @@ __device__ __forceinline__ void bead_cell(const KP &kp, const uint32_t *__restri
 // ===========================================================================
 // MD front
 // ===========================================================================
-constexpr int kFrontBlock = 1024;  // one level-1 tile of the cell sort
-
-__global__ __launch_bounds__(kFrontBlock) void k_md_front(MdFrontArgs a, KP kp) {
-    extern __shared__ uint32_t lh[];  // bucket histogram (buckets + 1), wave sums
+// One bead's share of the front: velocity Verlet, polarity reversal, Langevin
+// and the bonded terms on the new positions (all lanes of the wave call it:
+// chain neighbours and mirror beads arrive by shuffles).  Arrays are those
+// of the bead's system (a replica's slice in the batched step).
+__device__ __forceinline__ FrontLane md_front_lane(const KP &kp, const WaveWorm &m, const float4 *__restrict__ pos,
+                                                   const float4 *__restrict__ vel, const float4 *__restrict__ force,
+                                                   const float4 *__restrict__ forceOld, float dt, float lgvStd,
+                                                   int langevin, uint64_t seed, uint64_t offRev, uint64_t offLgv) {
     const SimParams &p = kp.p;
     const uint32_t n = p.filamentSize;
-    const BucketPlan &bp = a.bp;
-    const uint32_t B1 = bp.buckets + 1;
-    uint32_t *wsum = lh + B1;
-    for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) lh[b] = 0u;
-
-    const WaveWorm m = wave_worm(n, a.W);
     const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 P = m.active ? a.pos[m.bead] : zero4;
-    const float4 V = m.active ? a.vel[m.bead] : zero4;
-    const float4 F = m.active ? a.force[m.bead] : zero4;
-    const float4 FO = m.active ? a.forceOld[m.bead] : zero4;
+    const float4 P = m.active ? pos[m.bead] : zero4;
+    const float4 V = m.active ? vel[m.bead] : zero4;
+    const float4 F = m.active ? force[m.bead] : zero4;
+    const float4 FO = m.active ? forceOld[m.bead] : zero4;
 
     // velocity Verlet (filament_integrator, particles_kernel_impl.cuh:71-142)
+    FrontLane o;
     f3 r = mk(P), v = mk(V);
     const f3 ff = mk(F), fo = mk(FO);
-    const f3 dv = 0.5f * (ff + fo) * a.dt;
+    const f3 dv = 0.5f * (ff + fo) * dt;
     v += dv;
-    const f3 dr = v * a.dt + 0.5f * ff * a.dt * a.dt;
+    const f3 dr = v * dt + 0.5f * ff * dt * dt;
     r += dr;
     apply_boundaries(p, r, v, dr, p.particleRadius);
     f3 fold = ff;
@@ -72,7 +71,7 @@ __global__ __launch_bounds__(kFrontBlock) void k_md_front(MdFrontArgs a, KP kp) 
     // polarity reversal (reverseFilamentsKernel :820-857): bead p <-> n-1-p
     // of the worm, taken from the mirror lane; the force is zero here
     if (p.reverseProbability > 0.0f) {
-        const bool rev = m.active && !(uniform_at(a.seed, a.offRev, m.worm) > p.reverseProbability);
+        const bool rev = m.active && !(uniform_at(seed, offRev, m.worm) > p.reverseProbability);
         const int src = rev ? int(m.wl * n + (n - 1u - m.pp)) : int(m.lane);
         r = mk(__shfl(r.x, src, 64), __shfl(r.y, src, 64), __shfl(r.z, src, 64));
         v = mk(__shfl(v.x, src, 64), __shfl(v.y, src, 64), __shfl(v.z, src, 64));
@@ -84,13 +83,13 @@ __global__ __launch_bounds__(kFrontBlock) void k_md_front(MdFrontArgs a, KP kp) 
 
     // Langevin (langevinKernel :859-874), the force starts from zero
     f3 acc = mk(0.f, 0.f, 0.f);
-    if (a.langevin) {
-        const u4 b = philox_block(a.seed, a.offLgv + m.bead);
+    if (langevin) {
+        const u4 b = philox_block(seed, offLgv + m.bead);
         float z0, z1, z2, z3;
         box_muller(b.x, b.y, z0, z1);
         box_muller(b.z, b.w, z2, z3);
         (void)z3;
-        const f3 xi = mk(z0 * a.lgvStd + 0.0f, z1 * a.lgvStd + 0.0f, z2 * a.lgvStd + 0.0f);
+        const f3 xi = mk(z0 * lgvStd + 0.0f, z1 * lgvStd + 0.0f, z2 * lgvStd + 0.0f);
         f3 f = mk(0.f, 0.f, 0.f);
         f -= p.gamma * v;
         f += xi;
@@ -98,25 +97,58 @@ __global__ __launch_bounds__(kFrontBlock) void k_md_front(MdFrontArgs a, KP kp) 
     }
 
     // bonded forces and tangents on the new positions
-    f3 tg;
-    const bool writeTan = bonded_lane(kp, r, m.pp, acc, tg);
+    o.writeTan = bonded_lane(kp, r, m.pp, acc, o.tg);
+    o.r = r;
+    o.v = v;
+    o.acc = acc;
+    o.fold = fold;
+    o.pw = pw;
+    o.vw = vw;
+    o.fow = fow;
+    return o;
+}
 
+// stores of an active lane; returns the bead's cell (calcHashD)
+__device__ __forceinline__ uint32_t md_front_store(const KP &kp, const WaveWorm &m, const FrontLane &o,
+                                                   float4 *__restrict__ pos, float4 *__restrict__ vel,
+                                                   float4 *__restrict__ force, float4 *__restrict__ forceOld,
+                                                   float4 *__restrict__ tangent) {
+    const SimParams &p = kp.p;
+    pos[m.bead] = f4(o.r, o.pw);
+    vel[m.bead] = f4(o.v, o.vw);
+    force[m.bead] = f4(o.acc, 0.0f);
+    forceOld[m.bead] = f4(o.fold, o.fow);
+    if (o.writeTan) tangent[m.bead] = f4(o.tg, 0.0f);
+    return md_hash_wrapped(p, cell_coord(o.r.x, p.origin.x, p.cellSize.x), cell_coord(o.r.y, p.origin.y, p.cellSize.y),
+                           cell_coord(o.r.z, p.origin.z, p.cellSize.z));
+}
+
+constexpr int kFrontBlock = 1024;  // one level-1 tile of the cell sort
+
+// TILE: the workgroup is a tile of the cell sort (1024 threads, histogram in
+// LDS); otherwise the cells go to a.hash for the separate tile pass
+template <bool TILE>
+__global__ __launch_bounds__(TILE ? kFrontBlock : kBlock) void k_md_front(MdFrontArgs a, KP kp) {
+    extern __shared__ uint32_t lh[];  // bucket histogram (buckets + 1), wave sums
+    const BucketPlan &bp = a.bp;
+    const uint32_t B1 = bp.buckets + 1;
+    if (TILE)
+        for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) lh[b] = 0u;
+
+    const WaveWorm m = wave_worm(kp.p.filamentSize, a.W);
+    const FrontLane o = md_front_lane(kp, m, a.pos, a.vel, a.force, a.forceOld, a.dt, a.lgvStd, a.langevin, a.seed,
+                                      a.offRev, a.offLgv);
     uint32_t h = 0, rank = 0;
-    __syncthreads();  // histogram cleared
+    if (TILE) __syncthreads();  // histogram cleared
     if (m.active) {
-        a.pos[m.bead] = f4(r, pw);
-        a.vel[m.bead] = f4(v, vw);
-        a.force[m.bead] = f4(acc, 0.0f);
-        a.forceOld[m.bead] = f4(fold, fow);
-        if (writeTan) a.tangent[m.bead] = f4(tg, 0.0f);
-        // cell of the new position (calcHashD)
-        h = md_hash_wrapped(p, cell_coord(r.x, p.origin.x, p.cellSize.x), cell_coord(r.y, p.origin.y, p.cellSize.y),
-                            cell_coord(r.z, p.origin.z, p.cellSize.z));
+        h = md_front_store(kp, m, o, a.pos, a.vel, a.force, a.forceOld, a.tangent);
         if (a.hash) a.hash[m.bead] = h;
-        rank = atomicAdd(&lh[h >> bp.shift], 1u);
+        if (TILE) rank = atomicAdd(&lh[h >> bp.shift], 1u);
     }
+    if (!TILE) return;
     __syncthreads();
     // level 1 of the cell sort: this workgroup's beads are tile blockIdx.x
+    uint32_t *wsum = lh + B1;
     block_excl_scan(lh, bp.buckets, wsum);
     uint32_t *row = a.P + uint64_t(blockIdx.x) * B1;
     for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) row[b] = lh[b];
@@ -165,14 +197,16 @@ __device__ __forceinline__ void add_pair(const KP &kp, const Owner<MODE, TAN> &o
     }
 }
 
+// beadBase: the first bead id of the system (a replica's offset in the
+// batched step; worm and chain indices are the system's own)
 template <int MODE, bool TAN>
 __device__ __forceinline__ void add_run_global(const KP &kp, const Owner<MODE, TAN> &o, float rc2, float w, f3 &acc,
                                                uint32_t a, uint32_t b, const float4 *__restrict__ sPos,
                                                const float4 *__restrict__ sVel, const float4 *__restrict__ sTan,
-                                               const uint32_t *__restrict__ sIdx) {
+                                               const uint32_t *__restrict__ sIdx, uint32_t beadBase) {
     for (uint32_t j = a; j < b; ++j) {
         uint32_t fil2, ch2;
-        worm_of(kp, sIdx[j], fil2, ch2);
+        worm_of(kp, sIdx[j] - beadBase, fil2, ch2);
         add_pair<MODE, TAN>(kp, o, rc2, w, acc, mk(sPos[j]), fil2, ch2, MODE == 0 ? mk(sVel[j]) : mk(0.f, 0.f, 0.f),
                             TAN ? mk(sTan[j]) : mk(0.f, 0.f, 0.f));
     }
@@ -196,12 +230,14 @@ __device__ __forceinline__ void row_runs(const uint32_t *__restrict__ off, uint3
     }
 }
 
-// every candidate of the owner from global memory, in stencil order
+// every candidate of the owner from global memory, in stencil order (off:
+// the CSR offsets of the system's cells, indexed by its own cell ids)
 template <int MODE, bool TAN>
 __device__ __forceinline__ void add_all_global(const KP &kp, const Owner<MODE, TAN> &o, float rc2, f3 &acc, int gx,
                                                int gy, int gz, const uint32_t *__restrict__ off,
                                                const float4 *__restrict__ sPos, const float4 *__restrict__ sVel,
-                                               const float4 *__restrict__ sTan, const uint32_t *__restrict__ sIdx) {
+                                               const float4 *__restrict__ sTan, const uint32_t *__restrict__ sIdx,
+                                               uint32_t beadBase = 0) {
     const SimParams &p = kp.p;
     const uint32_t Gx = p.gridSize.x;
     const AxisCells ay = axis_cells(gy, p.gridSize.y, false);
@@ -217,8 +253,8 @@ __device__ __forceinline__ void add_all_global(const KP &kp, const Owner<MODE, T
             if (Gx >= 3) {
                 uint32_t a0, a1, b0, b1;
                 row_runs(off, base, gx, Gx, a0, a1, b0, b1);
-                add_run_global<MODE, TAN>(kp, o, rc2, wzy, acc, a0, a1, sPos, sVel, sTan, sIdx);
-                add_run_global<MODE, TAN>(kp, o, rc2, wzy, acc, b0, b1, sPos, sVel, sTan, sIdx);
+                add_run_global<MODE, TAN>(kp, o, rc2, wzy, acc, a0, a1, sPos, sVel, sTan, sIdx, beadBase);
+                add_run_global<MODE, TAN>(kp, o, rc2, wzy, acc, b0, b1, sPos, sVel, sTan, sIdx, beadBase);
             } else {
                 const AxisCells ax = axis_cells(gx, Gx, false);
 #pragma unroll
@@ -226,7 +262,7 @@ __device__ __forceinline__ void add_all_global(const KP &kp, const Owner<MODE, T
                     if (xi >= ax.n) break;
                     const uint32_t h = base + uint32_t(ax.c[xi]);
                     add_run_global<MODE, TAN>(kp, o, rc2, wzy * ax.w[xi], acc, off[h], off[h + 1], sPos, sVel,
-                                              sTan, sIdx);
+                                              sTan, sIdx, beadBase);
                 }
             }
         }
@@ -287,61 +323,67 @@ __global__ __launch_bounds__(kBlock) void k_collide_csr(float4 *__restrict__ for
 // ===========================================================================
 constexpr int kRunCap = 384;  // staged beads per row run
 
-template <int MODE, bool TAN, bool UNIT>
-__global__ __launch_bounds__(kBlock) void k_collide_rows(float4 *__restrict__ force, const float4 *__restrict__ sPos,
-                                                         const float4 *__restrict__ sVel,
-                                                         const float4 *__restrict__ sTan,
-                                                         const uint32_t *__restrict__ sIdx,
-                                                         const uint32_t *__restrict__ off,
-                                                         const uint32_t *__restrict__ sCell, uint32_t n, KP kp) {
-    __shared__ float4 lpos[3 * kRunCap];
-    __shared__ float4 lvel[MODE == 0 ? 3 * kRunCap : 1];
-    __shared__ float4 ltan[TAN ? 3 * kRunCap : 1];
-    __shared__ uint32_t rs[3], rl[3];
+// LDS of one pair-kernel workgroup
+template <int MODE, bool TAN>
+struct RowsLds {
+    float4 pos[3 * kRunCap];
+    float4 vel[MODE == 0 ? 3 * kRunCap : 1];
+    float4 tan[TAN ? 3 * kRunCap : 1];
+    uint32_t rs[3], rl[3];
+};
 
+// The workgroup's sorted beads [i0, min(i0 + blockDim, iEnd)) of one system
+// whose beads and cells start at beadBase / cellBase in the shared arrays
+// (zero for a single system); offR = off + cellBase.
+template <int MODE, bool TAN, bool UNIT>
+__device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uint32_t iEnd, uint32_t beadBase,
+                                                  uint32_t cellBase, float4 *__restrict__ force,
+                                                  const float4 *__restrict__ sPos, const float4 *__restrict__ sVel,
+                                                  const float4 *__restrict__ sTan, const uint32_t *__restrict__ sIdx,
+                                                  const uint32_t *__restrict__ offR,
+                                                  const uint32_t *__restrict__ sCell, RowsLds<MODE, TAN> &L) {
     const SimParams &p = kp.p;
     const uint32_t Gx = p.gridSize.x, Gy = p.gridSize.y;
     const uint32_t layer = Gx * Gy;
     const uint32_t C = layer * p.gridSize.z;
-    const uint32_t i0 = blockIdx.x * blockDim.x;
     const uint32_t i = i0 + threadIdx.x;
 
     if (threadIdx.x < 3) {
-        const uint32_t last = min(i0 + blockDim.x, n) - 1u;
-        const uint32_t h0 = sCell[i0], h1 = sCell[last];
+        const uint32_t last = min(i0 + blockDim.x, iEnd) - 1u;
+        const uint32_t h0 = sCell[i0] - cellBase, h1 = sCell[last] - cellBase;
         const int dy = int(threadIdx.x) - 1;
         const int64_t lo = int64_t(h0) - 1 + int64_t(dy) * Gx;
         const int64_t hi = int64_t(h1) + 2 + int64_t(dy) * Gx;  // exclusive
         const uint32_t clo = uint32_t(lo < 0 ? 0 : (lo > int64_t(C) ? C : lo));
         const uint32_t chi = uint32_t(hi < 0 ? 0 : (hi > int64_t(C) ? C : hi));
-        const uint32_t s = off[clo], e = off[chi];
-        rs[threadIdx.x] = s;
-        rl[threadIdx.x] = (e > s && e - s <= uint32_t(kRunCap)) ? e - s : 0u;
+        const uint32_t s = offR[clo], e = offR[chi];
+        L.rs[threadIdx.x] = s;
+        L.rl[threadIdx.x] = (e > s && e - s <= uint32_t(kRunCap)) ? e - s : 0u;
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-        const uint32_t s = rs[r], l = rl[r];
+        const uint32_t s = L.rs[r], l = L.rl[r];
         for (uint32_t k = threadIdx.x; k < l; k += blockDim.x) {
             const uint32_t j = s + k;
             const float4 P = sPos[j];
             uint32_t fil, ch;
-            worm_of(kp, sIdx[j], fil, ch);
-            lpos[r * kRunCap + k] = make_float4(P.x, P.y, P.z, __uint_as_float(fil));
+            worm_of(kp, sIdx[j] - beadBase, fil, ch);
+            L.pos[r * kRunCap + k] = make_float4(P.x, P.y, P.z, __uint_as_float(fil));
             if (MODE == 0) {
                 const float4 V = sVel[j];
-                lvel[r * kRunCap + k] = make_float4(V.x, V.y, V.z, __uint_as_float(ch));
+                L.vel[r * kRunCap + k] = make_float4(V.x, V.y, V.z, __uint_as_float(ch));
             }
-            if (TAN) ltan[r * kRunCap + k] = sTan[j];
+            if (TAN) L.tan[r * kRunCap + k] = sTan[j];
         }
     }
     __syncthreads();
-    if (i >= n) return;
+    if (i >= iEnd) return;
 
     const uint32_t me = sIdx[i];
-    const Owner<MODE, TAN> o = load_owner<MODE, TAN>(kp, i, me, sPos, sVel, sTan);
+    const Owner<MODE, TAN> o = load_owner<MODE, TAN>(kp, i, me - beadBase, sPos, sVel, sTan);
     const float rc2 = kp.e.wcaCutoff * kp.e.wcaCutoff;
-    const uint32_t c = sCell[i];
+    const uint32_t c = sCell[i] - cellBase;
     const uint32_t gz = c / layer, rem = c - gz * layer;
     const uint32_t gyu = rem / Gx;
     const int gx = int(rem - gyu * Gx), gy = int(gyu);
@@ -350,23 +392,23 @@ __global__ __launch_bounds__(kBlock) void k_collide_rows(float4 *__restrict__ fo
 
     // LDS start and length of every row's run; any row outside the stage
     // sends the bead down the global walk
-    uint32_t q[3] = {0u, 0u, 0u}, L[3] = {0u, 0u, 0u};
+    uint32_t q[3] = {0u, 0u, 0u}, Lr[3] = {0u, 0u, 0u};
     float w[3] = {0.f, 0.f, 0.f};
     bool fast = gx >= 1 && gx + 1 < int(Gx);
 #pragma unroll
     for (int yi = 0; yi < 3; ++yi) {
         if (yi < ay.n && fast) {
             const uint32_t base = (gz * Gy + uint32_t(ay.c[yi])) * Gx;
-            const uint32_t a = off[base + gx - 1], b = off[base + gx + 2];
+            const uint32_t a = offR[base + gx - 1], b = offR[base + gx + 2];
             bool hit = a == b;
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
-                if (!hit && rl[r] && a >= rs[r] && b <= rs[r] + rl[r]) {
-                    q[yi] = uint32_t(r) * kRunCap + (a - rs[r]);
+                if (!hit && L.rl[r] && a >= L.rs[r] && b <= L.rs[r] + L.rl[r]) {
+                    q[yi] = uint32_t(r) * kRunCap + (a - L.rs[r]);
                     hit = true;
                 }
             }
-            L[yi] = b - a;
+            Lr[yi] = b - a;
             w[yi] = wz * ay.w[yi];
             fast = hit;
         }
@@ -374,27 +416,103 @@ __global__ __launch_bounds__(kBlock) void k_collide_rows(float4 *__restrict__ fo
 
     f3 acc = mk(0.f, 0.f, 0.f);
     if (fast) {
-        const uint32_t L01 = L[0] + L[1], K = L01 + L[2];
-        const uint32_t d0 = q[0], d1 = q[1] - L[0], d2 = q[2] - L01;  // slot = t + d (mod 2^32)
+        const uint32_t L01 = Lr[0] + Lr[1], K = L01 + Lr[2];
+        const uint32_t d0 = q[0], d1 = q[1] - Lr[0], d2 = q[2] - L01;  // slot = t + d (mod 2^32)
         for (uint32_t t = 0; t < K; ++t) {
-            const bool in0 = t < L[0], in1 = t < L01;
+            const bool in0 = t < Lr[0], in1 = t < L01;
             const uint32_t s = t + (in0 ? d0 : (in1 ? d1 : d2));
             const float wt = UNIT ? 1.0f : (in0 ? w[0] : (in1 ? w[1] : w[2]));
-            const float4 P = lpos[s];
+            const float4 P = L.pos[s];
             f3 v2 = mk(0.f, 0.f, 0.f);
             uint32_t ch2 = 0u;
             if (MODE == 0) {
-                const float4 V = lvel[s];
+                const float4 V = L.vel[s];
                 v2 = mk(V);
                 ch2 = __float_as_uint(V.w);
             }
             add_pair<MODE, TAN, UNIT>(kp, o, rc2, wt, acc, mk(P), __float_as_uint(P.w), ch2, v2,
-                                      TAN ? mk(ltan[s]) : mk(0.f, 0.f, 0.f));
+                                      TAN ? mk(L.tan[s]) : mk(0.f, 0.f, 0.f));
         }
     } else {
-        add_all_global<MODE, TAN>(kp, o, rc2, acc, gx, gy, int(gz), off, sPos, sVel, sTan, sIdx);
+        add_all_global<MODE, TAN>(kp, o, rc2, acc, gx, gy, int(gz), offR, sPos, sVel, sTan, sIdx, beadBase);
     }
     add_force(force, me, acc);
+}
+
+template <int MODE, bool TAN, bool UNIT>
+__global__ __launch_bounds__(kBlock) void k_collide_rows(float4 *__restrict__ force, const float4 *__restrict__ sPos,
+                                                         const float4 *__restrict__ sVel,
+                                                         const float4 *__restrict__ sTan,
+                                                         const uint32_t *__restrict__ sIdx,
+                                                         const uint32_t *__restrict__ off,
+                                                         const uint32_t *__restrict__ sCell, uint32_t n, KP kp) {
+    __shared__ RowsLds<MODE, TAN> L;
+    collide_rows_tile<MODE, TAN, UNIT>(kp, blockIdx.x * blockDim.x, n, 0u, 0u, force, sPos, sVel, sTan, sIdx, off,
+                                       sCell, L);
+}
+
+// ===========================================================================
+// batched replicas: every replica keeps its own parameters, grid, worms and
+// Philox stream; beads, cells, front wavefronts and pair workgroups of the
+// replicas are concatenated, so one launch of each kernel steps them all.
+// A wavefront / workgroup belongs to one replica (found by bisection over
+// the descriptor table, wave-uniform, so the replica's parameters arrive by
+// scalar loads).
+// ===========================================================================
+__device__ __forceinline__ uint32_t replica_of_wave(const BatchRep *__restrict__ reps, uint32_t R, uint32_t wave) {
+    uint32_t lo = 0, hi = R;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (reps[mid].waveOffset <= wave) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t replica_of_block(const BatchRep *__restrict__ reps, uint32_t R, uint32_t blk) {
+    uint32_t lo = 0, hi = R;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (reps[mid].blockOffset <= blk) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_md_front_batch(BatchArrays a, const BatchRep *__restrict__ reps,
+                                                           uint32_t R, uint32_t waves,
+                                                           const uint64_t *__restrict__ stepCtr, float dt) {
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (wave >= waves) return;
+    const uint32_t r = __builtin_amdgcn_readfirstlane(replica_of_wave(reps, R, wave));
+    const BatchRep &rep = reps[r];
+    const KP &kp = rep.kp;
+    const WaveWorm m = wave_worm_at(wave - rep.waveOffset, kp.p.filamentSize, kp.p.numFilaments);
+    // the replica's generator position at this step
+    const uint64_t offRev = rep.rngBase + stepCtr[0] * rep.rngStride;
+    const uint64_t offLgv = offRev + rep.revBlocks;
+    float4 *pos = a.pos + rep.beadOffset, *vel = a.vel + rep.beadOffset;
+    float4 *force = a.force + rep.beadOffset, *forceOld = a.forceOld + rep.beadOffset;
+    const FrontLane o = md_front_lane(kp, m, pos, vel, force, forceOld, dt, rep.lgvStd, rep.langevin, rep.seed,
+                                      offRev, offLgv);
+    if (m.active) {
+        const uint32_t h = md_front_store(kp, m, o, pos, vel, force, forceOld, a.tangent + rep.beadOffset);
+        a.hash[rep.beadOffset + m.bead] = rep.cellOffset + h;
+    }
+}
+
+template <int MODE, bool TAN, bool UNIT>
+__global__ __launch_bounds__(kBlock) void k_collide_rows_batch(BatchArrays a, const BatchRep *__restrict__ reps,
+                                                               uint32_t R, uint64_t *__restrict__ stepCtr) {
+    __shared__ RowsLds<MODE, TAN> L;
+    // the step's last kernel advances the step counter the next front reads
+    if (blockIdx.x == 0 && threadIdx.x == 0) stepCtr[0] = stepCtr[0] + 1u;
+    const uint32_t r = __builtin_amdgcn_readfirstlane(replica_of_block(reps, R, blockIdx.x));
+    const BatchRep &rep = reps[r];
+    const uint32_t i0 = rep.beadOffset + (blockIdx.x - rep.blockOffset) * blockDim.x;
+    const uint32_t iEnd = rep.beadOffset + rep.kp.p.numFilaments * rep.kp.p.filamentSize;
+    collide_rows_tile<MODE, TAN, UNIT>(rep.kp, i0, iEnd, rep.beadOffset, rep.cellOffset, a.force, a.sPos, a.sVel,
+                                       a.sTan, a.ids, a.off + rep.cellOffset, a.sCell, L);
 }
 
 // ===========================================================================
@@ -404,8 +522,12 @@ hipError_t launch_md_front(const MdFrontArgs &a, const KP &kp, hipStream_t st) {
     const uint32_t n = kp.p.filamentSize;
     const uint32_t wpw = 64u / n;
     const uint64_t threads = ((uint64_t(a.W) + wpw - 1) / wpw) * 64u;
-    const size_t lds = (size_t(a.bp.buckets) + 1 + kFrontBlock / 64) * sizeof(uint32_t);
-    hipLaunchKernelGGL(k_md_front, dim3(grid_for(threads, kFrontBlock)), dim3(kFrontBlock), lds, st, a, kp);
+    if (a.pairs) {
+        const size_t lds = (size_t(a.bp.buckets) + 1 + kFrontBlock / 64) * sizeof(uint32_t);
+        hipLaunchKernelGGL(k_md_front<true>, dim3(grid_for(threads, kFrontBlock)), dim3(kFrontBlock), lds, st, a, kp);
+    } else {
+        hipLaunchKernelGGL(k_md_front<false>, dim3(grid_for(threads)), dim3(kBlock), 0, st, a, kp);
+    }
     return hipGetLastError();
 }
 
@@ -438,6 +560,33 @@ hipError_t launch_collide_off(float4 *force, const float4 *sPos, const float4 *s
         else NW_COLLIDE(k_collide_csr, 0, true);
     }
 #undef NW_COLLIDE
+    return hipGetLastError();
+}
+
+hipError_t launch_md_front_batch(const BatchArrays &a, const BatchRep *reps, uint32_t R, uint32_t waves,
+                                 const uint64_t *stepCtr, float dt, hipStream_t st) {
+    if (waves == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_md_front_batch, dim3(grid_for(uint64_t(waves) * 64u)), dim3(kBlock), 0, st, a, reps, R,
+                       waves, stepCtr, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_collide_batch(const BatchArrays &a, const BatchRep *reps, uint32_t R, uint32_t blocks, int mode,
+                                bool tan, bool unit, uint64_t *stepCtr, hipStream_t st) {
+    if (blocks == 0) return hipSuccess;
+    const dim3 g(blocks), b(kBlock);
+#define NW_BATCH_COLLIDE(...) hipLaunchKernelGGL((k_collide_rows_batch<__VA_ARGS__>), g, b, 0, st, a, reps, R, stepCtr)
+    if (mode == 1 && !tan) {
+        if (unit) NW_BATCH_COLLIDE(1, false, true);
+        else NW_BATCH_COLLIDE(1, false, false);
+    } else if (mode == 1) {
+        if (unit) NW_BATCH_COLLIDE(1, true, true);
+        else NW_BATCH_COLLIDE(1, true, false);
+    } else {
+        if (unit) NW_BATCH_COLLIDE(0, true, true);
+        else NW_BATCH_COLLIDE(0, true, false);
+    }
+#undef NW_BATCH_COLLIDE
     return hipGetLastError();
 }
 

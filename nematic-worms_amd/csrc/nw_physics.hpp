@@ -160,6 +160,19 @@ struct WaveWorm {
     bool active;
 };
 
+// lanes of wavefront `wave` of a system of W worms of n beads
+__device__ __forceinline__ WaveWorm wave_worm_at(uint32_t wave, uint32_t n, uint32_t W) {
+    WaveWorm m;
+    m.lane = threadIdx.x & 63u;
+    const uint32_t wpw = 64u / n;
+    m.wl = m.lane / n;
+    m.pp = m.lane - m.wl * n;
+    m.worm = wave * wpw + m.wl;
+    m.active = m.wl < wpw && m.worm < W;
+    m.bead = m.worm * n + m.pp;
+    return m;
+}
+
 __device__ __forceinline__ WaveWorm wave_worm(uint32_t n, uint32_t W) {
     WaveWorm m;
     m.lane = threadIdx.x & 63u;

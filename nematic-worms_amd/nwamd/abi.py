@@ -79,6 +79,12 @@ assert ctypes.sizeof(NWExtParams) == 64
 NW_OBS_SUMS = 19
 
 
+class NWReplicaParams(ctypes.Structure):
+    _fields_ = [("params", SimParams), ("ext", NWExtParams), ("seed", ctypes.c_uint64), ("offset", ctypes.c_uint64),
+                ("langevinGamma", ctypes.c_float), ("langevinKbT", ctypes.c_float), ("langevin", ctypes.c_int),
+                ("reserved", ctypes.c_uint32 * 5)]
+
+
 class NWObservables(ctypes.Structure):
     _fields_ = [("nematicS", ctypes.c_double), ("polarP", ctypes.c_double), ("bondS", ctypes.c_double),
                 ("polarSpeed", ctypes.c_double), ("meanV2", ctypes.c_double), ("beads", ctypes.c_double),
@@ -130,6 +136,10 @@ PROTOTYPES = {
     "nwFusedSrdStep": (None, [P, P, P, P, P, c_float, c_uint, c_uint, c_int]),
     "nwFusedPlanFits": (c_int, [c_uint, c_uint, c_uint]),
     "nwObservables": (None, [ctypes.c_void_p, ctypes.c_void_p, c_uint, ctypes.POINTER(NWObservables)]),
+    "nwBatchSetReplicas": (None, [ctypes.POINTER(NWReplicaParams), c_uint]),
+    "nwBatchLayout": (None, [ctypes.c_void_p, ctypes.c_void_p]),
+    "nwBatchMdStep": (None, [ctypes.c_void_p] * 9 + [c_float, c_uint, c_int]),
+    "nwBatchRngState": (None, [c_uint, ctypes.POINTER(ctypes.c_uint64)]),
     "nwSymEig3Max": (ctypes.c_double, [ctypes.POINTER(ctypes.c_double)]),
     "nwGetDeviceFlags": (c_uint, [c_int]),
     "nwTimerEnable": (None, [ctypes.c_char_p, c_int]),

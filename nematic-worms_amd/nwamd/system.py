@@ -56,6 +56,19 @@ def load():
             "nwSystemSetRngState": (None, [P, u64]),
             "nwSystemWriteOutputs": (None, [P, ctypes.c_char_p, ctypes.c_int, f]),
             "nwSystemObservables": (None, [P, ctypes.POINTER(abi.NWObservables)]),
+            "nwReplicaSetCreate": (P, [ctypes.c_void_p, u32, ctypes.c_void_p]),
+            "nwReplicaSetDestroy": (None, [P]),
+            "nwReplicaSetReset": (None, [P, ctypes.c_int, f]),
+            "nwReplicaSetUpdate": (None, [P, f, u32, ctypes.c_int]),
+            "nwReplicaSetSync": (None, [P]),
+            "nwReplicaSetNumReplicas": (u32, [P]),
+            "nwReplicaSetNumParticles": (u64, [P, ctypes.c_int]),
+            "nwReplicaSetGetArray": (None, [P, ctypes.c_int, ctypes.c_int, P]),
+            "nwReplicaSetSetArray": (None, [P, ctypes.c_int, ctypes.c_int, P]),
+            "nwReplicaSetParams": (None, [P, ctypes.c_int, ctypes.POINTER(abi.SimParams),
+                                          ctypes.POINTER(abi.NWExtParams)]),
+            "nwReplicaSetRngState": (None, [P, ctypes.c_int, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
+            "nwReplicaSetObservables": (None, [P, ctypes.c_int, ctypes.POINTER(abi.NWObservables)]),
         }
         for name, (res, args) in protos.items():
             fn = getattr(L, name)

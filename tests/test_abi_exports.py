@@ -7,7 +7,7 @@ import subprocess
 
 import pytest
 
-from nwamd import abi, system
+from nwamd import abi, replicas, system
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 INC = os.path.join(ROOT, "include")
@@ -74,6 +74,10 @@ int main(void) {
   F(SimParams, numParticles) F(SimParams, bondSpringK) F(SimParams, activity) F(SimParams, spring)
   F(SimParams, boundaryDamping) F(SolventParams, gridSize) F(SolventParams, randOffset) F(SolventParams, alpha)
   F(SolventParams, particleMass) F(NWSystemConfig, seed) F(NWSystemConfig, ext)
+  printf("NWObservables %zu\nNWReplicaParams %zu\nNWReplicaSpec %zu\nNWReplicaPhysics %zu\n", sizeof(NWObservables),
+         sizeof(NWReplicaParams), sizeof(NWReplicaSpec), sizeof(NWReplicaPhysics));
+  F(NWObservables, sums) F(NWReplicaParams, ext) F(NWReplicaParams, seed) F(NWReplicaParams, langevin)
+  F(NWReplicaSpec, seed) F(NWReplicaPhysics, langevinFdt)
   return 0;
 }
 """)
@@ -84,11 +88,15 @@ int main(void) {
     assert out["SimParams"] == "144" and out["SolventParams"] == "60"
     assert int(out["NWExtParams"]) == ctypes.sizeof(abi.NWExtParams)
     assert int(out["NWSystemConfig"]) == ctypes.sizeof(system.NWSystemConfig)
+    classes = {"SimParams": abi.SimParams, "SolventParams": abi.SolventParams, "NWSystemConfig": system.NWSystemConfig,
+               "NWObservables": abi.NWObservables, "NWReplicaParams": abi.NWReplicaParams,
+               "NWReplicaSpec": replicas.NWReplicaSpec, "NWReplicaPhysics": replicas.NWReplicaPhysics}
+    for t in ("NWObservables", "NWReplicaParams", "NWReplicaSpec", "NWReplicaPhysics"):
+        assert int(out[t]) == ctypes.sizeof(classes[t]), t
     for key, val in out.items():
         if "." in key:
             t, m = key.split(".")
-            cls = {"SimParams": abi.SimParams, "SolventParams": abi.SolventParams,
-                   "NWSystemConfig": system.NWSystemConfig}[t]
+            cls = classes[t]
             assert getattr(cls, m).offset == int(val), key
 
 
@@ -112,5 +120,6 @@ def test_fused_plan_limits():
     assert L.nwFusedPlanFits(0, 16, 10) == 1
     assert L.nwFusedPlanFits(1000, 16, 65) == 0
     assert L.nwFusedPlanFits(1000, 0, 10) == 0
-    assert L.nwFusedPlanFits(1000, 16384 * 2048 + 1, 10) == 0
+    assert L.nwFusedPlanFits(1000, 32768 * 2048 + 1, 10) == 0
+    assert L.nwFusedPlanFits(21_760_000, 45_000_000, 20) == 1  # 256 replicas: a tile pass of its own
     assert L.nwFusedPlanFits(2 ** 30, 2 ** 24, 0) == 0
