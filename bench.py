@@ -120,6 +120,37 @@ def build_workload(name, per_call=False):
     return s, cfg, w
 
 
+def batched_form(args):
+    """256 independent replicas (worms U[500, 8000] x U{10..30} beads, area
+    fraction 0.3, f_a log-uniform [0.1, 10], kappa U[1, 20], Langevin gamma 2,
+    kT 0.1, dt 1e-3): aggregate bead-steps/s of the batched step replayed as a
+    hipGraph, and of the same replicas one launch sequence per replica."""
+    from nwamd import replicas, system
+    specs = replicas.replica_specs_config2(args.replicas)
+    rs = replicas.ReplicaSet(specs, replicas.worm_physics(gamma=2.0, kT=0.1, fdt=True))
+    rs.reset(system.NW_INIT_LATTICE_RANDOM, 0.1)
+    N = rs.n()
+    out = dict(workload=f"{args.replicas} replicas", replicas=args.replicas, beads=N)
+    for key, form, steps in (("graph", replicas.NW_REPLICAS_GRAPH, args.batched_steps),
+                             ("one_per_launch", replicas.NW_REPLICAS_ONE_PER_LAUNCH, args.one_per_launch_steps)):
+        if steps <= 0:
+            continue
+        rs.update(args.dt, max(2, steps // 10), form)  # warm-up (the graph is captured here)
+        rs.sync()
+        t0 = time.perf_counter()
+        rs.update(args.dt, steps, form)
+        rs.sync()
+        el = time.perf_counter() - t0
+        out[key] = dict(value=round(N * steps / el, 1), unit="bead-steps/s", steps=steps,
+                        ms_per_step=round(el / steps * 1e3, 4))
+    if "graph" in out and "one_per_launch" in out:
+        out["speedup"] = round(out["graph"]["value"] / out["one_per_launch"]["value"], 2)
+    o = rs.observables(0)
+    out["replica0_S"] = round(o["nematicS"], 4)
+    rs.close()
+    return out
+
+
 def timers(lib, names, enable):
     for n in names:
         lib.nwTimerEnable(n.encode(), 1 if enable else 0)
@@ -154,6 +185,10 @@ def main():
     ap.add_argument("--workload", default="worms_srd_800k")
     ap.add_argument("--dt", type=float, default=1e-3)
     ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--replicas", type=int, default=256, help="replicas of the batched form")
+    ap.add_argument("--batched-steps", type=int, default=200, help="timed steps of the batched form (0: skip)")
+    ap.add_argument("--one-per-launch-steps", type=int, default=20,
+                    help="timed steps of the replicas one launch sequence each (0: skip)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="at the end, one elementwise kernel reading and writing this many MiB "
                          "(a known-size reference for HBM byte counters)")
@@ -236,6 +271,8 @@ def main():
                         form="the reference's per-call C-ABI sequence (ParticleSystem::update, 10+ launches/step)")
         s2.close()
 
+    batched = batched_form(args) if args.batched_steps > 0 else None
+
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "bead-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 4),
@@ -244,7 +281,8 @@ def main():
         "config": {"workload": args.workload, "beads": w["N"], "solvent": w["Ns"], "md_cells": w["C"],
                    "srd_cells": w["Cs"], "dt": dt,
                    "form": "plain: one replica, ParticleSystem::update through the fused C-ABI entry points"},
-        "roofline": roof, "cpu_baseline": cpu, "per_call_sequence": per_call, "kernels": kernels,
+        "roofline": roof, "cpu_baseline": cpu, "per_call_sequence": per_call, "batched": batched,
+        "kernels": kernels,
     }
     print(json.dumps(out), flush=True)
     if args.copy_probe_mb > 0:
