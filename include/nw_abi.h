@@ -275,6 +275,34 @@ void nwBatchMdStep(float *pos, float *vel, float *force, float *forceOld, float 
 /* Generator offset of one replica after the batched steps so far. */
 void nwBatchRngState(uint replica, uint64_t *offset);
 
+/* ------------------------------------------------------------------------
+   Differentiation through the fused MD step (no counterpart in the
+   reference): reverse-mode gradients of a loss of the final configuration
+   with respect to theta = (driveForce, bondBendingK, wcaEpsilon, gamma) --
+   the gamma of both the drag and the noise amplitude sqrt(2 gamma kT).
+   Dry worms: WCA pairs, chain bonds, bending, drive, Langevin; periodic box,
+   one-layer stencil, no extensile term, no reversal.
+   ------------------------------------------------------------------------ */
+/* `steps` nwFusedMdStep steps from (pos, vel, force, forceOld), in place,
+   with the SimParams / NWExtParams of the last setParameters /
+   nwSetExtParameters; the state and generator offset are checkpointed every
+   checkpointEvery steps for nwAdjointBackward.  langevinGamma must equal
+   SimParams.gamma when langevin is on. */
+void nwAdjointForward(float *pos, float *vel, float *force, float *forceOld, float *tangent, uint numFilaments,
+                      float deltaTime, float langevinGamma, float langevinKbT, int langevin, uint steps,
+                      uint checkpointEvery);
+/* The adjoint of the last nwAdjointForward run, backwards from dLdPos /
+   dLdVel (adjoints of the final positions / velocities, device float4 per
+   bead; dLdVel may be NULL): every segment is recomputed from its checkpoint
+   with the Philox noise replayed, then swept backwards.  grad[4] receives
+   d loss / d theta.  The generator is left where the forward run left it.
+   Synchronises. */
+void nwAdjointBackward(const float *dLdPos, const float *dLdVel, double grad[4]);
+/* Bond-tangent nematic order S of the worms (NWObservables.bondS) and, when
+   dSdPos is not NULL, dS/dpos (device float4 per bead, w = 0).
+   Synchronises. */
+double nwBondOrderGrad(const float *pos, uint numFilaments, float *dSdPos);
+
 /* Order parameters of a worm configuration (the reference's analysis,
    analysis/analyze_correlations.py:13-38, analysis/afa/utils.py:35-49, on
    the device).  Bead tangent t = unit(r[p+1] - r[p-1]), one-sided at the worm

@@ -15,6 +15,7 @@
 #include "nw_internal.hpp"
 #include "nw_bucket.hpp"
 #include "nw_observe.hpp"
+#include "nw_adjoint.hpp"
 
 namespace {
 
@@ -64,6 +65,23 @@ struct State {
         hipGraph_t graph = nullptr;
         std::vector<uintptr_t> key;  // arguments the graph was captured with
     } batch;
+    // adjoint of the fused MD step: checkpoints of the last nwAdjointForward
+    struct Adjoint {
+        bool ready = false;
+        KP kp{};
+        float dt = 0.f, gamma = 0.f, kT = 0.f, lgvStd = 0.f;
+        int langevin = 0;
+        uint32_t W = 0, N = 0, C = 0, steps = 0, K = 0;
+        uint64_t seed = 0, endOffset = 0;
+        std::vector<uint64_t> ckptOffset;  // generator offset at every checkpoint
+        float4 *ckpt = nullptr;            // checkpoints x 4 x N: pos, vel, force, forceOld
+        // one segment: states after every step and the cell lists of every step
+        float4 *segPos = nullptr, *segVel = nullptr, *segSPos = nullptr, *work = nullptr, *lam = nullptr;
+        uint32_t *segIds = nullptr, *segOff = nullptr, *segCell = nullptr;
+        double *partial = nullptr, *grad = nullptr;
+        uint64_t capCkpt = 0, capPos = 0, capVel = 0, capSPos = 0, capIds = 0, capOff = 0, capCell = 0,
+                 capWork = 0, capLam = 0, capPartial = 0;
+    } adj;
     std::map<std::string, Timer> timers;
 };
 
@@ -114,6 +132,14 @@ bool fail(const char *file, int line, int code, const char *fmt, ...) {
         if (!(cond)) {                                         \
             fail(__FILE__, __LINE__, -22, __VA_ARGS__);        \
             return;                                            \
+        }                                                      \
+    } while (0)
+
+#define NW_REQUIRE_R(cond, ret, ...)                           \
+    do {                                                       \
+        if (!(cond)) {                                         \
+            fail(__FILE__, __LINE__, -22, __VA_ARGS__);        \
+            return ret;                                        \
         }                                                      \
     } while (0)
 
@@ -456,31 +482,34 @@ void collideSolvent(float *pos, float *vel, float *solventCellCOM, float *random
 // ---------------------------------------------------------------------------
 // fused step
 // ---------------------------------------------------------------------------
-void nwFusedMdStep(float *pos, float *vel, float *force, float *forceOld, float *tangent, uint *gridParticleHash,
-                   uint *gridParticleIndex, float *sortedPos, float *sortedVel, float *sortedTangent,
-                   float deltaTime, float langevinGamma, float langevinKbT, int langevin, uint numFilaments) {
-    (void)gridParticleHash;
-    NW_INIT();
+// The fused MD step of nwFusedMdStep; offOut / cellOut (NULL: the library's
+// scratch) receive the CSR cell offsets and the cell of every sorted slot.
+static bool fused_md(const KP &kp, float *pos, float *vel, float *force, float *forceOld, float *tangent,
+                     uint *gridParticleHash, uint *gridParticleIndex, float *sortedPos, float *sortedVel,
+                     float *sortedTangent, float deltaTime, float langevinGamma, float langevinKbT, int langevin,
+                     uint numFilaments, uint32_t *offOut, uint32_t *cellOut) {
     State &s = S();
-    const KP kp = make_kp();
     const uint32_t n = kp.p.filamentSize;
-    NW_REQUIRE(n >= 1 && n <= 64, "nwFusedMdStep: worms of %u beads (1..64 supported)", n);
+    NW_REQUIRE_R(n >= 1 && n <= 64, false, "nwFusedMdStep: worms of %u beads (1..64 supported)", n);
     const uint64_t cells = uint64_t(kp.p.gridSize.x) * kp.p.gridSize.y * kp.p.gridSize.z;
-    NW_REQUIRE(cells > 0 && cells <= kp.p.numCells, "nwFusedMdStep: grid does not fit numCells");
+    NW_REQUIRE_R(cells > 0 && cells <= kp.p.numCells, false, "nwFusedMdStep: grid does not fit numCells");
     const uint64_t N = uint64_t(numFilaments) * n;
-    NW_REQUIRE(N < (1ull << 31), "nwFusedMdStep: too many beads");
+    NW_REQUIRE_R(N < (1ull << 31), false, "nwFusedMdStep: too many beads");
     const bool needVel = kp.e.pairPotential == NW_PAIR_REF_DEM;
     const bool needTan = needVel || kp.p.activity != 0.0f;
-    NW_REQUIRE(!needVel || sortedVel != nullptr, "nwFusedMdStep: sortedVel is NULL");
-    NW_REQUIRE(!needTan || sortedTangent != nullptr, "nwFusedMdStep: sortedTangent is NULL");
-    if (N == 0) return;
+    NW_REQUIRE_R(!needVel || sortedVel != nullptr, false, "nwFusedMdStep: sortedVel is NULL");
+    NW_REQUIRE_R(!needTan || sortedTangent != nullptr, false, "nwFusedMdStep: sortedTangent is NULL");
+    if (N == 0) return true;
     // the front's workgroups are the sort's tiles while the tile x bucket
     // table stays small; larger systems take a tile pass of their own
     BucketPlan bp = make_bucket_plan(N, cells, md_tile_items(n));
     const bool frontTiles = !*bucket_plan_problem(bp);
     if (!frontTiles) bp = make_bucket_plan(N, cells, 0);
-    NW_REQUIRE(!*bucket_plan_problem(bp), "nwFusedMdStep: %s", bucket_plan_problem(bp));
-    NW_HIP(ensure_bucket_scratch(s.bucket, bp));
+    NW_REQUIRE_R(!*bucket_plan_problem(bp), false, "nwFusedMdStep: %s", bucket_plan_problem(bp));
+    NW_HIP_R(ensure_bucket_scratch(s.bucket, bp), false);
+    BucketScratch sc = s.bucket;
+    if (offOut) sc.off = offOut;
+    if (cellOut) sc.sortedCell = cellOut;
 
     MdFrontArgs a;
     a.pos = reinterpret_cast<float4 *>(pos);
@@ -503,27 +532,37 @@ void nwFusedMdStep(float *pos, float *vel, float *force, float *forceOld, float 
     if (langevin) s.offset += N;
     {
         TimedScope ts("md_front", s.stream);
-        NW_HIP(launch_md_front(a, kp, s.stream));
+        NW_HIP_R(launch_md_front(a, kp, s.stream), false);
     }
     if (!frontTiles) {
         TimedScope ts("md_tiles", s.stream);
-        NW_HIP(launch_hash_tiles(a.hash, bp, s.bucket, s.stream));
+        NW_HIP_R(launch_hash_tiles(a.hash, bp, s.bucket, s.stream), false);
     }
     {
         TimedScope ts("md_sort", s.stream);
-        NW_HIP(launch_md_cells(bp, s.bucket, gridParticleIndex, a.pos, needVel ? a.vel : nullptr,
+        NW_HIP_R(launch_md_cells(bp, sc, gridParticleIndex, a.pos, needVel ? a.vel : nullptr,
                                needTan ? a.tangent : nullptr, reinterpret_cast<float4 *>(sortedPos),
                                needVel ? reinterpret_cast<float4 *>(sortedVel) : nullptr,
-                               needTan ? reinterpret_cast<float4 *>(sortedTangent) : nullptr, s.stream));
+                               needTan ? reinterpret_cast<float4 *>(sortedTangent) : nullptr, s.stream), false);
     }
     {
         TimedScope ts("md_collide", s.stream);
-        NW_HIP(launch_collide_off(a.force, reinterpret_cast<const float4 *>(sortedPos),
+        NW_HIP_R(launch_collide_off(a.force, reinterpret_cast<const float4 *>(sortedPos),
                                   reinterpret_cast<const float4 *>(sortedVel),
                                   reinterpret_cast<const float4 *>(sortedTangent), gridParticleIndex,
-                                  s.bucket.off, s.bucket.sortedCell, uint32_t(N), kp, s.stream));
+                                  sc.off, sc.sortedCell, uint32_t(N), kp, s.stream), false);
     }
     s.lastHashPtr = nullptr;
+    return true;
+}
+
+void nwFusedMdStep(float *pos, float *vel, float *force, float *forceOld, float *tangent, uint *gridParticleHash,
+                   uint *gridParticleIndex, float *sortedPos, float *sortedVel, float *sortedTangent,
+                   float deltaTime, float langevinGamma, float langevinKbT, int langevin, uint numFilaments) {
+    NW_INIT();
+    (void)fused_md(make_kp(), pos, vel, force, forceOld, tangent, gridParticleHash, gridParticleIndex, sortedPos,
+                   sortedVel, sortedTangent, deltaTime, langevinGamma, langevinKbT, langevin, numFilaments, nullptr,
+                   nullptr);
 }
 
 void nwFusedSrdStep(float *pos, float *vel, float *solventCellCOM, uint *gridParticleHash, uint *gridParticleIndex,
@@ -843,6 +882,231 @@ void nwBatchMdStep(float *pos, float *vel, float *force, float *forceOld, float 
         else NW_HIP(batch_step(b, a, gv, gt, a.sVel, a.sTan, deltaTime, s.stream));
         ++b.steps;
     }
+}
+
+// ---------------------------------------------------------------------------
+// adjoint of the fused MD step
+// ---------------------------------------------------------------------------
+extern "C++" {
+template <typename T>
+static bool grow(T *&p, uint64_t &cap, uint64_t need) {
+    if (need <= cap) return true;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc(reinterpret_cast<void **>(&p), need * sizeof(T)) != hipSuccess) return false;
+    cap = need;
+    return true;
+}
+}
+
+static bool copy_d2d(void *dst, const void *src, size_t bytes, hipStream_t st) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st) == hipSuccess;
+}
+
+void nwAdjointForward(float *pos, float *vel, float *force, float *forceOld, float *tangent, uint numFilaments,
+                      float deltaTime, float langevinGamma, float langevinKbT, int langevin, uint steps,
+                      uint checkpointEvery) {
+    NW_INIT();
+    State &s = S();
+    State::Adjoint &A = s.adj;
+    A.ready = false;
+    const KP kp = make_kp();
+    const SimParams &p = kp.p;
+    NW_REQUIRE(kp.e.pairPotential == NW_PAIR_WCA && kp.e.bondedMode == NW_BONDED_CHAIN,
+               "nwAdjointForward: WCA pairs with chain bonds only");
+    NW_REQUIRE(p.activity == 0.0f && p.reverseProbability == 0.0f,
+               "nwAdjointForward: no extensile pair term and no reversal");
+    NW_REQUIRE(kp.e.stencil == NW_STENCIL_2D || p.gridSize.z == 1, "nwAdjointForward: one-layer stencil only");
+    NW_REQUIRE(p.boundaryX == PERIODIC && p.boundaryY == PERIODIC && p.boundaryZ == PERIODIC,
+               "nwAdjointForward: periodic boxes only");
+    NW_REQUIRE(!langevin || langevinGamma == p.gamma,
+               "nwAdjointForward: the Langevin gamma must be SimParams.gamma (one gamma to differentiate)");
+    NW_REQUIRE(checkpointEvery >= 1 && steps >= 1, "nwAdjointForward: steps and checkpointEvery >= 1");
+    const uint32_t n = p.filamentSize;
+    NW_REQUIRE(n >= 1 && n <= 64, "nwAdjointForward: worms of 1..64 beads");
+    const uint64_t N = uint64_t(numFilaments) * n;
+    const uint64_t C = uint64_t(p.gridSize.x) * p.gridSize.y * p.gridSize.z;
+    A.kp = kp;
+    A.dt = deltaTime;
+    A.gamma = langevinGamma;
+    A.kT = langevinKbT;
+    A.lgvStd = sqrtf(2.0f * langevinGamma * langevinKbT);
+    A.langevin = langevin ? 1 : 0;
+    A.W = numFilaments;
+    A.N = uint32_t(N);
+    A.C = uint32_t(C);
+    A.steps = steps;
+    A.K = checkpointEvery;
+    A.seed = s.seed;
+    const uint32_t nck = (steps + checkpointEvery - 1) / checkpointEvery;
+    NW_REQUIRE(grow(A.ckpt, A.capCkpt, uint64_t(nck) * 4 * N) && grow(A.segSPos, A.capSPos, N) &&
+                   grow(A.segIds, A.capIds, N),
+               "nwAdjointForward: out of device memory");
+    A.ckptOffset.assign(nck, 0);
+    const size_t bytes = N * sizeof(float4);
+    for (uint32_t k = 0; k < steps; ++k) {
+        if (k % checkpointEvery == 0) {
+            const uint32_t c = k / checkpointEvery;
+            float4 *ck = A.ckpt + uint64_t(c) * 4 * N;
+            NW_REQUIRE(copy_d2d(ck, pos, bytes, s.stream) && copy_d2d(ck + N, vel, bytes, s.stream) &&
+                           copy_d2d(ck + 2 * N, force, bytes, s.stream) &&
+                           copy_d2d(ck + 3 * N, forceOld, bytes, s.stream),
+                       "nwAdjointForward: checkpoint copy failed");
+            A.ckptOffset[c] = s.offset;
+        }
+        // sorted positions and ids go to the segment store (scratch here)
+        if (!fused_md(kp, pos, vel, force, forceOld, tangent, nullptr, A.segIds, reinterpret_cast<float *>(A.segSPos),
+                      nullptr, nullptr, deltaTime, langevinGamma, langevinKbT, langevin, numFilaments, nullptr,
+                      nullptr))
+            return;
+    }
+    A.endOffset = s.offset;
+    A.ready = true;
+}
+
+static bool ensure_segment(State::Adjoint &A) {
+    const uint64_t N = A.N, K = A.K;
+    return grow(A.segPos, A.capPos, (K + 1) * N) && grow(A.segVel, A.capVel, (K + 1) * N) &&
+           grow(A.segSPos, A.capSPos, K * N) && grow(A.segIds, A.capIds, K * N) &&
+           grow(A.segOff, A.capOff, K * (uint64_t(A.C) + 2)) && grow(A.segCell, A.capCell, K * N) &&
+           grow(A.work, A.capWork, 3 * N) && grow(A.lam, A.capLam, 6 * N);
+}
+
+void nwAdjointBackward(const float *dLdPos, const float *dLdVel, double grad[4]) {
+    NW_INIT();
+    State &s = S();
+    State::Adjoint &A = s.adj;
+    NW_REQUIRE(A.ready, "nwAdjointBackward: no nwAdjointForward run to differentiate");
+    NW_REQUIRE(dLdPos != nullptr && grad != nullptr, "nwAdjointBackward: NULL argument");
+    const uint64_t N = A.N, K = A.K;
+    NW_REQUIRE(ensure_segment(A), "nwAdjointBackward: out of device memory");
+    const uint32_t bRows = adj_bonded_blocks(A.W, A.kp.p.filamentSize), pRows = adj_pair_blocks(A.N);
+    const uint64_t rowsPerStep = uint64_t(bRows) + pRows;
+    NW_REQUIRE(grow(A.partial, A.capPartial, K * rowsPerStep * kAdjParams), "nwAdjointBackward: out of memory");
+    if (!A.grad) NW_HIP(hipMalloc(&A.grad, kAdjParams * sizeof(double)));
+    NW_HIP(hipMemsetAsync(A.grad, 0, kAdjParams * sizeof(double), s.stream));
+    const size_t bytes = N * sizeof(float4);
+    float4 *lr = A.lam, *lv = A.lam + N, *lF[2] = {A.lam + 2 * N, A.lam + 3 * N}, *lG = A.lam + 4 * N,
+           *Rb = A.lam + 5 * N;
+    NW_HIP(hipMemcpyAsync(lr, dLdPos, bytes, hipMemcpyDeviceToDevice, s.stream));
+    if (dLdVel) NW_HIP(hipMemcpyAsync(lv, dLdVel, bytes, hipMemcpyDeviceToDevice, s.stream));
+    else NW_HIP(hipMemsetAsync(lv, 0, bytes, s.stream));
+    NW_HIP(hipMemsetAsync(lF[0], 0, bytes, s.stream));
+    NW_HIP(hipMemsetAsync(lG, 0, bytes, s.stream));
+    int cur = 0;
+    float4 *F = A.work, *G = A.work + N, *T = A.work + 2 * N;
+    const uint32_t nck = (A.steps + A.K - 1) / A.K;
+    std::vector<uint64_t> lgvOff(K);
+    for (int c = int(nck) - 1; c >= 0; --c) {
+        const uint32_t s0 = uint32_t(c) * A.K, len = std::min<uint32_t>(A.K, A.steps - s0);
+        const float4 *ck = A.ckpt + uint64_t(c) * 4 * N;
+        NW_HIP(hipMemcpyAsync(A.segPos, ck, bytes, hipMemcpyDeviceToDevice, s.stream));
+        NW_HIP(hipMemcpyAsync(A.segVel, ck + N, bytes, hipMemcpyDeviceToDevice, s.stream));
+        NW_HIP(hipMemcpyAsync(F, ck + 2 * N, bytes, hipMemcpyDeviceToDevice, s.stream));
+        NW_HIP(hipMemcpyAsync(G, ck + 3 * N, bytes, hipMemcpyDeviceToDevice, s.stream));
+        // recompute the segment, keeping every step's state and cell lists
+        s.offset = A.ckptOffset[c];
+        for (uint32_t j = 0; j < len; ++j) {
+            float4 *pj = A.segPos + uint64_t(j + 1) * N, *vj = A.segVel + uint64_t(j + 1) * N;
+            NW_HIP(hipMemcpyAsync(pj, A.segPos + uint64_t(j) * N, bytes, hipMemcpyDeviceToDevice, s.stream));
+            NW_HIP(hipMemcpyAsync(vj, A.segVel + uint64_t(j) * N, bytes, hipMemcpyDeviceToDevice, s.stream));
+            lgvOff[j] = s.offset;
+            if (!fused_md(A.kp, reinterpret_cast<float *>(pj), reinterpret_cast<float *>(vj),
+                          reinterpret_cast<float *>(F), reinterpret_cast<float *>(G), reinterpret_cast<float *>(T),
+                          nullptr, A.segIds + uint64_t(j) * N, reinterpret_cast<float *>(A.segSPos + uint64_t(j) * N),
+                          nullptr, nullptr, A.dt, A.gamma, A.kT, A.langevin, A.W,
+                          A.segOff + uint64_t(j) * (A.C + 2), A.segCell + uint64_t(j) * N))
+                return;
+        }
+        // the adjoint through the segment, last step first
+        for (int j = int(len) - 1; j >= 0; --j) {
+            double *part = A.partial + uint64_t(j) * rowsPerStep * kAdjParams;
+            AdjBondedArgs ab;
+            ab.pos = A.segPos + uint64_t(j + 1) * N;
+            ab.vel = A.segVel + uint64_t(j + 1) * N;
+            ab.lF = lF[cur];
+            ab.Rb = Rb;
+            ab.partial = part;
+            ab.W = A.W;
+            ab.langevin = A.langevin;
+            ab.lgvStd = A.lgvStd;
+            ab.gamma = A.gamma;
+            ab.seed = A.seed;
+            ab.offLgv = lgvOff[j];
+            NW_HIP(launch_adj_bonded(ab, A.kp, s.stream));
+            AdjPairArgs ap;
+            ap.sPos = A.segSPos + uint64_t(j) * N;
+            ap.ids = A.segIds + uint64_t(j) * N;
+            ap.off = A.segOff + uint64_t(j) * (A.C + 2);
+            ap.sCell = A.segCell + uint64_t(j) * N;
+            ap.lF = lF[cur];
+            ap.Rb = Rb;
+            ap.lFout = lF[1 - cur];
+            ap.lr = lr;
+            ap.lv = lv;
+            ap.lG = lG;
+            ap.partial = part + uint64_t(bRows) * kAdjParams;
+            ap.n = A.N;
+            ap.dt = A.dt;
+            NW_HIP(launch_adj_pair(ap, A.kp, s.stream));
+            cur = 1 - cur;
+        }
+        NW_HIP(launch_adj_total(A.partial, uint64_t(len) * rowsPerStep, A.grad, s.stream));
+    }
+    s.offset = A.endOffset;  // the generator continues after the forward run
+    NW_HIP(hipMemcpyAsync(grad, A.grad, kAdjParams * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    NW_HIP(hipStreamSynchronize(s.stream));
+}
+
+// unit eigenvector of the symmetric 3x3 a (a11 a12 a13 a22 a23 a33) for the
+// eigenvalue lam: the longest cross product of two rows of a - lam I
+static void sym_eigvec3(const double a[6], double lam, double e[3]) {
+    const double m[3][3] = {{a[0] - lam, a[1], a[2]}, {a[1], a[3] - lam, a[4]}, {a[2], a[4], a[5] - lam}};
+    double best = -1.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = i + 1; j < 3; ++j) {
+            const double c[3] = {m[i][1] * m[j][2] - m[i][2] * m[j][1], m[i][2] * m[j][0] - m[i][0] * m[j][2],
+                                 m[i][0] * m[j][1] - m[i][1] * m[j][0]};
+            const double l = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+            if (l > best) {
+                best = l;
+                e[0] = c[0];
+                e[1] = c[1];
+                e[2] = c[2];
+            }
+        }
+    const double l = std::sqrt(best);
+    if (l > 0) {
+        e[0] /= l;
+        e[1] /= l;
+        e[2] /= l;
+    } else {
+        e[0] = 1.0;
+        e[1] = e[2] = 0.0;
+    }
+}
+
+double nwBondOrderGrad(const float *pos, uint numFilaments, float *dSdPos) {
+    if (!ensure_init()) return 0.0;
+    State &s = S();
+    NW_REQUIRE_R(pos != nullptr, 0.0, "nwBondOrderGrad: NULL positions");
+    NWObservables o;
+    nwObservables(pos, pos, numFilaments, &o);
+    if (o.bonds <= 0) return 0.0;
+    const double nb = o.bonds;
+    const double Q[6] = {1.5 * o.sums[kObsBxx] / nb - 0.5, 1.5 * o.sums[kObsBxy] / nb, 1.5 * o.sums[kObsBxz] / nb,
+                         1.5 * o.sums[kObsByy] / nb - 0.5, 1.5 * o.sums[kObsByz] / nb, 1.5 * o.sums[kObsBzz] / nb - 0.5};
+    const double S = nwSymEig3Max(Q);
+    if (dSdPos) {
+        double e[3];
+        sym_eigvec3(Q, S, e);
+        NW_HIP_R(launch_bond_order_grad(reinterpret_cast<const float4 *>(pos), numFilaments, e, 3.0 / nb,
+                                        reinterpret_cast<float4 *>(dSdPos), make_kp(), s.stream),
+                 S);
+        NW_HIP_R(hipStreamSynchronize(s.stream), S);
+    }
+    return S;
 }
 
 int nwFusedPlanFits(uint numParticles, uint numCells, uint filamentSize) {

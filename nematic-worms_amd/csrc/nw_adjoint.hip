@@ -1,0 +1,328 @@
+// nw_adjoint.hip -- the adjoint of the fused MD step (gfx950): reverse-mode
+// derivatives of a loss of the final configuration with respect to the drive
+// f_a, the bending stiffness kappa, the WCA epsilon and the Langevin gamma.
+//
+// One step maps (r, v, F, G) to
+//   v' = v + (F + G) dt / 2,  r' = r + v' dt + F dt^2 / 2,  G' = F,
+//   F' = -gamma v' + sigma(gamma) z + B(r'; kappa, f_a) + P(r'; epsilon)
+// (z the replayed Philox normals, sigma = sqrt(2 gamma kT), B the bonded
+// terms, P the WCA pairs; periodic wrapping has unit derivative).  With the
+// adjoints (lr, lv, lF, lG) of the step's outputs:
+//   R = lr + (dB/dr' + dP/dr')^T lF          adjoint of r'
+//   V = lv - gamma lF + dt R                 adjoint of v'
+//   lr <- R, lv <- V, lF <- lG + dt^2 R / 2 + dt V / 2, lG <- dt V / 2
+//   grad += (dF'/dtheta)^T lF.
+// Bending, bonds and WCA are conservative: their Jacobian is symmetric, so
+// J^T lF = J lF, the derivative of the force along lF, evaluated with dual
+// numbers through the forward expressions.  The drive is not; its transpose
+// product is written out.  The bonded kernel runs whole worms per wavefront
+// (chain neighbours by shuffles), the pair kernel walks the forward step's
+// cell lists of r'.  Parameter partials are FP64 per workgroup, added in a
+// fixed order at the end.
+#include "nw_adjoint.hpp"
+#include "nw_physics.hpp"
+
+namespace nw {
+
+// ---------------------------------------------------------------------------
+// dual numbers: value and derivative along the adjoint direction
+// ---------------------------------------------------------------------------
+struct d1 {
+    float v, t;
+};
+struct d3 {
+    f3 v, t;
+};
+
+__device__ __forceinline__ d1 operator*(d1 a, d1 b) { return {a.v * b.v, a.t * b.v + a.v * b.t}; }
+__device__ __forceinline__ d1 operator+(d1 a, d1 b) { return {a.v + b.v, a.t + b.t}; }
+__device__ __forceinline__ d1 operator-(d1 a, d1 b) { return {a.v - b.v, a.t - b.t}; }
+__device__ __forceinline__ d1 operator/(d1 a, d1 b) {
+    const float q = a.v / b.v;
+    return {q, (a.t - q * b.t) / b.v};
+}
+__device__ __forceinline__ d1 dsqrt(d1 a) {
+    const float s = sqrtf(a.v);
+    return {s, a.t / (2.0f * s)};
+}
+__device__ __forceinline__ d3 operator+(d3 a, d3 b) { return {a.v + b.v, a.t + b.t}; }
+__device__ __forceinline__ d3 operator-(d3 a, d3 b) { return {a.v - b.v, a.t - b.t}; }
+__device__ __forceinline__ d3 operator-(d3 a) { return {mk(-a.v.x, -a.v.y, -a.v.z), mk(-a.t.x, -a.t.y, -a.t.z)}; }
+__device__ __forceinline__ d3 operator*(d3 a, d1 s) { return {a.v * s.v, a.t * s.v + a.v * s.t}; }
+__device__ __forceinline__ d3 operator*(d1 s, d3 a) { return a * s; }
+__device__ __forceinline__ d3 operator/(d3 a, d1 s) {
+    const f3 q = a.v / s.v;
+    return {q, (a.t - q * s.t) / s.v};
+}
+__device__ __forceinline__ d1 ddot(d3 a, d3 b) { return {dot(a.v, b.v), dot(a.t, b.v) + dot(a.v, b.t)}; }
+__device__ __forceinline__ d1 dconst(float c) { return {c, 0.0f}; }
+
+// difference b - a under the minimum image (the wrap is piecewise constant)
+__device__ __forceinline__ d3 dmin_image(const SimParams &p, d3 a, d3 b) {
+    return {min_image(p, a.v, b.v), b.t - a.t};
+}
+
+__device__ __forceinline__ d3 dshfl_down(d3 a, int d) { return {shfl_down3(a.v, d), shfl_down3(a.t, d)}; }
+__device__ __forceinline__ d3 dshfl_up(d3 a, int d) { return {shfl_up3(a.v, d), shfl_up3(a.t, d)}; }
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// the workgroup's FP64 partials (kAdjParams of them) into out[blockIdx]
+__device__ __forceinline__ void block_partials(double (&acc)[kAdjParams], double *__restrict__ out) {
+    __shared__ double ls[kBlock / 64][kAdjParams];
+#pragma unroll
+    for (int k = 0; k < kAdjParams; ++k) {
+        const double s = wave_sum_d(acc[k]);
+        if ((threadIdx.x & 63u) == 0u) ls[threadIdx.x >> 6][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < kAdjParams) {
+        double s = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) s += ls[w][threadIdx.x];
+        out[uint64_t(blockIdx.x) * kAdjParams + threadIdx.x] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// bonded adjoint: Rb = (dB/dr)^T lF for bead p and the kappa, f_a, gamma
+// partials; whole worms per wavefront as in the forward front
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_adj_bonded(AdjBondedArgs a, KP kp) {
+    const SimParams &p = kp.p;
+    const uint32_t n = p.filamentSize;
+    const WaveWorm m = wave_worm(n, a.W);
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    const f3 rv = mk(m.active ? a.pos[m.bead] : zero4);
+    const f3 lam = mk(m.active ? a.lF[m.bead] : zero4);
+    const d3 r = {rv, lam};
+    const uint32_t pp = m.pp;
+
+    // bending (filamentKernel): lane pp owns the triple (pp, pp+1, pp+2)
+    const d3 rn1 = dshfl_down(r, 1), rn2 = dshfl_down(r, 2);
+    d3 fi = {mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)}, fj = fi, fk = fi;
+    const bool hasTriple = n >= 3 && pp + 2u < n;
+    if (hasTriple) {
+        const d3 r_ij = dmin_image(p, r, rn1);
+        const d3 r_jk = dmin_image(p, rn1, rn2);
+        const d1 d_ij = dsqrt(ddot(r_ij, r_ij));
+        const d1 d_jk = dsqrt(ddot(r_jk, r_jk));
+        const d1 d_ij_jk = ddot(r_ij, r_jk);
+        const d1 A = dconst(p.bondBendingK) / (d_ij * d_jk);
+        const d3 B = r_ij * d_ij_jk / ddot(r_ij, r_ij);
+        const d3 C = r_jk * d_ij_jk / ddot(r_jk, r_jk);
+        fi = -(A * (r_jk - B));
+        fk = -(A * (C - r_ij));
+        fj = -(fi + fk);
+    }
+    const d3 fk2 = dshfl_up(fk, 2), fj1 = dshfl_up(fj, 1);
+    d3 bend = {mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)};
+    if (pp >= 2u) bend = bend + fk2;
+    if (pp >= 1u) bend = bend + fj1;
+    if (hasTriple) bend = bend + fi;
+
+    // chain bonds (conservative) and drive (its transpose product by hand)
+    d3 fb = {mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)};
+    f3 driveT = mk(0.f, 0.f, 0.f);   // share of bead pp+1: f_a Hd lF_pp
+    f3 that = mk(0.f, 0.f, 0.f);     // unit bond (pp, pp+1)
+    if (pp + 1u < n) {
+        const d3 d = dmin_image(p, r, rn1);
+        const d1 len = dsqrt(ddot(d, d));
+        fb = (dconst(p.bondSpringK) * (len - dconst(p.bondSpringL)) / len) * d;
+        that = d.v / len.v;
+        // Hd lF = (lF - t (t . lF)) / len
+        const f3 hl = (lam - that * dot(that, lam)) / len.v;
+        driveT = kp.e.driveForce * hl;
+    }
+    const d3 fbPrev = dshfl_up(fb, 1);
+    const f3 driveFromPrev = shfl_up3(driveT, 1);
+    d3 bond = {mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)};
+    if (pp >= 1u) bond = bond - fbPrev;
+    if (pp + 1u < n) bond = bond + fb;
+
+    double acc[kAdjParams] = {0.0, 0.0, 0.0, 0.0};
+    if (m.active) {
+        f3 R = bend.t + bond.t;
+        if (pp + 1u < n) R -= driveT;
+        if (pp >= 1u) R += driveFromPrev;
+        a.Rb[m.bead] = f4(R, 0.0f);
+        // parameter partials lF . dF/dtheta (F the force this step produced)
+        if (pp + 1u < n) acc[kAdjDrive] = double(dot(lam, that));
+        if (p.bondBendingK != 0.0f) acc[kAdjKappa] = double(dot(lam, bend.v)) / double(p.bondBendingK);
+    }
+    if (m.active && a.langevin) {
+        const f3 v = mk(a.vel[m.bead]);
+        const u4 b = philox_block(a.seed, a.offLgv + m.bead);
+        float z0, z1, z2, z3;
+        box_muller(b.x, b.y, z0, z1);
+        box_muller(b.z, b.w, z2, z3);
+        (void)z3;
+        const f3 xi = mk(z0 * a.lgvStd + 0.0f, z1 * a.lgvStd + 0.0f, z2 * a.lgvStd + 0.0f);
+        // F_lgv = -gamma v + sigma(gamma) z, d sigma / d gamma = sigma / (2 gamma)
+        acc[kAdjGamma] = -double(dot(lam, v)) + double(dot(lam, xi)) / (2.0 * double(a.gamma));
+    }
+    block_partials(acc, a.partial);  // every thread, once (a workgroup barrier inside)
+}
+
+// ---------------------------------------------------------------------------
+// pair adjoint and the step's adjoint update, per sorted bead: R = lr + Rb +
+// sum_j H_ij (lF_i - lF_j), H = f I + 2 f' d d^T for F_i = -f(|d|^2) d,
+// d = r_j - r_i; then the new (lr, lv, lF, lG) of the bead
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_adj_pair(AdjPairArgs a, KP kp) {
+    const SimParams &p = kp.p;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[kAdjParams] = {0.0, 0.0, 0.0, 0.0};
+    if (i < a.n) {
+        const uint32_t me = a.ids[i];
+        const f3 ri = mk(a.sPos[i]);
+        const f3 li = mk(a.lF[me]);
+        uint32_t fil, ch;
+        worm_of(kp, me, fil, ch);
+        const uint32_t c = a.sCell[i];
+        const uint32_t Gx = p.gridSize.x, Gy = p.gridSize.y, layer = Gx * Gy;
+        const uint32_t gz = c / layer, rem = c - gz * layer;
+        const uint32_t gyu = rem / Gx;
+        const int gx = int(rem - gyu * Gx), gy = int(gyu);
+        const AxisCells ay = axis_cells(gy, Gy, false);
+        const float rc2 = kp.e.wcaCutoff * kp.e.wcaCutoff;
+        const float eps = kp.e.wcaEpsilon, s2c = kp.e.wcaSigma * kp.e.wcaSigma;
+        f3 Rp = mk(0.f, 0.f, 0.f);
+        double epsPart = 0.0;
+        auto run = [&](uint32_t lo, uint32_t hi) {
+            for (uint32_t j = lo; j < hi; ++j) {
+                const uint32_t other = a.ids[j];
+                uint32_t fil2, ch2;
+                worm_of(kp, other, fil2, ch2);
+                if (fil2 == fil) continue;
+                const f3 d = min_image_pair(p, ri, mk(a.sPos[j]));
+                const float rr = dot(d, d);
+                if (!(rr < rc2)) continue;
+                const float u = __builtin_amdgcn_rcpf(rr);
+                const float s6 = s2c * u * (s2c * u) * (s2c * u);
+                const float f = 24.0f * eps * u * s6 * (2.0f * s6 - 1.0f);
+                const float fp = -24.0f * eps * u * u * s6 * (14.0f * s6 - 4.0f);  // df / d(r^2)
+                const f3 dl = li - mk(a.lF[other]);                                  // lF_i - lF_j
+                // H dl = f dl + 2 f' (d . dl) d
+                Rp += f * dl + (2.0f * fp * dot(d, dl)) * d;
+                epsPart += double(dot(li, d)) * double(-f) / double(eps);
+            }
+        };
+#pragma unroll
+        for (int yi = 0; yi < 3; ++yi) {
+            if (yi >= ay.n) break;
+            const uint32_t base = (gz * Gy + uint32_t(ay.c[yi])) * Gx;
+            const int cx = gx;
+            uint32_t ia0, ia1, ib0 = 0, ib1 = 0;
+            if (cx >= 1 && cx + 1 < int(Gx)) {
+                ia0 = base + cx - 1;
+                ia1 = base + cx + 2;
+            } else if (cx == 0) {
+                ia0 = base + Gx - 1;
+                ia1 = base + Gx;
+                ib0 = base;
+                ib1 = base + 2;
+            } else {
+                ia0 = base + Gx - 2;
+                ia1 = base + Gx;
+                ib0 = base;
+                ib1 = base + 1;
+            }
+            run(a.off[ia0], a.off[ia1]);
+            if (ib1 > ib0) run(a.off[ib0], a.off[ib1]);
+        }
+        acc[kAdjEpsilon] = epsPart;
+
+        // adjoint of the step for bead me
+        const f3 R = mk(a.lr[me]) + mk(a.Rb[me]) + Rp;
+        const f3 V = mk(a.lv[me]) - p.gamma * li + a.dt * R;
+        const f3 lG = mk(a.lG[me]);
+        a.lr[me] = f4(R, 0.0f);
+        a.lv[me] = f4(V, 0.0f);
+        a.lFout[me] = f4(lG + (0.5f * a.dt * a.dt) * R + (0.5f * a.dt) * V, 0.0f);
+        a.lG[me] = f4((0.5f * a.dt) * V, 0.0f);
+    }
+    block_partials(acc, a.partial);
+}
+
+// ---------------------------------------------------------------------------
+// d S / d r of the bond-tangent order S = largest eigenvalue of
+// <1.5 b b - 0.5 I>: with e its unit eigenvector, dS/db = 3 (e . b) e / Nb,
+// through b = d / |d|, d = r[p+1] - r[p]
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_bond_order_grad(const float4 *__restrict__ pos, uint32_t W,
+                                                            float ex, float ey, float ez, float scale,
+                                                            float4 *__restrict__ out, KP kp) {
+    const SimParams &p = kp.p;
+    const uint32_t n = p.filamentSize;
+    const WaveWorm m = wave_worm(n, W);
+    const f3 r = mk(m.active ? pos[m.bead] : make_float4(0.f, 0.f, 0.f, 0.f));
+    const f3 rn = shfl_down3(r, 1);
+    f3 g = mk(0.f, 0.f, 0.f);
+    if (m.active && m.pp + 1u < n) {
+        const f3 d = min_image(p, r, rn);
+        const float len = sqrtf(dot(d, d));
+        const f3 b = d / len;
+        const f3 e = mk(ex, ey, ez);
+        const f3 gb = (scale * dot(e, b)) * e;
+        g = (gb - b * dot(b, gb)) / len;  // dS/dd
+    }
+    const f3 gPrev = shfl_up3(g, 1);
+    if (!m.active) return;
+    f3 o = mk(0.f, 0.f, 0.f);
+    if (m.pp + 1u < n) o -= g;
+    if (m.pp >= 1u) o += gPrev;
+    out[m.bead] = f4(o, 0.0f);
+}
+
+// out[k] += sum over rows of partial[row][k], rows in order
+__global__ __launch_bounds__(kAdjParams) void k_adj_total(const double *__restrict__ partial, uint64_t rows,
+                                                          double *__restrict__ out) {
+    const uint32_t k = threadIdx.x;
+    if (k >= uint32_t(kAdjParams)) return;
+    double s = 0.0;
+    for (uint64_t r = 0; r < rows; ++r) s += partial[r * kAdjParams + k];
+    out[k] = out[k] + s;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+uint32_t adj_bonded_blocks(uint32_t W, uint32_t n) {
+    const uint32_t wpw = 64u / n;
+    return grid_for(((uint64_t(W) + wpw - 1) / wpw) * 64u);
+}
+
+uint32_t adj_pair_blocks(uint32_t N) { return grid_for(N); }
+
+hipError_t launch_adj_bonded(const AdjBondedArgs &a, const KP &kp, hipStream_t st) {
+    const uint32_t blocks = adj_bonded_blocks(a.W, kp.p.filamentSize);
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(k_adj_bonded, dim3(blocks), dim3(kBlock), 0, st, a, kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_pair(const AdjPairArgs &a, const KP &kp, hipStream_t st) {
+    const uint32_t blocks = adj_pair_blocks(a.n);
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(k_adj_pair, dim3(blocks), dim3(kBlock), 0, st, a, kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_bond_order_grad(const float4 *pos, uint32_t W, const double e[3], double scale, float4 *out,
+                                  const KP &kp, hipStream_t st) {
+    const uint32_t blocks = adj_bonded_blocks(W, kp.p.filamentSize);
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(k_bond_order_grad, dim3(blocks), dim3(kBlock), 0, st, pos, W, float(e[0]), float(e[1]),
+                       float(e[2]), float(scale), out, kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_total(const double *partial, uint64_t rows, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_adj_total, dim3(1), dim3(kAdjParams), 0, st, partial, rows, out);
+    return hipGetLastError();
+}
+
+}  // namespace nw
