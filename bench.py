@@ -151,6 +151,63 @@ def batched_form(args):
     return out
 
 
+def adjoint_form(args):
+    """Dry worms, 2,000 x 16 beads in a periodic 200 x 200 box (WCA + bonds
+    k 57 r0 0.8 + bending kappa 5 + drive 1 + Langevin gamma 2 kT 0.1 with
+    FDT noise, dt 1e-3): `steps` forward steps alone,
+    then the same number differentiated -- forward with checkpoints every
+    100 steps, loss = final bond-tangent nematic order S, adjoint sweep with
+    recompute and replayed noise -- giving d S / d (f_a, kappa, epsilon, gamma).
+    Starts from rows of straight worms (lattice start)."""
+    import ctypes
+    from nwamd import abi, system
+    from nwamd.abi import DeviceArray
+    lib = abi.load()
+    c = system.NWSystemConfig()
+    c.numFilaments, c.filamentSize = 2000, 16
+    G = int(200.0 / 2.0 ** (1.0 / 6.0))
+    c.gridSize = abi.UInt3(G, G, 1)
+    c.boxSize = abi.Float3(200.0, 200.0, 200.0 / G)
+    c.langevinFdt = 1
+    c.seed = 4242
+    c.ext = system.wca_ext(stencil=abi.NW_STENCIL_2D, srd_beads=0)
+    s = system.System(c)
+    system.apply_worm_physics(s, kappa=5.0, gamma=2.0, kT=0.1)
+    # 32,000 beads in 200 x 200 fill 63 % of the area: rows of straight
+    # worms with random polarity (rejection sampling would not finish)
+    s.reset(system.NW_INIT_LATTICE_RANDOM, 0.1)
+    N, W, steps, K, dt = s.n, c.numFilaments, args.adjoint_steps, 100, args.dt
+    s.update(dt, 20)
+    s.sync()
+    t0 = time.perf_counter()
+    s.update(dt, steps)
+    s.sync()
+    t_fwd = time.perf_counter() - t0
+    p = s.params
+    lib.setParameters(ctypes.byref(p))
+    lib.nwSetExtParameters(ctypes.byref(c.ext))
+    seed, off = s.rng_state()
+    lib.nwRngSetSeed(seed, off)
+    arr = [ctypes.c_void_p(s.device(k)) for k in (system.POSITION, system.VELOCITY, system.FORCE, system.FORCE_OLD,
+                                                  system.TANGENT)]
+    dS = DeviceArray(lib, 16 * N)
+    g = (ctypes.c_double * 4)()
+    lib.threadSync()
+    t0 = time.perf_counter()
+    lib.nwAdjointForward(*arr, W, dt, p.gamma, p.kbT / dt, 1, steps, K)
+    S = lib.nwBondOrderGrad(arr[0], W, dS.p)
+    t1 = time.perf_counter()
+    lib.nwAdjointBackward(dS.p, None, g)
+    t2 = time.perf_counter()
+    s.close()
+    return dict(workload="2000 worms x 16 beads, L 200, dry", beads=N, steps=steps, checkpoint_every=K,
+                forward_ms=round(t_fwd * 1e3, 2), forward_value=round(N * steps / t_fwd, 1),
+                checkpointed_forward_ms=round((t1 - t0) * 1e3, 2), backward_ms=round((t2 - t1) * 1e3, 2),
+                adjoint_over_forward=round((t2 - t0) / t_fwd, 3), backward_over_forward=round((t2 - t1) / t_fwd, 3),
+                value=round(N * steps / (t2 - t0), 1), unit="bead-steps/s (forward + adjoint)",
+                loss_S=round(S, 6), grad=dict(drive=g[0], kappa=g[1], epsilon=g[2], gamma=g[3]))
+
+
 def timers(lib, names, enable):
     for n in names:
         lib.nwTimerEnable(n.encode(), 1 if enable else 0)
@@ -189,6 +246,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=200, help="timed steps of the batched form (0: skip)")
     ap.add_argument("--one-per-launch-steps", type=int, default=20,
                     help="timed steps of the replicas one launch sequence each (0: skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=2000,
+                    help="steps of the differentiated dry-worm run (0: skip)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="at the end, one elementwise kernel reading and writing this many MiB "
                          "(a known-size reference for HBM byte counters)")
@@ -272,6 +331,7 @@ def main():
         s2.close()
 
     batched = batched_form(args) if args.batched_steps > 0 else None
+    adjoint = adjoint_form(args) if args.adjoint_steps > 0 else None
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "bead-steps/s", "n_gpus": 1,
@@ -281,7 +341,7 @@ def main():
         "config": {"workload": args.workload, "beads": w["N"], "solvent": w["Ns"], "md_cells": w["C"],
                    "srd_cells": w["Cs"], "dt": dt,
                    "form": "plain: one replica, ParticleSystem::update through the fused C-ABI entry points"},
-        "roofline": roof, "cpu_baseline": cpu, "per_call_sequence": per_call, "batched": batched,
+        "roofline": roof, "cpu_baseline": cpu, "per_call_sequence": per_call, "batched": batched, "adjoint": adjoint,
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)

@@ -44,7 +44,7 @@ void place_worms(const SimParams &p, float sigma, int mode, float kT, HostStream
         std::vector<float> cand(3 * n);
         for (uint w = 0; w < W; ++w) {
             bool placed = false;
-            for (int attempt = 0; attempt < 100000 && !placed; ++attempt) {
+            for (int attempt = 0; attempt < 20000 && !placed; ++attempt) {
                 const float hx = p.origin.x + frand() * Lx, hy = p.origin.y + frand() * Ly;
                 const float th = frand() * twopi;
                 bool ok = true;

@@ -487,7 +487,8 @@ void collideSolvent(float *pos, float *vel, float *solventCellCOM, float *random
 static bool fused_md(const KP &kp, float *pos, float *vel, float *force, float *forceOld, float *tangent,
                      uint *gridParticleHash, uint *gridParticleIndex, float *sortedPos, float *sortedVel,
                      float *sortedTangent, float deltaTime, float langevinGamma, float langevinKbT, int langevin,
-                     uint numFilaments, uint32_t *offOut, uint32_t *cellOut) {
+                     uint numFilaments, uint32_t *offOut, uint32_t *cellOut, const float *posIn = nullptr,
+                     const float *velIn = nullptr) {
     State &s = S();
     const uint32_t n = kp.p.filamentSize;
     NW_REQUIRE_R(n >= 1 && n <= 64, false, "nwFusedMdStep: worms of %u beads (1..64 supported)", n);
@@ -517,6 +518,8 @@ static bool fused_md(const KP &kp, float *pos, float *vel, float *force, float *
     a.force = reinterpret_cast<float4 *>(force);
     a.forceOld = reinterpret_cast<float4 *>(forceOld);
     a.tangent = reinterpret_cast<float4 *>(tangent);
+    a.posIn = reinterpret_cast<const float4 *>(posIn);
+    a.velIn = reinterpret_cast<const float4 *>(velIn);
     a.hash = gridParticleHash ? gridParticleHash : (frontTiles ? nullptr : s.bucket.hash);
     a.pairs = frontTiles ? s.bucket.pairs : nullptr;
     a.P = s.bucket.P;
@@ -917,7 +920,8 @@ void nwAdjointForward(float *pos, float *vel, float *force, float *forceOld, flo
                "nwAdjointForward: WCA pairs with chain bonds only");
     NW_REQUIRE(p.activity == 0.0f && p.reverseProbability == 0.0f,
                "nwAdjointForward: no extensile pair term and no reversal");
-    NW_REQUIRE(kp.e.stencil == NW_STENCIL_2D || p.gridSize.z == 1, "nwAdjointForward: one-layer stencil only");
+    NW_REQUIRE((kp.e.stencil == NW_STENCIL_2D || p.gridSize.z == 1) && p.gridSize.x >= 3,
+               "nwAdjointForward: one-layer stencil and gridSize.x >= 3 only");
     NW_REQUIRE(p.boundaryX == PERIODIC && p.boundaryY == PERIODIC && p.boundaryZ == PERIODIC,
                "nwAdjointForward: periodic boxes only");
     NW_REQUIRE(!langevin || langevinGamma == p.gamma,
@@ -970,7 +974,7 @@ static bool ensure_segment(State::Adjoint &A) {
     return grow(A.segPos, A.capPos, (K + 1) * N) && grow(A.segVel, A.capVel, (K + 1) * N) &&
            grow(A.segSPos, A.capSPos, K * N) && grow(A.segIds, A.capIds, K * N) &&
            grow(A.segOff, A.capOff, K * (uint64_t(A.C) + 2)) && grow(A.segCell, A.capCell, K * N) &&
-           grow(A.work, A.capWork, 3 * N) && grow(A.lam, A.capLam, 6 * N);
+           grow(A.work, A.capWork, 3 * N) && grow(A.lam, A.capLam, 5 * N);
 }
 
 void nwAdjointBackward(const float *dLdPos, const float *dLdVel, double grad[4]) {
@@ -981,14 +985,12 @@ void nwAdjointBackward(const float *dLdPos, const float *dLdVel, double grad[4])
     NW_REQUIRE(dLdPos != nullptr && grad != nullptr, "nwAdjointBackward: NULL argument");
     const uint64_t N = A.N, K = A.K;
     NW_REQUIRE(ensure_segment(A), "nwAdjointBackward: out of device memory");
-    const uint32_t bRows = adj_bonded_blocks(A.W, A.kp.p.filamentSize), pRows = adj_pair_blocks(A.N);
-    const uint64_t rowsPerStep = uint64_t(bRows) + pRows;
+    const uint64_t rowsPerStep = adj_step_blocks(A.W, A.kp.p.filamentSize);
     NW_REQUIRE(grow(A.partial, A.capPartial, K * rowsPerStep * kAdjParams), "nwAdjointBackward: out of memory");
     if (!A.grad) NW_HIP(hipMalloc(&A.grad, kAdjParams * sizeof(double)));
     NW_HIP(hipMemsetAsync(A.grad, 0, kAdjParams * sizeof(double), s.stream));
     const size_t bytes = N * sizeof(float4);
-    float4 *lr = A.lam, *lv = A.lam + N, *lF[2] = {A.lam + 2 * N, A.lam + 3 * N}, *lG = A.lam + 4 * N,
-           *Rb = A.lam + 5 * N;
+    float4 *lr = A.lam, *lv = A.lam + N, *lF[2] = {A.lam + 2 * N, A.lam + 3 * N}, *lG = A.lam + 4 * N;
     NW_HIP(hipMemcpyAsync(lr, dLdPos, bytes, hipMemcpyDeviceToDevice, s.stream));
     if (dLdVel) NW_HIP(hipMemcpyAsync(lv, dLdVel, bytes, hipMemcpyDeviceToDevice, s.stream));
     else NW_HIP(hipMemsetAsync(lv, 0, bytes, s.stream));
@@ -1008,48 +1010,41 @@ void nwAdjointBackward(const float *dLdPos, const float *dLdVel, double grad[4])
         // recompute the segment, keeping every step's state and cell lists
         s.offset = A.ckptOffset[c];
         for (uint32_t j = 0; j < len; ++j) {
+            // step j reads slot j and writes slot j + 1
             float4 *pj = A.segPos + uint64_t(j + 1) * N, *vj = A.segVel + uint64_t(j + 1) * N;
-            NW_HIP(hipMemcpyAsync(pj, A.segPos + uint64_t(j) * N, bytes, hipMemcpyDeviceToDevice, s.stream));
-            NW_HIP(hipMemcpyAsync(vj, A.segVel + uint64_t(j) * N, bytes, hipMemcpyDeviceToDevice, s.stream));
             lgvOff[j] = s.offset;
             if (!fused_md(A.kp, reinterpret_cast<float *>(pj), reinterpret_cast<float *>(vj),
                           reinterpret_cast<float *>(F), reinterpret_cast<float *>(G), reinterpret_cast<float *>(T),
                           nullptr, A.segIds + uint64_t(j) * N, reinterpret_cast<float *>(A.segSPos + uint64_t(j) * N),
                           nullptr, nullptr, A.dt, A.gamma, A.kT, A.langevin, A.W,
-                          A.segOff + uint64_t(j) * (A.C + 2), A.segCell + uint64_t(j) * N))
+                          A.segOff + uint64_t(j) * (A.C + 2), A.segCell + uint64_t(j) * N,
+                          reinterpret_cast<const float *>(A.segPos + uint64_t(j) * N),
+                          reinterpret_cast<const float *>(A.segVel + uint64_t(j) * N)))
                 return;
         }
         // the adjoint through the segment, last step first
         for (int j = int(len) - 1; j >= 0; --j) {
             double *part = A.partial + uint64_t(j) * rowsPerStep * kAdjParams;
-            AdjBondedArgs ab;
-            ab.pos = A.segPos + uint64_t(j + 1) * N;
-            ab.vel = A.segVel + uint64_t(j + 1) * N;
-            ab.lF = lF[cur];
-            ab.Rb = Rb;
-            ab.partial = part;
-            ab.W = A.W;
-            ab.langevin = A.langevin;
-            ab.lgvStd = A.lgvStd;
-            ab.gamma = A.gamma;
-            ab.seed = A.seed;
-            ab.offLgv = lgvOff[j];
-            NW_HIP(launch_adj_bonded(ab, A.kp, s.stream));
-            AdjPairArgs ap;
-            ap.sPos = A.segSPos + uint64_t(j) * N;
-            ap.ids = A.segIds + uint64_t(j) * N;
-            ap.off = A.segOff + uint64_t(j) * (A.C + 2);
-            ap.sCell = A.segCell + uint64_t(j) * N;
-            ap.lF = lF[cur];
-            ap.Rb = Rb;
-            ap.lFout = lF[1 - cur];
-            ap.lr = lr;
-            ap.lv = lv;
-            ap.lG = lG;
-            ap.partial = part + uint64_t(bRows) * kAdjParams;
-            ap.n = A.N;
-            ap.dt = A.dt;
-            NW_HIP(launch_adj_pair(ap, A.kp, s.stream));
+            AdjStepArgs as;
+            as.pos = A.segPos + uint64_t(j + 1) * N;
+            as.vel = A.segVel + uint64_t(j + 1) * N;
+            as.sPos = A.segSPos + uint64_t(j) * N;
+            as.ids = A.segIds + uint64_t(j) * N;
+            as.off = A.segOff + uint64_t(j) * (A.C + 2);
+            as.lF = lF[cur];
+            as.lFout = lF[1 - cur];
+            as.lr = lr;
+            as.lv = lv;
+            as.lG = lG;
+            as.partial = part;
+            as.W = A.W;
+            as.langevin = A.langevin;
+            as.lgvStd = A.lgvStd;
+            as.gamma = A.gamma;
+            as.dt = A.dt;
+            as.seed = A.seed;
+            as.offLgv = lgvOff[j];
+            NW_HIP(launch_adj_step(as, A.kp, s.stream));
             cur = 1 - cur;
         }
         NW_HIP(launch_adj_total(A.partial, uint64_t(len) * rowsPerStep, A.grad, s.stream));

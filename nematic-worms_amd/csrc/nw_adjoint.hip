@@ -87,10 +87,19 @@ __device__ __forceinline__ void block_partials(double (&acc)[kAdjParams], double
 }
 
 // ---------------------------------------------------------------------------
-// bonded adjoint: Rb = (dB/dr)^T lF for bead p and the kappa, f_a, gamma
-// partials; whole worms per wavefront as in the forward front
+// One adjoint step, whole worms per wavefront (bead order):
+//   bonded:  Rb = (dB/dr')^T lF by dual numbers through the forward bending
+//            and bond expressions (chain neighbours by shuffles), the drive's
+//            transpose product written out; kappa, f_a, gamma partials
+//   pairs:   Rp = sum_j w H_ij (lF_i - lF_j), H = f I + 2 f' d d^T for
+//            F_i = -f(|d|^2) d, d = r_j - r_i, over the forward step's cell
+//            lists of r' (the bead's cell recomputed as the forward hashed
+//            it; w the stencil weight of the row); epsilon partial
+//   update:  R = lr + Rb + Rp, V = lv - gamma lF + dt R, then the new
+//            (lr, lv, lF, lG) of the bead (lF into the other buffer: the
+//            neighbours still read the old one)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_adj_bonded(AdjBondedArgs a, KP kp) {
+__global__ __launch_bounds__(kBlock) void k_adj_step(AdjStepArgs a, KP kp) {
     const SimParams &p = kp.p;
     const uint32_t n = p.filamentSize;
     const WaveWorm m = wave_worm(n, a.W);
@@ -125,8 +134,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_bonded(AdjBondedArgs a, KP kp) {
 
     // chain bonds (conservative) and drive (its transpose product by hand)
     d3 fb = {mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)};
-    f3 driveT = mk(0.f, 0.f, 0.f);   // share of bead pp+1: f_a Hd lF_pp
-    f3 that = mk(0.f, 0.f, 0.f);     // unit bond (pp, pp+1)
+    f3 driveT = mk(0.f, 0.f, 0.f);  // share of bead pp+1: f_a Hd lF_pp
+    f3 that = mk(0.f, 0.f, 0.f);    // unit bond (pp, pp+1)
     if (pp + 1u < n) {
         const d3 d = dmin_image(p, r, rn1);
         const d1 len = dsqrt(ddot(d, d));
@@ -147,104 +156,78 @@ __global__ __launch_bounds__(kBlock) void k_adj_bonded(AdjBondedArgs a, KP kp) {
         f3 R = bend.t + bond.t;
         if (pp + 1u < n) R -= driveT;
         if (pp >= 1u) R += driveFromPrev;
-        a.Rb[m.bead] = f4(R, 0.0f);
         // parameter partials lF . dF/dtheta (F the force this step produced)
         if (pp + 1u < n) acc[kAdjDrive] = double(dot(lam, that));
         if (p.bondBendingK != 0.0f) acc[kAdjKappa] = double(dot(lam, bend.v)) / double(p.bondBendingK);
-    }
-    if (m.active && a.langevin) {
-        const f3 v = mk(a.vel[m.bead]);
-        const u4 b = philox_block(a.seed, a.offLgv + m.bead);
-        float z0, z1, z2, z3;
-        box_muller(b.x, b.y, z0, z1);
-        box_muller(b.z, b.w, z2, z3);
-        (void)z3;
-        const f3 xi = mk(z0 * a.lgvStd + 0.0f, z1 * a.lgvStd + 0.0f, z2 * a.lgvStd + 0.0f);
-        // F_lgv = -gamma v + sigma(gamma) z, d sigma / d gamma = sigma / (2 gamma)
-        acc[kAdjGamma] = -double(dot(lam, v)) + double(dot(lam, xi)) / (2.0 * double(a.gamma));
-    }
-    block_partials(acc, a.partial);  // every thread, once (a workgroup barrier inside)
-}
+        if (a.langevin) {
+            const f3 v = mk(a.vel[m.bead]);
+            const u4 b = philox_block(a.seed, a.offLgv + m.bead);
+            float z0, z1, z2, z3;
+            box_muller(b.x, b.y, z0, z1);
+            box_muller(b.z, b.w, z2, z3);
+            (void)z3;
+            const f3 xi = mk(z0 * a.lgvStd + 0.0f, z1 * a.lgvStd + 0.0f, z2 * a.lgvStd + 0.0f);
+            // F_lgv = -gamma v + sigma(gamma) z, d sigma / d gamma = sigma / (2 gamma)
+            acc[kAdjGamma] = -double(dot(lam, v)) + double(dot(lam, xi)) / (2.0 * double(a.gamma));
+        }
 
-// ---------------------------------------------------------------------------
-// pair adjoint and the step's adjoint update, per sorted bead: R = lr + Rb +
-// sum_j H_ij (lF_i - lF_j), H = f I + 2 f' d d^T for F_i = -f(|d|^2) d,
-// d = r_j - r_i; then the new (lr, lv, lF, lG) of the bead
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_adj_pair(AdjPairArgs a, KP kp) {
-    const SimParams &p = kp.p;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    double acc[kAdjParams] = {0.0, 0.0, 0.0, 0.0};
-    if (i < a.n) {
-        const uint32_t me = a.ids[i];
-        const f3 ri = mk(a.sPos[i]);
-        const f3 li = mk(a.lF[me]);
-        uint32_t fil, ch;
-        worm_of(kp, me, fil, ch);
-        const uint32_t c = a.sCell[i];
-        const uint32_t Gx = p.gridSize.x, Gy = p.gridSize.y, layer = Gx * Gy;
-        const uint32_t gz = c / layer, rem = c - gz * layer;
-        const uint32_t gyu = rem / Gx;
-        const int gx = int(rem - gyu * Gx), gy = int(gyu);
+        // WCA pairs over the cell lists of r'
+        const uint32_t me = m.bead;
+        const uint32_t Gx = p.gridSize.x, Gy = p.gridSize.y;
+        const int gx = wrap_mod(cell_coord(rv.x, p.origin.x, p.cellSize.x), Gx);
+        const int gy = wrap_mod(cell_coord(rv.y, p.origin.y, p.cellSize.y), Gy);
+        const int gz = wrap_mod(cell_coord(rv.z, p.origin.z, p.cellSize.z), p.gridSize.z);
         const AxisCells ay = axis_cells(gy, Gy, false);
+        const float wz = kp.e.stencil == NW_STENCIL_2D ? 1.0f : axis_cells(gz, p.gridSize.z, false).w[0];
         const float rc2 = kp.e.wcaCutoff * kp.e.wcaCutoff;
         const float eps = kp.e.wcaEpsilon, s2c = kp.e.wcaSigma * kp.e.wcaSigma;
+        const uint32_t fil = m.worm;
         f3 Rp = mk(0.f, 0.f, 0.f);
         double epsPart = 0.0;
-        auto run = [&](uint32_t lo, uint32_t hi) {
-            for (uint32_t j = lo; j < hi; ++j) {
-                const uint32_t other = a.ids[j];
-                uint32_t fil2, ch2;
-                worm_of(kp, other, fil2, ch2);
-                if (fil2 == fil) continue;
-                const f3 d = min_image_pair(p, ri, mk(a.sPos[j]));
-                const float rr = dot(d, d);
-                if (!(rr < rc2)) continue;
-                const float u = __builtin_amdgcn_rcpf(rr);
-                const float s6 = s2c * u * (s2c * u) * (s2c * u);
-                const float f = 24.0f * eps * u * s6 * (2.0f * s6 - 1.0f);
-                const float fp = -24.0f * eps * u * u * s6 * (14.0f * s6 - 4.0f);  // df / d(r^2)
-                const f3 dl = li - mk(a.lF[other]);                                  // lF_i - lF_j
-                // H dl = f dl + 2 f' (d . dl) d
-                Rp += f * dl + (2.0f * fp * dot(d, dl)) * d;
-                epsPart += double(dot(li, d)) * double(-f) / double(eps);
-            }
-        };
 #pragma unroll
         for (int yi = 0; yi < 3; ++yi) {
-            if (yi >= ay.n) break;
-            const uint32_t base = (gz * Gy + uint32_t(ay.c[yi])) * Gx;
-            const int cx = gx;
-            uint32_t ia0, ia1, ib0 = 0, ib1 = 0;
-            if (cx >= 1 && cx + 1 < int(Gx)) {
-                ia0 = base + cx - 1;
-                ia1 = base + cx + 2;
-            } else if (cx == 0) {
-                ia0 = base + Gx - 1;
-                ia1 = base + Gx;
-                ib0 = base;
-                ib1 = base + 2;
-            } else {
-                ia0 = base + Gx - 2;
-                ia1 = base + Gx;
-                ib0 = base;
-                ib1 = base + 1;
+            if (yi < ay.n) {
+                const float w = wz * ay.w[yi];
+                const uint32_t base = (uint32_t(gz) * Gy + uint32_t(ay.c[yi])) * Gx;
+                const bool mid = gx >= 1 && gx + 1 < int(Gx), left = gx == 0;
+                const uint32_t ia0 = mid ? base + gx - 1 : (left ? base + Gx - 1 : base + Gx - 2);
+                const uint32_t ia1 = mid ? base + gx + 2 : base + Gx;
+                const uint32_t ib1 = mid ? base : (left ? base + 2 : base + 1);
+                for (int part = 0; part < 2; ++part) {
+                    const uint32_t lo = part == 0 ? a.off[ia0] : a.off[base];
+                    const uint32_t hi = part == 0 ? a.off[ia1] : a.off[ib1];
+                    for (uint32_t j = lo; j < hi; ++j) {
+                        const uint32_t other = a.ids[j];
+                        uint32_t fil2, ch2;
+                        worm_of(kp, other, fil2, ch2);
+                        if (fil2 == fil) continue;
+                        const f3 d = min_image_pair(p, rv, mk(a.sPos[j]));
+                        const float rr = dot(d, d);
+                        if (!(rr < rc2)) continue;
+                        const float u = __builtin_amdgcn_rcpf(rr);
+                        const float s6 = s2c * u * (s2c * u) * (s2c * u);
+                        const float f = 24.0f * eps * u * s6 * (2.0f * s6 - 1.0f);
+                        const float fp = -24.0f * eps * u * u * s6 * (14.0f * s6 - 4.0f);  // df / d(r^2)
+                        const f3 dl = lam - mk(a.lF[other]);                                // lF_i - lF_j
+                        Rp += w * (f * dl + (2.0f * fp * dot(d, dl)) * d);                  // w H dl
+                        epsPart += double(w) * double(dot(lam, d)) * double(-f) / double(eps);
+                    }
+                }
             }
-            run(a.off[ia0], a.off[ia1]);
-            if (ib1 > ib0) run(a.off[ib0], a.off[ib1]);
         }
         acc[kAdjEpsilon] = epsPart;
+        (void)me;
 
-        // adjoint of the step for bead me
-        const f3 R = mk(a.lr[me]) + mk(a.Rb[me]) + Rp;
-        const f3 V = mk(a.lv[me]) - p.gamma * li + a.dt * R;
-        const f3 lG = mk(a.lG[me]);
-        a.lr[me] = f4(R, 0.0f);
-        a.lv[me] = f4(V, 0.0f);
-        a.lFout[me] = f4(lG + (0.5f * a.dt * a.dt) * R + (0.5f * a.dt) * V, 0.0f);
-        a.lG[me] = f4((0.5f * a.dt) * V, 0.0f);
+        // adjoint of the step for this bead
+        const f3 Rt = mk(a.lr[m.bead]) + R + Rp;
+        const f3 V = mk(a.lv[m.bead]) - p.gamma * lam + a.dt * Rt;
+        const f3 lG = mk(a.lG[m.bead]);
+        a.lr[m.bead] = f4(Rt, 0.0f);
+        a.lv[m.bead] = f4(V, 0.0f);
+        a.lFout[m.bead] = f4(lG + (0.5f * a.dt * a.dt) * Rt + (0.5f * a.dt) * V, 0.0f);
+        a.lG[m.bead] = f4((0.5f * a.dt) * V, 0.0f);
     }
-    block_partials(acc, a.partial);
+    block_partials(acc, a.partial);  // every thread, once (a workgroup barrier inside)
 }
 
 // ---------------------------------------------------------------------------
@@ -290,30 +273,21 @@ __global__ __launch_bounds__(kAdjParams) void k_adj_total(const double *__restri
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-uint32_t adj_bonded_blocks(uint32_t W, uint32_t n) {
+uint32_t adj_step_blocks(uint32_t W, uint32_t n) {
     const uint32_t wpw = 64u / n;
     return grid_for(((uint64_t(W) + wpw - 1) / wpw) * 64u);
 }
 
-uint32_t adj_pair_blocks(uint32_t N) { return grid_for(N); }
-
-hipError_t launch_adj_bonded(const AdjBondedArgs &a, const KP &kp, hipStream_t st) {
-    const uint32_t blocks = adj_bonded_blocks(a.W, kp.p.filamentSize);
+hipError_t launch_adj_step(const AdjStepArgs &a, const KP &kp, hipStream_t st) {
+    const uint32_t blocks = adj_step_blocks(a.W, kp.p.filamentSize);
     if (!blocks) return hipSuccess;
-    hipLaunchKernelGGL(k_adj_bonded, dim3(blocks), dim3(kBlock), 0, st, a, kp);
-    return hipGetLastError();
-}
-
-hipError_t launch_adj_pair(const AdjPairArgs &a, const KP &kp, hipStream_t st) {
-    const uint32_t blocks = adj_pair_blocks(a.n);
-    if (!blocks) return hipSuccess;
-    hipLaunchKernelGGL(k_adj_pair, dim3(blocks), dim3(kBlock), 0, st, a, kp);
+    hipLaunchKernelGGL(k_adj_step, dim3(blocks), dim3(kBlock), 0, st, a, kp);
     return hipGetLastError();
 }
 
 hipError_t launch_bond_order_grad(const float4 *pos, uint32_t W, const double e[3], double scale, float4 *out,
                                   const KP &kp, hipStream_t st) {
-    const uint32_t blocks = adj_bonded_blocks(W, kp.p.filamentSize);
+    const uint32_t blocks = adj_step_blocks(W, kp.p.filamentSize);
     if (!blocks) return hipSuccess;
     hipLaunchKernelGGL(k_bond_order_grad, dim3(blocks), dim3(kBlock), 0, st, pos, W, float(e[0]), float(e[1]),
                        float(e[2]), float(scale), out, kp);

@@ -8,32 +8,22 @@ namespace nw {
 // parameters of the gradient, in this order
 enum : int { kAdjDrive, kAdjKappa, kAdjEpsilon, kAdjGamma, kAdjParams };
 
-struct AdjBondedArgs {
-    const float4 *pos, *vel;  // r', v' of the step (bead order)
-    const float4 *lF;         // adjoint of the step's force
-    float4 *Rb;               // out: (dB/dr')^T lF
-    double *partial;          // out: one row of kAdjParams per workgroup
+struct AdjStepArgs {
+    const float4 *pos, *vel;        // r', v' of the step (bead order)
+    const float4 *sPos;             // r' in cell order
+    const uint32_t *ids, *off;      // the step's cell lists
+    const float4 *lF;               // adjoint of the step's force
+    float4 *lFout;                  // adjoint of the previous force (the other buffer)
+    float4 *lr, *lv, *lG;           // updated in place
+    double *partial;                // out: one row of kAdjParams per workgroup
     uint32_t W;
     int langevin;
-    float lgvStd, gamma;
-    uint64_t seed, offLgv;    // the step's Langevin draws
+    float lgvStd, gamma, dt;
+    uint64_t seed, offLgv;          // the step's Langevin draws
 };
 
-struct AdjPairArgs {
-    const float4 *sPos;       // r' in cell order
-    const uint32_t *ids, *off, *sCell;
-    const float4 *lF, *Rb;
-    float4 *lFout;            // adjoint of the previous step's force (new buffer)
-    float4 *lr, *lv, *lG;     // updated in place
-    double *partial;
-    uint32_t n;
-    float dt;
-};
-
-uint32_t adj_bonded_blocks(uint32_t W, uint32_t n);
-uint32_t adj_pair_blocks(uint32_t N);
-hipError_t launch_adj_bonded(const AdjBondedArgs &a, const KP &kp, hipStream_t st);
-hipError_t launch_adj_pair(const AdjPairArgs &a, const KP &kp, hipStream_t st);
+uint32_t adj_step_blocks(uint32_t W, uint32_t n);
+hipError_t launch_adj_step(const AdjStepArgs &a, const KP &kp, hipStream_t st);
 // d S / d r of the bond-tangent nematic order: e the unit eigenvector of the
 // largest eigenvalue of <1.5 b b - 0.5 I>, scale = 3 / bonds
 hipError_t launch_bond_order_grad(const float4 *pos, uint32_t W, const double e[3], double scale, float4 *out,

@@ -51,6 +51,7 @@ inline uint32_t md_tile_items(uint32_t filamentSize) { return 16u * (64u / filam
 
 struct MdFrontArgs {
     float4 *pos, *vel, *force, *forceOld, *tangent;
+    const float4 *posIn, *velIn;  // the step's input state when not pos / vel (NULL: in place)
     uint32_t *hash;  // may be NULL when the front is the tile pass
     uint2 *pairs;    // NULL: no tile pass in the front (cells to hash)
     uint32_t *P;

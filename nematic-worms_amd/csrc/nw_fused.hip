@@ -136,8 +136,8 @@ __global__ __launch_bounds__(TILE ? kFrontBlock : kBlock) void k_md_front(MdFron
         for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) lh[b] = 0u;
 
     const WaveWorm m = wave_worm(kp.p.filamentSize, a.W);
-    const FrontLane o = md_front_lane(kp, m, a.pos, a.vel, a.force, a.forceOld, a.dt, a.lgvStd, a.langevin, a.seed,
-                                      a.offRev, a.offLgv);
+    const FrontLane o = md_front_lane(kp, m, a.posIn ? a.posIn : a.pos, a.velIn ? a.velIn : a.vel, a.force,
+                                      a.forceOld, a.dt, a.lgvStd, a.langevin, a.seed, a.offRev, a.offLgv);
     uint32_t h = 0, rank = 0;
     if (TILE) __syncthreads();  // histogram cleared
     if (m.active) {

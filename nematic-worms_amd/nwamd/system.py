@@ -139,6 +139,10 @@ class System:
         self.lib.nwSystemRngState(self.h, ctypes.byref(s), ctypes.byref(o))
         return s.value, o.value
 
+    def device(self, which):
+        """Device pointer of one of the system's arrays (POSITION ... FORCE_OLD)."""
+        return self.lib.nwSystemDeviceArray(self.h, which)
+
     def observables(self):
         """nwObservables of the current configuration as a dict."""
         o = abi.NWObservables()
