@@ -48,7 +48,8 @@ ALG_BYTES = {
     "srd_stream": lambda w: 64 * w["Ns"],                      # RMW pos vel of the solvent
     # fused path (nwFusedMdStep / nwFusedSrdStep); Pm / Ps = bytes of the
     # sort's tile x bucket prefix matrix (written by the tile pass, read once)
-    "md_front": lambda w: 156 * w["N"] + w["Pm"],              # R pos vel f f_old, W + tangent, W cell, W pair
+    "md_front": lambda w: 148 * w["N"],                        # R pos vel f f_old, W + tangent, W cell
+    "md_tiles": lambda w: 12 * w["N"] + w["Pm"],               # R cell, W pair
     "md_sort": lambda w: 48 * w["N"] + 4 * w["C"] + w["Pm"],   # R pairs, W ids cells, gather pos, W offsets
     "md_collide": lambda w: (56 if w["wca"] else 88) * w["N"] + 4 * w["C"],  # R sPos sIdx sCell (+sVel sTan), RMW f
     "srdf_tiles": lambda w: 28 * w["NT"] + w["Ps"],            # R pos, W cell, W pair
@@ -74,14 +75,14 @@ def bucket_plan(n, cells, tile):
 
 # families that are one kernel (the roofline is quoted for a single kernel)
 SINGLE_KERNEL = {"integrate", "langevin", "filament", "hash", "collide", "srd_com", "srd_thermo",
-                 "srd_hash", "srd_collide", "srd_stream", "md_front", "md_sort", "md_collide", "srdf_tiles",
-                 "srdf_cells", "srdf_rotate"}
+                 "srd_hash", "srd_collide", "srd_stream", "md_front", "md_tiles", "md_sort", "md_collide",
+                 "srdf_tiles", "srdf_cells", "srdf_rotate"}
 
 
 # kernel behind each single-kernel family of the fused path (the names that
 # rocprofv3 reports), for the HBM traffic measured by its counters
-FAMILY_KERNEL = {"md_front": "k_md_front", "md_sort": "k_bucket_cells<false>",
-                 "md_collide": "k_collide_rows<1, false, true>", "srdf_tiles": "k_srd_tiles",
+FAMILY_KERNEL = {"md_front": "k_md_front", "md_tiles": "k_tiles_staged<false, 4>", "md_sort": "k_bucket_cells<false>",
+                 "md_collide": "k_collide_rows<1, false, true>", "srdf_tiles": "k_tiles_staged<true, 16>",
                  "srdf_cells": "k_bucket_cells<true>", "srdf_rotate": "k_srd_rotate"}
 
 
@@ -112,8 +113,7 @@ def build_workload(name, per_call=False):
     p = s.params
     w = dict(N=s.n, Ns=s.ns, NT=s.n + s.ns, C=p.numCells, Cs=s.solvent.numCells if cfg.srd else 0,
              wca=cfg.ext.pairPotential == 1)
-    n = p.filamentSize
-    B, tiles = bucket_plan(w["N"], w["C"], 16 * (64 // n) * n)
+    B, tiles = bucket_plan(w["N"], w["C"], 0)
     w["Pm"] = 4 * tiles * (B + 1)
     B, tiles = bucket_plan(w["NT"], w["Cs"], 0) if cfg.srd else (0, 0)
     w["Ps"] = 4 * tiles * (B + 1)

@@ -207,7 +207,7 @@ void nwRadixSortPairs(uint *dGridParticleHash, uint *dGridParticleIndex,
    SimParams.reverseProbability > 0), langevinThermostat (when `langevin`),
    filamentForces, calcHash, sortParticles, reorderDataAndFindCellStart and
    collideFilaments -- in the order ParticleSystem::update issues them
-   (simulation/particleSystem.cpp:257-330) -- in three launches, for worms of
+   (simulation/particleSystem.cpp:257-330) -- in four launches, for worms of
    1..64 beads.  No counterpart in the reference.  Draws advance the generator
    exactly as the separate calls do.  On return gridParticleIndex holds the
    bead ids in stable cell order, sortedPos (and sortedVel / sortedTangent,

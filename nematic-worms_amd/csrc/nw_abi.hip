@@ -501,11 +501,7 @@ static bool fused_md(const KP &kp, float *pos, float *vel, float *force, float *
     NW_REQUIRE_R(!needVel || sortedVel != nullptr, false, "nwFusedMdStep: sortedVel is NULL");
     NW_REQUIRE_R(!needTan || sortedTangent != nullptr, false, "nwFusedMdStep: sortedTangent is NULL");
     if (N == 0) return true;
-    // the front's workgroups are the sort's tiles while the tile x bucket
-    // table stays small; larger systems take a tile pass of their own
-    BucketPlan bp = make_bucket_plan(N, cells, md_tile_items(n));
-    const bool frontTiles = !*bucket_plan_problem(bp);
-    if (!frontTiles) bp = make_bucket_plan(N, cells, 0);
+    const BucketPlan bp = make_bucket_plan(N, cells, 0);
     NW_REQUIRE_R(!*bucket_plan_problem(bp), false, "nwFusedMdStep: %s", bucket_plan_problem(bp));
     NW_HIP_R(ensure_bucket_scratch(s.bucket, bp), false);
     BucketScratch sc = s.bucket;
@@ -520,10 +516,7 @@ static bool fused_md(const KP &kp, float *pos, float *vel, float *force, float *
     a.tangent = reinterpret_cast<float4 *>(tangent);
     a.posIn = reinterpret_cast<const float4 *>(posIn);
     a.velIn = reinterpret_cast<const float4 *>(velIn);
-    a.hash = gridParticleHash ? gridParticleHash : (frontTiles ? nullptr : s.bucket.hash);
-    a.pairs = frontTiles ? s.bucket.pairs : nullptr;
-    a.P = s.bucket.P;
-    a.bp = bp;
+    a.hash = gridParticleHash ? gridParticleHash : s.bucket.hash;
     a.dt = deltaTime;
     a.W = numFilaments;
     a.seed = s.seed;
@@ -537,7 +530,7 @@ static bool fused_md(const KP &kp, float *pos, float *vel, float *force, float *
         TimedScope ts("md_front", s.stream);
         NW_HIP_R(launch_md_front(a, kp, s.stream), false);
     }
-    if (!frontTiles) {
+    {
         TimedScope ts("md_tiles", s.stream);
         NW_HIP_R(launch_hash_tiles(a.hash, bp, s.bucket, s.stream), false);
     }
@@ -1106,8 +1099,6 @@ double nwBondOrderGrad(const float *pos, uint numFilaments, float *dSdPos) {
 
 int nwFusedPlanFits(uint numParticles, uint numCells, uint filamentSize) {
     if (filamentSize > 64 || numCells == 0) return 0;
-    if (filamentSize && !*bucket_plan_problem(make_bucket_plan(numParticles, numCells, md_tile_items(filamentSize))))
-        return 1;
     return *bucket_plan_problem(make_bucket_plan(numParticles, numCells, 0)) ? 0 : 1;
 }
 

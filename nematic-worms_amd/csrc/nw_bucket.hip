@@ -5,10 +5,11 @@
 // G/s measured), so a histogram with one global counter per cell costs more
 // than the data it sorts, and a global scan over a tile x bucket matrix costs
 // launches.  Here every counter lives in LDS and no global scan is needed:
-//   level 1   (k_tiles, or the MD front kernel in nw_fused.hip) a tile
-//             of consecutive particles counts its members per bucket in LDS,
-//             scans the counts and writes the prefix as one row of P, then
-//             its (cell, id) pairs grouped by bucket into its pair region
+//   level 1   (k_tiles_staged, k_tiles) a tile of consecutive particles
+//             counts its members per bucket in LDS, scans the counts and
+//             writes the prefix as one row of P, then its (cell, id) pairs
+//             grouped by bucket into its pair region (staged in LDS and
+//             written out coalesced when the tile fits)
 //   level 2   (k_bucket_cells) a workgroup per bucket, buckets mapped to
 //             XCDs in contiguous runs so neighbours share L2 lines of P: the
 //             column sum of P at the bucket is its first sorted slot, the
@@ -20,6 +21,9 @@
 //   rotate    (k_srd_rotate) particle order, coalesced: every particle
 //             rotates about its cell's momentum, solvent streams
 // Results are deterministic and equal to a stable sort by cell.
+#include <utility>
+#include <vector>
+
 #include "nw_bucket.hpp"
 #include "nw_physics.hpp"
 
@@ -134,6 +138,54 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(const float4 *__restrict__
         const uint32_t slot = atomicAdd(&lh[h >> bp.shift], 1u);
         pairs[i0 + slot] = make_uint2(h, i);
     }
+}
+
+// The same level 1 with each thread's ITEMS cells and ranks kept in
+// registers (one read of the cells) and the tile's pairs staged in LDS in
+// bucket order, then written out with coalesced stores (tiles of
+// 1024 x ITEMS particles).
+template <bool FROM_POS, int ITEMS>
+__global__ __launch_bounds__(kTileBlock) void k_tiles_staged(const float4 *__restrict__ pos,
+                                                             uint32_t *__restrict__ hash,
+                                                             uint2 *__restrict__ pairs, uint32_t *__restrict__ P,
+                                                             BucketPlan bp, KP kp) {
+    extern __shared__ __align__(16) uint32_t lh[];
+    const uint32_t B1 = bp.buckets + 1;
+    uint32_t *wsum = lh + B1;
+    uint2 *stage = reinterpret_cast<uint2 *>(lh + ((B1 + kTileBlock / 64 + 3) & ~3u));
+    for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) lh[b] = 0u;
+    __syncthreads();
+    const uint32_t t = blockIdx.x;
+    const uint32_t i0 = t * bp.tileItems;
+    const uint32_t i1 = min(i0 + bp.tileItems, bp.n);
+    uint32_t h[ITEMS], rk[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t i = i0 + uint32_t(k) * kTileBlock + threadIdx.x;
+        h[k] = 0u;
+        rk[k] = 0u;
+        if (i < i1) {
+            if (FROM_POS) {
+                h[k] = srd_hash(kp.p, kp.s, mk(pos[i]));
+                hash[i] = h[k];
+            } else {
+                h[k] = hash[i];
+            }
+            rk[k] = atomicAdd(&lh[h[k] >> bp.shift], 1u);
+        }
+    }
+    __syncthreads();
+    block_excl_scan(lh, bp.buckets, wsum);
+    uint32_t *row = P + uint64_t(t) * B1;
+    for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) row[b] = lh[b];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t i = i0 + uint32_t(k) * kTileBlock + threadIdx.x;
+        if (i < i1) stage[lh[h[k] >> bp.shift] + rk[k]] = make_uint2(h[k], i);
+    }
+    __syncthreads();
+    const uint32_t cnt = i1 - i0;
+    for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) pairs[i0 + k] = stage[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -398,20 +450,50 @@ hipError_t ensure_bucket_scratch(BucketScratch &s, const BucketPlan &bp) {
     return hipSuccess;
 }
 
+// LDS beyond 64 KiB is granted per kernel, once (outside any graph capture:
+// the first launch of a size runs directly)
+static hipError_t grant_lds(const void *fn, size_t lds) {
+    static std::vector<std::pair<const void *, size_t>> granted;
+    if (lds <= 65536) return hipSuccess;
+    for (auto &g : granted)
+        if (g.first == fn && g.second >= lds) return hipSuccess;
+    const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e == hipSuccess) granted.emplace_back(fn, lds);
+    return e;
+}
+
+template <bool FROM_POS, int ITEMS>
+static hipError_t launch_staged(const float4 *pos, uint32_t *hash, const BucketPlan &bp, BucketScratch &s,
+                                const KP &kp, hipStream_t st) {
+    const size_t lds = (((size_t(bp.buckets) + 1 + kTileBlock / 64 + 3) & ~size_t(3)) * sizeof(uint32_t)) +
+                       size_t(bp.tileItems) * sizeof(uint2);
+    const void *fn = reinterpret_cast<const void *>(&k_tiles_staged<FROM_POS, ITEMS>);
+    hipError_t e = grant_lds(fn, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_tiles_staged<FROM_POS, ITEMS>), dim3(bp.tiles), dim3(kTileBlock), lds, st, pos, hash,
+                       s.pairs, s.P, bp, kp);
+    return hipGetLastError();
+}
+
+// staged kernel when the tile is 4096 or 16384 particles and its pairs and
+// histogram fit the 160 KiB of LDS; otherwise the two-pass kernel
 static hipError_t launch_tiles(bool fromPos, const float4 *pos, uint32_t *hash, const BucketPlan &bp,
                                BucketScratch &s, const KP &kp, hipStream_t st) {
     if (bp.n == 0) return hipSuccess;
+    const size_t stagedLds = (size_t(bp.buckets) + 1 + kTileBlock / 64 + 3) * 4 + size_t(bp.tileItems) * 8;
+    if (stagedLds <= 160 * 1024) {
+        if (bp.tileItems == 4 * kTileBlock)
+            return fromPos ? launch_staged<true, 4>(pos, hash, bp, s, kp, st)
+                           : launch_staged<false, 4>(pos, hash, bp, s, kp, st);
+        if (bp.tileItems == 16 * kTileBlock)
+            return fromPos ? launch_staged<true, 16>(pos, hash, bp, s, kp, st)
+                           : launch_staged<false, 16>(pos, hash, bp, s, kp, st);
+    }
     const size_t lds = (size_t(bp.buckets) + 1 + kTileBlock / 64) * sizeof(uint32_t);
     const void *fn = fromPos ? reinterpret_cast<const void *>(&k_tiles<true>)
                              : reinterpret_cast<const void *>(&k_tiles<false>);
-    // LDS beyond 64 KiB is granted per kernel, once (outside any graph capture:
-    // the first launch of a size runs directly)
-    static size_t granted[2] = {65536, 65536};
-    if (lds > granted[fromPos ? 1 : 0]) {
-        const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-        if (e != hipSuccess) return e;
-        granted[fromPos ? 1 : 0] = lds;
-    }
+    const hipError_t e = grant_lds(fn, lds);
+    if (e != hipSuccess) return e;
     if (fromPos)
         hipLaunchKernelGGL(k_tiles<true>, dim3(bp.tiles), dim3(kTileBlock), lds, st, pos, hash, s.pairs, s.P, bp, kp);
     else

@@ -28,8 +28,7 @@ constexpr uint32_t kMaxBuckets = 32767;   // level-1 LDS histogram (128 KiB of t
 constexpr uint64_t kMaxMatrix = 1ull << 24;  // entries of P
 constexpr int kCellsBlock = 256;          // level-2 workgroup
 
-// tileItems == 0: chosen for the particle count (the SRD tiles); the MD
-// tiles are the fused front kernel's workgroups
+// tileItems == 0: chosen for the particle count and the cells
 BucketPlan make_bucket_plan(uint64_t n, uint64_t cells, uint32_t tileItems);
 // empty string when the plan fits the limits above, else the reason
 const char *bucket_plan_problem(const BucketPlan &bp);
@@ -46,16 +45,10 @@ struct BucketScratch {
 
 hipError_t ensure_bucket_scratch(BucketScratch &s, const BucketPlan &bp);
 
-// beads per fused-front workgroup (the MD tile): 16 waves of whole worms
-inline uint32_t md_tile_items(uint32_t filamentSize) { return 16u * (64u / filamentSize) * filamentSize; }
-
 struct MdFrontArgs {
     float4 *pos, *vel, *force, *forceOld, *tangent;
     const float4 *posIn, *velIn;  // the step's input state when not pos / vel (NULL: in place)
-    uint32_t *hash;  // may be NULL when the front is the tile pass
-    uint2 *pairs;    // NULL: no tile pass in the front (cells to hash)
-    uint32_t *P;
-    BucketPlan bp;
+    uint32_t *hash;               // out: the cell of every bead
     float dt, lgvStd;
     int langevin;
     uint32_t W;
@@ -69,10 +62,9 @@ struct FrontLane {
     bool writeTan;
 };
 
-// integrate + reversal + Langevin + bonded + tangent + MD cell (+ level 1
-// when a.pairs is given: the workgroups of 1024 threads are the tiles)
+// integrate + reversal + Langevin + bonded + tangent + MD cell
 hipError_t launch_md_front(const MdFrontArgs &a, const KP &kp, hipStream_t st);
-// level 1 over precomputed cells (any tile size)
+// level 1 over precomputed cells
 hipError_t launch_hash_tiles(const uint32_t *hash, const BucketPlan &bp, BucketScratch &s, hipStream_t st);
 // level 2 for MD: ids in stable cell order, CSR offsets, cell of every
 // slot, gathered records (sVel / sTan may be NULL)

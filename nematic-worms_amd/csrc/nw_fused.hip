@@ -5,7 +5,7 @@
 // gathers re-read what the sort just touched.  The fused step is
 //   k_md_front      integrate + reversal + Langevin + bonded + tangent + cell
 //                   hash (whole worms per wavefront, chain neighbours by
-//                   cross-lane shuffles) + the tile pass of the cell sort
+//                   cross-lane shuffles)
 //   cell sort       nw_bucket.hip (stable, atomic-free, gathers the records)
 //   k_collide_rows  pair forces, neighbour rows staged in LDS, one flattened
 //                   candidate list per bead (k_collide_csr without LDS for
@@ -123,36 +123,11 @@ __device__ __forceinline__ uint32_t md_front_store(const KP &kp, const WaveWorm 
                            cell_coord(o.r.z, p.origin.z, p.cellSize.z));
 }
 
-constexpr int kFrontBlock = 1024;  // one level-1 tile of the cell sort
-
-// TILE: the workgroup is a tile of the cell sort (1024 threads, histogram in
-// LDS); otherwise the cells go to a.hash for the separate tile pass
-template <bool TILE>
-__global__ __launch_bounds__(TILE ? kFrontBlock : kBlock) void k_md_front(MdFrontArgs a, KP kp) {
-    extern __shared__ uint32_t lh[];  // bucket histogram (buckets + 1), wave sums
-    const BucketPlan &bp = a.bp;
-    const uint32_t B1 = bp.buckets + 1;
-    if (TILE)
-        for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) lh[b] = 0u;
-
+__global__ __launch_bounds__(kBlock) void k_md_front(MdFrontArgs a, KP kp) {
     const WaveWorm m = wave_worm(kp.p.filamentSize, a.W);
     const FrontLane o = md_front_lane(kp, m, a.posIn ? a.posIn : a.pos, a.velIn ? a.velIn : a.vel, a.force,
                                       a.forceOld, a.dt, a.lgvStd, a.langevin, a.seed, a.offRev, a.offLgv);
-    uint32_t h = 0, rank = 0;
-    if (TILE) __syncthreads();  // histogram cleared
-    if (m.active) {
-        h = md_front_store(kp, m, o, a.pos, a.vel, a.force, a.forceOld, a.tangent);
-        if (a.hash) a.hash[m.bead] = h;
-        if (TILE) rank = atomicAdd(&lh[h >> bp.shift], 1u);
-    }
-    if (!TILE) return;
-    __syncthreads();
-    // level 1 of the cell sort: this workgroup's beads are tile blockIdx.x
-    uint32_t *wsum = lh + B1;
-    block_excl_scan(lh, bp.buckets, wsum);
-    uint32_t *row = a.P + uint64_t(blockIdx.x) * B1;
-    for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) row[b] = lh[b];
-    if (m.active) a.pairs[uint64_t(blockIdx.x) * bp.tileItems + lh[h >> bp.shift] + rank] = make_uint2(h, m.bead);
+    if (m.active) a.hash[m.bead] = md_front_store(kp, m, o, a.pos, a.vel, a.force, a.forceOld, a.tangent);
 }
 
 // ===========================================================================
@@ -522,12 +497,7 @@ hipError_t launch_md_front(const MdFrontArgs &a, const KP &kp, hipStream_t st) {
     const uint32_t n = kp.p.filamentSize;
     const uint32_t wpw = 64u / n;
     const uint64_t threads = ((uint64_t(a.W) + wpw - 1) / wpw) * 64u;
-    if (a.pairs) {
-        const size_t lds = (size_t(a.bp.buckets) + 1 + kFrontBlock / 64) * sizeof(uint32_t);
-        hipLaunchKernelGGL(k_md_front<true>, dim3(grid_for(threads, kFrontBlock)), dim3(kFrontBlock), lds, st, a, kp);
-    } else {
-        hipLaunchKernelGGL(k_md_front<false>, dim3(grid_for(threads)), dim3(kBlock), 0, st, a, kp);
-    }
+    hipLaunchKernelGGL(k_md_front, dim3(grid_for(threads)), dim3(kBlock), 0, st, a, kp);
     return hipGetLastError();
 }
 
