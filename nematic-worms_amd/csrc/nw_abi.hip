@@ -762,7 +762,7 @@ void nwBatchSetReplicas(const NWReplicaParams *reps, uint numReplicas) {
         N += uint64_t(W) * n;
         C += cells;
         waves += (uint64_t(W) + 64u / n - 1) / (64u / n);
-        blocks += (uint64_t(W) * n + kBlock - 1) / kBlock;
+        blocks += (uint64_t(W) * n + kPairBlock - 1) / kPairBlock;
         NW_REQUIRE(N < (1ull << 31) && C < (1ull << 32) - 2, "nwBatchSetReplicas: batch too large");
         b.beadOff[r + 1] = uint32_t(N);
         b.cellOff[r + 1] = uint32_t(C);

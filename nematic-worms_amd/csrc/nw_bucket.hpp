@@ -27,6 +27,7 @@ constexpr uint32_t kMaxTiles = 4096;      // level-2 segment tables in LDS
 constexpr uint32_t kMaxBuckets = 32767;   // level-1 LDS histogram (128 KiB of the 160 KiB)
 constexpr uint64_t kMaxMatrix = 1ull << 24;  // entries of P
 constexpr int kCellsBlock = 256;          // level-2 workgroup
+constexpr int kPairBlock = 256;           // sorted beads per LDS pair-kernel workgroup
 
 // tileItems == 0: chosen for the particle count and the cells
 BucketPlan make_bucket_plan(uint64_t n, uint64_t cells, uint32_t tileItems);

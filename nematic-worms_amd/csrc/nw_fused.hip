@@ -296,7 +296,7 @@ __global__ __launch_bounds__(kBlock) void k_collide_csr(float4 *__restrict__ for
 // run) takes the global-memory walk.  Same order and arithmetic as
 // k_collide_csr.
 // ===========================================================================
-constexpr int kRunCap = 384;  // staged beads per row run
+constexpr int kRunCap = kPairBlock + 128;  // staged beads per row run
 
 // LDS of one pair-kernel workgroup
 template <int MODE, bool TAN>
@@ -356,6 +356,7 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
     if (i >= iEnd) return;
 
     const uint32_t me = sIdx[i];
+    const float4 Fo = force[me];  // overlaps the candidate walk
     const Owner<MODE, TAN> o = load_owner<MODE, TAN>(kp, i, me - beadBase, sPos, sVel, sTan);
     const float rc2 = kp.e.wcaCutoff * kp.e.wcaCutoff;
     const uint32_t c = sCell[i] - cellBase;
@@ -411,11 +412,11 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
     } else {
         add_all_global<MODE, TAN>(kp, o, rc2, acc, gx, gy, int(gz), offR, sPos, sVel, sTan, sIdx, beadBase);
     }
-    add_force(force, me, acc);
+    force[me] = make_float4(Fo.x + acc.x, Fo.y + acc.y, Fo.z + acc.z, Fo.w + 0.0f);
 }
 
 template <int MODE, bool TAN, bool UNIT>
-__global__ __launch_bounds__(kBlock) void k_collide_rows(float4 *__restrict__ force, const float4 *__restrict__ sPos,
+__global__ __launch_bounds__(kPairBlock) void k_collide_rows(float4 *__restrict__ force, const float4 *__restrict__ sPos,
                                                          const float4 *__restrict__ sVel,
                                                          const float4 *__restrict__ sTan,
                                                          const uint32_t *__restrict__ sIdx,
@@ -477,7 +478,7 @@ __global__ __launch_bounds__(kBlock) void k_md_front_batch(BatchArrays a, const 
 }
 
 template <int MODE, bool TAN, bool UNIT>
-__global__ __launch_bounds__(kBlock) void k_collide_rows_batch(BatchArrays a, const BatchRep *__restrict__ reps,
+__global__ __launch_bounds__(kPairBlock) void k_collide_rows_batch(BatchArrays a, const BatchRep *__restrict__ reps,
                                                                uint32_t R, uint64_t *__restrict__ stepCtr) {
     __shared__ RowsLds<MODE, TAN> L;
     // the step's last kernel advances the step counter the next front reads
@@ -510,7 +511,7 @@ hipError_t launch_collide_off(float4 *force, const float4 *sPos, const float4 *s
     const bool lds = ldsOk && kp.e.collideKernel != NW_COLLIDE_GLOBAL;
     const bool wca = kp.e.pairPotential == NW_PAIR_WCA;
     const bool tan = !wca || kp.p.activity != 0.0f;
-    const dim3 g(grid_for(n)), b(kBlock);
+    const dim3 g(grid_for(n, lds ? kPairBlock : kBlock)), b(lds ? kPairBlock : kBlock);
 #define NW_COLLIDE(K, ...) hipLaunchKernelGGL((K<__VA_ARGS__>), g, b, 0, st, force, sPos, sVel, sTan, sIdx, off, sCell, n, kp)
     if (lds) {
         // every row weight is 1 for the one-layer stencil over >= 3 rows
@@ -544,7 +545,7 @@ hipError_t launch_md_front_batch(const BatchArrays &a, const BatchRep *reps, uin
 hipError_t launch_collide_batch(const BatchArrays &a, const BatchRep *reps, uint32_t R, uint32_t blocks, int mode,
                                 bool tan, bool unit, uint64_t *stepCtr, hipStream_t st) {
     if (blocks == 0) return hipSuccess;
-    const dim3 g(blocks), b(kBlock);
+    const dim3 g(blocks), b(kPairBlock);
 #define NW_BATCH_COLLIDE(...) hipLaunchKernelGGL((k_collide_rows_batch<__VA_ARGS__>), g, b, 0, st, a, reps, R, stepCtr)
     if (mode == 1 && !tan) {
         if (unit) NW_BATCH_COLLIDE(1, false, true);
