@@ -75,7 +75,7 @@ __host__ __device__ inline CellsLayout cells_layout(const BucketPlan &bp, bool s
     CellsLayout L;
     const uint32_t nc = 1u << bp.shift;
     uint32_t o = 0;
-    L.wsum = o; o += align16(4u * (kCellsBlock / 64 + 4));
+    L.wsum = o; o += align16(4u * 2u * (kCellsBlockMax / 64));
     L.cnt = o;  o += align16(4u * (nc + 1));
     L.cur = o;  o += align16(4u * nc);
     L.lid = o;  o += align16(4u * kBucketCap);
@@ -218,8 +218,62 @@ __device__ __forceinline__ uint32_t seg_of(const uint32_t *segStart, uint32_t ti
     return lo;
 }
 
-template <bool SRD>
-__global__ __launch_bounds__(kCellsBlock) void k_bucket_cells(CellsArgs o, BucketPlan bp) {
+
+// MD level 2, staged bucket of m members: ids, cells and the gathered
+// records of the sorted slots, in groups of four members per thread with the
+// gathers of a group in flight together; past the last member a thread
+// handles member m - 1 again (the same values to the same addresses)
+template <bool POS, bool VEL, bool TAN, int BLOCK, int G>
+__device__ __forceinline__ void md_gather_group(const CellsArgs &o, const uint32_t *lid, const uint16_t *lcell,
+                                                uint32_t sb, uint32_t cell0, uint32_t m, uint32_t j0) {
+    // records moved as plain 4-vectors (register values, never a copy of a
+    // float4 object through memory)
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 *pos = reinterpret_cast<const v4 *>(o.pos), *vel = reinterpret_cast<const v4 *>(o.vel),
+             *tan = reinterpret_cast<const v4 *>(o.tan);
+    v4 *sPos = reinterpret_cast<v4 *>(o.sPos), *sVel = reinterpret_cast<v4 *>(o.sVel),
+       *sTan = reinterpret_cast<v4 *>(o.sTan);
+    const uint32_t tid = threadIdx.x;
+    uint32_t *out = o.ids + sb;
+    {
+        uint32_t k[G], id[G];
+        v4 P[G], V[G], Tn[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            k[j] = min(tid + (j0 + uint32_t(j)) * BLOCK, m - 1u);
+            id[j] = lid[k[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            if (POS) P[j] = pos[id[j]];
+            if (VEL) V[j] = vel[id[j]];
+            if (TAN) Tn[j] = tan[id[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            out[k[j]] = id[j];
+            o.cellOf[sb + k[j]] = cell0 + uint32_t(lcell[k[j]]);
+            if (POS) sPos[sb + k[j]] = P[j];
+            if (VEL) sVel[sb + k[j]] = V[j];
+            if (TAN) sTan[sb + k[j]] = Tn[j];
+        }
+    }
+}
+
+template <bool POS, bool VEL, bool TAN, int BLOCK>
+__device__ __forceinline__ void md_gather_staged(const CellsArgs &o, const uint32_t *lid, const uint16_t *lcell,
+                                                 uint32_t sb, uint32_t cell0, uint32_t m) {
+    constexpr int PER = int(kBucketCap) / BLOCK, G = PER < 4 ? PER : 4;
+#pragma unroll
+    for (int g = 0; g < PER; g += G)
+        if (m > uint32_t(g) * BLOCK) md_gather_group<POS, VEL, TAN, BLOCK, G>(o, lid, lcell, sb, cell0, m, g);
+}
+
+// BLOCK threads per bucket; a staged bucket gives each thread at most
+// PER = kBucketCap / BLOCK members
+template <bool SRD, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_bucket_cells(CellsArgs o, BucketPlan bp) {
+    constexpr int PER = int(kBucketCap) / BLOCK;
     extern __shared__ __align__(16) unsigned char smem[];
     const CellsLayout L = cells_layout(bp, SRD);
     uint32_t *wsum = reinterpret_cast<uint32_t *>(smem + L.wsum);
@@ -247,10 +301,10 @@ __global__ __launch_bounds__(kCellsBlock) void k_bucket_cells(CellsArgs o, Bucke
         part += lo;
     }
     for (int d = 32; d >= 1; d >>= 1) part += __shfl_xor(part, d, 64);
-    if ((tid & 63u) == 0u) wsum[kCellsBlock / 64 + (tid >> 6)] = part;  // upper half of wsum
+    if ((tid & 63u) == 0u) wsum[BLOCK / 64 + (tid >> 6)] = part;  // upper half of wsum
     __syncthreads();
     uint32_t sb = 0;
-    for (uint32_t w = 0; w < kCellsBlock / 64; ++w) sb += wsum[kCellsBlock / 64 + w];
+    for (uint32_t w = 0; w < BLOCK / 64; ++w) sb += wsum[BLOCK / 64 + w];
     block_excl_scan(segStart, bp.tiles, wsum);
     const uint32_t m = segStart[bp.tiles];
     const uint32_t cell0 = b << bp.shift;
@@ -260,15 +314,32 @@ __global__ __launch_bounds__(kCellsBlock) void k_bucket_cells(CellsArgs o, Bucke
 
     const bool staged = m <= kBucketCap;
     const uint64_t T = bp.tileItems;
-    for (uint32_t k = tid; k < m; k += bd) {
-        const uint32_t t = seg_of(segStart, bp.tiles, k);
-        const uint2 q = o.pairs[uint64_t(t) * T + segBase[t] + (k - segStart[t])];
-        const uint32_t lc = q.x - cell0;
-        if (staged) {
-            tmpId[k] = q.y;
-            tmpCell[k] = uint16_t(lc);
+    if (staged && m) {
+        // every load of the thread's members in flight before the first use
+        // (indices past the last member load the last member again, unused)
+        uint2 q[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t k = min(tid + uint32_t(j) * BLOCK, m - 1u);
+            const uint32_t t = seg_of(segStart, bp.tiles, k);
+            q[j] = o.pairs[uint64_t(t) * T + segBase[t] + (k - segStart[t])];
         }
-        atomicAdd(&cnt[lc], 1u);
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t k = tid + uint32_t(j) * BLOCK;
+            if (k < m) {
+                const uint32_t lc = q[j].x - cell0;
+                tmpId[k] = q[j].y;
+                tmpCell[k] = uint16_t(lc);
+                atomicAdd(&cnt[lc], 1u);
+            }
+        }
+    } else if (!staged) {
+        for (uint32_t k = tid; k < m; k += bd) {
+            const uint32_t t = seg_of(segStart, bp.tiles, k);
+            const uint2 q = o.pairs[uint64_t(t) * T + segBase[t] + (k - segStart[t])];
+            atomicAdd(&cnt[q.x - cell0], 1u);
+        }
     }
     __syncthreads();
     block_excl_scan(cnt, nc, wsum);
@@ -310,10 +381,17 @@ __global__ __launch_bounds__(kCellsBlock) void k_bucket_cells(CellsArgs o, Bucke
     __syncthreads();
 
     if (!SRD) {
+        if (staged) {
+            if (!o.sPos) md_gather_staged<false, false, false, BLOCK>(o, lid, lcell, sb, cell0, m);
+            else if (o.sVel && o.sTan) md_gather_staged<true, true, true, BLOCK>(o, lid, lcell, sb, cell0, m);
+            else if (o.sVel) md_gather_staged<true, true, false, BLOCK>(o, lid, lcell, sb, cell0, m);
+            else if (o.sTan) md_gather_staged<true, false, true, BLOCK>(o, lid, lcell, sb, cell0, m);
+            else md_gather_staged<true, false, false, BLOCK>(o, lid, lcell, sb, cell0, m);
+            return;
+        }
         for (uint32_t k = tid; k < m; k += bd) {
             const uint32_t id = arr[k];
-            if (staged) out[k] = id;
-            o.cellOf[sb + k] = cell0 + (staged ? uint32_t(lcell[k]) : seg_of(cnt, nc, k));
+            o.cellOf[sb + k] = cell0 + seg_of(cnt, nc, k);
             if (o.sPos) {
                 o.sPos[sb + k] = o.pos[id];
                 if (o.sVel) o.sVel[sb + k] = o.vel[id];
@@ -324,14 +402,24 @@ __global__ __launch_bounds__(kCellsBlock) void k_bucket_cells(CellsArgs o, Bucke
     }
 
     // SRD: stage the members' velocities (region A is dead now)
-    if (staged) {
-        for (uint32_t k = tid; k < m; k += bd) {
-            const uint32_t id = lid[k];
-            out[k] = id;
-            lv[k] = o.srdVel[id];
+    if (staged && m) {
+        float4 v[PER];
+        uint32_t id[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) id[j] = lid[min(tid + uint32_t(j) * BLOCK, m - 1u)];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) v[j] = o.srdVel[id[j]];
+        // lv holds kBucketCap entries: slots past the last member take
+        // copies, never read
+#pragma unroll
+        for (int j = 0; j < PER; ++j) lv[tid + uint32_t(j) * BLOCK] = v[j];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t k = tid + uint32_t(j) * BLOCK;
+            if (k < m) out[k] = id[j];
         }
-        __syncthreads();
     }
+    __syncthreads();
     // one thread per cell, sums in sorted order: the arithmetic of
     // isokineticThermostatSRD and cellCenterOfMomentumSRD
     // (particles_kernel_impl.cuh:559-637)
@@ -528,7 +616,8 @@ hipError_t launch_md_cells(const BucketPlan &bp, BucketScratch &s, uint32_t *ids
     o.sVel = sVel;
     o.sTan = sTan;
     const CellsLayout L = cells_layout(bp, false);
-    hipLaunchKernelGGL(k_bucket_cells<false>, dim3(bp.buckets), dim3(kCellsBlock), L.total, st, o, bp);
+    hipLaunchKernelGGL((k_bucket_cells<false, kMdCellsBlock>), dim3(bp.buckets), dim3(kMdCellsBlock), L.total, st, o,
+                       bp);
     return hipGetLastError();
 }
 
@@ -549,7 +638,8 @@ hipError_t launch_srd_cells(const BucketPlan &bp, BucketScratch &s, uint32_t *id
     o.seed = seed;
     o.offset = offset;
     const CellsLayout L = cells_layout(bp, true);
-    hipLaunchKernelGGL(k_bucket_cells<true>, dim3(bp.buckets), dim3(kCellsBlock), L.total, st, o, bp);
+    hipLaunchKernelGGL((k_bucket_cells<true, kSrdCellsBlock>), dim3(bp.buckets), dim3(kSrdCellsBlock), L.total, st,
+                       o, bp);
     return hipGetLastError();
 }
 

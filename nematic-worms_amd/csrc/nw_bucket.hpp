@@ -26,7 +26,9 @@ constexpr uint32_t kBucketCap = 2048;     // bucket members staged in LDS (more:
 constexpr uint32_t kMaxTiles = 4096;      // level-2 segment tables in LDS
 constexpr uint32_t kMaxBuckets = 32767;   // level-1 LDS histogram (128 KiB of the 160 KiB)
 constexpr uint64_t kMaxMatrix = 1ull << 24;  // entries of P
-constexpr int kCellsBlock = 256;          // level-2 workgroup
+constexpr int kMdCellsBlock = 256;        // level-2 workgroup, MD sort
+constexpr int kSrdCellsBlock = 256;       // level-2 workgroup, SRD sort
+constexpr int kCellsBlockMax = 1024;
 constexpr int kPairBlock = 256;           // sorted beads per LDS pair-kernel workgroup
 
 // tileItems == 0: chosen for the particle count and the cells
