@@ -458,41 +458,57 @@ __global__ __launch_bounds__(BLOCK) void k_bucket_cells(CellsArgs o, BucketPlan 
 // :651-691, solvent_integrator :145-205); the isokinetic factor of the cell
 // scales the velocity first, as isokineticThermostatSRD writes it back
 // ---------------------------------------------------------------------------
+constexpr int kRotatePer = 4;  // particles per thread, their loads in flight together
+
 __global__ __launch_bounds__(kBlock) void k_srd_rotate(const uint32_t *__restrict__ hash,
                                                        const float4 *__restrict__ com, const float *__restrict__ lam,
                                                        float4 *__restrict__ pos, float4 *__restrict__ vel,
-                                                       uint32_t NT, uint32_t N, float dt, float cosA, float sinA,
-                                                       int iso, KP kp) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= NT) return;
-    const bool solvent = i >= N;
+                                                       uint32_t i0, uint32_t NT, uint32_t N, float dt, float cosA,
+                                                       float sinA, int iso, KP kp) {
+    // particles i0 + blockIdx.x * kBlock * kRotatePer + j * kBlock + threadIdx.x;
+    // past NT a thread loads particle NT - 1 again and stores nothing
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 *com4 = reinterpret_cast<const v4 *>(com);
+    v4 *pos4 = reinterpret_cast<v4 *>(pos), *vel4 = reinterpret_cast<v4 *>(vel);
     const bool rotBeads = kp.e.srdCollideFilaments != 0;
-    if (!solvent && !rotBeads && !iso) return;
-    const uint32_t c = hash[i];
-    float4 V = vel[i];
-    if (iso) {
-        const float l = lam[c];
-        V = f4(l * mk(V), V.w);
+    const uint32_t base = i0 + blockIdx.x * (kBlock * kRotatePer) + threadIdx.x;
+    uint32_t c[kRotatePer];
+    v4 V[kRotatePer], C[kRotatePer], Pp[kRotatePer];
+    float l[kRotatePer];
+#pragma unroll
+    for (int j = 0; j < kRotatePer; ++j) c[j] = hash[min(base + uint32_t(j) * kBlock, NT - 1u)];
+#pragma unroll
+    for (int j = 0; j < kRotatePer; ++j) {
+        const uint32_t i = min(base + uint32_t(j) * kBlock, NT - 1u);
+        V[j] = vel4[i];
+        Pp[j] = pos4[i];
+        C[j] = com4[c[j]];
+        l[j] = iso ? lam[c[j]] : 1.0f;
     }
-    if (!solvent && !rotBeads) {
-        vel[i] = V;
-        return;
+#pragma unroll
+    for (int j = 0; j < kRotatePer; ++j) {
+        const uint32_t i = base + uint32_t(j) * kBlock;
+        if (i >= NT) continue;
+        const bool solvent = i >= N;
+        if (!solvent && !rotBeads && !iso) continue;
+        f3 v = mk(V[j].x, V[j].y, V[j].z);
+        if (iso) v = l[j] * v;
+        if (solvent || rotBeads) {
+            const f3 u = mk(C[j].x, C[j].y, C[j].z);
+            const float sn = C[j].w * sinA;
+            const f3 d = v - u;
+            const f3 rot = mk(d.x * cosA - d.y * sn, d.x * sn + d.y * cosA, d.z);
+            v = u + rot;
+        }
+        if (solvent) {
+            f3 r = mk(Pp[j].x, Pp[j].y, Pp[j].z);
+            const f3 dr = dt * v;
+            r += dr;
+            apply_boundaries(kp.p, r, v, dr, 0.0f);
+            pos4[i] = v4{r.x, r.y, r.z, Pp[j].w};
+        }
+        vel4[i] = v4{v.x, v.y, v.z, V[j].w};
     }
-    const float4 C = com[c];
-    const f3 u = mk(C);
-    const float sn = C.w * sinA;
-    const f3 d = mk(V) - u;
-    const f3 rot = mk(d.x * cosA - d.y * sn, d.x * sn + d.y * cosA, d.z);
-    f3 v = u + rot;
-    if (solvent) {
-        const float4 Pp = pos[i];
-        f3 r = mk(Pp);
-        const f3 dr = dt * v;
-        r += dr;
-        apply_boundaries(kp.p, r, v, dr, 0.0f);
-        pos[i] = f4(r, Pp.w);
-    }
-    vel[i] = f4(v, V.w);
 }
 
 // ---------------------------------------------------------------------------
@@ -646,9 +662,12 @@ hipError_t launch_srd_cells(const BucketPlan &bp, BucketScratch &s, uint32_t *id
 hipError_t launch_srd_rotate(const uint32_t *hash, const float4 *com, const float *lam, float4 *pos, float4 *vel,
                              uint32_t NT, uint32_t N, float dt, float cosA, float sinA, int iso, const KP &kp,
                              hipStream_t st) {
-    if (NT == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_srd_rotate, dim3(grid_for(NT)), dim3(kBlock), 0, st, hash, com, lam, pos, vel, NT, N, dt,
-                       cosA, sinA, iso, kp);
+    // beads that are neither rotated nor rescaled are not visited
+    const uint32_t i0 = (kp.e.srdCollideFilaments != 0 || iso) ? 0u : min(N, NT);
+    if (NT == i0) return hipSuccess;
+    const uint32_t per = uint32_t(kBlock) * kRotatePer;
+    hipLaunchKernelGGL(k_srd_rotate, dim3((NT - i0 + per - 1) / per), dim3(kBlock), 0, st, hash, com, lam, pos, vel,
+                       i0, NT, N, dt, cosA, sinA, iso, kp);
     return hipGetLastError();
 }
 
