@@ -317,11 +317,24 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
                                                   const float4 *__restrict__ sTan, const uint32_t *__restrict__ sIdx,
                                                   const uint32_t *__restrict__ offR,
                                                   const uint32_t *__restrict__ sCell, RowsLds<MODE, TAN> &L) {
+    // records move as plain 4-vectors (register values)
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 *pos4 = reinterpret_cast<const v4 *>(sPos), *vel4 = reinterpret_cast<const v4 *>(sVel),
+             *tan4 = reinterpret_cast<const v4 *>(sTan);
     const SimParams &p = kp.p;
     const uint32_t Gx = p.gridSize.x, Gy = p.gridSize.y;
     const uint32_t layer = Gx * Gy;
     const uint32_t C = layer * p.gridSize.z;
     const uint32_t i = i0 + threadIdx.x;
+    const bool own = i < iEnd;
+    const uint32_t ic = own ? i : iEnd - 1u;  // lanes past the end read the last bead, unused
+
+    // the bead's own sorted records, in flight with the row bounds
+    const uint32_t me = sIdx[ic];
+    const uint32_t cOwn = sCell[ic] - cellBase;
+    const v4 Pown = pos4[ic];
+    const v4 Vown = MODE == 0 ? vel4[ic] : v4{0.f, 0.f, 0.f, 0.f};
+    const v4 Town = TAN ? tan4[ic] : v4{0.f, 0.f, 0.f, 0.f};
 
     if (threadIdx.x < 3) {
         const uint32_t last = min(i0 + blockDim.x, iEnd) - 1u;
@@ -336,46 +349,76 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
         L.rl[threadIdx.x] = (e > s && e - s <= uint32_t(kRunCap)) ? e - s : 0u;
     }
     __syncthreads();
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        const uint32_t s = L.rs[r], l = L.rl[r];
-        for (uint32_t k = threadIdx.x; k < l; k += blockDim.x) {
-            const uint32_t j = s + k;
-            const float4 P = sPos[j];
-            uint32_t fil, ch;
-            worm_of(kp, sIdx[j] - beadBase, fil, ch);
-            L.pos[r * kRunCap + k] = make_float4(P.x, P.y, P.z, __uint_as_float(fil));
-            if (MODE == 0) {
-                const float4 V = sVel[j];
-                L.vel[r * kRunCap + k] = make_float4(V.x, V.y, V.z, __uint_as_float(ch));
-            }
-            if (TAN) L.tan[r * kRunCap + k] = sTan[j];
-        }
-    }
-    __syncthreads();
-    if (i >= iEnd) return;
 
-    const uint32_t me = sIdx[i];
-    const float4 Fo = force[me];  // overlaps the candidate walk
-    const Owner<MODE, TAN> o = load_owner<MODE, TAN>(kp, i, me - beadBase, sPos, sVel, sTan);
-    const float rc2 = kp.e.wcaCutoff * kp.e.wcaCutoff;
-    const uint32_t c = sCell[i] - cellBase;
-    const uint32_t gz = c / layer, rem = c - gz * layer;
+    // in flight together: the old force, the bounds of the bead's three rows
+    // and the records of the three staged runs
+    const v4 Fo = reinterpret_cast<const v4 *>(force)[me];
+    const uint32_t gz = cOwn / layer, rem = cOwn - gz * layer;
     const uint32_t gyu = rem / Gx;
     const int gx = int(rem - gyu * Gx), gy = int(gyu);
     const AxisCells ay = axis_cells(gy, Gy, false);
     const float wz = kp.e.stencil == NW_STENCIL_2D ? 1.0f : axis_cells(int(gz), p.gridSize.z, false).w[0];
+    const bool inner = gx >= 1 && gx + 1 < int(Gx);
+    uint32_t ra[3] = {0u, 0u, 0u}, rb[3] = {0u, 0u, 0u};
+#pragma unroll
+    for (int yi = 0; yi < 3; ++yi) {
+        if (yi < ay.n && inner) {
+            const uint32_t base = (gz * Gy + uint32_t(ay.c[yi])) * Gx;
+            ra[yi] = offR[base + gx - 1];
+            rb[yi] = offR[base + gx + 2];
+        }
+    }
+    constexpr int RH = (kRunCap + kPairBlock - 1) / kPairBlock;  // staged records per thread and row
+    v4 SP[3][RH], SV[3][RH], ST[3][RH];
+    uint32_t SI[3][RH];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const uint32_t s = L.rs[r], l = L.rl[r];
+#pragma unroll
+        for (int h = 0; h < RH; ++h) {
+            // an empty run reads bead 0 (in bounds, unused)
+            const uint32_t j = l ? s + min(threadIdx.x + uint32_t(h) * kPairBlock, l - 1u) : 0u;
+            SP[r][h] = pos4[j];
+            SI[r][h] = sIdx[j];
+            if (MODE == 0) SV[r][h] = vel4[j];
+            if (TAN) ST[r][h] = tan4[j];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const uint32_t l = L.rl[r];
+#pragma unroll
+        for (int h = 0; h < RH; ++h) {
+            const uint32_t k = threadIdx.x + uint32_t(h) * kPairBlock;
+            if (k < l) {
+                uint32_t fil, ch;
+                worm_of(kp, SI[r][h] - beadBase, fil, ch);
+                L.pos[r * kRunCap + k] = make_float4(SP[r][h].x, SP[r][h].y, SP[r][h].z, __uint_as_float(fil));
+                if (MODE == 0)
+                    L.vel[r * kRunCap + k] = make_float4(SV[r][h].x, SV[r][h].y, SV[r][h].z, __uint_as_float(ch));
+                if (TAN) L.tan[r * kRunCap + k] = make_float4(ST[r][h].x, ST[r][h].y, ST[r][h].z, ST[r][h].w);
+            }
+        }
+    }
+    __syncthreads();
+    if (!own) return;
+
+    Owner<MODE, TAN> o;
+    o.r = mk(Pown.x, Pown.y, Pown.z);
+    o.v = mk(Vown.x, Vown.y, Vown.z);
+    o.t = mk(Town.x, Town.y, Town.z);
+    worm_of(kp, me - beadBase, o.fil, o.ch);
+    const float rc2 = kp.e.wcaCutoff * kp.e.wcaCutoff;
 
     // LDS start and length of every row's run; any row outside the stage
     // sends the bead down the global walk
     uint32_t q[3] = {0u, 0u, 0u}, Lr[3] = {0u, 0u, 0u};
     float w[3] = {0.f, 0.f, 0.f};
-    bool fast = gx >= 1 && gx + 1 < int(Gx);
+    bool fast = inner;
 #pragma unroll
     for (int yi = 0; yi < 3; ++yi) {
         if (yi < ay.n && fast) {
-            const uint32_t base = (gz * Gy + uint32_t(ay.c[yi])) * Gx;
-            const uint32_t a = offR[base + gx - 1], b = offR[base + gx + 2];
+            const uint32_t a = ra[yi], b = rb[yi];
             bool hit = a == b;
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
