@@ -54,7 +54,8 @@ ALG_BYTES = {
     "md_collide": lambda w: (56 if w["wca"] else 88) * w["N"] + 4 * w["C"],  # R sPos sIdx sCell (+sVel sTan), RMW f
     "srdf_tiles": lambda w: 28 * w["NT"] + w["Ps"],            # R pos, W cell, W pair
     "srdf_cells": lambda w: 28 * w["NT"] + 20 * w["Cs"] + w["Ps"],  # R pairs, W ids, gather vel, W offsets com
-    "srdf_rotate": lambda w: 36 * w["NT"] + 32 * w["Ns"] + 16 * w["Cs"],  # R cell, RMW vel (+pos), R com
+    # R cell, RMW vel (+pos), R com; beads only when rotated or rescaled
+    "srdf_rotate": lambda w: 36 * w["Nrot"] + 32 * w["Ns"] + 16 * w["Cs"],
 }
 
 
@@ -99,7 +100,7 @@ def measured_traffic(workload, family):
 
 
 def build_workload(name, per_call=False):
-    from nwamd import system
+    from nwamd import abi, system
     if name == "worms_srd_800k":
         cfg = system.config_srd_800k()
     elif name == "worms_dry_4k":
@@ -113,6 +114,8 @@ def build_workload(name, per_call=False):
     p = s.params
     w = dict(N=s.n, Ns=s.ns, NT=s.n + s.ns, C=p.numCells, Cs=s.solvent.numCells if cfg.srd else 0,
              wca=cfg.ext.pairPotential == 1)
+    rot_beads = bool(cfg.ext.srdCollideFilaments) or cfg.ext.srdThermostat == abi.NW_SRD_THERMO_ISOKINETIC
+    w["Nrot"] = w["NT"] if rot_beads else w["Ns"]
     B, tiles = bucket_plan(w["N"], w["C"], 0)
     w["Pm"] = 4 * tiles * (B + 1)
     B, tiles = bucket_plan(w["NT"], w["Cs"], 0) if cfg.srd else (0, 0)

@@ -336,16 +336,18 @@ double nwSymEig3Max(const double a[6]);
 /* Device-side anomaly flags (bit 0: sort key above bound).  Synchronises. */
 uint nwGetDeviceFlags(int clear);
 
-/* Kernel timing with HIP events recorded on the library stream around the
-   launches of the named kernel family while enabled: enable = 0 stops,
-   enable = k >= 1 (re)starts timing every k-th launch (an event pair costs a
-   few microseconds of stream time; sampling keeps that out of most steps).
-   Names of the per-call
-   entry points: "integrate", "langevin", "filament", "hash", "sort",
-   "reorder", "collide", "srd_hash", "srd_sort", "srd_reorder", "srd_com",
-   "srd_thermo", "srd_collide", "srd_stream", "reverse"; of the fused steps:
-   "md_front", "md_tiles" (systems too large for the front's tiles),
-   "md_sort", "md_collide", "srdf_tiles", "srdf_cells", "srdf_rotate". */
+/* Kernel timing with HIP events on the library stream while enabled:
+   enable = 0 stops, enable = k >= 1 (re)starts timing every k-th launch of
+   the named kernel family.  The per-call families record an event pair
+   around their launches (a few microseconds of stream time each; sampling
+   keeps that out of most steps); each fused family is one kernel, and its
+   events ride on that kernel's dispatch packet (hipExtLaunchKernelGGL), so
+   they time its execution alone.  Names of the per-call entry points:
+   "integrate", "langevin", "filament", "hash", "sort", "reorder", "collide",
+   "srd_hash", "srd_sort", "srd_reorder", "srd_com", "srd_thermo",
+   "srd_collide", "srd_stream", "reverse"; of the fused steps: "md_front",
+   "md_tiles" (the sort's first level), "md_sort", "md_collide",
+   "srdf_tiles", "srdf_cells", "srdf_rotate". */
 void nwTimerEnable(const char *kernel, int enable);
 /* Synchronises; returns total milliseconds and launch count since enable. */
 void nwTimerRead(const char *kernel, double *totalMs, uint64_t *launches);

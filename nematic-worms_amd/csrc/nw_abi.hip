@@ -195,11 +195,15 @@ KP make_kp_for(const SimParams &p, const SolventParams &sp, const NWExtParams &e
 
 KP make_kp() { return make_kp_for(S().params, S().solvent, S().ext); }
 
+// A timed family's launches between two events on the stream; a family of
+// a single launch (single = true) hands the events to that launch's dispatch
+// packet instead, which times the kernel alone.
 struct TimedScope {
     Timer *t = nullptr;
     hipStream_t st;
+    bool single = false;
     std::pair<hipEvent_t, hipEvent_t> ev{};
-    TimedScope(const char *name, hipStream_t stream) : st(stream) {
+    TimedScope(const char *name, hipStream_t stream, bool singleLaunch = false) : st(stream), single(singleLaunch) {
         auto it = S().timers.find(name);
         if (it == S().timers.end() || !it->second.enabled) return;
         if (it->second.seen++ % it->second.stride != 0) return;
@@ -211,11 +215,28 @@ struct TimedScope {
             (void)hipEventCreate(&ev.first);
             (void)hipEventCreate(&ev.second);
         }
-        (void)hipEventRecord(ev.first, st);
+        if (single) {
+            LaunchTiming &lt = launch_timing();
+            lt.start = ev.first;
+            lt.stop = ev.second;
+            lt.used = false;
+        } else {
+            (void)hipEventRecord(ev.first, st);
+        }
     }
     ~TimedScope() {
         if (!t) return;
-        (void)hipEventRecord(ev.second, st);
+        if (single) {
+            LaunchTiming &lt = launch_timing();
+            const bool used = lt.used;
+            lt = LaunchTiming{};
+            if (!used) {  // nothing launched (an empty system)
+                t->pool.push_back(ev);
+                return;
+            }
+        } else {
+            (void)hipEventRecord(ev.second, st);
+        }
         t->pending.push_back(ev);
         t->launches++;
     }
@@ -224,6 +245,13 @@ struct TimedScope {
 uint64_t blocks_for(uint64_t n) { return (n + 3) / 4; }
 
 }  // namespace
+
+namespace nw {
+LaunchTiming &launch_timing() {
+    static thread_local LaunchTiming lt;
+    return lt;
+}
+}  // namespace nw
 
 extern "C" {
 
@@ -527,22 +555,22 @@ static bool fused_md(const KP &kp, float *pos, float *vel, float *force, float *
     a.lgvStd = sqrtf(2.0f * langevinGamma * langevinKbT);
     if (langevin) s.offset += N;
     {
-        TimedScope ts("md_front", s.stream);
+        TimedScope ts("md_front", s.stream, true);
         NW_HIP_R(launch_md_front(a, kp, s.stream), false);
     }
     {
-        TimedScope ts("md_tiles", s.stream);
+        TimedScope ts("md_tiles", s.stream, true);
         NW_HIP_R(launch_hash_tiles(a.hash, bp, s.bucket, s.stream), false);
     }
     {
-        TimedScope ts("md_sort", s.stream);
+        TimedScope ts("md_sort", s.stream, true);
         NW_HIP_R(launch_md_cells(bp, sc, gridParticleIndex, a.pos, needVel ? a.vel : nullptr,
                                needTan ? a.tangent : nullptr, reinterpret_cast<float4 *>(sortedPos),
                                needVel ? reinterpret_cast<float4 *>(sortedVel) : nullptr,
                                needTan ? reinterpret_cast<float4 *>(sortedTangent) : nullptr, s.stream), false);
     }
     {
-        TimedScope ts("md_collide", s.stream);
+        TimedScope ts("md_collide", s.stream, true);
         NW_HIP_R(launch_collide_off(a.force, reinterpret_cast<const float4 *>(sortedPos),
                                   reinterpret_cast<const float4 *>(sortedVel),
                                   reinterpret_cast<const float4 *>(sortedTangent), gridParticleIndex,
@@ -584,15 +612,15 @@ void nwFusedSrdStep(float *pos, float *vel, float *solventCellCOM, uint *gridPar
     uint32_t *hash = gridParticleHash ? gridParticleHash : s.bucket.hash;
     const int iso = isokinetic ? 1 : 0;
     {
-        TimedScope ts("srdf_tiles", s.stream);
+        TimedScope ts("srdf_tiles", s.stream, true);
         NW_HIP(launch_srd_tiles(P, hash, bp, s.bucket, kp, s.stream));
     }
     {
-        TimedScope ts("srdf_cells", s.stream);
+        TimedScope ts("srdf_cells", s.stream, true);
         NW_HIP(launch_srd_cells(bp, s.bucket, gridParticleIndex, V, com, iso, s.seed, off, kp, s.stream));
     }
     {
-        TimedScope ts("srdf_rotate", s.stream);
+        TimedScope ts("srdf_rotate", s.stream, true);
         NW_HIP(launch_srd_rotate(hash, com, s.bucket.lam, P, V, uint32_t(NT), numParticles, deltaTime, cosA, sinA,
                                  iso, kp, s.stream));
     }

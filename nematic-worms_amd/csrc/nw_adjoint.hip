@@ -260,14 +260,29 @@ __global__ __launch_bounds__(kBlock) void k_bond_order_grad(const float4 *__rest
     out[m.bead] = f4(o, 0.0f);
 }
 
-// out[k] += sum over rows of partial[row][k], rows in order
-__global__ __launch_bounds__(kAdjParams) void k_adj_total(const double *__restrict__ partial, uint64_t rows,
-                                                          double *__restrict__ out) {
-    const uint32_t k = threadIdx.x;
-    if (k >= uint32_t(kAdjParams)) return;
+// out[k] += sum over rows of partial[row][k]: thread t adds the entries
+// t, t + kTotalBlock, ... of the row-major table (all of parameter
+// t mod kAdjParams) in order, then a fixed tree over the threads of each
+// parameter -- the same order on every call
+constexpr uint32_t kTotalBlock = 1024;
+static_assert(kTotalBlock % kAdjParams == 0 && (kAdjParams & (kAdjParams - 1)) == 0,
+              "each thread keeps one parameter; the tree halves down to kAdjParams");
+
+__global__ __launch_bounds__(kTotalBlock) void k_adj_total(const double *__restrict__ partial, uint64_t rows,
+                                                           double *__restrict__ out) {
+    __shared__ double red[kTotalBlock];
+    const uint32_t t = threadIdx.x;
+    const uint64_t n = rows * kAdjParams;
     double s = 0.0;
-    for (uint64_t r = 0; r < rows; ++r) s += partial[r * kAdjParams + k];
-    out[k] = out[k] + s;
+#pragma unroll 8
+    for (uint64_t e = t; e < n; e += kTotalBlock) s += partial[e];
+    red[t] = s;
+    __syncthreads();
+    for (uint32_t h = kTotalBlock / 2; h >= uint32_t(kAdjParams); h >>= 1) {
+        if (t < h) red[t] += red[t + h];
+        __syncthreads();
+    }
+    if (t < uint32_t(kAdjParams)) out[t] = out[t] + red[t];
 }
 
 // ---------------------------------------------------------------------------
@@ -295,7 +310,7 @@ hipError_t launch_bond_order_grad(const float4 *pos, uint32_t W, const double e[
 }
 
 hipError_t launch_adj_total(const double *partial, uint64_t rows, double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_adj_total, dim3(1), dim3(kAdjParams), 0, st, partial, rows, out);
+    hipLaunchKernelGGL(k_adj_total, dim3(1), dim3(kTotalBlock), 0, st, partial, rows, out);
     return hipGetLastError();
 }
 

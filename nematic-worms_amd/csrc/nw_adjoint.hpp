@@ -28,7 +28,7 @@ hipError_t launch_adj_step(const AdjStepArgs &a, const KP &kp, hipStream_t st);
 // largest eigenvalue of <1.5 b b - 0.5 I>, scale = 3 / bonds
 hipError_t launch_bond_order_grad(const float4 *pos, uint32_t W, const double e[3], double scale, float4 *out,
                                   const KP &kp, hipStream_t st);
-// out[k] += the column sums of partial (rows x kAdjParams), in row order
+// out[k] += the column sums of partial (rows x kAdjParams), in a fixed order
 hipError_t launch_adj_total(const double *partial, uint64_t rows, double *out, hipStream_t st);
 
 }  // namespace nw

@@ -574,7 +574,7 @@ static hipError_t launch_staged(const float4 *pos, uint32_t *hash, const BucketP
     const void *fn = reinterpret_cast<const void *>(&k_tiles_staged<FROM_POS, ITEMS>);
     hipError_t e = grant_lds(fn, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_tiles_staged<FROM_POS, ITEMS>), dim3(bp.tiles), dim3(kTileBlock), lds, st, pos, hash,
+    NW_LAUNCH((k_tiles_staged<FROM_POS, ITEMS>), dim3(bp.tiles), dim3(kTileBlock), lds, st, pos, hash,
                        s.pairs, s.P, bp, kp);
     return hipGetLastError();
 }
@@ -599,9 +599,9 @@ static hipError_t launch_tiles(bool fromPos, const float4 *pos, uint32_t *hash, 
     const hipError_t e = grant_lds(fn, lds);
     if (e != hipSuccess) return e;
     if (fromPos)
-        hipLaunchKernelGGL(k_tiles<true>, dim3(bp.tiles), dim3(kTileBlock), lds, st, pos, hash, s.pairs, s.P, bp, kp);
+        NW_LAUNCH(k_tiles<true>, dim3(bp.tiles), dim3(kTileBlock), lds, st, pos, hash, s.pairs, s.P, bp, kp);
     else
-        hipLaunchKernelGGL(k_tiles<false>, dim3(bp.tiles), dim3(kTileBlock), lds, st, pos, hash, s.pairs, s.P, bp,
+        NW_LAUNCH(k_tiles<false>, dim3(bp.tiles), dim3(kTileBlock), lds, st, pos, hash, s.pairs, s.P, bp,
                            kp);
     return hipGetLastError();
 }
@@ -632,7 +632,7 @@ hipError_t launch_md_cells(const BucketPlan &bp, BucketScratch &s, uint32_t *ids
     o.sVel = sVel;
     o.sTan = sTan;
     const CellsLayout L = cells_layout(bp, false);
-    hipLaunchKernelGGL((k_bucket_cells<false, kMdCellsBlock>), dim3(bp.buckets), dim3(kMdCellsBlock), L.total, st, o,
+    NW_LAUNCH((k_bucket_cells<false, kMdCellsBlock>), dim3(bp.buckets), dim3(kMdCellsBlock), L.total, st, o,
                        bp);
     return hipGetLastError();
 }
@@ -654,7 +654,7 @@ hipError_t launch_srd_cells(const BucketPlan &bp, BucketScratch &s, uint32_t *id
     o.seed = seed;
     o.offset = offset;
     const CellsLayout L = cells_layout(bp, true);
-    hipLaunchKernelGGL((k_bucket_cells<true, kSrdCellsBlock>), dim3(bp.buckets), dim3(kSrdCellsBlock), L.total, st,
+    NW_LAUNCH((k_bucket_cells<true, kSrdCellsBlock>), dim3(bp.buckets), dim3(kSrdCellsBlock), L.total, st,
                        o, bp);
     return hipGetLastError();
 }
@@ -666,7 +666,7 @@ hipError_t launch_srd_rotate(const uint32_t *hash, const float4 *com, const floa
     const uint32_t i0 = (kp.e.srdCollideFilaments != 0 || iso) ? 0u : min(N, NT);
     if (NT == i0) return hipSuccess;
     const uint32_t per = uint32_t(kBlock) * kRotatePer;
-    hipLaunchKernelGGL(k_srd_rotate, dim3((NT - i0 + per - 1) / per), dim3(kBlock), 0, st, hash, com, lam, pos, vel,
+    NW_LAUNCH(k_srd_rotate, dim3((NT - i0 + per - 1) / per), dim3(kBlock), 0, st, hash, com, lam, pos, vel,
                        i0, NT, N, dt, cosA, sinA, iso, kp);
     return hipGetLastError();
 }

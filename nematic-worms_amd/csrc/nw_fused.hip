@@ -541,7 +541,7 @@ hipError_t launch_md_front(const MdFrontArgs &a, const KP &kp, hipStream_t st) {
     const uint32_t n = kp.p.filamentSize;
     const uint32_t wpw = 64u / n;
     const uint64_t threads = ((uint64_t(a.W) + wpw - 1) / wpw) * 64u;
-    hipLaunchKernelGGL(k_md_front, dim3(grid_for(threads)), dim3(kBlock), 0, st, a, kp);
+    NW_LAUNCH(k_md_front, dim3(grid_for(threads)), dim3(kBlock), 0, st, a, kp);
     return hipGetLastError();
 }
 
@@ -555,7 +555,7 @@ hipError_t launch_collide_off(float4 *force, const float4 *sPos, const float4 *s
     const bool wca = kp.e.pairPotential == NW_PAIR_WCA;
     const bool tan = !wca || kp.p.activity != 0.0f;
     const dim3 g(grid_for(n, lds ? kPairBlock : kBlock)), b(lds ? kPairBlock : kBlock);
-#define NW_COLLIDE(K, ...) hipLaunchKernelGGL((K<__VA_ARGS__>), g, b, 0, st, force, sPos, sVel, sTan, sIdx, off, sCell, n, kp)
+#define NW_COLLIDE(K, ...) NW_LAUNCH((K<__VA_ARGS__>), g, b, 0, st, force, sPos, sVel, sTan, sIdx, off, sCell, n, kp)
     if (lds) {
         // every row weight is 1 for the one-layer stencil over >= 3 rows
         const bool unit = kp.e.stencil == NW_STENCIL_2D && kp.p.gridSize.y >= 3;

@@ -3,6 +3,7 @@
 // status; none synchronises.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "nw_device.hpp"
@@ -10,6 +11,28 @@
 namespace nw {
 
 constexpr int kBlock = 256;
+
+// Events that the next NW_LAUNCH on this thread attaches to its own dispatch
+// packet (hipExtLaunchKernelGGL): the elapsed time between them is the
+// kernel's execution alone.  A timing scope around a launcher that makes a
+// single launch sets them; `used` tells it that the launch took them.
+struct LaunchTiming {
+    hipEvent_t start = nullptr, stop = nullptr;
+    bool used = false;
+};
+LaunchTiming &launch_timing();
+
+#define NW_LAUNCH(K, G, B, SH, ST, ...)                                                                  \
+    do {                                                                                                 \
+        ::nw::LaunchTiming &lt_ = ::nw::launch_timing();                                                 \
+        if (lt_.start) {                                                                                 \
+            hipExtLaunchKernelGGL(K, G, B, SH, ST, lt_.start, lt_.stop, 0u, __VA_ARGS__);                 \
+            lt_.start = lt_.stop = nullptr;                                                              \
+            lt_.used = true;                                                                             \
+        } else {                                                                                         \
+            hipLaunchKernelGGL(K, G, B, SH, ST, __VA_ARGS__);                                            \
+        }                                                                                                \
+    } while (0)
 
 inline unsigned grid_for(uint64_t n, unsigned block = kBlock) {
     return unsigned((n + block - 1) / block);

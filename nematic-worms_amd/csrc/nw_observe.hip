@@ -86,13 +86,23 @@ __global__ __launch_bounds__(kObsBlock) void k_observe(const float4 *__restrict_
 }
 
 // out[k] = sum over rows of partial[row][k], rows in order
-__global__ __launch_bounds__(kObsSums) void k_observe_total(const double *__restrict__ partial, uint32_t rows,
-                                                            double *__restrict__ out) {
-    const uint32_t k = threadIdx.x;
-    if (k >= uint32_t(kObsSums)) return;
+// thread (g, k) adds rows g, g + kObsGroups, ... of sum k in order; thread k
+// then adds the groups' results in group order (the same order every call)
+constexpr uint32_t kObsGroups = 32;
+
+__global__ __launch_bounds__(kObsGroups * kObsSums) void k_observe_total(const double *__restrict__ partial,
+                                                                         uint32_t rows, double *__restrict__ out) {
+    __shared__ double red[kObsGroups * kObsSums];
+    const uint32_t g = threadIdx.x / kObsSums, k = threadIdx.x - g * kObsSums;
     double s = 0.0;
-    for (uint32_t r = 0; r < rows; ++r) s += partial[uint64_t(r) * kObsSums + k];
-    out[k] = s;
+    for (uint32_t r = g; r < rows; r += kObsGroups) s += partial[uint64_t(r) * kObsSums + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < uint32_t(kObsSums)) {
+        double t = 0.0;
+        for (uint32_t q = 0; q < kObsGroups; ++q) t += red[q * kObsSums + threadIdx.x];
+        out[threadIdx.x] = t;
+    }
 }
 
 uint32_t observe_rows(uint32_t W, uint32_t n) {
@@ -108,7 +118,7 @@ hipError_t launch_observe(const float4 *pos, const float4 *vel, uint32_t W, doub
     hipLaunchKernelGGL(k_observe, dim3(rows), dim3(kObsBlock), 0, st, pos, vel, W, partial, kp);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_observe_total, dim3(1), dim3(kObsSums), 0, st, partial, rows, out);
+    hipLaunchKernelGGL(k_observe_total, dim3(1), dim3(kObsGroups * kObsSums), 0, st, partial, rows, out);
     return hipGetLastError();
 }
 

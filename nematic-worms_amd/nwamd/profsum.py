@@ -1,7 +1,9 @@
 """Summaries of rocprofv3 CSV output for the worm step kernels.
 
-  python -m nwamd.profsum trace <kernel_trace.csv> [--last N]
-      per-kernel dispatch durations (markdown table)
+  python -m nwamd.profsum trace <kernel_trace.csv> [--last N] [--by-grid [--fold-below G]]
+      per-kernel dispatch durations (markdown table); --by-grid keeps the
+      launches of one kernel over different grid sizes (work-items) apart,
+      --fold-below G merges each kernel's launches over grids below G
   python -m nwamd.profsum traffic <FETCH_SIZE counter csv> <WRITE_SIZE counter csv> --copy NAME --copy-bytes B
       HBM bytes per dispatch from the two counter passes, each scaled by what
       the same counter reads for a copy kernel of known size (NAME: a
@@ -33,18 +35,25 @@ def kernel_key(name):
     return name[:cut].replace("nw::", "")
 
 
-def trace(path, last=0):
+def trace(path, last=0, by_grid=False, fold_below=0):
     rows = list(csv.DictReader(open(path)))
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
     if last:
         rows = rows[-last:]
     agg = defaultdict(list)
     for r in rows:
-        agg[kernel_key(r["Kernel_Name"])].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+        key = kernel_key(r["Kernel_Name"])
+        if by_grid:
+            g = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+            key = (key, f"< {fold_below}" if g < fold_below else str(g))
+        agg[key].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
     tot = sum(sum(v) for v in agg.values())
-    out = ["| kernel | calls | avg us | min us | max us | total ms | % |", "|---|---:|---:|---:|---:|---:|---:|"]
+    head = "| kernel | grid | calls |" if by_grid else "| kernel | calls |"
+    out = [head + " avg us | min us | max us | total ms | % |",
+           "|---|" + ("---:|" if by_grid else "") + "---:|---:|---:|---:|---:|---:|"]
     for k, v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
-        out.append(f"| `{k}` | {len(v)} | {sum(v) / len(v) / 1e3:.2f} | {min(v) / 1e3:.2f} | {max(v) / 1e3:.2f} | "
+        name = f"`{k[0]}` | {k[1]}" if by_grid else f"`{k}`"
+        out.append(f"| {name} | {len(v)} | {sum(v) / len(v) / 1e3:.2f} | {min(v) / 1e3:.2f} | {max(v) / 1e3:.2f} | "
                    f"{sum(v) / 1e6:.3f} | {100 * sum(v) / tot:.1f} |")
     out.append(f"\nTotal kernel time {tot / 1e6:.3f} ms over {len(rows)} dispatches.")
     return "\n".join(out)
@@ -85,6 +94,8 @@ def main(argv=None):
     t = sub.add_parser("trace")
     t.add_argument("csv")
     t.add_argument("--last", type=int, default=0, help="only the last N dispatches")
+    t.add_argument("--by-grid", action="store_true", help="one row per kernel and grid size")
+    t.add_argument("--fold-below", type=int, default=0, help="with --by-grid: one row for all grids below this")
     p = sub.add_parser("traffic")
     p.add_argument("fetch_csv")
     p.add_argument("write_csv")
@@ -93,7 +104,7 @@ def main(argv=None):
     p.add_argument("--json")
     a = ap.parse_args(argv)
     if a.cmd == "trace":
-        print(trace(a.csv, a.last))
+        print(trace(a.csv, a.last, a.by_grid, a.fold_below))
     else:
         text, res, cal = traffic(a.fetch_csv, a.write_csv, a.copy, a.copy_bytes)
         print(text)
