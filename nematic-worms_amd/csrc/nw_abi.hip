@@ -171,25 +171,28 @@ bool ensure_init() {
         if (!ensure_init()) return; \
     } while (0)
 
-// fast division by the worm length: q = mulhi(i, m) with m = ceil(2^32 / n)
-// is exact for every i < 2^32 / e, e = m * n - 2^32 (Granlund & Montgomery 1994).
+// fast division by a constant n: q = mulhi(i, m) with m = ceil(2^32 / n) is
+// exact for every i < 2^32 / e, e = m * n - 2^32 (Granlund & Montgomery
+// 1994); limit 0 sends every division to the hardware sequence
+static void fast_div_for(uint64_t n, uint32_t &magic, uint32_t &limit) {
+    magic = limit = 0;
+    if (n < 2) return;
+    const uint64_t two32 = 1ull << 32;
+    const uint64_t m = (two32 + n - 1) / n;
+    const uint64_t e = m * n - two32;
+    if (m >= two32) return;
+    magic = uint32_t(m);
+    limit = e == 0 ? 0xffffffffu : uint32_t(std::min<uint64_t>(0xffffffffull, (two32 - 1) / e));
+}
+
 KP make_kp_for(const SimParams &p, const SolventParams &sp, const NWExtParams &ext) {
     KP kp{};
     kp.p = p;
     kp.s = sp;
     kp.e = ext;
-    const uint64_t n = p.filamentSize;
-    if (n >= 2) {
-        const uint64_t two32 = 1ull << 32;
-        const uint64_t m = (two32 + n - 1) / n;
-        const uint64_t e = m * n - two32;
-        kp.fdivMagic = uint32_t(m);
-        kp.fdivLimit = e == 0 ? 0xffffffffu : uint32_t(std::min<uint64_t>(0xffffffffull, (two32 - 1) / e));
-        if (m >= two32) kp.fdivLimit = 0;
-    } else {
-        kp.fdivMagic = 0;
-        kp.fdivLimit = 0;
-    }
+    fast_div_for(p.filamentSize, kp.fdivMagic, kp.fdivLimit);
+    fast_div_for(p.gridSize.x, kp.gxMagic, kp.gxLimit);
+    fast_div_for(uint64_t(p.gridSize.x) * p.gridSize.y, kp.layerMagic, kp.layerLimit);
     return kp;
 }
 

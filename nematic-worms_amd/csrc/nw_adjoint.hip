@@ -177,8 +177,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_step(AdjStepArgs a, KP kp) {
         const int gx = wrap_mod(cell_coord(rv.x, p.origin.x, p.cellSize.x), Gx);
         const int gy = wrap_mod(cell_coord(rv.y, p.origin.y, p.cellSize.y), Gy);
         const int gz = wrap_mod(cell_coord(rv.z, p.origin.z, p.cellSize.z), p.gridSize.z);
-        const AxisCells ay = axis_cells(gy, Gy, false);
-        const float wz = kp.e.stencil == NW_STENCIL_2D ? 1.0f : axis_cells(gz, p.gridSize.z, false).w[0];
+        const AxisCells ay = axis_cells_in(gy, Gy, false);
+        const float wz = kp.e.stencil == NW_STENCIL_2D ? 1.0f : axis_cells_in(gz, p.gridSize.z, false).w[0];
         const float rc2 = kp.e.wcaCutoff * kp.e.wcaCutoff;
         const float eps = kp.e.wcaEpsilon, s2c = kp.e.wcaSigma * kp.e.wcaSigma;
         const uint32_t fil = m.worm;

@@ -41,7 +41,25 @@ struct KP {
     NWExtParams e;
     uint32_t fdivMagic;   // ceil(2^32 / filamentSize): q = mulhi(i, m)
     uint32_t fdivLimit;   // mulhi quotient is exact for i < fdivLimit (0: always divide)
+    uint32_t gxMagic, gxLimit;        // the same for gridSize.x (cell index -> row)
+    uint32_t layerMagic, layerLimit;  // and for gridSize.x * gridSize.y (-> layer)
 };
+
+// x / d by the multiplier of a KP (exact below limit; the hardware sequence
+// otherwise)
+__device__ __forceinline__ uint32_t div_magic(uint32_t x, uint32_t magic, uint32_t limit, uint32_t d) {
+    return x < limit ? __umulhi(x, magic) : x / d;
+}
+
+// coordinates of cell c of the MD grid
+__device__ __forceinline__ void cell_xyz(const KP &kp, uint32_t c, int &gx, int &gy, int &gz) {
+    const uint32_t Gx = kp.p.gridSize.x, layer = kp.p.gridSize.x * kp.p.gridSize.y;
+    const uint32_t z = div_magic(c, kp.layerMagic, kp.layerLimit, layer), rem = c - z * layer;
+    const uint32_t y = div_magic(rem, kp.gxMagic, kp.gxLimit, Gx);
+    gx = int(rem - y * Gx);
+    gy = int(y);
+    gz = int(z);
+}
 
 // i / filamentSize and i % filamentSize for bead ids.
 __device__ __forceinline__ void worm_of(const KP &kp, uint32_t i, uint32_t &w, uint32_t &c) {
@@ -103,12 +121,27 @@ __device__ __forceinline__ f3 tangent_of(const SimParams &p, f3 a, f3 b) {
 // --------------------------------------------------------------------------
 __device__ __forceinline__ int wrap_mod(int g, uint32_t G) {
     if ((G & (G - 1u)) == 0u) return g & int(G - 1u);
-    int m = g % int(G);
-    return m < 0 ? m + int(G) : m;
+    const int Gi = int(G);
+    // coordinates of positions in or next to the box: one add or subtract
+    if (g >= -Gi && g < 2 * Gi) return g < 0 ? g + Gi : (g >= Gi ? g - Gi : g);
+    const int m = g % Gi;
+    return m < 0 ? m + Gi : m;
+}
+
+// floor(d / c) of the correctly rounded quotient.  d * rcp(c) is within
+// 2^-22 |t| of d / c (hardware reciprocal 1 ulp, one rounding each for the
+// product and the quotient), so farther than 2^-19 |t| from an integer both
+// have the same floor; closer (and for |t| below 2^-100, NaN or infinity)
+// the quotient is formed exactly.
+__device__ __forceinline__ int floor_div(float d, float c) {
+    const float t = d * __builtin_amdgcn_rcpf(c);
+    const float f = floorf(t);
+    if (fminf(t - f, f + 1.0f - t) > fabsf(t) * 0x1p-19f + 0x1p-100f) return int(f);
+    return int(floorf(d / c));
 }
 
 __device__ __forceinline__ int cell_coord(float p, float o, float c) {
-    return int(floorf((p - o) / c));
+    return floor_div(p - o, c);
 }
 
 __device__ __forceinline__ uint32_t md_hash_wrapped(const SimParams &p, int gx, int gy, int gz) {
@@ -142,9 +175,9 @@ __device__ __forceinline__ f3 position_relative_to_box(const SimParams &p, f3 q)
 __device__ __forceinline__ uint32_t srd_hash(const SimParams &p, const SolventParams &s, f3 q) {
     const f3 shifted = q + mk(s.randOffset.x, s.randOffset.y, s.randOffset.z);
     const f3 r = position_relative_to_box(p, shifted);
-    int gx = int(floorf(r.x / s.cellSize.x));
-    int gy = int(floorf(r.y / s.cellSize.y));
-    int gz = int(floorf(r.z / s.cellSize.z));
+    int gx = floor_div(r.x, s.cellSize.x);
+    int gy = floor_div(r.y, s.cellSize.y);
+    int gz = floor_div(r.z, s.cellSize.z);
     if (p.boundaryX == PERIODIC) gx = wrap_mod(gx, s.gridSize.x);
     else gx = min(max(gx, 0), int(s.gridSize.x) - 1);
     if (p.boundaryY == PERIODIC) gy = wrap_mod(gy, s.gridSize.y);
@@ -193,9 +226,10 @@ __device__ __forceinline__ float u01(uint32_t x) {
 __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float &z0, float &z1) {
     const float u1 = u01(a), u2 = u01(b);
     const float r = sqrtf(-2.0f * logf(u1));
-    const float th = 6.2831853071795864769f * u2;
+    // angle 2 pi u2 as sin / cos of pi (2 u2): 2 u2 is exact, so no rounding
+    // of the angle, and no range reduction
     float s, c;
-    sincosf(th, &s, &c);
+    sincospif(2.0f * u2, &s, &c);
     z0 = r * c;
     z1 = r * s;
 }

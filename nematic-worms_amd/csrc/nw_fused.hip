@@ -23,13 +23,7 @@ __device__ __forceinline__ void bead_cell(const KP &kp, const uint32_t *__restri
                                           int &gy, int &gz) {
     const SimParams &p = kp.p;
     if (sCell) {
-        const uint32_t c = sCell[i];
-        const uint32_t Gx = p.gridSize.x, layer = p.gridSize.x * p.gridSize.y;
-        const uint32_t z = c / layer, rem = c - z * layer;
-        const uint32_t y = rem / Gx;
-        gx = int(rem - y * Gx);
-        gy = int(y);
-        gz = int(z);
+        cell_xyz(kp, sCell[i], gx, gy, gz);
     } else {
         gx = wrap_mod(cell_coord(r.x, p.origin.x, p.cellSize.x), p.gridSize.x);
         gy = wrap_mod(cell_coord(r.y, p.origin.y, p.cellSize.y), p.gridSize.y);
@@ -215,8 +209,8 @@ __device__ __forceinline__ void add_all_global(const KP &kp, const Owner<MODE, T
                                                uint32_t beadBase = 0) {
     const SimParams &p = kp.p;
     const uint32_t Gx = p.gridSize.x;
-    const AxisCells ay = axis_cells(gy, p.gridSize.y, false);
-    const AxisCells az = axis_cells(gz, p.gridSize.z, kp.e.stencil == NW_STENCIL_2D);
+    const AxisCells ay = axis_cells_in(gy, p.gridSize.y, false);
+    const AxisCells az = axis_cells_in(gz, p.gridSize.z, kp.e.stencil == NW_STENCIL_2D);
 #pragma unroll
     for (int zi = 0; zi < 3; ++zi) {
         if (zi >= az.n) break;
@@ -231,7 +225,7 @@ __device__ __forceinline__ void add_all_global(const KP &kp, const Owner<MODE, T
                 add_run_global<MODE, TAN>(kp, o, rc2, wzy, acc, a0, a1, sPos, sVel, sTan, sIdx, beadBase);
                 add_run_global<MODE, TAN>(kp, o, rc2, wzy, acc, b0, b1, sPos, sVel, sTan, sIdx, beadBase);
             } else {
-                const AxisCells ax = axis_cells(gx, Gx, false);
+                const AxisCells ax = axis_cells_in(gx, Gx, false);
 #pragma unroll
                 for (int xi = 0; xi < 3; ++xi) {
                     if (xi >= ax.n) break;
@@ -353,11 +347,11 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
     // in flight together: the old force, the bounds of the bead's three rows
     // and the records of the three staged runs
     const v4 Fo = reinterpret_cast<const v4 *>(force)[me];
-    const uint32_t gz = cOwn / layer, rem = cOwn - gz * layer;
-    const uint32_t gyu = rem / Gx;
-    const int gx = int(rem - gyu * Gx), gy = int(gyu);
-    const AxisCells ay = axis_cells(gy, Gy, false);
-    const float wz = kp.e.stencil == NW_STENCIL_2D ? 1.0f : axis_cells(int(gz), p.gridSize.z, false).w[0];
+    int gx, gy, gzi;
+    cell_xyz(kp, cOwn, gx, gy, gzi);
+    const uint32_t gz = uint32_t(gzi);
+    const AxisCells ay = axis_cells_in(gy, Gy, false);
+    const float wz = kp.e.stencil == NW_STENCIL_2D ? 1.0f : axis_cells_in(gzi, p.gridSize.z, false).w[0];
     const bool inner = gx >= 1 && gx + 1 < int(Gx);
     uint32_t ra[3] = {0u, 0u, 0u}, rb[3] = {0u, 0u, 0u};
 #pragma unroll

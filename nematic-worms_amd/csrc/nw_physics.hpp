@@ -95,6 +95,33 @@ __device__ __forceinline__ AxisCells axis_cells(int g, uint32_t G, bool onlyCent
     return a;
 }
 
+// axis_cells of a coordinate already in [0, G): the neighbours wrap by
+// selects (the same cells as wrap_mod gives)
+__device__ __forceinline__ AxisCells axis_cells_in(int g, uint32_t G, bool onlyCentre) {
+    AxisCells a;
+    if (onlyCentre) {
+        a.c[0] = g;
+        a.w[0] = 1.0f;
+        a.n = 1;
+        return a;
+    }
+    const int Gi = int(G);
+    const int m = g == 0 ? Gi - 1 : g - 1, c = g, p = g + 1 == Gi ? 0 : g + 1;
+    if (G >= 3) {
+        a.c[0] = m; a.c[1] = c; a.c[2] = p;
+        a.w[0] = a.w[1] = a.w[2] = 1.0f;
+        a.n = 3;
+    } else if (G == 2) {
+        a.c[0] = m; a.w[0] = 2.0f;
+        a.c[1] = c; a.w[1] = 1.0f;
+        a.n = 2;
+    } else {
+        a.c[0] = c; a.w[0] = 3.0f;
+        a.n = 1;
+    }
+    return a;
+}
+
 // DEM soft sphere (collideParticles)
 __device__ __forceinline__ f3 dem_force(const SimParams &p, f3 rA, f3 rB, f3 vA, f3 vB) {
     const f3 rel = min_image_pair(p, rA, rB);
