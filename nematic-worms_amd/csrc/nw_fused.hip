@@ -199,6 +199,76 @@ __device__ __forceinline__ void row_runs(const uint32_t *__restrict__ off, uint3
     }
 }
 
+// The candidates of a one-layer stencil (Gx >= 3) from global memory: the
+// row pieces (three rows, one or two x runs each, in stencil order) first,
+// then the flattened candidate list four at a time with the loads of the
+// four issued together.  The same candidates, order and arithmetic as the
+// row-by-row walk below.
+constexpr int kGlobalBatch = 4;
+
+template <int MODE, bool TAN>
+__device__ __forceinline__ void add_layer_global(const KP &kp, const Owner<MODE, TAN> &o, float rc2, f3 &acc,
+                                                 int gx, const AxisCells &ay, uint32_t zc, float wz,
+                                                 const uint32_t *__restrict__ off, const float4 *__restrict__ sPos,
+                                                 const float4 *__restrict__ sVel, const float4 *__restrict__ sTan,
+                                                 const uint32_t *__restrict__ sIdx, uint32_t beadBase) {
+    const SimParams &p = kp.p;
+    const uint32_t Gx = p.gridSize.x;
+    uint32_t ps[6], pl[6];  // piece start slot and length
+    float pw[6];
+#pragma unroll
+    for (int yi = 0; yi < 3; ++yi) {
+        ps[2 * yi] = pl[2 * yi] = ps[2 * yi + 1] = pl[2 * yi + 1] = 0u;
+        pw[2 * yi] = pw[2 * yi + 1] = 0.0f;
+        if (yi < ay.n) {
+            uint32_t a0, a1, b0, b1;
+            row_runs(off, (zc * p.gridSize.y + uint32_t(ay.c[yi])) * Gx, gx, Gx, a0, a1, b0, b1);
+            ps[2 * yi] = a0;
+            pl[2 * yi] = a1 - a0;
+            ps[2 * yi + 1] = b0;
+            pl[2 * yi + 1] = b1 - b0;
+            pw[2 * yi] = pw[2 * yi + 1] = wz * ay.w[yi];
+        }
+    }
+    uint32_t pre[7];  // first flattened index of each piece
+    pre[0] = 0u;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) pre[k + 1] = pre[k] + pl[k];
+    const uint32_t K = pre[6];
+    for (uint32_t t0 = 0; t0 < K; t0 += kGlobalBatch) {
+        uint32_t I[kGlobalBatch];
+        float4 P[kGlobalBatch], V[kGlobalBatch], T[kGlobalBatch];
+        float W[kGlobalBatch];
+#pragma unroll
+        for (int u = 0; u < kGlobalBatch; ++u) {
+            const uint32_t t = min(t0 + uint32_t(u), K - 1u);
+            uint32_t j = 0u;
+            float w = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                if (t >= pre[k] && t < pre[k + 1]) {
+                    j = ps[k] + (t - pre[k]);
+                    w = pw[k];
+                }
+            }
+            I[u] = sIdx[j];
+            P[u] = sPos[j];
+            if (MODE == 0) V[u] = sVel[j];
+            if (TAN) T[u] = sTan[j];
+            W[u] = w;
+        }
+#pragma unroll
+        for (int u = 0; u < kGlobalBatch; ++u) {
+            if (t0 + uint32_t(u) < K) {
+                uint32_t fil2, ch2;
+                worm_of(kp, I[u] - beadBase, fil2, ch2);
+                add_pair<MODE, TAN>(kp, o, rc2, W[u], acc, mk(P[u]), fil2, ch2,
+                                    MODE == 0 ? mk(V[u]) : mk(0.f, 0.f, 0.f), TAN ? mk(T[u]) : mk(0.f, 0.f, 0.f));
+            }
+        }
+    }
+}
+
 // every candidate of the owner from global memory, in stencil order (off:
 // the CSR offsets of the system's cells, indexed by its own cell ids)
 template <int MODE, bool TAN>
@@ -211,6 +281,11 @@ __device__ __forceinline__ void add_all_global(const KP &kp, const Owner<MODE, T
     const uint32_t Gx = p.gridSize.x;
     const AxisCells ay = axis_cells_in(gy, p.gridSize.y, false);
     const AxisCells az = axis_cells_in(gz, p.gridSize.z, kp.e.stencil == NW_STENCIL_2D);
+    if (az.n == 1 && Gx >= 3) {
+        add_layer_global<MODE, TAN>(kp, o, rc2, acc, gx, ay, uint32_t(az.c[0]), az.w[0], off, sPos, sVel, sTan, sIdx,
+                                    beadBase);
+        return;
+    }
 #pragma unroll
     for (int zi = 0; zi < 3; ++zi) {
         if (zi >= az.n) break;

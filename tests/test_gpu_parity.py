@@ -64,6 +64,38 @@ def test_calc_hash_md_bit_exact(nwlib, grid, cell, twod):
     assert h.max() < p.numCells
 
 
+def face_positions(p, cell, n_per_axis, outside=False):
+    """Positions on and within two ulps of the cell faces o + k c along every
+    axis (quotients at or next to integers: the exact-division fallback of the
+    cell index), plus, with outside, faces up to two boxes away."""
+    ks = np.arange(-n_per_axis if outside else 0, n_per_axis + 1, dtype=np.float32)
+    pts = []
+    for a, (o, L) in enumerate(zip((p.origin.x, p.origin.y, p.origin.z), (p.boxSize.x, p.boxSize.y, p.boxSize.z))):
+        g = np.float32(o) + ks * np.float32(cell[a])
+        if outside:
+            g = np.concatenate([g, g + np.float32(L), g - np.float32(L)])
+        g = np.concatenate([g, np.nextafter(g, np.float32(np.inf)), np.nextafter(g, np.float32(-np.inf)),
+                            np.nextafter(np.nextafter(g, np.float32(np.inf)), np.float32(np.inf)),
+                            np.nextafter(np.nextafter(g, np.float32(-np.inf)), np.float32(-np.inf))])
+        pts.append(g.astype(np.float32))
+    n = max(len(x) for x in pts)
+    pos = np.zeros((n, 4), np.float32)
+    for a in range(3):
+        pos[:, a] = np.resize(pts[a], n)
+    return pos
+
+
+@pytest.mark.parametrize("grid,cell", [((89, 37, 3), 1.1236), ((890, 890, 1), 1.1235955), ((33, 7, 5), 0.7)])
+def test_calc_hash_md_cell_faces_bit_exact(nwlib, grid, cell):
+    p = sim_params(grid=grid, cell=cell, W=10, n=10)
+    pos = face_positions(p, (cell, cell, cell), max(grid), outside=True)
+    set_params(nwlib, p)
+    h, _, _ = run_hash(nwlib, p, pos)
+    oh, _ = orc.calc_hash(p, None, pos)
+    assert np.array_equal(h, oh)
+    assert h.max() < p.numCells
+
+
 @pytest.mark.parametrize("srd_grid,offset", [((8, 8, 1), (0.3, 1.7, 0.0)), ((100, 60, 1), (-0.5, 0.49, 0.0)),
                                              ((8, 8, 2), (1.99, 0.0, 0.5))])
 def test_calc_hash_srd_bit_exact(nwlib, srd_grid, offset):
@@ -78,6 +110,13 @@ def test_calc_hash_srd_bit_exact(nwlib, srd_grid, offset):
     oh, _ = orc.calc_hash(p, s, pos, srd=True)
     assert np.array_equal(h, oh)
     assert h.max() < s.numCells
+    # positions on the shifted SRD cell faces and within two ulps of them
+    faces = face_positions(p, cell, max(srd_grid))
+    for a, off in enumerate(offset):
+        faces[:, a] -= np.float32(off)
+    h, _, _ = run_hash(nwlib, p, faces, srd=True)
+    oh, _ = orc.calc_hash(p, s, faces, srd=True)
+    assert np.array_equal(h, oh)
 
 
 # ----------------------------------------------------------------- sorting
