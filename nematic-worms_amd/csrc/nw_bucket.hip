@@ -92,13 +92,6 @@ __host__ __device__ inline CellsLayout cells_layout(const BucketPlan &bp, bool s
     return L;
 }
 
-// blocks i = x + 8k of a launch are spread round-robin over the 8 XCDs; map
-// them so every XCD takes one contiguous run of buckets (a bijection)
-__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t i, uint32_t G) {
-    const uint32_t q = G >> 3, r = G & 7u;
-    const uint32_t x = i & 7u, k = i >> 3;
-    return x < r ? x * (q + 1u) + k : r * (q + 1u) + (x - r) * q + k;
-}
 
 // ---------------------------------------------------------------------------
 // level 1 as a kernel of its own: the cell of every particle (SRD: from the

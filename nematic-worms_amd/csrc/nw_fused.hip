@@ -535,8 +535,11 @@ __global__ __launch_bounds__(kPairBlock) void k_collide_rows(float4 *__restrict_
                                                          const uint32_t *__restrict__ off,
                                                          const uint32_t *__restrict__ sCell, uint32_t n, KP kp) {
     __shared__ RowsLds<MODE, TAN> L;
-    collide_rows_tile<MODE, TAN, UNIT>(kp, blockIdx.x * blockDim.x, n, 0u, 0u, force, sPos, sVel, sTan, sIdx, off,
-                                       sCell, L);
+    // neighbouring tiles share row runs: a contiguous range of tiles per XCD
+    // keeps their re-reads in that XCD's L2
+    const uint32_t tile = xcd_contiguous(blockIdx.x, gridDim.x);
+    collide_rows_tile<MODE, TAN, UNIT>(kp, tile * blockDim.x, n, 0u, 0u, force, sPos, sVel, sTan, sIdx, off, sCell,
+                                       L);
 }
 
 // ===========================================================================

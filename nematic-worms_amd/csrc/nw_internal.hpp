@@ -38,6 +38,14 @@ inline unsigned grid_for(uint64_t n, unsigned block = kBlock) {
     return unsigned((n + block - 1) / block);
 }
 
+// blocks i = x + 8k of a launch are spread round-robin over the 8 XCDs; map
+// them so every XCD takes one contiguous run of work items (a bijection)
+__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t i, uint32_t G) {
+    const uint32_t q = G >> 3, r = G & 7u;
+    const uint32_t x = i & 7u, k = i >> 3;
+    return x < r ? x * (q + 1u) + k : r * (q + 1u) + (x - r) * q + k;
+}
+
 // device anomaly flags (bit 0: sort key >= bound, bit 1: hash >= numCells in reorder)
 uint32_t *device_flags();
 
