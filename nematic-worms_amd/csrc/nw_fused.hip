@@ -373,7 +373,6 @@ struct RowsLds {
     float4 pos[3 * kRunCap];
     float4 vel[MODE == 0 ? 3 * kRunCap : 1];
     float4 tan[TAN ? 3 * kRunCap : 1];
-    uint32_t rs[3], rl[3];
 };
 
 // The workgroup's sorted beads [i0, min(i0 + blockDim, iEnd)) of one system
@@ -405,19 +404,22 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
     const v4 Vown = MODE == 0 ? vel4[ic] : v4{0.f, 0.f, 0.f, 0.f};
     const v4 Town = TAN ? tan4[ic] : v4{0.f, 0.f, 0.f, 0.f};
 
-    if (threadIdx.x < 3) {
+    // the three row runs (the same for the whole workgroup: scalar loads)
+    uint32_t rs[3], rl[3];
+    {
         const uint32_t last = min(i0 + blockDim.x, iEnd) - 1u;
         const uint32_t h0 = sCell[i0] - cellBase, h1 = sCell[last] - cellBase;
-        const int dy = int(threadIdx.x) - 1;
-        const int64_t lo = int64_t(h0) - 1 + int64_t(dy) * Gx;
-        const int64_t hi = int64_t(h1) + 2 + int64_t(dy) * Gx;  // exclusive
-        const uint32_t clo = uint32_t(lo < 0 ? 0 : (lo > int64_t(C) ? C : lo));
-        const uint32_t chi = uint32_t(hi < 0 ? 0 : (hi > int64_t(C) ? C : hi));
-        const uint32_t s = offR[clo], e = offR[chi];
-        L.rs[threadIdx.x] = s;
-        L.rl[threadIdx.x] = (e > s && e - s <= uint32_t(kRunCap)) ? e - s : 0u;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int64_t lo = int64_t(h0) - 1 + int64_t(r - 1) * Gx;
+            const int64_t hi = int64_t(h1) + 2 + int64_t(r - 1) * Gx;  // exclusive
+            const uint32_t clo = uint32_t(lo < 0 ? 0 : (lo > int64_t(C) ? C : lo));
+            const uint32_t chi = uint32_t(hi < 0 ? 0 : (hi > int64_t(C) ? C : hi));
+            const uint32_t s = offR[clo], e = offR[chi];
+            rs[r] = s;
+            rl[r] = (e > s && e - s <= uint32_t(kRunCap)) ? e - s : 0u;
+        }
     }
-    __syncthreads();
 
     // in flight together: the old force, the bounds of the bead's three rows
     // and the records of the three staged runs
@@ -442,7 +444,7 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
     uint32_t SI[3][RH];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-        const uint32_t s = L.rs[r], l = L.rl[r];
+        const uint32_t s = rs[r], l = rl[r];
 #pragma unroll
         for (int h = 0; h < RH; ++h) {
             // an empty run reads bead 0 (in bounds, unused)
@@ -455,7 +457,7 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-        const uint32_t l = L.rl[r];
+        const uint32_t l = rl[r];
 #pragma unroll
         for (int h = 0; h < RH; ++h) {
             const uint32_t k = threadIdx.x + uint32_t(h) * kPairBlock;
@@ -491,8 +493,8 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
             bool hit = a == b;
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
-                if (!hit && L.rl[r] && a >= L.rs[r] && b <= L.rs[r] + L.rl[r]) {
-                    q[yi] = uint32_t(r) * kRunCap + (a - L.rs[r]);
+                if (!hit && rl[r] && a >= rs[r] && b <= rs[r] + rl[r]) {
+                    q[yi] = uint32_t(r) * kRunCap + (a - rs[r]);
                     hit = true;
                 }
             }
