@@ -600,9 +600,11 @@ __global__ __launch_bounds__(kPairBlock) void k_collide_rows_batch(BatchArrays a
     __shared__ RowsLds<MODE, TAN> L;
     // the step's last kernel advances the step counter the next front reads
     if (blockIdx.x == 0 && threadIdx.x == 0) stepCtr[0] = stepCtr[0] + 1u;
-    const uint32_t r = __builtin_amdgcn_readfirstlane(replica_of_block(reps, R, blockIdx.x));
+    // tiles in contiguous runs per XCD (neighbouring tiles share row runs)
+    const uint32_t tile = xcd_contiguous(blockIdx.x, gridDim.x);
+    const uint32_t r = __builtin_amdgcn_readfirstlane(replica_of_block(reps, R, tile));
     const BatchRep &rep = reps[r];
-    const uint32_t i0 = rep.beadOffset + (blockIdx.x - rep.blockOffset) * blockDim.x;
+    const uint32_t i0 = rep.beadOffset + (tile - rep.blockOffset) * blockDim.x;
     const uint32_t iEnd = rep.beadOffset + rep.kp.p.numFilaments * rep.kp.p.filamentSize;
     collide_rows_tile<MODE, TAN, UNIT>(rep.kp, i0, iEnd, rep.beadOffset, rep.cellOffset, a.force, a.sPos, a.sVel,
                                        a.sTan, a.ids, a.off + rep.cellOffset, a.sCell, L);
