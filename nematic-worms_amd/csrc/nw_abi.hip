@@ -61,6 +61,9 @@ struct State {
         bool tan = false, unit = true;
         uint64_t steps = 0;
         BucketScratch scratch;
+        std::vector<SortRep> sort;  // every replica's own sort plan
+        SortRep *sortDev = nullptr;
+        BatchSortPlan sortPlan;
         hipGraphExec_t exec = nullptr;
         hipGraph_t graph = nullptr;
         std::vector<uintptr_t> key;  // arguments the graph was captured with
@@ -818,10 +821,16 @@ void nwBatchSetReplicas(const NWReplicaParams *reps, uint numReplicas) {
     }
     if (!b.stepCtr) NW_HIP(hipMalloc(&b.stepCtr, sizeof(uint64_t)));
     NW_HIP(hipMemset(b.stepCtr, 0, sizeof(uint64_t)));
+    b.sort.assign(numReplicas, SortRep{});
+    b.sortPlan = BatchSortPlan{};
+    if (b.sortDev) (void)hipFree(b.sortDev);
+    b.sortDev = nullptr;
     if (N) {
-        const BucketPlan bp = make_bucket_plan(N, C, 0);
-        NW_REQUIRE(!*bucket_plan_problem(bp), "nwBatchSetReplicas: %s", bucket_plan_problem(bp));
-        NW_HIP(ensure_bucket_scratch(b.scratch, bp));
+        const char *why = make_sort_reps(b.beadOff.data(), b.cellOff.data(), numReplicas, b.sort.data(), b.sortPlan);
+        NW_REQUIRE(!*why, "nwBatchSetReplicas: %s", why);
+        NW_HIP(hipMalloc(&b.sortDev, sizeof(SortRep) * numReplicas));
+        NW_HIP(hipMemcpy(b.sortDev, b.sort.data(), sizeof(SortRep) * numReplicas, hipMemcpyHostToDevice));
+        NW_HIP(ensure_batch_sort_scratch(b.scratch, b.sortPlan));
     }
 }
 
@@ -842,11 +851,12 @@ void nwBatchRngState(uint replica, uint64_t *offset) {
 // the launches of one batched step
 static hipError_t batch_step(State::Batch &b, const BatchArrays &a, const float4 *vel, const float4 *tan,
                              float4 *sVel, float4 *sTan, float dt, hipStream_t st) {
-    const BucketPlan bp = make_bucket_plan(b.N, b.cells, 0);
     hipError_t e;
     if ((e = launch_md_front_batch(a, b.dev, b.R, b.waves, b.stepCtr, dt, st)) != hipSuccess) return e;
-    if ((e = launch_hash_tiles(a.hash, bp, b.scratch, st)) != hipSuccess) return e;
-    if ((e = launch_md_cells(bp, b.scratch, a.ids, a.pos, vel, tan, a.sPos, sVel, sTan, st)) != hipSuccess) return e;
+    if ((e = launch_hash_tiles_batch(a.hash, b.sortDev, b.R, b.sortPlan, b.scratch, st)) != hipSuccess) return e;
+    if ((e = launch_md_cells_batch(b.sortDev, b.R, b.sortPlan, b.scratch, a.ids, a.pos, vel, tan, a.sPos, sVel, sTan,
+                                   st)) != hipSuccess)
+        return e;
     return launch_collide_batch(a, b.dev, b.R, b.blocks, b.mode, b.tan, b.unit, b.stepCtr, st);
 }
 

@@ -21,6 +21,7 @@
 //   rotate    (k_srd_rotate) particle order, coalesced: every particle
 //             rotates about its cell's momentum, solvent streams
 // Results are deterministic and equal to a stable sort by cell.
+#include <algorithm>
 #include <utility>
 #include <vector>
 
@@ -137,18 +138,20 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(const float4 *__restrict__
 // registers (one read of the cells) and the tile's pairs staged in LDS in
 // bucket order, then written out with coalesced stores (tiles of
 // 1024 x ITEMS particles).
+// Tile t of one system: hash / pairs / P start at the system's own first
+// entries, its cells start at cellBase in the hash values and its bead ids at
+// idBase (zero for a single system); pairs carry the system's own cell and the
+// global id.
 template <bool FROM_POS, int ITEMS>
-__global__ __launch_bounds__(kTileBlock) void k_tiles_staged(const float4 *__restrict__ pos,
-                                                             uint32_t *__restrict__ hash,
-                                                             uint2 *__restrict__ pairs, uint32_t *__restrict__ P,
-                                                             BucketPlan bp, KP kp) {
-    extern __shared__ __align__(16) uint32_t lh[];
+__device__ __forceinline__ void tiles_staged_body(const float4 *__restrict__ pos, uint32_t *__restrict__ hash,
+                                                  uint2 *__restrict__ pairs, uint32_t *__restrict__ P,
+                                                  const BucketPlan &bp, const KP &kp, uint32_t t, uint32_t cellBase,
+                                                  uint32_t idBase, uint32_t *lh) {
     const uint32_t B1 = bp.buckets + 1;
     uint32_t *wsum = lh + B1;
     uint2 *stage = reinterpret_cast<uint2 *>(lh + ((B1 + kTileBlock / 64 + 3) & ~3u));
     for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) lh[b] = 0u;
     __syncthreads();
-    const uint32_t t = blockIdx.x;
     const uint32_t i0 = t * bp.tileItems;
     const uint32_t i1 = min(i0 + bp.tileItems, bp.n);
     uint32_t h[ITEMS], rk[ITEMS];
@@ -162,7 +165,7 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles_staged(const float4 *__res
                 h[k] = srd_hash(kp.p, kp.s, mk(pos[i]));
                 hash[i] = h[k];
             } else {
-                h[k] = hash[i];
+                h[k] = hash[i] - cellBase;
             }
             rk[k] = atomicAdd(&lh[h[k] >> bp.shift], 1u);
         }
@@ -174,11 +177,57 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles_staged(const float4 *__res
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         const uint32_t i = i0 + uint32_t(k) * kTileBlock + threadIdx.x;
-        if (i < i1) stage[lh[h[k] >> bp.shift] + rk[k]] = make_uint2(h[k], i);
+        if (i < i1) stage[lh[h[k] >> bp.shift] + rk[k]] = make_uint2(h[k], idBase + i);
     }
     __syncthreads();
     const uint32_t cnt = i1 - i0;
     for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) pairs[i0 + k] = stage[k];
+}
+
+template <bool FROM_POS, int ITEMS>
+__global__ __launch_bounds__(kTileBlock) void k_tiles_staged(const float4 *__restrict__ pos,
+                                                             uint32_t *__restrict__ hash,
+                                                             uint2 *__restrict__ pairs, uint32_t *__restrict__ P,
+                                                             BucketPlan bp, KP kp) {
+    extern __shared__ __align__(16) uint32_t lh[];
+    tiles_staged_body<FROM_POS, ITEMS>(pos, hash, pairs, P, bp, kp, blockIdx.x, 0u, 0u, lh);
+}
+
+// the replica that owns item v of a launch over all replicas (offsets
+// ascending, wave-uniform)
+__device__ __forceinline__ uint32_t sort_rep_of(const SortRep *__restrict__ reps, uint32_t R, uint32_t v,
+                                                bool byTile) {
+    uint32_t lo = 0, hi = R;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((byTile ? reps[mid].tileOff : reps[mid].bucketOff) <= v) lo = mid;
+        else hi = mid;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+__device__ __forceinline__ BucketPlan plan_of(const SortRep &r) {
+    BucketPlan bp;
+    bp.n = r.n;
+    bp.cells = r.cells;
+    bp.shift = r.shift;
+    bp.buckets = r.buckets;
+    bp.tileItems = kBatchTileItems;
+    bp.tiles = r.tiles;
+    return bp;
+}
+
+// level 1 of the batched sort: every replica's tiles in one launch, each
+// against its own buckets and prefix matrix
+__global__ __launch_bounds__(kTileBlock) void k_tiles_batch(const uint32_t *__restrict__ hash,
+                                                            uint2 *__restrict__ pairs, uint32_t *__restrict__ P,
+                                                            const SortRep *__restrict__ reps, uint32_t R) {
+    extern __shared__ __align__(16) uint32_t lh[];
+    const SortRep &r = reps[sort_rep_of(reps, R, blockIdx.x, true)];
+    const BucketPlan bp = plan_of(r);
+    tiles_staged_body<false, kBatchTileItems / kTileBlock>(nullptr, const_cast<uint32_t *>(hash) + r.beadOffset,
+                                                            pairs + r.beadOffset, P + r.matOff, bp, KP{},
+                                                            blockIdx.x - r.tileOff, r.cellOffset, r.beadOffset, lh);
 }
 
 // ---------------------------------------------------------------------------
@@ -188,6 +237,9 @@ struct CellsArgs {
     const uint2 *pairs;
     const uint32_t *P;
     uint32_t *ids, *off, *cellOf;
+    // added to the cells written to cellOf and to the slots written to off
+    // (a replica's first cell and first bead; zero for a single system)
+    uint32_t cellBase, slotBase;
     // MD gathers (sPos NULL: none)
     const float4 *pos, *vel, *tan;
     float4 *sPos, *sVel, *sTan;
@@ -245,7 +297,7 @@ __device__ __forceinline__ void md_gather_group(const CellsArgs &o, const uint32
 #pragma unroll
         for (int j = 0; j < G; ++j) {
             out[k[j]] = id[j];
-            o.cellOf[sb + k[j]] = cell0 + uint32_t(lcell[k[j]]);
+            o.cellOf[sb + k[j]] = o.cellBase + cell0 + uint32_t(lcell[k[j]]);
             if (POS) sPos[sb + k[j]] = P[j];
             if (VEL) sVel[sb + k[j]] = V[j];
             if (TAN) sTan[sb + k[j]] = Tn[j];
@@ -264,10 +316,11 @@ __device__ __forceinline__ void md_gather_staged(const CellsArgs &o, const uint3
 
 // BLOCK threads per bucket; a staged bucket gives each thread at most
 // PER = kBucketCap / BLOCK members
+// bucket b of one system (see CellsArgs for the bases of a replica)
 template <bool SRD, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_bucket_cells(CellsArgs o, BucketPlan bp) {
+__device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const BucketPlan &bp, uint32_t b,
+                                                  unsigned char *smem) {
     constexpr int PER = int(kBucketCap) / BLOCK;
-    extern __shared__ __align__(16) unsigned char smem[];
     const CellsLayout L = cells_layout(bp, SRD);
     uint32_t *wsum = reinterpret_cast<uint32_t *>(smem + L.wsum);
     uint32_t *cnt = reinterpret_cast<uint32_t *>(smem + L.cnt);
@@ -280,7 +333,6 @@ __global__ __launch_bounds__(BLOCK) void k_bucket_cells(CellsArgs o, BucketPlan 
     uint16_t *tmpCell = reinterpret_cast<uint16_t *>(smem + L.tmpCell);
     float4 *lv = reinterpret_cast<float4 *>(smem + L.lv);
 
-    const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
     const uint32_t B1 = bp.buckets + 1;
     const uint32_t tid = threadIdx.x, bd = blockDim.x;
 
@@ -368,9 +420,9 @@ __global__ __launch_bounds__(BLOCK) void k_bucket_cells(CellsArgs o, BucketPlan 
             }
             arr[w] = x;
         }
-        o.off[cell0 + c] = sb + a;
+        o.off[cell0 + c] = o.slotBase + sb + a;
     }
-    if (tid == 0 && cell0 + nc == bp.cells) o.off[bp.cells] = bp.n;
+    if (tid == 0 && cell0 + nc == bp.cells) o.off[bp.cells] = o.slotBase + bp.n;
     __syncthreads();
 
     if (!SRD) {
@@ -384,7 +436,7 @@ __global__ __launch_bounds__(BLOCK) void k_bucket_cells(CellsArgs o, BucketPlan 
         }
         for (uint32_t k = tid; k < m; k += bd) {
             const uint32_t id = arr[k];
-            o.cellOf[sb + k] = cell0 + seg_of(cnt, nc, k);
+            o.cellOf[sb + k] = o.cellBase + cell0 + seg_of(cnt, nc, k);
             if (o.sPos) {
                 o.sPos[sb + k] = o.pos[id];
                 if (o.sVel) o.sVel[sb + k] = o.vel[id];
@@ -444,6 +496,35 @@ __global__ __launch_bounds__(BLOCK) void k_bucket_cells(CellsArgs o, BucketPlan 
         o.com[cell] = f4(mom, sgn);
         if (o.iso) o.lam[cell] = lambda;
     }
+}
+
+template <bool SRD, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_bucket_cells(CellsArgs o, BucketPlan bp) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    bucket_cells_body<SRD, BLOCK>(o, bp, xcd_contiguous(blockIdx.x, gridDim.x), smem);
+}
+
+// level 2 of the batched sort: every replica's buckets in one launch; the
+// arrays are offset to the replica's first bead and cell, the bead ids in the
+// pairs are global
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_bucket_cells_batch(CellsArgs a, const SortRep *__restrict__ reps,
+                                                              uint32_t R) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t v = xcd_contiguous(blockIdx.x, gridDim.x);
+    const SortRep &r = reps[sort_rep_of(reps, R, v, false)];
+    CellsArgs o = a;
+    o.pairs = a.pairs + r.beadOffset;
+    o.P = a.P + r.matOff;
+    o.ids = a.ids + r.beadOffset;
+    o.off = a.off + r.cellOffset;
+    o.cellOf = a.cellOf + r.beadOffset;
+    if (o.sPos) o.sPos = a.sPos + r.beadOffset;
+    if (o.sVel) o.sVel = a.sVel + r.beadOffset;
+    if (o.sTan) o.sTan = a.sTan + r.beadOffset;
+    o.cellBase = r.cellOffset;
+    o.slotBase = r.beadOffset;
+    bucket_cells_body<false, BLOCK>(o, plan_of(r), v - r.bucketOff, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -606,6 +687,119 @@ hipError_t launch_srd_tiles(const float4 *pos, uint32_t *hash, const BucketPlan 
 
 hipError_t launch_hash_tiles(const uint32_t *hash, const BucketPlan &bp, BucketScratch &s, hipStream_t st) {
     return launch_tiles(false, nullptr, const_cast<uint32_t *>(hash), bp, s, KP{}, st);
+}
+
+// ---------------------------------------------------------------------------
+// the batched sort
+// ---------------------------------------------------------------------------
+static size_t batch_tiles_lds(uint32_t maxBuckets) {
+    return ((size_t(maxBuckets) + 1 + kTileBlock / 64 + 3) & ~size_t(3)) * sizeof(uint32_t) +
+           size_t(kBatchTileItems) * sizeof(uint2);
+}
+
+static CellsLayout batch_cells_layout(const BatchSortPlan &tot) {
+    BucketPlan bp{};
+    bp.shift = tot.maxShift;
+    bp.tiles = tot.maxTiles;
+    return cells_layout(bp, false);
+}
+
+const char *make_sort_reps(const uint32_t *beadOff, const uint32_t *cellOff, uint32_t R, SortRep *out,
+                           BatchSortPlan &tot) {
+    tot = BatchSortPlan{};
+    for (uint32_t r = 0; r < R; ++r) {
+        const uint32_t n = beadOff[r + 1] - beadOff[r], cells = cellOff[r + 1] - cellOff[r];
+        if (cells == 0) return "a replica without cells";
+        const BucketPlan bp = make_bucket_plan(n, cells, kBatchTileItems);
+        const char *why = bucket_plan_problem(bp);
+        if (*why) return why;
+        SortRep &q = out[r];
+        q.beadOffset = beadOff[r];
+        q.n = n;
+        q.cellOffset = cellOff[r];
+        q.cells = cells;
+        q.shift = bp.shift;
+        q.buckets = bp.buckets;
+        q.tiles = bp.tiles;
+        q.tileOff = tot.tiles;
+        q.bucketOff = tot.buckets;
+        q.pad = 0;
+        q.matOff = tot.matrix;
+        tot.tiles += bp.tiles;
+        tot.buckets += bp.buckets;
+        tot.matrix += uint64_t(bp.tiles) * (bp.buckets + 1);
+        tot.maxBuckets = std::max(tot.maxBuckets, bp.buckets);
+        tot.maxShift = std::max(tot.maxShift, bp.shift);
+        tot.maxTiles = std::max(tot.maxTiles, bp.tiles);
+        tot.n = uint64_t(beadOff[r + 1]);
+        tot.cells = uint64_t(cellOff[r + 1]);
+    }
+    if (batch_tiles_lds(tot.maxBuckets) > 160 * 1024) return "a replica with too many cells for the batched sort";
+    if (batch_cells_layout(tot).total > 160 * 1024) return "a replica with too many particles for the batched sort";
+    return "";
+}
+
+hipError_t ensure_batch_sort_scratch(BucketScratch &s, const BatchSortPlan &tot) {
+    hipError_t e;
+    if (tot.n > s.cap_n) {
+        const uint64_t c = tot.n + tot.n / 8 + 1024;
+        if (s.hash) (void)hipFree(s.hash);
+        if (s.sortedCell) (void)hipFree(s.sortedCell);
+        s.hash = s.sortedCell = nullptr;
+        s.cap_n = 0;
+        if ((e = hipMalloc(&s.hash, c * 4)) != hipSuccess) return e;
+        if ((e = hipMalloc(&s.sortedCell, c * 4)) != hipSuccess) return e;
+        s.cap_n = c;
+    }
+    if ((e = regrow(s.pairs, s.cap_pairs, tot.n, 8)) != hipSuccess) return e;
+    if ((e = regrow(s.P, s.cap_mat, tot.matrix, 4)) != hipSuccess) return e;
+    if (tot.cells + 2 > s.cap_cells) {
+        const uint64_t c = tot.cells + tot.cells / 8 + 1024;
+        if (s.off) (void)hipFree(s.off);
+        if (s.lam) (void)hipFree(s.lam);
+        s.off = nullptr;
+        s.lam = nullptr;
+        s.cap_cells = 0;
+        if ((e = hipMalloc(&s.off, (c + 2) * 4)) != hipSuccess) return e;
+        if ((e = hipMalloc(&s.lam, c * 4)) != hipSuccess) return e;
+        s.cap_cells = c;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_hash_tiles_batch(const uint32_t *hash, const SortRep *reps, uint32_t R, const BatchSortPlan &tot,
+                                   BucketScratch &s, hipStream_t st) {
+    if (tot.tiles == 0) return hipSuccess;
+    const size_t lds = batch_tiles_lds(tot.maxBuckets);
+    const hipError_t e = grant_lds(reinterpret_cast<const void *>(&k_tiles_batch), lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_tiles_batch, dim3(tot.tiles), dim3(kTileBlock), lds, st, hash, s.pairs, s.P, reps, R);
+    return hipGetLastError();
+}
+
+hipError_t launch_md_cells_batch(const SortRep *reps, uint32_t R, const BatchSortPlan &tot, BucketScratch &s,
+                                 uint32_t *ids, const float4 *pos, const float4 *vel, const float4 *tan, float4 *sPos,
+                                 float4 *sVel, float4 *sTan, hipStream_t st) {
+    if (tot.buckets == 0) return hipSuccess;
+    CellsArgs o{};
+    o.pairs = s.pairs;
+    o.P = s.P;
+    o.ids = ids;
+    o.off = s.off;
+    o.cellOf = s.sortedCell;
+    o.pos = pos;
+    o.vel = vel;
+    o.tan = tan;
+    o.sPos = sPos;
+    o.sVel = sVel;
+    o.sTan = sTan;
+    const CellsLayout L = batch_cells_layout(tot);
+    const void *fn = reinterpret_cast<const void *>(&k_bucket_cells_batch<kMdCellsBlock>);
+    const hipError_t e = grant_lds(fn, L.total);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_bucket_cells_batch<kMdCellsBlock>), dim3(tot.buckets), dim3(kMdCellsBlock), L.total, st, o,
+                       reps, R);
+    return hipGetLastError();
 }
 
 hipError_t launch_md_cells(const BucketPlan &bp, BucketScratch &s, uint32_t *ids, const float4 *pos,

@@ -108,6 +108,32 @@ struct BatchRep {
     uint32_t numBeads;     // numFilaments x filamentSize
 };
 
+// The batched sort: every replica sorted against its own plan (tiles of
+// kBatchTileItems of its beads, buckets of its own cells, its own prefix
+// matrix) with one launch per level for all replicas.  Replica r's pairs sit
+// at its bead offset, its matrix at matOff; tiles and buckets of all
+// replicas are numbered consecutively (tileOff, bucketOff).
+constexpr uint32_t kBatchTileItems = 4096;
+struct SortRep {
+    uint32_t beadOffset, n, cellOffset, cells;
+    uint32_t shift, buckets, tiles, tileOff, bucketOff, pad;
+    uint64_t matOff;
+};
+struct BatchSortPlan {
+    uint32_t tiles = 0, buckets = 0, maxBuckets = 0, maxShift = 0, maxTiles = 0;
+    uint64_t matrix = 0, n = 0, cells = 0;
+};
+// host: the replicas' plans from their bead and cell offsets (R + 1 entries
+// each); returns "" or what does not fit
+const char *make_sort_reps(const uint32_t *beadOff, const uint32_t *cellOff, uint32_t R, SortRep *out,
+                           BatchSortPlan &tot);
+hipError_t ensure_batch_sort_scratch(BucketScratch &s, const BatchSortPlan &tot);
+hipError_t launch_hash_tiles_batch(const uint32_t *hash, const SortRep *reps, uint32_t R, const BatchSortPlan &tot,
+                                   BucketScratch &s, hipStream_t st);
+hipError_t launch_md_cells_batch(const SortRep *reps, uint32_t R, const BatchSortPlan &tot, BucketScratch &s,
+                                 uint32_t *ids, const float4 *pos, const float4 *vel, const float4 *tan, float4 *sPos,
+                                 float4 *sVel, float4 *sTan, hipStream_t st);
+
 struct BatchArrays {
     float4 *pos, *vel, *force, *forceOld, *tangent;
     uint32_t *hash;   // global cell of every bead
