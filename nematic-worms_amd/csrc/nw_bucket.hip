@@ -35,15 +35,15 @@ constexpr int kTileBlock = 1024;
 // ---------------------------------------------------------------------------
 // plan
 // ---------------------------------------------------------------------------
-BucketPlan make_bucket_plan(uint64_t n, uint64_t cells, uint32_t tileItems) {
+BucketPlan make_bucket_plan(uint64_t n, uint64_t cells, uint32_t tileItems, uint32_t members) {
     BucketPlan b{};
     b.n = uint32_t(n);
     b.cells = uint32_t(cells);
-    // at most ~1536 members per bucket on average (the LDS stage holds 2048),
-    // at most 2048 cells per bucket
+    // at most `members` members per bucket on average, at most 2048 cells per
+    // bucket
     const double perCell = cells ? double(n) / double(cells) : 1.0;
     int S = 0;
-    while (S < 11 && double(1u << (S + 1)) * perCell <= 1536.0) ++S;
+    while (S < 11 && double(1u << (S + 1)) * perCell <= double(members)) ++S;
     b.shift = uint32_t(S);
     b.buckets = uint32_t((cells + (1u << S) - 1) >> S);
     if (tileItems == 0) {
@@ -72,23 +72,24 @@ struct CellsLayout {
 
 __host__ __device__ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
 
-__host__ __device__ inline CellsLayout cells_layout(const BucketPlan &bp, bool srd) {
+// cap: members a bucket stages in LDS (more: the global-memory path)
+__host__ __device__ inline CellsLayout cells_layout(const BucketPlan &bp, bool srd, uint32_t cap = kBucketCap) {
     CellsLayout L;
     const uint32_t nc = 1u << bp.shift;
     uint32_t o = 0;
     L.wsum = o; o += align16(4u * 2u * (kCellsBlockMax / 64));
     L.cnt = o;  o += align16(4u * (nc + 1));
     L.cur = o;  o += align16(4u * nc);
-    L.lid = o;  o += align16(4u * kBucketCap);
-    L.lcell = o; if (!srd) o += align16(2u * kBucketCap);
+    L.lid = o;  o += align16(4u * cap);
+    L.lcell = o; if (!srd) o += align16(2u * cap);
     // region A (segment tables, staged pairs) -- SRD velocities alias it
     const uint32_t a0 = o;
     L.segStart = o; o += align16(4u * (bp.tiles + 1));
     L.segBase = o;  o += align16(4u * bp.tiles);
-    L.tmpId = o;    o += align16(4u * kBucketCap);
-    L.tmpCell = o;  o += align16(2u * kBucketCap);
+    L.tmpId = o;    o += align16(4u * cap);
+    L.tmpCell = o;  o += align16(2u * cap);
     L.lv = a0;
-    if (srd && o < a0 + 16u * kBucketCap) o = a0 + 16u * kBucketCap;
+    if (srd && o < a0 + 16u * cap) o = a0 + 16u * cap;
     L.total = o;
     return L;
 }
@@ -142,14 +143,14 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(const float4 *__restrict__
 // entries, its cells start at cellBase in the hash values and its bead ids at
 // idBase (zero for a single system); pairs carry the system's own cell and the
 // global id.
-template <bool FROM_POS, int ITEMS>
+template <bool FROM_POS, int ITEMS, int TB = kTileBlock>
 __device__ __forceinline__ void tiles_staged_body(const float4 *__restrict__ pos, uint32_t *__restrict__ hash,
                                                   uint2 *__restrict__ pairs, uint32_t *__restrict__ P,
                                                   const BucketPlan &bp, const KP &kp, uint32_t t, uint32_t cellBase,
                                                   uint32_t idBase, uint32_t *lh) {
     const uint32_t B1 = bp.buckets + 1;
     uint32_t *wsum = lh + B1;
-    uint2 *stage = reinterpret_cast<uint2 *>(lh + ((B1 + kTileBlock / 64 + 3) & ~3u));
+    uint2 *stage = reinterpret_cast<uint2 *>(lh + ((B1 + TB / 64 + 3) & ~3u));
     for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) lh[b] = 0u;
     __syncthreads();
     const uint32_t i0 = t * bp.tileItems;
@@ -157,7 +158,7 @@ __device__ __forceinline__ void tiles_staged_body(const float4 *__restrict__ pos
     uint32_t h[ITEMS], rk[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
-        const uint32_t i = i0 + uint32_t(k) * kTileBlock + threadIdx.x;
+        const uint32_t i = i0 + uint32_t(k) * TB + threadIdx.x;
         h[k] = 0u;
         rk[k] = 0u;
         if (i < i1) {
@@ -176,7 +177,7 @@ __device__ __forceinline__ void tiles_staged_body(const float4 *__restrict__ pos
     for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) row[b] = lh[b];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
-        const uint32_t i = i0 + uint32_t(k) * kTileBlock + threadIdx.x;
+        const uint32_t i = i0 + uint32_t(k) * TB + threadIdx.x;
         if (i < i1) stage[lh[h[k] >> bp.shift] + rk[k]] = make_uint2(h[k], idBase + i);
     }
     __syncthreads();
@@ -218,14 +219,18 @@ __device__ __forceinline__ BucketPlan plan_of(const SortRep &r) {
 }
 
 // level 1 of the batched sort: every replica's tiles in one launch, each
-// against its own buckets and prefix matrix
-__global__ __launch_bounds__(kTileBlock) void k_tiles_batch(const uint32_t *__restrict__ hash,
-                                                            uint2 *__restrict__ pairs, uint32_t *__restrict__ P,
-                                                            const SortRep *__restrict__ reps, uint32_t R) {
+// against its own buckets and prefix matrix (256 threads, 16 beads each: the
+// LDS stage, not the wave count, bounds the tiles a CU holds)
+constexpr int kBatchTileBlock = 256;
+
+__global__ __launch_bounds__(kBatchTileBlock) void k_tiles_batch(const uint32_t *__restrict__ hash,
+                                                                 uint2 *__restrict__ pairs, uint32_t *__restrict__ P,
+                                                                 const SortRep *__restrict__ reps, uint32_t R) {
     extern __shared__ __align__(16) uint32_t lh[];
     const SortRep &r = reps[sort_rep_of(reps, R, blockIdx.x, true)];
     const BucketPlan bp = plan_of(r);
-    tiles_staged_body<false, kBatchTileItems / kTileBlock>(nullptr, const_cast<uint32_t *>(hash) + r.beadOffset,
+    tiles_staged_body<false, kBatchTileItems / kBatchTileBlock, kBatchTileBlock>(
+        nullptr, const_cast<uint32_t *>(hash) + r.beadOffset,
                                                             pairs + r.beadOffset, P + r.matOff, bp, KP{},
                                                             blockIdx.x - r.tileOff, r.cellOffset, r.beadOffset, lh);
 }
@@ -305,23 +310,23 @@ __device__ __forceinline__ void md_gather_group(const CellsArgs &o, const uint32
     }
 }
 
-template <bool POS, bool VEL, bool TAN, int BLOCK>
+template <bool POS, bool VEL, bool TAN, int BLOCK, int CAP>
 __device__ __forceinline__ void md_gather_staged(const CellsArgs &o, const uint32_t *lid, const uint16_t *lcell,
                                                  uint32_t sb, uint32_t cell0, uint32_t m) {
-    constexpr int PER = int(kBucketCap) / BLOCK, G = PER < 4 ? PER : 4;
+    constexpr int PER = CAP / BLOCK, G = PER < 4 ? PER : 4;
 #pragma unroll
     for (int g = 0; g < PER; g += G)
         if (m > uint32_t(g) * BLOCK) md_gather_group<POS, VEL, TAN, BLOCK, G>(o, lid, lcell, sb, cell0, m, g);
 }
 
-// BLOCK threads per bucket; a staged bucket gives each thread at most
-// PER = kBucketCap / BLOCK members
-// bucket b of one system (see CellsArgs for the bases of a replica)
-template <bool SRD, int BLOCK>
+// Bucket b of one system (see CellsArgs for the bases of a replica), BLOCK
+// threads; a bucket of at most CAP members is staged in LDS, each thread
+// holding at most PER = CAP / BLOCK of them.
+template <bool SRD, int BLOCK, int CAP = int(kBucketCap)>
 __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const BucketPlan &bp, uint32_t b,
                                                   unsigned char *smem) {
-    constexpr int PER = int(kBucketCap) / BLOCK;
-    const CellsLayout L = cells_layout(bp, SRD);
+    constexpr int PER = CAP / BLOCK;
+    const CellsLayout L = cells_layout(bp, SRD, CAP);
     uint32_t *wsum = reinterpret_cast<uint32_t *>(smem + L.wsum);
     uint32_t *cnt = reinterpret_cast<uint32_t *>(smem + L.cnt);
     uint32_t *cur = reinterpret_cast<uint32_t *>(smem + L.cur);
@@ -357,7 +362,7 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
     for (uint32_t c = tid; c < nc; c += bd) cnt[c] = 0u;
     __syncthreads();
 
-    const bool staged = m <= kBucketCap;
+    const bool staged = m <= uint32_t(CAP);
     const uint64_t T = bp.tileItems;
     if (staged && m) {
         // every load of the thread's members in flight before the first use
@@ -427,11 +432,11 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
 
     if (!SRD) {
         if (staged) {
-            if (!o.sPos) md_gather_staged<false, false, false, BLOCK>(o, lid, lcell, sb, cell0, m);
-            else if (o.sVel && o.sTan) md_gather_staged<true, true, true, BLOCK>(o, lid, lcell, sb, cell0, m);
-            else if (o.sVel) md_gather_staged<true, true, false, BLOCK>(o, lid, lcell, sb, cell0, m);
-            else if (o.sTan) md_gather_staged<true, false, true, BLOCK>(o, lid, lcell, sb, cell0, m);
-            else md_gather_staged<true, false, false, BLOCK>(o, lid, lcell, sb, cell0, m);
+            if (!o.sPos) md_gather_staged<false, false, false, BLOCK, CAP>(o, lid, lcell, sb, cell0, m);
+            else if (o.sVel && o.sTan) md_gather_staged<true, true, true, BLOCK, CAP>(o, lid, lcell, sb, cell0, m);
+            else if (o.sVel) md_gather_staged<true, true, false, BLOCK, CAP>(o, lid, lcell, sb, cell0, m);
+            else if (o.sTan) md_gather_staged<true, false, true, BLOCK, CAP>(o, lid, lcell, sb, cell0, m);
+            else md_gather_staged<true, false, false, BLOCK, CAP>(o, lid, lcell, sb, cell0, m);
             return;
         }
         for (uint32_t k = tid; k < m; k += bd) {
@@ -454,7 +459,7 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
         for (int j = 0; j < PER; ++j) id[j] = lid[min(tid + uint32_t(j) * BLOCK, m - 1u)];
 #pragma unroll
         for (int j = 0; j < PER; ++j) v[j] = o.srdVel[id[j]];
-        // lv holds kBucketCap entries: slots past the last member take
+        // lv holds CAP entries: slots past the last member take
         // copies, never read
 #pragma unroll
         for (int j = 0; j < PER; ++j) lv[tid + uint32_t(j) * BLOCK] = v[j];
@@ -507,7 +512,7 @@ __global__ __launch_bounds__(BLOCK) void k_bucket_cells(CellsArgs o, BucketPlan 
 // level 2 of the batched sort: every replica's buckets in one launch; the
 // arrays are offset to the replica's first bead and cell, the bead ids in the
 // pairs are global
-template <int BLOCK>
+template <int BLOCK, int CAP>
 __global__ __launch_bounds__(BLOCK) void k_bucket_cells_batch(CellsArgs a, const SortRep *__restrict__ reps,
                                                               uint32_t R) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -524,7 +529,7 @@ __global__ __launch_bounds__(BLOCK) void k_bucket_cells_batch(CellsArgs a, const
     if (o.sTan) o.sTan = a.sTan + r.beadOffset;
     o.cellBase = r.cellOffset;
     o.slotBase = r.beadOffset;
-    bucket_cells_body<false, BLOCK>(o, plan_of(r), v - r.bucketOff, smem);
+    bucket_cells_body<false, BLOCK, CAP>(o, plan_of(r), v - r.bucketOff, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -693,7 +698,7 @@ hipError_t launch_hash_tiles(const uint32_t *hash, const BucketPlan &bp, BucketS
 // the batched sort
 // ---------------------------------------------------------------------------
 static size_t batch_tiles_lds(uint32_t maxBuckets) {
-    return ((size_t(maxBuckets) + 1 + kTileBlock / 64 + 3) & ~size_t(3)) * sizeof(uint32_t) +
+    return ((size_t(maxBuckets) + 1 + kBatchTileBlock / 64 + 3) & ~size_t(3)) * sizeof(uint32_t) +
            size_t(kBatchTileItems) * sizeof(uint2);
 }
 
@@ -701,7 +706,7 @@ static CellsLayout batch_cells_layout(const BatchSortPlan &tot) {
     BucketPlan bp{};
     bp.shift = tot.maxShift;
     bp.tiles = tot.maxTiles;
-    return cells_layout(bp, false);
+    return cells_layout(bp, false, kBatchBucketCap);
 }
 
 const char *make_sort_reps(const uint32_t *beadOff, const uint32_t *cellOff, uint32_t R, SortRep *out,
@@ -710,7 +715,7 @@ const char *make_sort_reps(const uint32_t *beadOff, const uint32_t *cellOff, uin
     for (uint32_t r = 0; r < R; ++r) {
         const uint32_t n = beadOff[r + 1] - beadOff[r], cells = cellOff[r + 1] - cellOff[r];
         if (cells == 0) return "a replica without cells";
-        const BucketPlan bp = make_bucket_plan(n, cells, kBatchTileItems);
+        const BucketPlan bp = make_bucket_plan(n, cells, kBatchTileItems, kBatchMembers);
         const char *why = bucket_plan_problem(bp);
         if (*why) return why;
         SortRep &q = out[r];
@@ -773,7 +778,7 @@ hipError_t launch_hash_tiles_batch(const uint32_t *hash, const SortRep *reps, ui
     const size_t lds = batch_tiles_lds(tot.maxBuckets);
     const hipError_t e = grant_lds(reinterpret_cast<const void *>(&k_tiles_batch), lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_tiles_batch, dim3(tot.tiles), dim3(kTileBlock), lds, st, hash, s.pairs, s.P, reps, R);
+    hipLaunchKernelGGL(k_tiles_batch, dim3(tot.tiles), dim3(kBatchTileBlock), lds, st, hash, s.pairs, s.P, reps, R);
     return hipGetLastError();
 }
 
@@ -794,11 +799,11 @@ hipError_t launch_md_cells_batch(const SortRep *reps, uint32_t R, const BatchSor
     o.sVel = sVel;
     o.sTan = sTan;
     const CellsLayout L = batch_cells_layout(tot);
-    const void *fn = reinterpret_cast<const void *>(&k_bucket_cells_batch<kMdCellsBlock>);
+    const void *fn = reinterpret_cast<const void *>(&k_bucket_cells_batch<kMdCellsBlock, int(kBatchBucketCap)>);
     const hipError_t e = grant_lds(fn, L.total);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_bucket_cells_batch<kMdCellsBlock>), dim3(tot.buckets), dim3(kMdCellsBlock), L.total, st, o,
-                       reps, R);
+    hipLaunchKernelGGL((k_bucket_cells_batch<kMdCellsBlock, int(kBatchBucketCap)>), dim3(tot.buckets),
+                       dim3(kMdCellsBlock), L.total, st, o, reps, R);
     return hipGetLastError();
 }
 

@@ -32,7 +32,9 @@ constexpr int kCellsBlockMax = 1024;
 constexpr int kPairBlock = 256;           // sorted beads per LDS pair-kernel workgroup
 
 // tileItems == 0: chosen for the particle count and the cells
-BucketPlan make_bucket_plan(uint64_t n, uint64_t cells, uint32_t tileItems);
+// members: the average bucket population aimed at (the LDS stage of a
+// single system holds kBucketCap = 2048)
+BucketPlan make_bucket_plan(uint64_t n, uint64_t cells, uint32_t tileItems, uint32_t members = 1536);
 // empty string when the plan fits the limits above, else the reason
 const char *bucket_plan_problem(const BucketPlan &bp);
 
@@ -114,6 +116,9 @@ struct BatchRep {
 // at its bead offset, its matrix at matOff; tiles and buckets of all
 // replicas are numbered consecutively (tileOff, bucketOff).
 constexpr uint32_t kBatchTileItems = 4096;
+// the batched level 2 stages at most 1024 members per bucket and aims at 512
+// (a smaller LDS stage: more buckets in flight per CU)
+constexpr uint32_t kBatchBucketCap = 1024, kBatchMembers = 512;
 struct SortRep {
     uint32_t beadOffset, n, cellOffset, cells;
     uint32_t shift, buckets, tiles, tileOff, bucketOff, pad;
