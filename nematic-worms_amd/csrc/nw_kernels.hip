@@ -146,13 +146,23 @@ __global__ __launch_bounds__(kBlock) void k_collide(float4 *__restrict__ force, 
     const AxisCells az = axis_cells(gz, p.gridSize.z, twoD);
     const float rc2 = kp.e.wcaCutoff * kp.e.wcaCutoff;
 
+    // the neighbour cells by selects over scalars (a runtime index into the
+    // AxisCells arrays would put them in scratch memory)
+    const int cx0 = ax.c[0], cx1 = ax.c[1], cx2 = ax.c[2], cy0 = ay.c[0], cy1 = ay.c[1], cy2 = ay.c[2];
+    const int cz0 = az.c[0], cz1 = az.c[1], cz2 = az.c[2];
+    const float wx0 = ax.w[0], wx1 = ax.w[1], wx2 = ax.w[2], wy0 = ay.w[0], wy1 = ay.w[1], wy2 = ay.w[2];
+    const float wz0 = az.w[0], wz1 = az.w[1], wz2 = az.w[2];
     f3 force_acc = mk(0.f, 0.f, 0.f);
     for (int zi = 0; zi < az.n; ++zi) {
+        const int cz = zi == 0 ? cz0 : (zi == 1 ? cz1 : cz2);
+        const float wz = zi == 0 ? wz0 : (zi == 1 ? wz1 : wz2);
         for (int yi = 0; yi < ay.n; ++yi) {
-            const uint32_t rowBase = (uint32_t(az.c[zi]) * p.gridSize.y + uint32_t(ay.c[yi])) * p.gridSize.x;
+            const int cy = yi == 0 ? cy0 : (yi == 1 ? cy1 : cy2);
+            const float wy = yi == 0 ? wy0 : (yi == 1 ? wy1 : wy2);
+            const uint32_t rowBase = (uint32_t(cz) * p.gridSize.y + uint32_t(cy)) * p.gridSize.x;
             for (int xi = 0; xi < ax.n; ++xi) {
-                const float wgt = az.w[zi] * ay.w[yi] * ax.w[xi];
-                const uint32_t h = rowBase + uint32_t(ax.c[xi]);
+                const float wgt = wz * wy * (xi == 0 ? wx0 : (xi == 1 ? wx1 : wx2));
+                const uint32_t h = rowBase + uint32_t(xi == 0 ? cx0 : (xi == 1 ? cx1 : cx2));
                 const uint32_t start = cellStart[h];
                 if (start == 0xffffffffu) continue;
                 const uint32_t end = cellEnd[h];
