@@ -382,8 +382,14 @@ void sortParticles(uint *dGridParticleHash, uint *dGridParticleIndex, uint numPa
     State &s = S();
     TimedScope ts(s.lastHashSrd ? "srd_sort" : "sort", s.stream);
     if (s.lastHashPtr == dGridParticleHash && s.lastHashBound > 0) {
-        NW_HIP(counting_sort_pairs(dGridParticleHash, dGridParticleIndex, numParticles, s.lastHashBound, s.sort,
-                                   s.stream));
+        // calcHash wrote these keys: every one below the cell count
+        const hipError_t e = bucket_sort_pairs(dGridParticleHash, dGridParticleIndex, numParticles, s.lastHashBound,
+                                               s.bucket, s.stream);
+        if (e == hipErrorNotSupported)
+            NW_HIP(counting_sort_pairs(dGridParticleHash, dGridParticleIndex, numParticles, s.lastHashBound, s.sort,
+                                       s.stream));
+        else
+            NW_HIP(e);
     } else {
         NW_HIP(radix_sort_pairs(dGridParticleHash, dGridParticleIndex, numParticles, 32, s.sort, s.stream));
     }

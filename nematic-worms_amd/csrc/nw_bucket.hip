@@ -166,7 +166,8 @@ __device__ __forceinline__ void tiles_staged_body(const float4 *__restrict__ pos
                 h[k] = srd_hash(kp.p, kp.s, mk(pos[i]));
                 hash[i] = h[k];
             } else {
-                h[k] = hash[i] - cellBase;
+                // clamped: a key outside the grid sorts as its last cell
+                h[k] = min(hash[i] - cellBase, bp.cells - 1u);
             }
             rk[k] = atomicAdd(&lh[h[k] >> bp.shift], 1u);
         }
@@ -611,10 +612,12 @@ hipError_t ensure_bucket_scratch(BucketScratch &s, const BucketPlan &bp) {
         const uint64_t c = uint64_t(bp.n) + bp.n / 8 + 1024;
         if (s.hash) (void)hipFree(s.hash);
         if (s.sortedCell) (void)hipFree(s.sortedCell);
-        s.hash = s.sortedCell = nullptr;
+        if (s.aux) (void)hipFree(s.aux);
+        s.hash = s.sortedCell = s.aux = nullptr;
         s.cap_n = 0;
         if ((e = hipMalloc(&s.hash, c * 4)) != hipSuccess) return e;
         if ((e = hipMalloc(&s.sortedCell, c * 4)) != hipSuccess) return e;
+        if ((e = hipMalloc(&s.aux, c * 4)) != hipSuccess) return e;
         s.cap_n = c;
     }
     if ((e = regrow(s.pairs, s.cap_pairs, uint64_t(bp.tiles) * bp.tileItems, 8)) != hipSuccess) return e;
@@ -750,10 +753,12 @@ hipError_t ensure_batch_sort_scratch(BucketScratch &s, const BatchSortPlan &tot)
         const uint64_t c = tot.n + tot.n / 8 + 1024;
         if (s.hash) (void)hipFree(s.hash);
         if (s.sortedCell) (void)hipFree(s.sortedCell);
-        s.hash = s.sortedCell = nullptr;
+        if (s.aux) (void)hipFree(s.aux);
+        s.hash = s.sortedCell = s.aux = nullptr;
         s.cap_n = 0;
         if ((e = hipMalloc(&s.hash, c * 4)) != hipSuccess) return e;
         if ((e = hipMalloc(&s.sortedCell, c * 4)) != hipSuccess) return e;
+        if ((e = hipMalloc(&s.aux, c * 4)) != hipSuccess) return e;
         s.cap_n = c;
     }
     if ((e = regrow(s.pairs, s.cap_pairs, tot.n, 8)) != hipSuccess) return e;
@@ -804,6 +809,34 @@ hipError_t launch_md_cells_batch(const SortRep *reps, uint32_t R, const BatchSor
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_bucket_cells_batch<kMdCellsBlock, int(kBatchBucketCap)>), dim3(tot.buckets),
                        dim3(kMdCellsBlock), L.total, st, o, reps, R);
+    return hipGetLastError();
+}
+
+// keys[k] = cell of slot k, vals[k] = the value of the particle in slot k
+__global__ __launch_bounds__(kBlock) void k_sort_finish(uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                        const uint32_t *__restrict__ pos,
+                                                        const uint32_t *__restrict__ cell,
+                                                        const uint32_t *__restrict__ valsIn, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    keys[k] = cell[k];
+    vals[k] = valsIn[pos[k]];
+}
+
+hipError_t bucket_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t n, uint32_t bound, BucketScratch &s,
+                             hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const BucketPlan bp = make_bucket_plan(n, bound, 0);
+    if (*bucket_plan_problem(bp)) return hipErrorNotSupported;
+    hipError_t e;
+    if ((e = ensure_bucket_scratch(s, bp)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(s.aux, vals, size_t(n) * 4, hipMemcpyDeviceToDevice, st)) != hipSuccess) return e;
+    if ((e = launch_hash_tiles(keys, bp, s, st)) != hipSuccess) return e;
+    // sorted positions into s.hash (free here: the keys come from the caller)
+    if ((e = launch_md_cells(bp, s, s.hash, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, st)) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_sort_finish, dim3(grid_for(n)), dim3(kBlock), 0, st, keys, vals, s.hash, s.sortedCell, s.aux,
+                       n);
     return hipGetLastError();
 }
 

@@ -45,6 +45,7 @@ struct BucketScratch {
     uint32_t *sortedCell = nullptr;  // cell of each sorted slot
     uint32_t *off = nullptr;         // CSR cell offsets (cells + 1)
     float *lam = nullptr;            // isokinetic factor per SRD cell
+    uint32_t *aux = nullptr;         // per particle: the values of a key-value sort
     uint64_t cap_n = 0, cap_pairs = 0, cap_mat = 0, cap_cells = 0;
 };
 
@@ -71,6 +72,13 @@ struct FrontLane {
 hipError_t launch_md_front(const MdFrontArgs &a, const KP &kp, hipStream_t st);
 // level 1 over precomputed cells
 hipError_t launch_hash_tiles(const uint32_t *hash, const BucketPlan &bp, BucketScratch &s, hipStream_t st);
+// stable sort of (keys, vals) by key, every key < bound (the per-call
+// sortParticles after calcHash): the two-level bucket sort of the positions,
+// then keys from the slots' cells and vals gathered by position; a key >=
+// bound (never written by calcHash) sorts as bound - 1.
+// hipErrorNotSupported when the bound does not fit the bucket sort.
+hipError_t bucket_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t n, uint32_t bound, BucketScratch &s,
+                             hipStream_t st);
 // level 2 for MD: ids in stable cell order, CSR offsets, cell of every
 // slot, gathered records (sVel / sTan may be NULL)
 hipError_t launch_md_cells(const BucketPlan &bp, BucketScratch &s, uint32_t *ids, const float4 *pos,
