@@ -291,6 +291,20 @@ void nwBatchRngState(uint replica, uint64_t *offset);
 void nwAdjointForward(float *pos, float *vel, float *force, float *forceOld, float *tangent, uint numFilaments,
                       float deltaTime, float langevinGamma, float langevinKbT, int langevin, uint steps,
                       uint checkpointEvery);
+/* nwAdjointForward with SRD collisions (no counterpart in the reference):
+   pos / vel hold the numFilaments worms' beads followed by numSolvent solvent
+   particles, as nwFusedSrdStep; after steps 0, srdGap, 2 srdGap, ... of the
+   run the fused SRD collision (time step deltaTime * srdGap, the
+   SolventParams of the last setSolventParameters with the grid shift
+   srdShifts[3c .. 3c+2] for collision c, ceil(steps / srdGap) of them)
+   follows the MD step, as ParticleSystem::update orders them
+   (simulation/particleSystem.cpp:249-339).  No Langevin thermostat (the
+   reference applies it only without solvent, particleSystem.cpp:276); no
+   isokinetic rescaling.  nwAdjointBackward then differentiates through the
+   collisions with their cells and rotation signs held fixed. */
+void nwAdjointForwardSrd(float *pos, float *vel, float *force, float *forceOld, float *tangent, uint numFilaments,
+                         uint numSolvent, float deltaTime, uint srdGap, const float *srdShifts, uint steps,
+                         uint checkpointEvery);
 /* The adjoint of the last nwAdjointForward run, backwards from dLdPos /
    dLdVel (adjoints of the final positions / velocities, device float4 per
    bead; dLdVel may be NULL): every segment is recomputed from its checkpoint
@@ -302,6 +316,11 @@ void nwAdjointBackward(const float *dLdPos, const float *dLdVel, double grad[4])
    dSdPos is not NULL, dS/dpos (device float4 per bead, w = 0).
    Synchronises. */
 double nwBondOrderGrad(const float *pos, uint numFilaments, float *dSdPos);
+
+/* Polar speed P = <v . t> of the worms (NWObservables.polarSpeed) and, when
+   dPdPos is not NULL, dP/dpos and (dPdVel not NULL) dP/dvel (device float4
+   per bead, w = 0).  Synchronises. */
+double nwPolarSpeedGrad(const float *pos, const float *vel, uint numFilaments, float *dPdPos, float *dPdVel);
 
 /* Order parameters of a worm configuration (the reference's analysis,
    analysis/analyze_correlations.py:13-38, analysis/afa/utils.py:35-49, on

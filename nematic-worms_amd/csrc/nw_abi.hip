@@ -77,7 +77,21 @@ struct State {
         uint32_t W = 0, N = 0, C = 0, steps = 0, K = 0;
         uint64_t seed = 0, endOffset = 0;
         std::vector<uint64_t> ckptOffset;  // generator offset at every checkpoint
-        float4 *ckpt = nullptr;            // checkpoints x 4 x N: pos, vel, force, forceOld
+        // checkpoints x (2 NT + 2 N): pos, vel (beads then solvent), force, forceOld
+        float4 *ckpt = nullptr;
+        // SRD collisions of the run (nwAdjointForwardSrd): the solvent follows
+        // the beads, a collision after steps 0, gap, 2 gap, ...
+        bool srd = false;
+        uint32_t Ns = 0, NT = 0, Cs = 0, gap = 0;
+        int rotBeads = 0;
+        float cosA = 0.f, sinA = 0.f;
+        std::vector<float> shifts;  // the grid shift of every collision, 3 floats each
+        // a segment's collisions: cell of every particle, cell order, CSR
+        // offsets, (u, sign) per cell; the full state while recomputing
+        uint32_t *collHash = nullptr, *collIds = nullptr, *collOff = nullptr;
+        float4 *collCom = nullptr, *cellSum = nullptr, *wPos = nullptr, *wVel = nullptr;
+        uint64_t capCollHash = 0, capCollIds = 0, capCollOff = 0, capCollCom = 0, capCellSum = 0, capWPos = 0,
+                 capWVel = 0;
         // one segment: states after every step and the cell lists of every step
         float4 *segPos = nullptr, *segVel = nullptr, *segSPos = nullptr, *work = nullptr, *lam = nullptr;
         uint32_t *segIds = nullptr, *segOff = nullptr, *segCell = nullptr;
@@ -601,20 +615,24 @@ void nwFusedMdStep(float *pos, float *vel, float *force, float *forceOld, float 
                    nullptr);
 }
 
-void nwFusedSrdStep(float *pos, float *vel, float *solventCellCOM, uint *gridParticleHash, uint *gridParticleIndex,
-                    float deltaTime, uint numParticles, uint numSolvent, int isokinetic) {
-    NW_INIT();
+// The fused SRD collision with the parameters kp; offOut (may be NULL: the
+// library's scratch) receives the CSR offsets of the cell order.
+static bool fused_srd(const KP &kp, float *pos, float *vel, float *solventCellCOM, uint *gridParticleHash,
+                      uint *gridParticleIndex, uint32_t *offOut, float deltaTime, uint numParticles, uint numSolvent,
+                      int isokinetic) {
     State &s = S();
-    const KP kp = make_kp();
     const uint64_t NT = uint64_t(numParticles) + numSolvent;
-    NW_REQUIRE(NT < (1ull << 31), "nwFusedSrdStep: too many particles");
+    NW_REQUIRE_R(NT < (1ull << 31), false, "nwFusedSrdStep: too many particles");
     const uint64_t cells = uint64_t(kp.s.gridSize.x) * kp.s.gridSize.y * kp.s.gridSize.z;
-    NW_REQUIRE(cells > 0 && cells == kp.s.numCells, "nwFusedSrdStep: SRD grid does not match numCells");
-    NW_REQUIRE(gridParticleIndex != nullptr && solventCellCOM != nullptr, "nwFusedSrdStep: NULL output buffer");
-    if (NT == 0) return;
+    NW_REQUIRE_R(cells > 0 && cells == kp.s.numCells, false, "nwFusedSrdStep: SRD grid does not match numCells");
+    NW_REQUIRE_R(gridParticleIndex != nullptr && solventCellCOM != nullptr, false,
+                 "nwFusedSrdStep: NULL output buffer");
+    if (NT == 0) return true;
     const BucketPlan bp = make_bucket_plan(NT, cells, 0);
-    NW_REQUIRE(!*bucket_plan_problem(bp), "nwFusedSrdStep: %s", bucket_plan_problem(bp));
-    NW_HIP(ensure_bucket_scratch(s.bucket, bp));
+    NW_REQUIRE_R(!*bucket_plan_problem(bp), false, "nwFusedSrdStep: %s", bucket_plan_problem(bp));
+    NW_HIP_R(ensure_bucket_scratch(s.bucket, bp), false);
+    BucketScratch sc = s.bucket;
+    if (offOut) sc.off = offOut;
     const uint64_t off = s.offset;
     s.offset += blocks_for(cells);
     const double alpha = double(kp.s.alpha);
@@ -625,18 +643,27 @@ void nwFusedSrdStep(float *pos, float *vel, float *solventCellCOM, uint *gridPar
     const int iso = isokinetic ? 1 : 0;
     {
         TimedScope ts("srdf_tiles", s.stream, true);
-        NW_HIP(launch_srd_tiles(P, hash, bp, s.bucket, kp, s.stream));
+        NW_HIP_R(launch_srd_tiles(P, hash, bp, s.bucket, kp, s.stream), false);
     }
     {
         TimedScope ts("srdf_cells", s.stream, true);
-        NW_HIP(launch_srd_cells(bp, s.bucket, gridParticleIndex, V, com, iso, s.seed, off, kp, s.stream));
+        NW_HIP_R(launch_srd_cells(bp, sc, gridParticleIndex, V, com, iso, s.seed, off, kp, s.stream), false);
     }
     {
         TimedScope ts("srdf_rotate", s.stream, true);
-        NW_HIP(launch_srd_rotate(hash, com, s.bucket.lam, P, V, uint32_t(NT), numParticles, deltaTime, cosA, sinA,
-                                 iso, kp, s.stream));
+        NW_HIP_R(launch_srd_rotate(hash, com, s.bucket.lam, P, V, uint32_t(NT), numParticles, deltaTime, cosA, sinA,
+                                   iso, kp, s.stream),
+                 false);
     }
     s.lastHashPtr = nullptr;
+    return true;
+}
+
+void nwFusedSrdStep(float *pos, float *vel, float *solventCellCOM, uint *gridParticleHash, uint *gridParticleIndex,
+                    float deltaTime, uint numParticles, uint numSolvent, int isokinetic) {
+    NW_INIT();
+    (void)fused_srd(make_kp(), pos, vel, solventCellCOM, gridParticleHash, gridParticleIndex, nullptr, deltaTime,
+                    numParticles, numSolvent, isokinetic);
 }
 
 // ---------------------------------------------------------------------------
@@ -947,13 +974,24 @@ static bool copy_d2d(void *dst, const void *src, size_t bytes, hipStream_t st) {
     return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st) == hipSuccess;
 }
 
-void nwAdjointForward(float *pos, float *vel, float *force, float *forceOld, float *tangent, uint numFilaments,
-                      float deltaTime, float langevinGamma, float langevinKbT, int langevin, uint steps,
-                      uint checkpointEvery) {
-    NW_INIT();
+static uint32_t collisions_in(uint32_t steps, uint32_t gap) { return gap ? (steps + gap - 1) / gap : 0; }
+
+// the SolventParams of collision c of the run
+static KP collision_kp(const State::Adjoint &A, uint32_t c) {
+    KP kp = A.kp;
+    kp.s.randOffset.x = A.shifts[3 * size_t(c)];
+    kp.s.randOffset.y = A.shifts[3 * size_t(c) + 1];
+    kp.s.randOffset.z = A.shifts[3 * size_t(c) + 2];
+    return kp;
+}
+
+// The forward run of nwAdjointForward / nwAdjointForwardSrd (srd: numSolvent,
+// gap and shifts set in A by the caller, A.srd true).
+static void adjoint_forward(float *pos, float *vel, float *force, float *forceOld, float *tangent,
+                            uint numFilaments, float deltaTime, float langevinGamma, float langevinKbT, int langevin,
+                            uint steps, uint checkpointEvery) {
     State &s = S();
     State::Adjoint &A = s.adj;
-    A.ready = false;
     const KP kp = make_kp();
     const SimParams &p = kp.p;
     NW_REQUIRE(kp.e.pairPotential == NW_PAIR_WCA && kp.e.bondedMode == NW_BONDED_CHAIN,
@@ -971,6 +1009,18 @@ void nwAdjointForward(float *pos, float *vel, float *force, float *forceOld, flo
     NW_REQUIRE(n >= 1 && n <= 64, "nwAdjointForward: worms of 1..64 beads");
     const uint64_t N = uint64_t(numFilaments) * n;
     const uint64_t C = uint64_t(p.gridSize.x) * p.gridSize.y * p.gridSize.z;
+    if (A.srd) {
+        NW_REQUIRE(!langevin, "nwAdjointForwardSrd: no Langevin thermostat with the solvent");
+        NW_REQUIRE(kp.s.gridSize.x * kp.s.gridSize.y * kp.s.gridSize.z == kp.s.numCells && kp.s.numCells > 0,
+                   "nwAdjointForwardSrd: SRD grid does not match numCells");
+        NW_REQUIRE(N + A.Ns < (1ull << 31), "nwAdjointForwardSrd: too many particles");
+        const double alpha = double(kp.s.alpha);
+        A.cosA = float(std::cos(alpha));
+        A.sinA = float(std::sin(alpha));
+        A.rotBeads = kp.e.srdCollideFilaments != 0 ? 1 : 0;
+        A.Cs = kp.s.numCells;
+    }
+    A.NT = uint32_t(N + (A.srd ? A.Ns : 0));
     A.kp = kp;
     A.dt = deltaTime;
     A.gamma = langevinGamma;
@@ -984,18 +1034,22 @@ void nwAdjointForward(float *pos, float *vel, float *force, float *forceOld, flo
     A.K = checkpointEvery;
     A.seed = s.seed;
     const uint32_t nck = (steps + checkpointEvery - 1) / checkpointEvery;
-    NW_REQUIRE(grow(A.ckpt, A.capCkpt, uint64_t(nck) * 4 * N) && grow(A.segSPos, A.capSPos, N) &&
-                   grow(A.segIds, A.capIds, N),
+    const uint64_t NT = A.NT, stride = 2 * NT + 2 * N;
+    NW_REQUIRE(grow(A.ckpt, A.capCkpt, uint64_t(nck) * stride) && grow(A.segSPos, A.capSPos, N) &&
+                   grow(A.segIds, A.capIds, NT),
                "nwAdjointForward: out of device memory");
+    if (A.srd)
+        NW_REQUIRE(grow(A.collCom, A.capCollCom, A.Cs) && grow(A.collHash, A.capCollHash, NT),
+                   "nwAdjointForwardSrd: out of device memory");
     A.ckptOffset.assign(nck, 0);
-    const size_t bytes = N * sizeof(float4);
+    const size_t bytesT = NT * sizeof(float4), bytes = N * sizeof(float4);
     for (uint32_t k = 0; k < steps; ++k) {
         if (k % checkpointEvery == 0) {
             const uint32_t c = k / checkpointEvery;
-            float4 *ck = A.ckpt + uint64_t(c) * 4 * N;
-            NW_REQUIRE(copy_d2d(ck, pos, bytes, s.stream) && copy_d2d(ck + N, vel, bytes, s.stream) &&
-                           copy_d2d(ck + 2 * N, force, bytes, s.stream) &&
-                           copy_d2d(ck + 3 * N, forceOld, bytes, s.stream),
+            float4 *ck = A.ckpt + uint64_t(c) * stride;
+            NW_REQUIRE(copy_d2d(ck, pos, bytesT, s.stream) && copy_d2d(ck + NT, vel, bytesT, s.stream) &&
+                           copy_d2d(ck + 2 * NT, force, bytes, s.stream) &&
+                           copy_d2d(ck + 2 * NT + N, forceOld, bytes, s.stream),
                        "nwAdjointForward: checkpoint copy failed");
             A.ckptOffset[c] = s.offset;
         }
@@ -1004,17 +1058,57 @@ void nwAdjointForward(float *pos, float *vel, float *force, float *forceOld, flo
                       nullptr, nullptr, deltaTime, langevinGamma, langevinKbT, langevin, numFilaments, nullptr,
                       nullptr))
             return;
+        if (A.srd && k % A.gap == 0 &&
+            !fused_srd(collision_kp(A, k / A.gap), pos, vel, reinterpret_cast<float *>(A.collCom), A.collHash,
+                       A.segIds, nullptr, deltaTime * float(A.gap), uint32_t(N), A.Ns, 0))
+            return;
     }
     A.endOffset = s.offset;
     A.ready = true;
 }
 
+void nwAdjointForward(float *pos, float *vel, float *force, float *forceOld, float *tangent, uint numFilaments,
+                      float deltaTime, float langevinGamma, float langevinKbT, int langevin, uint steps,
+                      uint checkpointEvery) {
+    NW_INIT();
+    State::Adjoint &A = S().adj;
+    A.ready = false;
+    A.srd = false;
+    A.Ns = A.gap = 0;
+    A.shifts.clear();
+    adjoint_forward(pos, vel, force, forceOld, tangent, numFilaments, deltaTime, langevinGamma, langevinKbT, langevin,
+                    steps, checkpointEvery);
+}
+
+void nwAdjointForwardSrd(float *pos, float *vel, float *force, float *forceOld, float *tangent, uint numFilaments,
+                         uint numSolvent, float deltaTime, uint srdGap, const float *srdShifts, uint steps,
+                         uint checkpointEvery) {
+    NW_INIT();
+    State::Adjoint &A = S().adj;
+    A.ready = false;
+    NW_REQUIRE(srdGap >= 1 && srdShifts != nullptr, "nwAdjointForwardSrd: srdGap >= 1 and the shifts are required");
+    A.srd = true;
+    A.Ns = numSolvent;
+    A.gap = srdGap;
+    A.shifts.assign(srdShifts, srdShifts + 3 * size_t(collisions_in(steps, srdGap)));
+    adjoint_forward(pos, vel, force, forceOld, tangent, numFilaments, deltaTime, S().params.gamma, 0.0f, 0, steps,
+                    checkpointEvery);
+}
+
 static bool ensure_segment(State::Adjoint &A) {
-    const uint64_t N = A.N, K = A.K;
-    return grow(A.segPos, A.capPos, (K + 1) * N) && grow(A.segVel, A.capVel, (K + 1) * N) &&
-           grow(A.segSPos, A.capSPos, K * N) && grow(A.segIds, A.capIds, K * N) &&
-           grow(A.segOff, A.capOff, K * (uint64_t(A.C) + 2)) && grow(A.segCell, A.capCell, K * N) &&
-           grow(A.work, A.capWork, 3 * N) && grow(A.lam, A.capLam, 5 * N);
+    const uint64_t N = A.N, K = A.K, NT = A.NT;
+    if (!(grow(A.segPos, A.capPos, (K + 1) * N) && grow(A.segVel, A.capVel, (K + 1) * N) &&
+          grow(A.segSPos, A.capSPos, K * N) && grow(A.segIds, A.capIds, std::max(K * N, NT)) &&
+          grow(A.segOff, A.capOff, K * (uint64_t(A.C) + 2)) && grow(A.segCell, A.capCell, K * N) &&
+          grow(A.work, A.capWork, 3 * N) && grow(A.lam, A.capLam, 4 * N + NT)))
+        return false;
+    if (!A.srd) return true;
+    // at most this many collisions in a segment of K steps
+    const uint64_t cpS = (K + A.gap - 1) / A.gap + 1;
+    return grow(A.collHash, A.capCollHash, cpS * NT) && grow(A.collIds, A.capCollIds, cpS * NT) &&
+           grow(A.collOff, A.capCollOff, cpS * (uint64_t(A.Cs) + 2)) &&
+           grow(A.collCom, A.capCollCom, cpS * A.Cs) && grow(A.cellSum, A.capCellSum, A.Cs) &&
+           grow(A.wPos, A.capWPos, NT) && grow(A.wVel, A.capWVel, NT);
 }
 
 void nwAdjointBackward(const float *dLdPos, const float *dLdVel, double grad[4]) {
@@ -1029,30 +1123,63 @@ void nwAdjointBackward(const float *dLdPos, const float *dLdVel, double grad[4])
     NW_REQUIRE(grow(A.partial, A.capPartial, K * rowsPerStep * kAdjParams), "nwAdjointBackward: out of memory");
     if (!A.grad) NW_HIP(hipMalloc(&A.grad, kAdjParams * sizeof(double)));
     NW_HIP(hipMemsetAsync(A.grad, 0, kAdjParams * sizeof(double), s.stream));
+    const uint64_t NT = A.NT, stride = 2 * NT + 2 * N;
     const size_t bytes = N * sizeof(float4);
-    float4 *lr = A.lam, *lv = A.lam + N, *lF[2] = {A.lam + 2 * N, A.lam + 3 * N}, *lG = A.lam + 4 * N;
+    // lv covers every particle (the solvent's adjoints follow the beads')
+    float4 *lr = A.lam, *lF[2] = {A.lam + N, A.lam + 2 * N}, *lG = A.lam + 3 * N, *lv = A.lam + 4 * N;
     NW_HIP(hipMemcpyAsync(lr, dLdPos, bytes, hipMemcpyDeviceToDevice, s.stream));
+    NW_HIP(hipMemsetAsync(lv, 0, NT * sizeof(float4), s.stream));
     if (dLdVel) NW_HIP(hipMemcpyAsync(lv, dLdVel, bytes, hipMemcpyDeviceToDevice, s.stream));
-    else NW_HIP(hipMemsetAsync(lv, 0, bytes, s.stream));
     NW_HIP(hipMemsetAsync(lF[0], 0, bytes, s.stream));
     NW_HIP(hipMemsetAsync(lG, 0, bytes, s.stream));
     int cur = 0;
     float4 *F = A.work, *G = A.work + N, *T = A.work + 2 * N;
     const uint32_t nck = (A.steps + A.K - 1) / A.K;
     std::vector<uint64_t> lgvOff(K);
+    std::vector<int> collSlot(K);  // the collision store slot of each step of the segment, -1: none
     for (int c = int(nck) - 1; c >= 0; --c) {
         const uint32_t s0 = uint32_t(c) * A.K, len = std::min<uint32_t>(A.K, A.steps - s0);
-        const float4 *ck = A.ckpt + uint64_t(c) * 4 * N;
+        const float4 *ck = A.ckpt + uint64_t(c) * stride;
         NW_HIP(hipMemcpyAsync(A.segPos, ck, bytes, hipMemcpyDeviceToDevice, s.stream));
-        NW_HIP(hipMemcpyAsync(A.segVel, ck + N, bytes, hipMemcpyDeviceToDevice, s.stream));
-        NW_HIP(hipMemcpyAsync(F, ck + 2 * N, bytes, hipMemcpyDeviceToDevice, s.stream));
-        NW_HIP(hipMemcpyAsync(G, ck + 3 * N, bytes, hipMemcpyDeviceToDevice, s.stream));
+        NW_HIP(hipMemcpyAsync(A.segVel, ck + NT, bytes, hipMemcpyDeviceToDevice, s.stream));
+        NW_HIP(hipMemcpyAsync(F, ck + 2 * NT, bytes, hipMemcpyDeviceToDevice, s.stream));
+        NW_HIP(hipMemcpyAsync(G, ck + 2 * NT + N, bytes, hipMemcpyDeviceToDevice, s.stream));
+        if (A.srd) {
+            NW_HIP(hipMemcpyAsync(A.wPos, ck, NT * sizeof(float4), hipMemcpyDeviceToDevice, s.stream));
+            NW_HIP(hipMemcpyAsync(A.wVel, ck + NT, NT * sizeof(float4), hipMemcpyDeviceToDevice, s.stream));
+        }
         // recompute the segment, keeping every step's state and cell lists
         s.offset = A.ckptOffset[c];
+        int slots = 0;
         for (uint32_t j = 0; j < len; ++j) {
             // step j reads slot j and writes slot j + 1
             float4 *pj = A.segPos + uint64_t(j + 1) * N, *vj = A.segVel + uint64_t(j + 1) * N;
             lgvOff[j] = s.offset;
+            collSlot[j] = -1;
+            if (A.srd) {
+                // the full state steps in place; the beads' copies fill the slots
+                if (!fused_md(A.kp, reinterpret_cast<float *>(A.wPos), reinterpret_cast<float *>(A.wVel),
+                              reinterpret_cast<float *>(F), reinterpret_cast<float *>(G),
+                              reinterpret_cast<float *>(T), nullptr, A.segIds + uint64_t(j) * N,
+                              reinterpret_cast<float *>(A.segSPos + uint64_t(j) * N), nullptr, nullptr, A.dt, A.gamma,
+                              A.kT, A.langevin, A.W, A.segOff + uint64_t(j) * (A.C + 2),
+                              A.segCell + uint64_t(j) * N, nullptr, nullptr))
+                    return;
+                NW_HIP(hipMemcpyAsync(pj, A.wPos, bytes, hipMemcpyDeviceToDevice, s.stream));
+                NW_HIP(hipMemcpyAsync(vj, A.wVel, bytes, hipMemcpyDeviceToDevice, s.stream));
+                const uint32_t k = s0 + j;
+                if (k % A.gap == 0) {
+                    const int q = slots++;
+                    collSlot[j] = q;
+                    if (!fused_srd(collision_kp(A, k / A.gap), reinterpret_cast<float *>(A.wPos),
+                                   reinterpret_cast<float *>(A.wVel),
+                                   reinterpret_cast<float *>(A.collCom + uint64_t(q) * A.Cs),
+                                   A.collHash + uint64_t(q) * NT, A.collIds + uint64_t(q) * NT,
+                                   A.collOff + uint64_t(q) * (A.Cs + 2), A.dt * float(A.gap), A.N, A.Ns, 0))
+                        return;
+                }
+                continue;
+            }
             if (!fused_md(A.kp, reinterpret_cast<float *>(pj), reinterpret_cast<float *>(vj),
                           reinterpret_cast<float *>(F), reinterpret_cast<float *>(G), reinterpret_cast<float *>(T),
                           nullptr, A.segIds + uint64_t(j) * N, reinterpret_cast<float *>(A.segSPos + uint64_t(j) * N),
@@ -1064,6 +1191,13 @@ void nwAdjointBackward(const float *dLdPos, const float *dLdVel, double grad[4])
         }
         // the adjoint through the segment, last step first
         for (int j = int(len) - 1; j >= 0; --j) {
+            if (collSlot[j] >= 0) {
+                // the step's collision came after its MD part: undo it first
+                const uint64_t q = uint64_t(collSlot[j]);
+                NW_HIP(launch_srd_adj(A.collIds + q * NT, A.collOff + q * (A.Cs + 2), A.collHash + q * NT,
+                                      A.collCom + q * A.Cs, A.wVel, lv, A.cellSum, A.N, uint32_t(NT), A.Cs, A.cosA,
+                                      A.sinA, A.rotBeads, s.stream));
+            }
             double *part = A.partial + uint64_t(j) * rowsPerStep * kAdjParams;
             AdjStepArgs as;
             as.pos = A.segPos + uint64_t(j + 1) * N;
@@ -1142,6 +1276,22 @@ double nwBondOrderGrad(const float *pos, uint numFilaments, float *dSdPos) {
         NW_HIP_R(hipStreamSynchronize(s.stream), S);
     }
     return S;
+}
+
+double nwPolarSpeedGrad(const float *pos, const float *vel, uint numFilaments, float *dPdPos, float *dPdVel) {
+    if (!ensure_init()) return 0.0;
+    State &s = S();
+    NW_REQUIRE_R(pos != nullptr && vel != nullptr, 0.0, "nwPolarSpeedGrad: NULL positions or velocities");
+    NWObservables o;
+    nwObservables(pos, vel, numFilaments, &o);
+    if (o.beads <= 0) return 0.0;
+    if (dPdPos)
+        NW_HIP_R(launch_polar_speed_grad(reinterpret_cast<const float4 *>(pos), reinterpret_cast<const float4 *>(vel),
+                                         numFilaments, 1.0 / o.beads, reinterpret_cast<float4 *>(dPdPos),
+                                         reinterpret_cast<float4 *>(dPdVel), make_kp(), s.stream),
+                 o.polarSpeed);
+    NW_HIP_R(hipStreamSynchronize(s.stream), o.polarSpeed);
+    return o.polarSpeed;
 }
 
 int nwFusedPlanFits(uint numParticles, uint numCells, uint filamentSize) {

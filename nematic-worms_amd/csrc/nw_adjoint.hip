@@ -220,7 +220,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_step(AdjStepArgs a, KP kp) {
 
         // adjoint of the step for this bead
         const f3 Rt = mk(a.lr[m.bead]) + R + Rp;
-        const f3 V = mk(a.lv[m.bead]) - p.gamma * lam + a.dt * Rt;
+        // the drag -gamma v' is part of F' only with the Langevin thermostat on
+        const f3 V = mk(a.lv[m.bead]) - (a.langevin ? a.gamma : 0.0f) * lam + a.dt * Rt;
         const f3 lG = mk(a.lG[m.bead]);
         a.lr[m.bead] = f4(Rt, 0.0f);
         a.lv[m.bead] = f4(V, 0.0f);
@@ -258,6 +259,100 @@ __global__ __launch_bounds__(kBlock) void k_bond_order_grad(const float4 *__rest
     if (m.pp + 1u < n) o -= g;
     if (m.pp >= 1u) o += gPrev;
     out[m.bead] = f4(o, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// The adjoint of an SRD collision with its cells and rotation signs held
+// fixed.  The collision is linear in the velocities: for particle i of cell c
+// (mass m_i, cell mass M_c, rotation R_c by s_c alpha in the xy plane)
+//   u_c = sum_j m_j v_j / M_c,  v_i' = u_c + R_c (v_i - u_c)  (rotated)
+//                               v_i' = v_i                    (not rotated:
+//                               beads when they only join the momentum sum)
+// so with L_c the sum of the adjoints of the rotated members
+//   lv_i <- (R_c^T or I) lv_i' + (m_i / M_c) (I - R_c^T) L_c.
+// ---------------------------------------------------------------------------
+// per cell, members in the forward step's cell order: (L_c, M_c)
+__global__ __launch_bounds__(kBlock) void k_srd_adj_cells(const uint32_t *__restrict__ ids,
+                                                          const uint32_t *__restrict__ off,
+                                                          const float4 *__restrict__ lv,
+                                                          const float4 *__restrict__ mass,
+                                                          float4 *__restrict__ cellSum, uint32_t N, uint32_t cells,
+                                                          int rotBeads) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    const uint32_t lo = off[c], hi = off[c + 1];
+    if (lo == hi) return;  // no particle reads an empty cell
+    f3 L = mk(0.f, 0.f, 0.f);
+    float M = 0.0f;
+    for (uint32_t k = lo; k < hi; ++k) {
+        const uint32_t i = ids[k];
+        M += mass[i].w;
+        if (rotBeads || i >= N) L += mk(lv[i]);
+    }
+    cellSum[c] = f4(L, M);
+}
+
+// R^T x for the rotation by the angle with cosine cs and sine sn (xy plane)
+__device__ __forceinline__ f3 rot_t(f3 x, float cs, float sn) {
+    return mk(x.x * cs + x.y * sn, -x.x * sn + x.y * cs, x.z);
+}
+
+// per particle, in place
+__global__ __launch_bounds__(kBlock) void k_srd_adj_apply(const uint32_t *__restrict__ hash,
+                                                          const float4 *__restrict__ com,
+                                                          const float4 *__restrict__ cellSum,
+                                                          const float4 *__restrict__ mass, float4 *__restrict__ lv,
+                                                          uint32_t N, uint32_t NT, float cosA, float sinA,
+                                                          int rotBeads) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= NT) return;
+    const uint32_t c = hash[j];
+    const float4 S = cellSum[c];
+    const float sn = com[c].w * sinA;  // the cell's rotation sign times sin(alpha)
+    const f3 l = mk(lv[j]);
+    f3 out = (rotBeads || j >= N) ? rot_t(l, cosA, sn) : l;
+    const f3 Lc = mk(S.x, S.y, S.z);
+    out += (mass[j].w / S.w) * (Lc - rot_t(Lc, cosA, sn));
+    lv[j] = f4(out, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// d P / d (r, v) of the polar speed P = <v . t> over the beads of worms of
+// >= 2 beads (t = unit(r[p+1] - r[p-1]), one-sided at the worm ends, minimum
+// image): dP/dv = t / Nb, dP/dd = (v - t (t . v)) / (|d| Nb) through
+// d = b - a (a = r[p-1] or r[p], b = r[p+1] or r[p])
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_polar_speed_grad(const float4 *__restrict__ pos,
+                                                             const float4 *__restrict__ vel, uint32_t W,
+                                                             float scale, float4 *__restrict__ dPos,
+                                                             float4 *__restrict__ dVel, KP kp) {
+    const SimParams &p = kp.p;
+    const uint32_t n = p.filamentSize;
+    const WaveWorm m = wave_worm(n, W);
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    const f3 r = mk(m.active ? pos[m.bead] : zero4);
+    const f3 v = mk(m.active ? vel[m.bead] : zero4);
+    const f3 rn = shfl_down3(r, 1), rp = shfl_up3(r, 1);
+    f3 g = mk(0.f, 0.f, 0.f), t = g;
+    if (m.active && n >= 2u) {
+        const f3 a = m.pp == 0u ? r : rp;
+        const f3 b = m.pp + 1u == n ? r : rn;
+        const f3 d = min_image(p, a, b);
+        const float len = sqrtf(dot(d, d));
+        t = d / len;
+        g = (scale / len) * (v - t * dot(t, v));
+    }
+    const f3 gPrev = shfl_up3(g, 1), gNext = shfl_down3(g, 1);
+    if (!m.active) return;
+    f3 o = mk(0.f, 0.f, 0.f);
+    if (n >= 2u) {
+        if (m.pp >= 1u) o += gPrev;       // b of bead pp - 1
+        if (m.pp + 1u < n) o -= gNext;    // a of bead pp + 1
+        if (m.pp == 0u) o -= g;           // a of the first bead
+        if (m.pp + 1u == n) o += g;       // b of the last bead
+    }
+    dPos[m.bead] = f4(o, 0.0f);
+    if (dVel) dVel[m.bead] = f4(scale * t, 0.0f);
 }
 
 // out[k] += sum over rows of partial[row][k]: thread t adds the entries
@@ -306,6 +401,26 @@ hipError_t launch_bond_order_grad(const float4 *pos, uint32_t W, const double e[
     if (!blocks) return hipSuccess;
     hipLaunchKernelGGL(k_bond_order_grad, dim3(blocks), dim3(kBlock), 0, st, pos, W, float(e[0]), float(e[1]),
                        float(e[2]), float(scale), out, kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_srd_adj(const uint32_t *ids, const uint32_t *off, const uint32_t *hash, const float4 *com,
+                          const float4 *mass, float4 *lv, float4 *cellSum, uint32_t N, uint32_t NT, uint32_t cells,
+                          float cosA, float sinA, int rotBeads, hipStream_t st) {
+    if (NT == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_srd_adj_cells, dim3(grid_for(cells)), dim3(kBlock), 0, st, ids, off, lv, mass, cellSum, N,
+                       cells, rotBeads);
+    hipLaunchKernelGGL(k_srd_adj_apply, dim3(grid_for(NT)), dim3(kBlock), 0, st, hash, com, cellSum, mass, lv, N, NT,
+                       cosA, sinA, rotBeads);
+    return hipGetLastError();
+}
+
+hipError_t launch_polar_speed_grad(const float4 *pos, const float4 *vel, uint32_t W, double scale, float4 *dPos,
+                                   float4 *dVel, const KP &kp, hipStream_t st) {
+    const uint32_t blocks = adj_step_blocks(W, kp.p.filamentSize);
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(k_polar_speed_grad, dim3(blocks), dim3(kBlock), 0, st, pos, vel, W, float(scale), dPos, dVel,
+                       kp);
     return hipGetLastError();
 }
 

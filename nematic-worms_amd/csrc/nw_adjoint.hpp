@@ -28,6 +28,16 @@ hipError_t launch_adj_step(const AdjStepArgs &a, const KP &kp, hipStream_t st);
 // largest eigenvalue of <1.5 b b - 0.5 I>, scale = 3 / bonds
 hipError_t launch_bond_order_grad(const float4 *pos, uint32_t W, const double e[3], double scale, float4 *out,
                                   const KP &kp, hipStream_t st);
+// adjoint of one SRD collision (cells, signs fixed) on the velocity adjoints
+// lv of all NT particles, beads first: ids / off the collision's cell order
+// and CSR offsets, hash the cell of every particle, com its (u, sign) per
+// cell, mass.w the particle masses; cellSum: scratch of `cells` float4
+hipError_t launch_srd_adj(const uint32_t *ids, const uint32_t *off, const uint32_t *hash, const float4 *com,
+                          const float4 *mass, float4 *lv, float4 *cellSum, uint32_t N, uint32_t NT, uint32_t cells,
+                          float cosA, float sinA, int rotBeads, hipStream_t st);
+// scale * d(sum of v . t)/d(r, v) of the polar speed (dVel may be NULL)
+hipError_t launch_polar_speed_grad(const float4 *pos, const float4 *vel, uint32_t W, double scale, float4 *dPos,
+                                   float4 *dVel, const KP &kp, hipStream_t st);
 // out[k] += the column sums of partial (rows x kAdjParams), in a fixed order
 hipError_t launch_adj_total(const double *partial, uint64_t rows, double *out, hipStream_t st);
 

@@ -142,7 +142,11 @@ PROTOTYPES = {
     "nwBatchRngState": (None, [c_uint, ctypes.POINTER(ctypes.c_uint64)]),
     "nwAdjointForward": (None, [ctypes.c_void_p] * 5 + [c_uint, c_float, c_float, c_float, c_int, c_uint, c_uint]),
     "nwAdjointBackward": (None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "nwAdjointForwardSrd": (None, [ctypes.c_void_p] * 5 + [c_uint, c_uint, c_float, c_uint,
+                                                         ctypes.POINTER(c_float), c_uint, c_uint]),
     "nwBondOrderGrad": (ctypes.c_double, [ctypes.c_void_p, c_uint, ctypes.c_void_p]),
+    "nwPolarSpeedGrad": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_void_p, c_uint, ctypes.c_void_p,
+                                           ctypes.c_void_p]),
     "nwSymEig3Max": (ctypes.c_double, [ctypes.POINTER(ctypes.c_double)]),
     "nwGetDeviceFlags": (c_uint, [c_int]),
     "nwTimerEnable": (None, [ctypes.c_char_p, c_int]),

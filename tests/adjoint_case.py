@@ -37,3 +37,47 @@ def initial_state(seed=5):
     w = rng.normal(0, 1, (N, 4))
     w[:, 3] = 0
     return pos, vel, np.zeros((N, 4), np.float32), w.astype(np.float32)
+
+
+# ---- with SRD solvent: the same worms in a 12 x 12 SRD grid of unit cells,
+# 3 solvent particles per cell, a collision after steps 0, 5, 10, ...
+SRD_GAP = 5
+SRD_PER_CELL = 3
+
+
+def srd_system(theta=THETA):
+    from nwtest import solvent_params
+    p, e = system(theta)
+    e.srdCollideFilaments = 1
+    s = solvent_params(p, (12, 12, 1), (1.0, 1.0, p.boxSize.z), alpha=np.pi / 2, kbT=KT,
+                       ns=12 * 12 * SRD_PER_CELL)
+    return p, e, s
+
+
+def srd_initial_state(seed=6):
+    """beads then solvent (pos, vel: w = radius, mass 1), forces of the beads,
+    loss weights of the beads' positions and velocities"""
+    pos, vel, zero, w = initial_state()
+    p, _, s = srd_system()
+    rng = np.random.default_rng(seed)
+    Ns = s.numParticles
+    sp = np.zeros((Ns, 4), np.float32)
+    sp[:, :2] = rng.random((Ns, 2)) * p.boxSize.x + p.origin.x
+    sp[:, 2] = pos[0, 2]
+    sp[:, 3] = 0.5
+    sv = np.zeros((Ns, 4), np.float32)
+    sv[:, :2] = rng.normal(0, np.sqrt(KT), (Ns, 2))
+    sv[:, 3] = 1.0
+    wv = rng.normal(0, 1, w.shape)
+    wv[:, 3] = 0
+    return np.concatenate([pos, sp]), np.concatenate([vel, sv]), zero, w, wv.astype(np.float32)
+
+
+def srd_shifts(steps):
+    """the grid shift of every collision of a run (ParticleSystem's centred
+    shift stream)"""
+    import orc
+    p, _, s = srd_system()
+    cell = (s.cellSize.x, s.cellSize.y, s.cellSize.z)
+    return np.array([orc.srd_shift(SEED, c, True, cell) for c in range((steps + SRD_GAP - 1) // SRD_GAP)],
+                    np.float32)
