@@ -275,6 +275,22 @@ void nwBatchMdStep(float *pos, float *vel, float *force, float *forceOld, float 
 /* Generator offset of one replica after the batched steps so far. */
 void nwBatchRngState(uint replica, uint64_t *offset);
 
+/* SRD solvent of the batch (no counterpart in the reference): replica r's
+   numSolvent solvent particles follow all beads of the batch, in replica
+   order, and its SRD cells follow the previous replicas' cells (offsets:
+   nwBatchSolventLayout).  solvent.randOffset is not read (each collision
+   brings its grid shifts).  One rotation angle and one srdCollideFilaments
+   over the batch.  Replaced by the next nwBatchSetReplicas. */
+typedef struct NWReplicaSolvent {
+    SolventParams solvent;  /* the replica's SRD grid, cell size, alpha */
+    uint numSolvent;
+    uint reserved[3];
+} NWReplicaSolvent;
+void nwBatchSetSolvent(const NWReplicaSolvent *solvent, uint numReplicas);
+/* numReplicas + 1 entries each: first solvent particle (counted from the
+   first bead) and first SRD cell of every replica. */
+void nwBatchSolventLayout(uint *solventOffset, uint *srdCellOffset);
+
 /* ------------------------------------------------------------------------
    Differentiation through the fused MD step (no counterpart in the
    reference): reverse-mode gradients of a loss of the final configuration
@@ -312,6 +328,29 @@ void nwAdjointForwardSrd(float *pos, float *vel, float *force, float *forceOld, 
    d loss / d theta.  The generator is left where the forward run left it.
    Synchronises. */
 void nwAdjointBackward(const float *dLdPos, const float *dLdVel, double grad[4]);
+/* Differentiation through batched steps (no counterpart in the reference):
+   `steps` batched MD steps of the current batch (WCA chain worms, no
+   extensile term, no reversal, periodic boxes) over pos / vel (all beads,
+   then, with srdGap > 0, the solvent of nwBatchSetSolvent), force,
+   forceOld, tangent (beads); with srdGap > 0, after steps 0, srdGap, ... the
+   SRD collision of every replica (time step deltaTime * srdGap, grid shift
+   srdShifts[(c * numReplicas + r) * 3 ..] for collision c of replica r, its
+   signs drawn from the replica's stream at offset + c * ceil(cells / 4)):
+   replica r's trajectory is the one nwAdjointForward / nwAdjointForwardSrd
+   gives it alone from the same seed and offset, bit for bit.  Checkpoints
+   every checkpointEvery steps.  With the solvent: no Langevin thermostat. */
+void nwBatchAdjointForward(float *pos, float *vel, float *force, float *forceOld, float *tangent, float deltaTime,
+                           uint srdGap, const float *srdShifts, uint steps, uint checkpointEvery);
+/* The adjoint of the last nwBatchAdjointForward run from dLdPos / dLdVel
+   (beads, device float4; dLdVel may be NULL; the solvent's adjoints start
+   at zero): grad[4 r + k] = d loss / d theta_k of replica r's own
+   (driveForce, bondBendingK, wcaEpsilon, gamma).  Collisions are held fixed
+   (cells and signs).  Synchronises. */
+void nwBatchAdjointBackward(const float *dLdPos, const float *dLdVel, double *grad);
+/* Polar speed P[r] of every replica and, when dPdPos is not NULL, dP_r /
+   dpos (and dPdVel: dP_r / dvel) of its beads.  Synchronises. */
+void nwBatchPolarSpeedGrad(const float *pos, const float *vel, float *dPdPos, float *dPdVel, double *P);
+
 /* Bond-tangent nematic order S of the worms (NWObservables.bondS) and, when
    dSdPos is not NULL, dS/dpos (device float4 per bead, w = 0).
    Synchronises. */

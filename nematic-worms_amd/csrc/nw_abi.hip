@@ -99,6 +99,33 @@ struct State {
         uint64_t capCkpt = 0, capPos = 0, capVel = 0, capSPos = 0, capIds = 0, capOff = 0, capCell = 0,
                  capWork = 0, capLam = 0, capPartial = 0;
     } adj;
+    // SRD solvent of the batch (nwBatchSetSolvent)
+    struct BatchSolvent {
+        std::vector<SrdRep> reps;
+        SrdRep *dev = nullptr;
+        std::vector<uint32_t> solvOff, cellOff;  // R + 1 each: first solvent particle, first SRD cell
+        uint32_t R = 0, Ns = 0, cells = 0;
+        float cosA = 0.f, sinA = 0.f;
+        int rotBeads = 0;
+        BucketScratch sort;  // the SRD cell sort of the whole batch
+    } bsol;
+    // adjoint of a batched run: checkpoints of the last nwBatchAdjointForward
+    struct BatchAdjoint {
+        bool ready = false, srd = false;
+        uint32_t steps = 0, K = 0, gap = 0, N = 0, NT = 0, C = 0, Cs = 0, R = 0, waves = 0;
+        uint64_t ctr0 = 0;  // the batch's step count before the run
+        float dt = 0.f;
+        float4 *ckpt = nullptr;   // checkpoints x (2 NT + 2 N): pos, vel, force, forceOld
+        float *shifts = nullptr;  // collisions x R x 3 grid shifts
+        float4 *segPos = nullptr, *segVel = nullptr, *segSPos = nullptr, *work = nullptr, *lam = nullptr;
+        float4 *wPos = nullptr, *wVel = nullptr, *collCom = nullptr, *cellSum = nullptr;
+        uint32_t *segIds = nullptr, *segOff = nullptr, *hash = nullptr, *sCell = nullptr;
+        uint32_t *collHash = nullptr, *collIds = nullptr, *collOff = nullptr;
+        double *partial = nullptr, *grad = nullptr;
+        uint64_t capCkpt = 0, capShifts = 0, capPos = 0, capVel = 0, capSPos = 0, capWork = 0, capLam = 0,
+                 capWPos = 0, capWVel = 0, capCollCom = 0, capCellSum = 0, capIds = 0, capOff = 0, capHash = 0,
+                 capSCell = 0, capCollHash = 0, capCollIds = 0, capCollOff = 0, capPartial = 0, capGrad = 0;
+    } badj;
     std::map<std::string, Timer> timers;
 };
 
@@ -734,11 +761,17 @@ double nwSymEig3Max(const double a[6]) {
     return q + 2.0 * p * std::cos(std::acos(r) / 3.0);
 }
 
+static void observe_kp(const KP &kp, const float *pos, const float *vel, uint numFilaments, NWObservables *out);
+
 void nwObservables(const float *pos, const float *vel, uint numFilaments, NWObservables *out) {
     NW_INIT();
     NW_REQUIRE(out != nullptr && pos != nullptr && vel != nullptr, "nwObservables: NULL argument");
+    observe_kp(make_kp(), pos, vel, numFilaments, out);
+}
+
+// the observables of the worms of kp's SimParams
+static void observe_kp(const KP &kp, const float *pos, const float *vel, uint numFilaments, NWObservables *out) {
     State &s = S();
-    const KP kp = make_kp();
     const uint32_t n = kp.p.filamentSize;
     NW_REQUIRE(n >= 1 && n <= 64, "nwObservables: worms of %u beads (1..64 supported)", n);
     std::memset(out, 0, sizeof(*out));
@@ -797,6 +830,8 @@ void nwBatchSetReplicas(const NWReplicaParams *reps, uint numReplicas) {
     State::Batch &b = s.batch;
     NW_REQUIRE(numReplicas == 0 || reps != nullptr, "nwBatchSetReplicas: NULL replicas");
     batch_drop_graph(b);
+    s.bsol.R = 0;  // a new batch has no solvent until nwBatchSetSolvent
+    s.badj.ready = false;
     b.reps.assign(numReplicas, BatchRep{});
     b.beadOff.assign(numReplicas + 1, 0u);
     b.cellOff.assign(numReplicas + 1, 0u);
@@ -824,6 +859,7 @@ void nwBatchSetReplicas(const NWReplicaParams *reps, uint numReplicas) {
         d.revBlocks = p.reverseProbability > 0.0f ? uint32_t(blocks_for(W)) : 0u;
         d.rngStride = d.revBlocks + (q.langevin ? uint64_t(W) * n : 0u);
         d.lgvStd = sqrtf(2.0f * q.langevinGamma * q.langevinKbT);
+        d.lgvGamma = q.langevinGamma;
         d.langevin = q.langevin ? 1 : 0;
         d.numBeads = W * n;
         N += uint64_t(W) * n;
@@ -1225,6 +1261,337 @@ void nwAdjointBackward(const float *dLdPos, const float *dLdVel, double grad[4])
     }
     s.offset = A.endOffset;  // the generator continues after the forward run
     NW_HIP(hipMemcpyAsync(grad, A.grad, kAdjParams * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    NW_HIP(hipStreamSynchronize(s.stream));
+}
+
+// ---------------------------------------------------------------------------
+// SRD solvent and adjoint of batched replicas
+// ---------------------------------------------------------------------------
+void nwBatchSetSolvent(const NWReplicaSolvent *solvent, uint numReplicas) {
+    NW_INIT();
+    State &s = S();
+    State::Batch &b = s.batch;
+    State::BatchSolvent &q = s.bsol;
+    s.badj.ready = false;
+    q.R = q.Ns = q.cells = 0;
+    q.reps.clear();
+    if (q.dev) (void)hipFree(q.dev);
+    q.dev = nullptr;
+    if (numReplicas == 0) return;
+    NW_REQUIRE(solvent != nullptr && numReplicas == b.R,
+               "nwBatchSetSolvent: one solvent per replica of the current batch (%u)", b.R);
+    q.reps.assign(numReplicas, SrdRep{});
+    q.solvOff.assign(numReplicas + 1, b.N);
+    q.cellOff.assign(numReplicas + 1, 0u);
+    uint64_t NT = b.N, C = 0;
+    const float alpha = solvent[0].solvent.alpha;
+    const int rot = b.reps[0].kp.e.srdCollideFilaments != 0 ? 1 : 0;
+    for (uint r = 0; r < numReplicas; ++r) {
+        const SolventParams &sp = solvent[r].solvent;
+        const uint64_t cells = uint64_t(sp.gridSize.x) * sp.gridSize.y * sp.gridSize.z;
+        NW_REQUIRE(cells > 0 && cells == sp.numCells, "nwBatchSetSolvent: replica %u: SRD grid does not match numCells",
+                   r);
+        NW_REQUIRE(sp.alpha == alpha, "nwBatchSetSolvent: replica %u: one rotation angle per batch", r);
+        NW_REQUIRE((b.reps[r].kp.e.srdCollideFilaments != 0 ? 1 : 0) == rot,
+                   "nwBatchSetSolvent: replica %u: one srdCollideFilaments per batch", r);
+        SrdRep &d = q.reps[r];
+        d.kp = make_kp_for(b.reps[r].kp.p, sp, b.reps[r].kp.e);
+        d.beadOffset = b.beadOff[r];
+        d.numBeads = b.reps[r].numBeads;
+        d.solvOffset = uint32_t(NT);
+        d.numSolvent = solvent[r].numSolvent;
+        d.cellOffset = uint32_t(C);
+        d.cells = uint32_t(cells);
+        d.seed = b.reps[r].seed;
+        d.rngBase = b.reps[r].rngBase;
+        d.cellBlocks = blocks_for(cells);
+        NT += solvent[r].numSolvent;
+        C += cells;
+        NW_REQUIRE(NT < (1ull << 31) && C < (1ull << 32) - 2, "nwBatchSetSolvent: batch too large");
+        q.solvOff[r + 1] = uint32_t(NT);
+        q.cellOff[r + 1] = uint32_t(C);
+    }
+    const BucketPlan bp = make_bucket_plan(NT, C, 0);
+    NW_REQUIRE(!*bucket_plan_problem(bp), "nwBatchSetSolvent: %s", bucket_plan_problem(bp));
+    q.Ns = uint32_t(NT - b.N);
+    q.cells = uint32_t(C);
+    q.cosA = float(std::cos(double(alpha)));
+    q.sinA = float(std::sin(double(alpha)));
+    q.rotBeads = rot;
+    NW_HIP(hipMalloc(&q.dev, sizeof(SrdRep) * numReplicas));
+    NW_HIP(hipMemcpy(q.dev, q.reps.data(), sizeof(SrdRep) * numReplicas, hipMemcpyHostToDevice));
+    q.R = numReplicas;
+}
+
+void nwBatchSolventLayout(uint *solventOffset, uint *srdCellOffset) {
+    const State::BatchSolvent &q = S().bsol;
+    for (uint r = 0; r <= q.R; ++r) {
+        if (solventOffset) solventOffset[r] = q.solvOff.empty() ? 0u : q.solvOff[r];
+        if (srdCellOffset) srdCellOffset[r] = q.cellOff.empty() ? 0u : q.cellOff[r];
+    }
+}
+
+// One batched MD step writing its cell lists into a (ids, off, sCell, sPos),
+// so a replayed step keeps them: the WCA chain worms of the adjoint (no
+// velocity or tangent gathers).
+static hipError_t batch_step_into(State::Batch &b, const BatchArrays &a, float dt, hipStream_t st) {
+    BucketScratch sc = b.scratch;
+    sc.off = a.off;
+    sc.sortedCell = a.sCell;
+    hipError_t e;
+    if ((e = launch_md_front_batch(a, b.dev, b.R, b.waves, b.stepCtr, dt, st)) != hipSuccess) return e;
+    if ((e = launch_hash_tiles_batch(a.hash, b.sortDev, b.R, b.sortPlan, b.scratch, st)) != hipSuccess) return e;
+    if ((e = launch_md_cells_batch(b.sortDev, b.R, b.sortPlan, sc, a.ids, a.pos, nullptr, nullptr, a.sPos, nullptr,
+                                   nullptr, st)) != hipSuccess)
+        return e;
+    return launch_collide_batch(a, b.dev, b.R, b.blocks, b.mode, b.tan, b.unit, b.stepCtr, st);
+}
+
+// Collision c of every replica of the batch over the full arrays (all beads,
+// then the solvent): cell of every particle into hash, the cell order into
+// ids, its CSR offsets into offOut, (u, sign) per cell into com.
+static bool batch_srd_collision(State &s, uint32_t c, float4 *P, float4 *V, uint32_t *hash, uint32_t *ids,
+                                uint32_t *offOut, float4 *com, const float *shift, float dtSrd) {
+    State::Batch &b = s.batch;
+    State::BatchSolvent &q = s.bsol;
+    const uint32_t NT = b.N + q.Ns;
+    const BucketPlan bp = make_bucket_plan(NT, q.cells, 0);
+    NW_HIP_R(ensure_bucket_scratch(q.sort, bp), false);
+    BucketScratch sc = q.sort;
+    sc.off = offOut;
+    NW_HIP_R(launch_srd_hash_batch(P, hash, q.dev, q.R, b.N, NT, shift, s.stream), false);
+    NW_HIP_R(launch_hash_tiles(hash, bp, q.sort, s.stream), false);
+    NW_HIP_R(launch_srd_cells_batch(bp, sc, ids, V, com, q.dev, q.R, c, s.stream), false);
+    NW_HIP_R(launch_srd_rotate_batch(hash, com, P, V, q.dev, q.R, b.N, NT, dtSrd, q.cosA, q.sinA, q.rotBeads,
+                                     s.stream),
+             false);
+    return true;
+}
+
+void nwBatchAdjointForward(float *pos, float *vel, float *force, float *forceOld, float *tangent, float deltaTime,
+                           uint srdGap, const float *srdShifts, uint steps, uint checkpointEvery) {
+    NW_INIT();
+    State &s = S();
+    State::Batch &b = s.batch;
+    State::BatchSolvent &q = s.bsol;
+    State::BatchAdjoint &A = s.badj;
+    A.ready = false;
+    NW_REQUIRE(b.R > 0 && b.N > 0, "nwBatchAdjointForward: no batch (nwBatchSetReplicas)");
+    NW_REQUIRE(pos && vel && force && forceOld && tangent, "nwBatchAdjointForward: NULL array");
+    NW_REQUIRE(checkpointEvery >= 1 && steps >= 1, "nwBatchAdjointForward: steps and checkpointEvery >= 1");
+    NW_REQUIRE(b.mode == 1 && !b.tan, "nwBatchAdjointForward: WCA pairs without the extensile term only");
+    for (uint32_t r = 0; r < b.R; ++r) {
+        const KP &kp = b.reps[r].kp;
+        NW_REQUIRE(kp.e.bondedMode == NW_BONDED_CHAIN && kp.p.reverseProbability == 0.0f,
+                   "nwBatchAdjointForward: replica %u: chain bonds and no reversal only", r);
+        NW_REQUIRE(kp.p.boundaryX == PERIODIC && kp.p.boundaryY == PERIODIC && kp.p.boundaryZ == PERIODIC,
+                   "nwBatchAdjointForward: replica %u: periodic boxes only", r);
+        NW_REQUIRE(!b.reps[r].langevin || b.reps[r].lgvGamma == kp.p.gamma,
+                   "nwBatchAdjointForward: replica %u: the Langevin gamma must be SimParams.gamma", r);
+    }
+    const bool srd = srdGap > 0;
+    if (srd) {
+        NW_REQUIRE(q.R == b.R, "nwBatchAdjointForward: SRD collisions need the batch's solvent (nwBatchSetSolvent)");
+        NW_REQUIRE(srdShifts != nullptr, "nwBatchAdjointForward: NULL grid shifts");
+        for (uint32_t r = 0; r < b.R; ++r)
+            NW_REQUIRE(b.reps[r].rngStride == 0,
+                       "nwBatchAdjointForward: replica %u: no Langevin thermostat with the solvent", r);
+    }
+    const uint64_t N = b.N, NT = N + (srd ? q.Ns : 0), C = b.cells, Cs = srd ? q.cells : 0;
+    A.srd = srd;
+    A.gap = srdGap;
+    A.steps = steps;
+    A.K = checkpointEvery;
+    A.dt = deltaTime;
+    A.N = b.N;
+    A.NT = uint32_t(NT);
+    A.C = b.cells;
+    A.Cs = uint32_t(Cs);
+    A.R = b.R;
+    A.waves = b.waves;
+    A.ctr0 = b.steps;
+    const uint32_t nck = (steps + checkpointEvery - 1) / checkpointEvery;
+    const uint64_t stride = 2 * NT + 2 * N;
+    NW_REQUIRE(grow(A.ckpt, A.capCkpt, uint64_t(nck) * stride) && grow(A.hash, A.capHash, NT) &&
+                   grow(A.sCell, A.capSCell, N) && grow(A.segIds, A.capIds, N) && grow(A.segSPos, A.capSPos, N) &&
+                   grow(A.segOff, A.capOff, C + 2),
+               "nwBatchAdjointForward: out of device memory");
+    if (srd) {
+        const uint64_t nColl = collisions_in(steps, srdGap);
+        NW_REQUIRE(grow(A.shifts, A.capShifts, nColl * b.R * 3) && grow(A.collHash, A.capCollHash, NT) &&
+                       grow(A.collIds, A.capCollIds, NT) && grow(A.collOff, A.capCollOff, Cs + 2) &&
+                       grow(A.collCom, A.capCollCom, Cs),
+                   "nwBatchAdjointForward: out of device memory");
+        NW_HIP(hipMemcpy(A.shifts, srdShifts, nColl * b.R * 3 * sizeof(float), hipMemcpyHostToDevice));
+    }
+    BatchArrays a{};
+    a.pos = reinterpret_cast<float4 *>(pos);
+    a.vel = reinterpret_cast<float4 *>(vel);
+    a.force = reinterpret_cast<float4 *>(force);
+    a.forceOld = reinterpret_cast<float4 *>(forceOld);
+    a.tangent = reinterpret_cast<float4 *>(tangent);
+    a.hash = A.hash;
+    a.ids = A.segIds;
+    a.off = A.segOff;
+    a.sCell = A.sCell;
+    a.sPos = A.segSPos;
+    const size_t bytesT = NT * sizeof(float4), bytes = N * sizeof(float4);
+    for (uint32_t k = 0; k < steps; ++k) {
+        if (k % checkpointEvery == 0) {
+            float4 *ck = A.ckpt + uint64_t(k / checkpointEvery) * stride;
+            NW_REQUIRE(copy_d2d(ck, pos, bytesT, s.stream) && copy_d2d(ck + NT, vel, bytesT, s.stream) &&
+                           copy_d2d(ck + 2 * NT, force, bytes, s.stream) &&
+                           copy_d2d(ck + 2 * NT + N, forceOld, bytes, s.stream),
+                       "nwBatchAdjointForward: checkpoint copy failed");
+        }
+        {
+            TimedScope ts("batch_adj_fwd_md", s.stream);
+            NW_HIP(batch_step_into(b, a, deltaTime, s.stream));
+        }
+        ++b.steps;
+        if (srd && k % srdGap == 0) {
+            TimedScope ts("batch_adj_fwd_srd", s.stream);
+            const uint32_t c = k / srdGap;
+            if (!batch_srd_collision(s, c, a.pos, a.vel, A.collHash, A.collIds, A.collOff, A.collCom,
+                                     A.shifts + uint64_t(c) * b.R * 3, deltaTime * float(srdGap)))
+                return;
+        }
+    }
+    A.ready = true;
+}
+
+static bool ensure_batch_segment(State::BatchAdjoint &A) {
+    const uint64_t N = A.N, K = A.K, NT = A.NT;
+    if (!(grow(A.segPos, A.capPos, (K + 1) * N) && grow(A.segVel, A.capVel, (K + 1) * N) &&
+          grow(A.segSPos, A.capSPos, K * N) && grow(A.segIds, A.capIds, K * N) &&
+          grow(A.segOff, A.capOff, K * (uint64_t(A.C) + 2)) && grow(A.work, A.capWork, 3 * N) &&
+          grow(A.lam, A.capLam, 4 * N + NT) && grow(A.wPos, A.capWPos, NT) && grow(A.wVel, A.capWVel, NT) &&
+          grow(A.partial, A.capPartial, uint64_t(A.waves) * kAdjParams) &&
+          grow(A.grad, A.capGrad, uint64_t(A.R) * kAdjParams)))
+        return false;
+    if (!A.srd) return true;
+    const uint64_t cpS = (K + A.gap - 1) / A.gap + 1;  // collisions in a segment, at most
+    return grow(A.collHash, A.capCollHash, cpS * NT) && grow(A.collIds, A.capCollIds, cpS * NT) &&
+           grow(A.collOff, A.capCollOff, cpS * (uint64_t(A.Cs) + 2)) &&
+           grow(A.collCom, A.capCollCom, cpS * A.Cs) && grow(A.cellSum, A.capCellSum, A.Cs);
+}
+
+void nwBatchAdjointBackward(const float *dLdPos, const float *dLdVel, double *grad) {
+    NW_INIT();
+    State &s = S();
+    State::Batch &b = s.batch;
+    State::BatchSolvent &q = s.bsol;
+    State::BatchAdjoint &A = s.badj;
+    NW_REQUIRE(A.ready, "nwBatchAdjointBackward: no nwBatchAdjointForward run to differentiate");
+    NW_REQUIRE(dLdPos != nullptr && grad != nullptr, "nwBatchAdjointBackward: NULL argument");
+    NW_REQUIRE(b.R == A.R && b.N == A.N, "nwBatchAdjointBackward: the batch changed since the forward run");
+    NW_REQUIRE(ensure_batch_segment(A), "nwBatchAdjointBackward: out of device memory");
+    const uint64_t N = A.N, NT = A.NT, K = A.K, C = A.C, stride = 2 * NT + 2 * N;
+    const size_t bytes = N * sizeof(float4);
+    float4 *lr = A.lam, *lF[2] = {A.lam + N, A.lam + 2 * N}, *lG = A.lam + 3 * N, *lv = A.lam + 4 * N;
+    NW_HIP(hipMemcpyAsync(lr, dLdPos, bytes, hipMemcpyDeviceToDevice, s.stream));
+    NW_HIP(hipMemsetAsync(lv, 0, NT * sizeof(float4), s.stream));
+    if (dLdVel) NW_HIP(hipMemcpyAsync(lv, dLdVel, bytes, hipMemcpyDeviceToDevice, s.stream));
+    NW_HIP(hipMemsetAsync(lF[0], 0, bytes, s.stream));
+    NW_HIP(hipMemsetAsync(lG, 0, bytes, s.stream));
+    NW_HIP(hipMemsetAsync(A.partial, 0, uint64_t(A.waves) * kAdjParams * sizeof(double), s.stream));
+    int cur = 0;
+    float4 *F = A.work, *G = A.work + N, *T = A.work + 2 * N;
+    const uint32_t nck = (A.steps + A.K - 1) / A.K;
+    std::vector<int> collSlot(K);
+    for (int c = int(nck) - 1; c >= 0; --c) {
+        const uint32_t s0 = uint32_t(c) * A.K, len = std::min<uint32_t>(A.K, A.steps - s0);
+        const float4 *ck = A.ckpt + uint64_t(c) * stride;
+        NW_HIP(hipMemcpyAsync(A.wPos, ck, NT * sizeof(float4), hipMemcpyDeviceToDevice, s.stream));
+        NW_HIP(hipMemcpyAsync(A.wVel, ck + NT, NT * sizeof(float4), hipMemcpyDeviceToDevice, s.stream));
+        NW_HIP(hipMemcpyAsync(F, ck + 2 * NT, bytes, hipMemcpyDeviceToDevice, s.stream));
+        NW_HIP(hipMemcpyAsync(G, ck + 2 * NT + N, bytes, hipMemcpyDeviceToDevice, s.stream));
+        NW_HIP(launch_set_step(b.stepCtr, A.ctr0 + s0, s.stream));
+        // recompute the segment: the full state in place, every step's bead
+        // state and cell lists and every collision's cells kept
+        int slots = 0;
+        for (uint32_t j = 0; j < len; ++j) {
+            BatchArrays a{};
+            a.pos = A.wPos;
+            a.vel = A.wVel;
+            a.force = F;
+            a.forceOld = G;
+            a.tangent = T;
+            a.hash = A.hash;
+            a.ids = A.segIds + uint64_t(j) * N;
+            a.off = A.segOff + uint64_t(j) * (C + 2);
+            a.sCell = A.sCell;
+            a.sPos = A.segSPos + uint64_t(j) * N;
+            NW_HIP(batch_step_into(b, a, A.dt, s.stream));
+            NW_HIP(hipMemcpyAsync(A.segPos + uint64_t(j + 1) * N, A.wPos, bytes, hipMemcpyDeviceToDevice, s.stream));
+            NW_HIP(hipMemcpyAsync(A.segVel + uint64_t(j + 1) * N, A.wVel, bytes, hipMemcpyDeviceToDevice, s.stream));
+            collSlot[j] = -1;
+            const uint32_t k = s0 + j;
+            if (A.srd && k % A.gap == 0) {
+                const int sl = slots++;
+                collSlot[j] = sl;
+                const uint64_t u = uint64_t(sl);
+                if (!batch_srd_collision(s, k / A.gap, A.wPos, A.wVel, A.collHash + u * NT, A.collIds + u * NT,
+                                         A.collOff + u * (A.Cs + 2), A.collCom + u * A.Cs,
+                                         A.shifts + uint64_t(k / A.gap) * A.R * 3, A.dt * float(A.gap)))
+                    return;
+            }
+        }
+        // the adjoint through the segment, last step first
+        for (int j = int(len) - 1; j >= 0; --j) {
+            if (collSlot[j] >= 0) {
+                const uint64_t u = uint64_t(collSlot[j]);
+                NW_HIP(launch_srd_adj(A.collIds + u * NT, A.collOff + u * (A.Cs + 2), A.collHash + u * NT,
+                                      A.collCom + u * A.Cs, A.wVel, lv, A.cellSum, A.N, uint32_t(NT), A.Cs, q.cosA,
+                                      q.sinA, q.rotBeads, s.stream));
+            }
+            AdjStepArgs as{};
+            as.pos = A.segPos + uint64_t(j + 1) * N;
+            as.vel = A.segVel + uint64_t(j + 1) * N;
+            as.sPos = A.segSPos + uint64_t(j) * N;
+            as.ids = A.segIds + uint64_t(j) * N;
+            as.off = A.segOff + uint64_t(j) * (C + 2);
+            as.lF = lF[cur];
+            as.lFout = lF[1 - cur];
+            as.lr = lr;
+            as.lv = lv;
+            as.lG = lG;
+            as.partial = A.partial;
+            as.dt = A.dt;
+            NW_HIP(launch_adj_step_batch(as, b.dev, b.R, A.waves, A.ctr0 + s0 + uint32_t(j), s.stream));
+            cur = 1 - cur;
+        }
+    }
+    NW_HIP(launch_set_step(b.stepCtr, A.ctr0 + A.steps, s.stream));  // where the forward run left it
+    NW_HIP(launch_adj_total_batch(A.partial, b.dev, b.R, A.waves, A.grad, s.stream));
+    NW_HIP(hipMemcpyAsync(grad, A.grad, uint64_t(A.R) * kAdjParams * sizeof(double), hipMemcpyDeviceToHost,
+                          s.stream));
+    NW_HIP(hipStreamSynchronize(s.stream));
+}
+
+void nwBatchPolarSpeedGrad(const float *pos, const float *vel, float *dPdPos, float *dPdVel, double *P) {
+    NW_INIT();
+    State &s = S();
+    State::Batch &b = s.batch;
+    NW_REQUIRE(pos != nullptr && vel != nullptr && P != nullptr, "nwBatchPolarSpeedGrad: NULL argument");
+    for (uint32_t r = 0; r < b.R; ++r) {
+        const BatchRep &rep = b.reps[r];
+        const size_t o = size_t(rep.beadOffset) * 4;
+        NWObservables obs;
+        observe_kp(rep.kp, pos + o, vel + o, rep.kp.p.numFilaments, &obs);
+        P[r] = obs.polarSpeed;
+        if (!dPdPos) continue;
+        if (obs.beads > 0) {
+            NW_HIP(launch_polar_speed_grad(reinterpret_cast<const float4 *>(pos + o),
+                                           reinterpret_cast<const float4 *>(vel + o), rep.kp.p.numFilaments,
+                                           1.0 / obs.beads, reinterpret_cast<float4 *>(dPdPos + o),
+                                           dPdVel ? reinterpret_cast<float4 *>(dPdVel + o) : nullptr, rep.kp,
+                                           s.stream));
+        } else {
+            NW_HIP(hipMemsetAsync(dPdPos + o, 0, size_t(rep.numBeads) * 16, s.stream));
+            if (dPdVel) NW_HIP(hipMemsetAsync(dPdVel + o, 0, size_t(rep.numBeads) * 16, s.stream));
+        }
+    }
     NW_HIP(hipStreamSynchronize(s.stream));
 }
 

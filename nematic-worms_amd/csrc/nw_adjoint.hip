@@ -99,10 +99,13 @@ __device__ __forceinline__ void block_partials(double (&acc)[kAdjParams], double
 //            (lr, lv, lF, lG) of the bead (lF into the other buffer: the
 //            neighbours still read the old one)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_adj_step(AdjStepArgs a, KP kp) {
+// The lanes m of one system: a's bead arrays (pos, vel, lF, lFout, lr, lv,
+// lG) and off start at the system's first bead and cell; ids and sPos are the
+// step's sorted slots, whose ids exceed the system's bead index by idBase.
+__device__ __forceinline__ void adj_step_lanes(const AdjStepArgs &a, const KP &kp, const WaveWorm &m,
+                                               uint32_t idBase, double (&acc)[kAdjParams]) {
     const SimParams &p = kp.p;
     const uint32_t n = p.filamentSize;
-    const WaveWorm m = wave_worm(n, a.W);
     const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
     const f3 rv = mk(m.active ? a.pos[m.bead] : zero4);
     const f3 lam = mk(m.active ? a.lF[m.bead] : zero4);
@@ -151,7 +154,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_step(AdjStepArgs a, KP kp) {
     if (pp >= 1u) bond = bond - fbPrev;
     if (pp + 1u < n) bond = bond + fb;
 
-    double acc[kAdjParams] = {0.0, 0.0, 0.0, 0.0};
     if (m.active) {
         f3 R = bend.t + bond.t;
         if (pp + 1u < n) R -= driveT;
@@ -197,7 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_step(AdjStepArgs a, KP kp) {
                     const uint32_t lo = part == 0 ? a.off[ia0] : a.off[base];
                     const uint32_t hi = part == 0 ? a.off[ia1] : a.off[ib1];
                     for (uint32_t j = lo; j < hi; ++j) {
-                        const uint32_t other = a.ids[j];
+                        const uint32_t other = a.ids[j] - idBase;
                         uint32_t fil2, ch2;
                         worm_of(kp, other, fil2, ch2);
                         if (fil2 == fil) continue;
@@ -228,7 +230,74 @@ __global__ __launch_bounds__(kBlock) void k_adj_step(AdjStepArgs a, KP kp) {
         a.lFout[m.bead] = f4(lG + (0.5f * a.dt * a.dt) * Rt + (0.5f * a.dt) * V, 0.0f);
         a.lG[m.bead] = f4((0.5f * a.dt) * V, 0.0f);
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_adj_step(AdjStepArgs a, KP kp) {
+    const WaveWorm m = wave_worm(kp.p.filamentSize, a.W);
+    double acc[kAdjParams] = {0.0, 0.0, 0.0, 0.0};
+    adj_step_lanes(a, kp, m, 0u, acc);
     block_partials(acc, a.partial);  // every thread, once (a workgroup barrier inside)
+}
+
+// One adjoint step of every replica of a batch (a: the concatenated arrays;
+// the wavefronts of the batched front kernel).  Each wavefront adds its FP64
+// partials to its own row of a.partial (one row of kAdjParams per
+// wavefront, accumulated over the steps in step order).
+__global__ __launch_bounds__(kBlock) void k_adj_step_batch(AdjStepArgs a0, const BatchRep *__restrict__ reps,
+                                                           uint32_t R, uint32_t waves, uint64_t step) {
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (wave >= waves) return;
+    const uint32_t r = __builtin_amdgcn_readfirstlane(replica_of_wave(reps, R, wave));
+    const BatchRep &rep = reps[r];
+    const KP &kp = rep.kp;
+    const WaveWorm m = wave_worm_at(wave - rep.waveOffset, kp.p.filamentSize, kp.p.numFilaments);
+    AdjStepArgs a = a0;
+    const uint32_t b0 = rep.beadOffset;
+    a.pos += b0;
+    a.vel += b0;
+    a.lF += b0;
+    a.lFout += b0;
+    a.lr += b0;
+    a.lv += b0;
+    a.lG += b0;
+    a.off += rep.cellOffset;
+    a.W = kp.p.numFilaments;
+    a.langevin = rep.langevin;
+    a.lgvStd = rep.lgvStd;
+    a.gamma = kp.p.gamma;
+    a.seed = rep.seed;
+    a.offLgv = rep.rngBase + step * rep.rngStride + rep.revBlocks;
+    double acc[kAdjParams] = {0.0, 0.0, 0.0, 0.0};
+    adj_step_lanes(a, kp, m, b0, acc);
+#pragma unroll
+    for (int k = 0; k < kAdjParams; ++k) {
+        const double t = wave_sum_d(acc[k]);
+        if ((threadIdx.x & 63u) == 0u) a0.partial[uint64_t(wave) * kAdjParams + k] += t;
+    }
+}
+
+// out[r][k] = sum of the rows of replica r's wavefronts, in a fixed order
+constexpr uint32_t kRepTotalBlock = 256;
+__global__ __launch_bounds__(kRepTotalBlock) void k_adj_total_batch(const double *__restrict__ partial,
+                                                                    const BatchRep *__restrict__ reps, uint32_t R,
+                                                                    uint32_t waves, double *__restrict__ out) {
+    __shared__ double red[kAdjParams][kRepTotalBlock];
+    const uint32_t r = blockIdx.x, t = threadIdx.x;
+    const uint32_t lo = reps[r].waveOffset, hi = r + 1 < R ? reps[r + 1].waveOffset : waves;
+    double acc[kAdjParams] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t w = lo + t; w < hi; w += kRepTotalBlock)
+#pragma unroll
+        for (int k = 0; k < kAdjParams; ++k) acc[k] += partial[uint64_t(w) * kAdjParams + k];
+#pragma unroll
+    for (int k = 0; k < kAdjParams; ++k) red[k][t] = acc[k];
+    __syncthreads();
+    for (uint32_t h = kRepTotalBlock / 2; h >= 1u; h >>= 1) {
+        if (t < h)
+#pragma unroll
+            for (int k = 0; k < kAdjParams; ++k) red[k][t] += red[k][t + h];
+        __syncthreads();
+    }
+    if (t < uint32_t(kAdjParams)) out[uint64_t(r) * kAdjParams + t] = red[t][0];
 }
 
 // ---------------------------------------------------------------------------
@@ -421,6 +490,21 @@ hipError_t launch_polar_speed_grad(const float4 *pos, const float4 *vel, uint32_
     if (!blocks) return hipSuccess;
     hipLaunchKernelGGL(k_polar_speed_grad, dim3(blocks), dim3(kBlock), 0, st, pos, vel, W, float(scale), dPos, dVel,
                        kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_step_batch(const AdjStepArgs &a, const BatchRep *reps, uint32_t R, uint32_t waves,
+                                 uint64_t step, hipStream_t st) {
+    if (waves == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adj_step_batch, dim3(grid_for(uint64_t(waves) * 64u)), dim3(kBlock), 0, st, a, reps, R,
+                       waves, step);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_total_batch(const double *partial, const BatchRep *reps, uint32_t R, uint32_t waves,
+                                  double *out, hipStream_t st) {
+    if (R == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adj_total_batch, dim3(R), dim3(kRepTotalBlock), 0, st, partial, reps, R, waves, out);
     return hipGetLastError();
 }
 

@@ -1,6 +1,7 @@
 // nw_adjoint.hpp -- kernels of the adjoint of the fused MD step (nw_adjoint.hip)
 #pragma once
 
+#include "nw_bucket.hpp"
 #include "nw_internal.hpp"
 
 namespace nw {
@@ -38,6 +39,15 @@ hipError_t launch_srd_adj(const uint32_t *ids, const uint32_t *off, const uint32
 // scale * d(sum of v . t)/d(r, v) of the polar speed (dVel may be NULL)
 hipError_t launch_polar_speed_grad(const float4 *pos, const float4 *vel, uint32_t W, double scale, float4 *dPos,
                                    float4 *dVel, const KP &kp, hipStream_t st);
+// one adjoint step of every replica of a batch (a: concatenated arrays,
+// off over the concatenated cells, ids global; the replicas' own W,
+// Langevin and Philox from reps; a.partial: one row per front wavefront,
+// added to)
+hipError_t launch_adj_step_batch(const AdjStepArgs &a, const BatchRep *reps, uint32_t R, uint32_t waves,
+                                 uint64_t step, hipStream_t st);
+// out[r][k] = the sum of replica r's wavefront rows
+hipError_t launch_adj_total_batch(const double *partial, const BatchRep *reps, uint32_t R, uint32_t waves,
+                                  double *out, hipStream_t st);
 // out[k] += the column sums of partial (rows x kAdjParams), in a fixed order
 hipError_t launch_adj_total(const double *partial, uint64_t rows, double *out, hipStream_t st);
 

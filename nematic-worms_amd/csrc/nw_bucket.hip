@@ -256,7 +256,21 @@ struct CellsArgs {
     int iso;
     float kbT;
     uint64_t seed, offset;
+    // batched replicas (NULL: one system): each cell's sign from its replica's stream
+    const SrdRep *srdReps;
+    uint32_t srdR, collision;
 };
+
+// the replica that owns SRD cell `cell` (cell offsets ascending)
+__device__ __forceinline__ uint32_t srd_rep_of_cell(const SrdRep *__restrict__ reps, uint32_t R, uint32_t cell) {
+    uint32_t lo = 0, hi = R;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (reps[mid].cellOffset <= cell) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
 
 // index of the largest t in [0, tiles) with segStart[t] <= k
 __device__ __forceinline__ uint32_t seg_of(const uint32_t *segStart, uint32_t tiles, uint32_t k) {
@@ -498,7 +512,15 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
             if (M > 0.0f) mom = mom * (1.0f / M);
         }
         const uint32_t cell = cell0 + c;
-        const float sgn = uniform_at(o.seed, o.offset, cell) > 0.5f ? 1.0f : -1.0f;
+        uint64_t seed = o.seed, offset = o.offset;
+        uint32_t e = cell;
+        if (o.srdReps) {
+            const SrdRep &q = o.srdReps[srd_rep_of_cell(o.srdReps, o.srdR, cell)];
+            seed = q.seed;
+            offset = q.rngBase + uint64_t(o.collision) * q.cellBlocks;
+            e = cell - q.cellOffset;
+        }
+        const float sgn = uniform_at(seed, offset, e) > 0.5f ? 1.0f : -1.0f;
         o.com[cell] = f4(mom, sgn);
         if (o.iso) o.lam[cell] = lambda;
     }
@@ -589,6 +611,62 @@ __global__ __launch_bounds__(kBlock) void k_srd_rotate(const uint32_t *__restric
         }
         vel4[i] = v4{v.x, v.y, v.z, V[j].w};
     }
+}
+
+// ---------------------------------------------------------------------------
+// batched SRD: particles of every replica in one launch, each with its
+// replica's parameters (found by bisection: particles of a wavefront mostly
+// share a replica)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t srd_rep_of_particle(const SrdRep *__restrict__ reps, uint32_t R, uint32_t i,
+                                                        bool bead) {
+    uint32_t lo = 0, hi = R;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((bead ? reps[mid].beadOffset : reps[mid].solvOffset) <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_srd_hash_batch(const float4 *__restrict__ pos, uint32_t *__restrict__ hash,
+                                                           const SrdRep *__restrict__ reps, uint32_t R, uint32_t Nb,
+                                                           uint32_t NT, const float *__restrict__ shift) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NT) return;
+    const uint32_t r = srd_rep_of_particle(reps, R, i, i < Nb);
+    const SrdRep &q = reps[r];
+    const f3 sh = mk(shift[3 * r], shift[3 * r + 1], shift[3 * r + 2]);
+    hash[i] = q.cellOffset + srd_hash_at(q.kp.p, q.kp.s, sh, mk(pos[i]));
+}
+
+__global__ __launch_bounds__(kBlock) void k_srd_rotate_batch(const uint32_t *__restrict__ hash,
+                                                             const float4 *__restrict__ com, float4 *__restrict__ pos,
+                                                             float4 *__restrict__ vel, const SrdRep *__restrict__ reps,
+                                                             uint32_t R, uint32_t Nb, uint32_t NT, float dt,
+                                                             float cosA, float sinA, int rotBeads) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NT) return;
+    const bool solvent = i >= Nb;
+    if (!solvent && !rotBeads) return;
+    const float4 V = vel[i], C = com[hash[i]];
+    // the arithmetic of k_srd_rotate
+    f3 v = mk(V.x, V.y, V.z);
+    const f3 u = mk(C.x, C.y, C.z);
+    const float sn = C.w * sinA;
+    const f3 d = v - u;
+    const f3 rot = mk(d.x * cosA - d.y * sn, d.x * sn + d.y * cosA, d.z);
+    v = u + rot;
+    if (solvent) {
+        const SimParams &p = reps[srd_rep_of_particle(reps, R, i, false)].kp.p;
+        const float4 P = pos[i];
+        f3 r = mk(P.x, P.y, P.z);
+        const f3 dr = dt * v;
+        r += dr;
+        apply_boundaries(p, r, v, dr, 0.0f);
+        pos[i] = make_float4(r.x, r.y, r.z, P.w);
+    }
+    vel[i] = make_float4(v.x, v.y, v.z, V.w);
 }
 
 // ---------------------------------------------------------------------------
@@ -881,6 +959,43 @@ hipError_t launch_srd_cells(const BucketPlan &bp, BucketScratch &s, uint32_t *id
     const CellsLayout L = cells_layout(bp, true);
     NW_LAUNCH((k_bucket_cells<true, kSrdCellsBlock>), dim3(bp.buckets), dim3(kSrdCellsBlock), L.total, st,
                        o, bp);
+    return hipGetLastError();
+}
+
+hipError_t launch_srd_cells_batch(const BucketPlan &bp, BucketScratch &s, uint32_t *ids, const float4 *vel,
+                                  float4 *com, const SrdRep *reps, uint32_t R, uint32_t collision, hipStream_t st) {
+    if (bp.n == 0) return hipSuccess;
+    CellsArgs o{};
+    o.pairs = s.pairs;
+    o.P = s.P;
+    o.ids = ids;
+    o.off = s.off;
+    o.cellOf = s.sortedCell;
+    o.srdVel = vel;
+    o.com = com;
+    o.lam = s.lam;
+    o.iso = 0;
+    o.srdReps = reps;
+    o.srdR = R;
+    o.collision = collision;
+    const CellsLayout L = cells_layout(bp, true);
+    NW_LAUNCH((k_bucket_cells<true, kSrdCellsBlock>), dim3(bp.buckets), dim3(kSrdCellsBlock), L.total, st, o, bp);
+    return hipGetLastError();
+}
+
+hipError_t launch_srd_hash_batch(const float4 *pos, uint32_t *hash, const SrdRep *reps, uint32_t R, uint32_t Nb,
+                                 uint32_t NT, const float *shift, hipStream_t st) {
+    if (NT == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_srd_hash_batch, dim3(grid_for(NT)), dim3(kBlock), 0, st, pos, hash, reps, R, Nb, NT, shift);
+    return hipGetLastError();
+}
+
+hipError_t launch_srd_rotate_batch(const uint32_t *hash, const float4 *com, float4 *pos, float4 *vel,
+                                   const SrdRep *reps, uint32_t R, uint32_t Nb, uint32_t NT, float dt, float cosA,
+                                   float sinA, int rotBeads, hipStream_t st) {
+    if (NT == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_srd_rotate_batch, dim3(grid_for(NT)), dim3(kBlock), 0, st, hash, com, pos, vel, reps, R, Nb,
+                       NT, dt, cosA, sinA, rotBeads);
     return hipGetLastError();
 }
 

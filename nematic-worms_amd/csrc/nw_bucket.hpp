@@ -114,9 +114,32 @@ struct BatchRep {
     uint64_t rngStride;    // blocks drawn per step (reversal + Langevin)
     uint32_t revBlocks;    // reversal blocks per step (0 without reversal)
     float lgvStd;          // sqrt(2 gamma kT) of the Langevin noise
+    float lgvGamma;        // the Langevin gamma
     int32_t langevin;
     uint32_t numBeads;     // numFilaments x filamentSize
 };
+
+// the replica of a front wavefront / pair workgroup of a batched launch
+// (offsets ascending; bisection over the descriptor table)
+__device__ __forceinline__ uint32_t replica_of_wave(const BatchRep *__restrict__ reps, uint32_t R, uint32_t wave) {
+    uint32_t lo = 0, hi = R;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (reps[mid].waveOffset <= wave) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t replica_of_block(const BatchRep *__restrict__ reps, uint32_t R, uint32_t blk) {
+    uint32_t lo = 0, hi = R;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (reps[mid].blockOffset <= blk) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
 
 // The batched sort: every replica sorted against its own plan (tiles of
 // kBatchTileItems of its beads, buckets of its own cells, its own prefix
@@ -147,6 +170,31 @@ hipError_t launch_md_cells_batch(const SortRep *reps, uint32_t R, const BatchSor
                                  uint32_t *ids, const float4 *pos, const float4 *vel, const float4 *tan, float4 *sPos,
                                  float4 *sVel, float4 *sTan, hipStream_t st);
 
+// SRD collisions of batched replicas: one cell sort over every replica's
+// particles (all beads of the batch, then every replica's solvent) and the
+// concatenated SRD cells; per replica its parameters, layout and Philox
+// stream (collision c draws its rotation signs at rngBase + c cellBlocks)
+struct SrdRep {
+    KP kp;                             // SimParams (box, boundaries) and SolventParams (SRD grid)
+    uint32_t beadOffset, numBeads;     // its beads: particles beadOffset ..
+    uint32_t solvOffset, numSolvent;   // its solvent: particles solvOffset .. (after all beads)
+    uint32_t cellOffset, cells;        // its SRD cells in the concatenated cell space
+    uint64_t seed, rngBase, cellBlocks;
+};
+// cell of every particle (Nb beads, then the solvent; NT in all) with the
+// collision's grid shift of replica r at shift[3 r ..]
+hipError_t launch_srd_hash_batch(const float4 *pos, uint32_t *hash, const SrdRep *reps, uint32_t R, uint32_t Nb,
+                                 uint32_t NT, const float *shift, hipStream_t st);
+// level 2 of the batched SRD sort: launch_srd_cells with every cell's sign
+// from its replica's stream at collision c (no isokinetic rescaling)
+hipError_t launch_srd_cells_batch(const BucketPlan &bp, BucketScratch &s, uint32_t *ids, const float4 *vel,
+                                  float4 *com, const SrdRep *reps, uint32_t R, uint32_t collision, hipStream_t st);
+// rotation about the cell momentum (beads when rotBeads) and the solvent's
+// streaming in its replica's box
+hipError_t launch_srd_rotate_batch(const uint32_t *hash, const float4 *com, float4 *pos, float4 *vel,
+                                   const SrdRep *reps, uint32_t R, uint32_t Nb, uint32_t NT, float dt, float cosA,
+                                   float sinA, int rotBeads, hipStream_t st);
+
 struct BatchArrays {
     float4 *pos, *vel, *force, *forceOld, *tangent;
     uint32_t *hash;   // global cell of every bead
@@ -156,6 +204,8 @@ struct BatchArrays {
     float4 *sPos, *sVel, *sTan;
 };
 
+// sets the device step counter of a batch (replaying a stretch of steps)
+hipError_t launch_set_step(uint64_t *ctr, uint64_t v, hipStream_t st);
 hipError_t launch_md_front_batch(const BatchArrays &a, const BatchRep *reps, uint32_t R, uint32_t waves,
                                  const uint64_t *stepCtr, float dt, hipStream_t st);
 // the last kernel of a batched step: advances *stepCtr

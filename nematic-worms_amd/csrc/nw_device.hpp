@@ -172,8 +172,9 @@ __device__ __forceinline__ f3 position_relative_to_box(const SimParams &p, f3 q)
 // SRD cell of a position (calcGridPosSRD :241-249 + calcGridHashSRD :261-276).
 // Periodic axes wrap; a non-periodic axis is clamped into [0, G) (the
 // reference leaves it unwrapped, which can index past the cell table).
-__device__ __forceinline__ uint32_t srd_hash(const SimParams &p, const SolventParams &s, f3 q) {
-    const f3 shifted = q + mk(s.randOffset.x, s.randOffset.y, s.randOffset.z);
+// the grid shifted by `shift` (the collision's randOffset)
+__device__ __forceinline__ uint32_t srd_hash_at(const SimParams &p, const SolventParams &s, f3 shift, f3 q) {
+    const f3 shifted = q + shift;
     const f3 r = position_relative_to_box(p, shifted);
     int gx = floor_div(r.x, s.cellSize.x);
     int gy = floor_div(r.y, s.cellSize.y);
@@ -185,6 +186,10 @@ __device__ __forceinline__ uint32_t srd_hash(const SimParams &p, const SolventPa
     if (p.boundaryZ == PERIODIC) gz = wrap_mod(gz, s.gridSize.z);
     else gz = min(max(gz, 0), int(s.gridSize.z) - 1);
     return (uint32_t(gz) * s.gridSize.y) * s.gridSize.x + uint32_t(gy) * s.gridSize.x + uint32_t(gx);
+}
+
+__device__ __forceinline__ uint32_t srd_hash(const SimParams &p, const SolventParams &s, f3 q) {
+    return srd_hash_at(p, s, mk(s.randOffset.x, s.randOffset.y, s.randOffset.z), q);
 }
 
 // --------------------------------------------------------------------------

@@ -552,25 +552,6 @@ __global__ __launch_bounds__(kPairBlock) void k_collide_rows(float4 *__restrict_
 // the descriptor table, wave-uniform, so the replica's parameters arrive by
 // scalar loads).
 // ===========================================================================
-__device__ __forceinline__ uint32_t replica_of_wave(const BatchRep *__restrict__ reps, uint32_t R, uint32_t wave) {
-    uint32_t lo = 0, hi = R;
-    while (hi - lo > 1u) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (reps[mid].waveOffset <= wave) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ uint32_t replica_of_block(const BatchRep *__restrict__ reps, uint32_t R, uint32_t blk) {
-    uint32_t lo = 0, hi = R;
-    while (hi - lo > 1u) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (reps[mid].blockOffset <= blk) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
 
 __global__ __launch_bounds__(kBlock) void k_md_front_batch(BatchArrays a, const BatchRep *__restrict__ reps,
                                                            uint32_t R, uint32_t waves,
@@ -650,6 +631,16 @@ hipError_t launch_collide_off(float4 *force, const float4 *sPos, const float4 *s
         else NW_COLLIDE(k_collide_csr, 0, true);
     }
 #undef NW_COLLIDE
+    return hipGetLastError();
+}
+
+// the batch's device step counter (one lane stores)
+__global__ void k_set_step(uint64_t *__restrict__ ctr, uint64_t v) {
+    if (threadIdx.x == 0) ctr[0] = v;
+}
+
+hipError_t launch_set_step(uint64_t *ctr, uint64_t v, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, st, ctr, v);
     return hipGetLastError();
 }
 

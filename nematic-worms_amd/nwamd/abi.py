@@ -85,6 +85,10 @@ class NWReplicaParams(ctypes.Structure):
                 ("reserved", ctypes.c_uint32 * 5)]
 
 
+class NWReplicaSolvent(ctypes.Structure):
+    _fields_ = [("solvent", SolventParams), ("numSolvent", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+
+
 class NWObservables(ctypes.Structure):
     _fields_ = [("nematicS", ctypes.c_double), ("polarP", ctypes.c_double), ("bondS", ctypes.c_double),
                 ("polarSpeed", ctypes.c_double), ("meanV2", ctypes.c_double), ("beads", ctypes.c_double),
@@ -144,6 +148,12 @@ PROTOTYPES = {
     "nwAdjointBackward": (None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "nwAdjointForwardSrd": (None, [ctypes.c_void_p] * 5 + [c_uint, c_uint, c_float, c_uint,
                                                          ctypes.POINTER(c_float), c_uint, c_uint]),
+    "nwBatchSetSolvent": (None, [ctypes.POINTER(NWReplicaSolvent), c_uint]),
+    "nwBatchSolventLayout": (None, [ctypes.c_void_p, ctypes.c_void_p]),
+    "nwBatchAdjointForward": (None, [ctypes.c_void_p] * 5 + [c_float, c_uint, ctypes.POINTER(c_float), c_uint,
+                                                           c_uint]),
+    "nwBatchAdjointBackward": (None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "nwBatchPolarSpeedGrad": (None, [ctypes.c_void_p] * 4 + [ctypes.POINTER(ctypes.c_double)]),
     "nwBondOrderGrad": (ctypes.c_double, [ctypes.c_void_p, c_uint, ctypes.c_void_p]),
     "nwPolarSpeedGrad": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_void_p, c_uint, ctypes.c_void_p,
                                            ctypes.c_void_p]),
