@@ -211,6 +211,106 @@ def adjoint_form(args):
                 loss_S=round(S, 6), grad=dict(drive=g[0], kappa=g[1], epsilon=g[2], gamma=g[3]))
 
 
+def batch_adjoint_form(args):
+    """Batched replicas differentiated through SRD collisions: R replicas of
+    1,000 worms x 12 beads in periodic 120 x 120 boxes (WCA + bonds k 57 r0
+    0.8 + bending kappa 5 + drive f_a uniform in [0.5, 5] per replica, dt
+    1e-3, no Langevin thermostat: the solvent takes its place), each with SRD
+    solvent at 3 per unit cell (43,200 particles, Maxwellian kT 0.1, mass 1;
+    beads join the collisions), a collision every 10 steps (90 degrees,
+    random sign per cell, random grid shift per replica and collision).
+    `steps` batched steps with checkpoints every 50, loss = sum over replicas
+    of (polar speed - 0.5)^2, adjoint sweep with the collisions' cells and
+    signs held fixed, giving d loss / d f_a of every replica.  Starts from
+    rows of straight worms (lattice start)."""
+    import ctypes
+    import math
+
+    import numpy as np
+    from nwamd import abi, replicas, system
+    from nwamd.abi import DeviceArray
+    lib = abi.load()
+    R, steps, K, gap, dt, kT = args.batch_adjoint_replicas, args.batch_adjoint_steps, 50, 10, args.dt, 0.1
+    rng = np.random.default_rng(31337)
+    specs = (replicas.NWReplicaSpec * R)()
+    for r in range(R):
+        specs[r].numFilaments, specs[r].filamentSize, specs[r].boxSize = 1000, 12, 120.0
+        specs[r].driveForce = float(rng.uniform(0.5, 5.0))
+        specs[r].bendingK, specs[r].activity = 5.0, 0.0
+        specs[r].seed = int(rng.integers(1, 2 ** 62))
+    rs = replicas.ReplicaSet(specs, replicas.worm_physics(gamma=2.0, kT=kT, fdt=True))
+    rs.reset(system.NW_INIT_LATTICE_RANDOM, kT)
+    reps = (abi.NWReplicaParams * R)()
+    sol = (abi.NWReplicaSolvent * R)()
+    beads, bvel, spos, svel = [], [], [], []
+    for r in range(R):
+        p, e = rs.params(r)
+        e.srdCollideFilaments, e.srdThermostat = 1, abi.NW_SRD_THERMO_NONE
+        reps[r].params, reps[r].ext = p, e
+        reps[r].seed, reps[r].offset = rs.rng_state(r)
+        reps[r].langevin = 0
+        beads.append(rs.get(system.POSITION, r))
+        bvel.append(rs.get(system.VELOCITY, r))
+        sp = abi.SolventParams()
+        sp.enabled = True
+        sp.gridSize = abi.UInt3(120, 120, 1)
+        sp.numCells = 120 * 120
+        sp.cellSize = abi.Float3(1.0, 1.0, p.boxSize.z)
+        sp.alpha, sp.kbT, sp.particleMass = math.pi / 2, kT, 1.0
+        Ns = 3 * sp.numCells
+        sp.numParticles = Ns
+        sol[r].solvent, sol[r].numSolvent = sp, Ns
+        q = np.zeros((Ns, 4), np.float32)
+        q[:, :2] = rng.random((Ns, 2)) * 120.0 + p.origin.x
+        q[:, 2] = p.origin.z + 0.5 * p.boxSize.z
+        q[:, 3] = 0.5
+        v = np.zeros((Ns, 4), np.float32)
+        v[:, :2] = rng.normal(0.0, math.sqrt(kT), (Ns, 2))
+        v[:, 3] = 1.0
+        spos.append(q)
+        svel.append(v)
+    rs.close()
+    lib.nwBatchSetReplicas(reps, R)
+    lib.nwBatchSetSolvent(sol, R)
+    N = sum(len(b) for b in beads)
+    Nsol = sum(len(q) for q in spos)
+    P = DeviceArray.from_numpy(lib, np.concatenate(beads + spos))
+    V = DeviceArray.from_numpy(lib, np.concatenate(bvel + svel))
+    zero = np.zeros((N, 4), np.float32)
+    F, G, T = (DeviceArray.from_numpy(lib, zero) for _ in range(3))
+    dP, dV = DeviceArray(lib, 16 * N), DeviceArray(lib, 16 * N)
+    nColl = (steps + gap - 1) // gap
+    sh = np.zeros((nColl, R, 3), np.float32)
+    sh[:, :, :2] = rng.random((nColl, R, 2)) - 0.5
+    sh = np.ascontiguousarray(sh.reshape(-1))
+    shp = sh.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    g = (ctypes.c_double * (4 * R))()
+    Pr = (ctypes.c_double * R)()
+    # warm-up: a short differentiated run (allocations, first launches)
+    lib.nwBatchAdjointForward(P.p, V.p, F.p, G.p, T.p, dt, gap, shp, 2 * gap, gap)
+    lib.nwBatchPolarSpeedGrad(P.p, V.p, dP.p, dV.p, Pr)
+    lib.nwBatchAdjointBackward(dP.p, dV.p, g)
+    lib.threadSync()
+    t0 = time.perf_counter()
+    lib.nwBatchAdjointForward(P.p, V.p, F.p, G.p, T.p, dt, gap, shp, steps, K)
+    lib.threadSync()
+    t1 = time.perf_counter()
+    lib.nwBatchPolarSpeedGrad(P.p, V.p, dP.p, dV.p, Pr)
+    lib.nwBatchAdjointBackward(dP.p, dV.p, g)
+    t2 = time.perf_counter()
+    Pv = np.array(Pr[:R])
+    dfa = 2.0 * (Pv - 0.5) * np.array(g[:])[0::4]  # d loss / d f_a of every replica
+    return dict(workload=f"{R} replicas x 1000 worms x 12 beads, L 120, SRD 3 per cell", replicas=R, beads=N,
+                solvent=Nsol, steps=steps, checkpoint_every=K, srd_gap=gap,
+                forward_ms=round((t1 - t0) * 1e3, 2), backward_ms=round((t2 - t1) * 1e3, 2),
+                backward_over_forward=round((t2 - t1) / (t1 - t0), 3),
+                adjoint_over_forward=round((t2 - t0) / (t1 - t0), 3),
+                value=round(N * steps / (t2 - t0), 1), unit="bead-steps/s (forward + adjoint)",
+                loss=round(float(((Pv - 0.5) ** 2).sum()), 6),
+                dloss_dfa=dict(replica0=float(dfa[0]), replica1=float(dfa[1]), mean=float(dfa.mean()),
+                               min=float(dfa.min()), max=float(dfa.max())))
+
+
 def timers(lib, names, enable):
     for n in names:
         lib.nwTimerEnable(n.encode(), 1 if enable else 0)
@@ -251,6 +351,9 @@ def main():
                     help="timed steps of the replicas one launch sequence each (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=2000,
                     help="steps of the differentiated dry-worm run (0: skip)")
+    ap.add_argument("--batch-adjoint-steps", type=int, default=1000,
+                    help="steps of the batched run differentiated through SRD collisions (0: skip)")
+    ap.add_argument("--batch-adjoint-replicas", type=int, default=64)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="at the end, one elementwise kernel reading and writing this many MiB "
                          "(a known-size reference for HBM byte counters)")
@@ -335,6 +438,7 @@ def main():
 
     batched = batched_form(args) if args.batched_steps > 0 else None
     adjoint = adjoint_form(args) if args.adjoint_steps > 0 else None
+    batch_adjoint = batch_adjoint_form(args) if args.batch_adjoint_steps > 0 else None
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "bead-steps/s", "n_gpus": 1,
@@ -345,6 +449,7 @@ def main():
                    "srd_cells": w["Cs"], "dt": dt,
                    "form": "plain: one replica, ParticleSystem::update through the fused C-ABI entry points"},
         "roofline": roof, "cpu_baseline": cpu, "per_call_sequence": per_call, "batched": batched, "adjoint": adjoint,
+        "batched_adjoint_srd": batch_adjoint,
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)

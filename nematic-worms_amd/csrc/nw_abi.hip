@@ -1499,6 +1499,9 @@ void nwBatchAdjointBackward(const float *dLdPos, const float *dLdVel, double *gr
     float4 *F = A.work, *G = A.work + N, *T = A.work + 2 * N;
     const uint32_t nck = (A.steps + A.K - 1) / A.K;
     std::vector<int> collSlot(K);
+    // the step velocities are read only for the Langevin partials
+    bool lgv = false;
+    for (uint32_t r = 0; r < b.R; ++r) lgv = lgv || b.reps[r].langevin != 0;
     for (int c = int(nck) - 1; c >= 0; --c) {
         const uint32_t s0 = uint32_t(c) * A.K, len = std::min<uint32_t>(A.K, A.steps - s0);
         const float4 *ck = A.ckpt + uint64_t(c) * stride;
@@ -1524,7 +1527,9 @@ void nwBatchAdjointBackward(const float *dLdPos, const float *dLdVel, double *gr
             a.sPos = A.segSPos + uint64_t(j) * N;
             NW_HIP(batch_step_into(b, a, A.dt, s.stream));
             NW_HIP(hipMemcpyAsync(A.segPos + uint64_t(j + 1) * N, A.wPos, bytes, hipMemcpyDeviceToDevice, s.stream));
-            NW_HIP(hipMemcpyAsync(A.segVel + uint64_t(j + 1) * N, A.wVel, bytes, hipMemcpyDeviceToDevice, s.stream));
+            if (lgv)
+                NW_HIP(hipMemcpyAsync(A.segVel + uint64_t(j + 1) * N, A.wVel, bytes, hipMemcpyDeviceToDevice,
+                                      s.stream));
             collSlot[j] = -1;
             const uint32_t k = s0 + j;
             if (A.srd && k % A.gap == 0) {

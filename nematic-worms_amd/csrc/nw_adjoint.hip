@@ -212,12 +212,12 @@ __device__ __forceinline__ void adj_step_lanes(const AdjStepArgs &a, const KP &k
                         const float fp = -24.0f * eps * u * u * s6 * (14.0f * s6 - 4.0f);  // df / d(r^2)
                         const f3 dl = lam - mk(a.lF[other]);                                // lF_i - lF_j
                         Rp += w * (f * dl + (2.0f * fp * dot(d, dl)) * d);                  // w H dl
-                        epsPart += double(w) * double(dot(lam, d)) * double(-f) / double(eps);
+                        epsPart += double(w) * double(dot(lam, d)) * double(f);
                     }
                 }
             }
         }
-        acc[kAdjEpsilon] = epsPart;
+        acc[kAdjEpsilon] = -epsPart / double(eps);  // F is linear in epsilon
         (void)me;
 
         // adjoint of the step for this bead
