@@ -65,6 +65,9 @@ __device__ __forceinline__ d3 dmin_image(const SimParams &p, d3 a, d3 b) {
 __device__ __forceinline__ d3 dshfl_down(d3 a, int d) { return {shfl_down3(a.v, d), shfl_down3(a.t, d)}; }
 __device__ __forceinline__ d3 dshfl_up(d3 a, int d) { return {shfl_up3(a.v, d), shfl_up3(a.t, d)}; }
 
+// candidates of the adjoint's pair walk loaded together
+constexpr int kAdjBatch = 4;
+
 __device__ __forceinline__ double wave_sum_d(double v) {
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
     return v;
@@ -186,35 +189,74 @@ __device__ __forceinline__ void adj_step_lanes(const AdjStepArgs &a, const KP &k
         const uint32_t fil = m.worm;
         f3 Rp = mk(0.f, 0.f, 0.f);
         double epsPart = 0.0;
+        // the six candidate runs (two per stencil row, the row's three cells
+        // with the periodic wrap), flattened; candidates in groups of
+        // kAdjBatch with all their loads in flight together (every slot of a
+        // run holds a bead of this system, so a group past the last candidate
+        // reloads the last one and discards it)
+        uint32_t ps[6], pl[6];
+        float pw[6];
 #pragma unroll
         for (int yi = 0; yi < 3; ++yi) {
+            ps[2 * yi] = pl[2 * yi] = ps[2 * yi + 1] = pl[2 * yi + 1] = 0u;
+            pw[2 * yi] = pw[2 * yi + 1] = 0.0f;
             if (yi < ay.n) {
-                const float w = wz * ay.w[yi];
                 const uint32_t base = (uint32_t(gz) * Gy + uint32_t(ay.c[yi])) * Gx;
                 const bool mid = gx >= 1 && gx + 1 < int(Gx), left = gx == 0;
                 const uint32_t ia0 = mid ? base + gx - 1 : (left ? base + Gx - 1 : base + Gx - 2);
                 const uint32_t ia1 = mid ? base + gx + 2 : base + Gx;
                 const uint32_t ib1 = mid ? base : (left ? base + 2 : base + 1);
-                for (int part = 0; part < 2; ++part) {
-                    const uint32_t lo = part == 0 ? a.off[ia0] : a.off[base];
-                    const uint32_t hi = part == 0 ? a.off[ia1] : a.off[ib1];
-                    for (uint32_t j = lo; j < hi; ++j) {
-                        const uint32_t other = a.ids[j] - idBase;
-                        uint32_t fil2, ch2;
-                        worm_of(kp, other, fil2, ch2);
-                        if (fil2 == fil) continue;
-                        const f3 d = min_image_pair(p, rv, mk(a.sPos[j]));
-                        const float rr = dot(d, d);
-                        if (!(rr < rc2)) continue;
-                        const float u = __builtin_amdgcn_rcpf(rr);
-                        const float s6 = s2c * u * (s2c * u) * (s2c * u);
-                        const float f = 24.0f * eps * u * s6 * (2.0f * s6 - 1.0f);
-                        const float fp = -24.0f * eps * u * u * s6 * (14.0f * s6 - 4.0f);  // df / d(r^2)
-                        const f3 dl = lam - mk(a.lF[other]);                                // lF_i - lF_j
-                        Rp += w * (f * dl + (2.0f * fp * dot(d, dl)) * d);                  // w H dl
-                        epsPart += double(w) * double(dot(lam, d)) * double(f);
+                const uint32_t a0 = a.off[ia0], a1 = a.off[ia1], b0 = a.off[base], b1 = a.off[ib1];
+                ps[2 * yi] = a0;
+                pl[2 * yi] = a1 - a0;
+                ps[2 * yi + 1] = b0;
+                pl[2 * yi + 1] = b1 - b0;
+                pw[2 * yi] = pw[2 * yi + 1] = wz * ay.w[yi];
+            }
+        }
+        uint32_t pre[7];
+        pre[0] = 0u;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) pre[k + 1] = pre[k] + pl[k];
+        const uint32_t K = pre[6];
+        for (uint32_t t0 = 0; t0 < K; t0 += kAdjBatch) {
+            uint32_t I[kAdjBatch];
+            float4 Q[kAdjBatch], LF[kAdjBatch];
+            float W[kAdjBatch];
+#pragma unroll
+            for (int u = 0; u < kAdjBatch; ++u) {
+                const uint32_t t = min(t0 + uint32_t(u), K - 1u);
+                uint32_t j = 0u;
+                float w = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    if (t >= pre[k] && t < pre[k + 1]) {
+                        j = ps[k] + (t - pre[k]);
+                        w = pw[k];
                     }
                 }
+                I[u] = a.ids[j] - idBase;
+                Q[u] = a.sPos[j];
+                W[u] = w;
+            }
+#pragma unroll
+            for (int u = 0; u < kAdjBatch; ++u) LF[u] = a.lF[I[u]];
+#pragma unroll
+            for (int u = 0; u < kAdjBatch; ++u) {
+                if (t0 + uint32_t(u) >= K) continue;
+                uint32_t fil2, ch2;
+                worm_of(kp, I[u], fil2, ch2);
+                if (fil2 == fil) continue;
+                const f3 d = min_image_pair(p, rv, mk(Q[u]));
+                const float rr = dot(d, d);
+                if (!(rr < rc2)) continue;
+                const float uu = __builtin_amdgcn_rcpf(rr);
+                const float s6 = s2c * uu * (s2c * uu) * (s2c * uu);
+                const float f = 24.0f * eps * uu * s6 * (2.0f * s6 - 1.0f);
+                const float fp = -24.0f * eps * uu * uu * s6 * (14.0f * s6 - 4.0f);  // df / d(r^2)
+                const f3 dl = lam - mk(LF[u]);                                       // lF_i - lF_j
+                Rp += W[u] * (f * dl + (2.0f * fp * dot(d, dl)) * d);                // w H dl
+                epsPart += double(W[u]) * double(dot(lam, d)) * double(f);
             }
         }
         acc[kAdjEpsilon] = -epsPart / double(eps);  // F is linear in epsilon
