@@ -115,3 +115,119 @@ class _BondOrderFn(torch.autograd.Function):
 def bond_order(pos, run):
     """Bond-tangent nematic order S of the worms at `pos` (differentiable)."""
     return _BondOrderFn.apply(pos, run)
+
+
+# ---- batched replicas with SRD solvent -------------------------------------
+class ReplicaSrdRun:
+    """`steps` batched steps of R replicas with SRD solvent as a
+    differentiable function of every replica's drive f_a:
+
+        run = ReplicaSrdRun(reps, solvent, dt, gap, shifts, steps, checkpoint_every)
+        fa = torch.tensor([...], dtype=torch.float64, requires_grad=True)   # (R,)
+        pos, vel = run(fa, pos0, vel0)       # all beads, then every replica's solvent
+        P = polar_speed(pos, vel, run)       # (R,) polar speed of every replica
+        ((P - 0.5) ** 2).sum().backward()    # fa.grad = d loss / d f_a
+
+    reps: NWReplicaParams[R] (no Langevin thermostat), solvent:
+    NWReplicaSolvent[R], shifts: (collisions, R, 3) grid shifts.  The forward
+    pass runs nwBatchAdjointForward, the backward pass nwBatchAdjointBackward
+    (collisions' cells and signs held fixed); the loss may depend on the
+    beads' final positions and velocities, not on the solvent's.  The other
+    parameters' gradients of the last backward pass: `grad_theta` (R x 4:
+    f_a, kappa, epsilon, gamma)."""
+
+    def __init__(self, reps, solvent, dt, gap, shifts, steps, checkpoint_every=50):
+        self.reps, self.solvent = reps, solvent
+        self.R = len(reps)
+        self.dt, self.gap, self.steps, self.K = dt, gap, steps, checkpoint_every
+        self.shifts = torch.as_tensor(shifts, dtype=torch.float32).reshape(-1).contiguous()
+        self.bead_off = None
+        self.grad_theta = None
+
+    def set_batch(self, fa):
+        lib = abi.load()
+        R = self.R
+        reps = (abi.NWReplicaParams * R)()
+        for r in range(R):
+            reps[r] = self.reps[r]
+            reps[r].ext.driveForce = float(fa[r])
+        lib.nwBatchSetReplicas(reps, R)
+        sol = (abi.NWReplicaSolvent * R)(*self.solvent)
+        lib.nwBatchSetSolvent(sol, R)
+        off = (ctypes.c_uint * (R + 1))()
+        lib.nwBatchLayout(off, None)
+        self.bead_off = list(off)
+        return lib
+
+    def __call__(self, fa, pos0, vel0):
+        return _ReplicaSrdFn.apply(fa, pos0, vel0, self)
+
+
+class _ReplicaSrdFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, fa, pos0, vel0, run):
+        lib = run.set_batch(fa.detach().cpu())
+        N = run.bead_off[-1]
+        pos = pos0.detach().clone().contiguous()
+        vel = vel0.detach().clone().contiguous()
+        zeros = [torch.zeros((N, 4), dtype=torch.float32, device=pos.device) for _ in range(3)]
+        sh = run.shifts
+        with _Stream(lib):
+            lib.nwBatchAdjointForward(_ptr(pos), _ptr(vel), _ptr(zeros[0]), _ptr(zeros[1]), _ptr(zeros[2]), run.dt,
+                                      run.gap, ctypes.cast(sh.data_ptr(), ctypes.POINTER(ctypes.c_float)),
+                                      run.steps, run.K)
+        _last_run[0] += 1
+        ctx.run_id = _last_run[0]
+        ctx.run = run
+        return pos, vel
+
+    @staticmethod
+    def backward(ctx, grad_pos, grad_vel):
+        if ctx.run_id != _last_run[0]:
+            raise RuntimeError("ReplicaSrdRun: only the last forward run can be differentiated")
+        run = ctx.run
+        N = run.bead_off[-1]
+        for g in (grad_pos, grad_vel):
+            if g is not None and bool(torch.any(g[N:] != 0)):
+                raise RuntimeError("ReplicaSrdRun: the loss may not depend on the solvent's final state")
+        lib = abi.load()
+        gp = grad_pos[:N].detach().to(torch.float32).contiguous() if grad_pos is not None else None
+        gv = grad_vel[:N].detach().to(torch.float32).contiguous() if grad_vel is not None else None
+        if gp is None:
+            gp = torch.zeros((N, 4), dtype=torch.float32, device=grad_vel.device)
+        g = (ctypes.c_double * (4 * run.R))()
+        with _Stream(lib):
+            lib.nwBatchAdjointBackward(_ptr(gp), _ptr(gv) if gv is not None else None, g)
+        run.grad_theta = torch.tensor(list(g), dtype=torch.float64).reshape(run.R, 4)
+        return run.grad_theta[:, 0].clone(), None, None, None
+
+
+class _PolarSpeedFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, pos, vel, run):
+        lib = abi.load()
+        N = run.bead_off[-1]
+        pos, vel = pos.detach().contiguous(), vel.detach().contiguous()
+        gP = torch.zeros_like(pos)
+        gV = torch.zeros_like(vel)
+        P = (ctypes.c_double * run.R)()
+        with _Stream(lib):
+            lib.nwBatchPolarSpeedGrad(_ptr(pos), _ptr(vel), _ptr(gP), _ptr(gV), P)
+        counts = torch.tensor([run.bead_off[r + 1] - run.bead_off[r] for r in range(run.R)], device=pos.device)
+        ctx.save_for_backward(gP[:N], gV[:N], counts)
+        ctx.NT = pos.shape[0]
+        return torch.tensor(list(P), dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, g):
+        gP, gV, counts = ctx.saved_tensors
+        scale = torch.repeat_interleave(g.to(gP.device, torch.float32), counts)[:, None]
+        N = gP.shape[0]
+        pad = torch.zeros((ctx.NT - N, 4), dtype=torch.float32, device=gP.device)
+        return torch.cat([scale * gP, pad]), torch.cat([scale * gV, pad]), None
+
+
+def polar_speed(pos, vel, run):
+    """Polar speed <v . t> of every replica of `run` (differentiable in the
+    beads' positions and velocities)."""
+    return _PolarSpeedFn.apply(pos, vel, run)
