@@ -328,6 +328,18 @@ void nwAdjointForwardSrd(float *pos, float *vel, float *force, float *forceOld, 
    d loss / d theta.  The generator is left where the forward run left it.
    Synchronises. */
 void nwAdjointBackward(const float *dLdPos, const float *dLdVel, double grad[4]);
+/* One SRD collision of every replica of the batch (nwFusedSrdStep over all
+   replicas in one launch per kernel; no counterpart in the reference): pos /
+   vel hold all beads, then the solvent of nwBatchSetSolvent; shifts
+   (host, 3 per replica) the replicas' grid shifts; replica r draws its
+   rotation signs from its stream at offset + collision * ceil(cells / 4)
+   (cells: its SRD cells), as nwFusedSrdStep does after nwRngSetSeed(seed,
+   that offset).  solventCellCOM receives (u, sign) per SRD cell of the
+   concatenated cell space, gridParticleHash the cell of every particle,
+   gridParticleIndex the particles in cell order.  No isokinetic rescaling. */
+void nwBatchSrdStep(float *pos, float *vel, float *solventCellCOM, uint *gridParticleHash, uint *gridParticleIndex,
+                    float deltaTime, const float *shifts, uint collision);
+
 /* Differentiation through batched steps (no counterpart in the reference):
    `steps` batched MD steps of the current batch (WCA chain worms, no
    extensile term, no reversal, periodic boxes) over pos / vel (all beads,

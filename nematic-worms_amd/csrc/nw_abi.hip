@@ -108,6 +108,8 @@ struct State {
         float cosA = 0.f, sinA = 0.f;
         int rotBeads = 0;
         BucketScratch sort;  // the SRD cell sort of the whole batch
+        float *shift = nullptr;  // nwBatchSrdStep: the replicas' grid shifts on the device
+        uint64_t capShift = 0;
     } bsol;
     // adjoint of a batched run: checkpoints of the last nwBatchAdjointForward
     struct BatchAdjoint {
@@ -1358,7 +1360,7 @@ static bool batch_srd_collision(State &s, uint32_t c, float4 *P, float4 *V, uint
     const BucketPlan bp = make_bucket_plan(NT, q.cells, 0);
     NW_HIP_R(ensure_bucket_scratch(q.sort, bp), false);
     BucketScratch sc = q.sort;
-    sc.off = offOut;
+    if (offOut) sc.off = offOut;
     NW_HIP_R(launch_srd_hash_batch(P, hash, q.dev, q.R, b.N, NT, shift, s.stream), false);
     NW_HIP_R(launch_hash_tiles(hash, bp, q.sort, s.stream), false);
     NW_HIP_R(launch_srd_cells_batch(bp, sc, ids, V, com, q.dev, q.R, c, s.stream), false);
@@ -1366,6 +1368,24 @@ static bool batch_srd_collision(State &s, uint32_t c, float4 *P, float4 *V, uint
                                      s.stream),
              false);
     return true;
+}
+
+void nwBatchSrdStep(float *pos, float *vel, float *solventCellCOM, uint *gridParticleHash, uint *gridParticleIndex,
+                    float deltaTime, const float *shifts, uint collision) {
+    NW_INIT();
+    State &s = S();
+    State::Batch &b = s.batch;
+    State::BatchSolvent &q = s.bsol;
+    NW_REQUIRE(b.R > 0 && q.R == b.R, "nwBatchSrdStep: no batch solvent (nwBatchSetReplicas, nwBatchSetSolvent)");
+    NW_REQUIRE(pos && vel && solventCellCOM && gridParticleHash && gridParticleIndex && shifts,
+               "nwBatchSrdStep: NULL argument");
+    NW_REQUIRE(grow(q.shift, q.capShift, uint64_t(b.R) * 3), "nwBatchSrdStep: out of device memory");
+    // a synchronous copy: the previous collision's kernels may still read the buffer
+    NW_HIP(hipStreamSynchronize(s.stream));
+    NW_HIP(hipMemcpy(q.shift, shifts, size_t(b.R) * 3 * sizeof(float), hipMemcpyHostToDevice));
+    (void)batch_srd_collision(s, collision, reinterpret_cast<float4 *>(pos), reinterpret_cast<float4 *>(vel),
+                              gridParticleHash, gridParticleIndex, nullptr, reinterpret_cast<float4 *>(solventCellCOM),
+                              q.shift, deltaTime);
 }
 
 void nwBatchAdjointForward(float *pos, float *vel, float *force, float *forceOld, float *tangent, float deltaTime,

@@ -150,6 +150,7 @@ PROTOTYPES = {
                                                          ctypes.POINTER(c_float), c_uint, c_uint]),
     "nwBatchSetSolvent": (None, [ctypes.POINTER(NWReplicaSolvent), c_uint]),
     "nwBatchSolventLayout": (None, [ctypes.c_void_p, ctypes.c_void_p]),
+    "nwBatchSrdStep": (None, [ctypes.c_void_p] * 5 + [c_float, ctypes.POINTER(c_float), c_uint]),
     "nwBatchAdjointForward": (None, [ctypes.c_void_p] * 5 + [c_float, c_uint, ctypes.POINTER(c_float), c_uint,
                                                            c_uint]),
     "nwBatchAdjointBackward": (None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),

@@ -158,3 +158,58 @@ def test_batched_polar_speed_equals_every_replica_alone(nwlib):
         sl = slice(bo[k], bo[k + 1])
         assert np.array_equal(G[sl], ga.download(np.float32, x[3].shape))
         assert np.array_equal(GV[sl], gb.download(np.float32, x[3].shape))
+
+
+def test_batched_srd_step_equals_every_replica_alone(nwlib):
+    """nwBatchSrdStep (collision 2 of the replicas' streams) against
+    nwFusedSrdStep on each replica alone with the matching generator offset:
+    positions, velocities, cell momenta and signs, cells and cell order bit
+    for bit."""
+    lib = nwlib
+    R = len(THETAS)
+    ins = [replica_inputs(k, True) for k in range(R)]
+    reps = (abi.NWReplicaParams * R)()
+    sol = (abi.NWReplicaSolvent * R)()
+    for k, x in enumerate(ins):
+        reps[k].params, reps[k].ext = x[0], x[1]
+        reps[k].seed, reps[k].offset = SEEDS[k]
+        sol[k].solvent, sol[k].numSolvent = x[2], len(x[5])
+    lib.nwBatchSetReplicas(reps, R)
+    lib.nwBatchSetSolvent(sol, R)
+    bo, so, co = (np.zeros(R + 1, np.uint32) for _ in range(3))
+    lib.nwBatchLayout(bo.ctypes.data, None)
+    lib.nwBatchSolventLayout(so.ctypes.data, co.ctypes.data)
+    P = np.concatenate([x[3] for x in ins] + [x[5] for x in ins])
+    V = np.concatenate([x[4] for x in ins] + [x[6] for x in ins])
+    NT, C = len(P), int(co[-1])
+    shifts = np.ascontiguousarray(np.stack([shifts_of(k, 15, ins[k][2])[2] for k in range(R)]).reshape(-1))
+    dt = float(np.float32(case.DT) * np.float32(case.SRD_GAP))
+    collision = 2
+    dP, dV = DeviceArray.from_numpy(lib, P), DeviceArray.from_numpy(lib, V)
+    com, h, ids = DeviceArray(lib, 16 * C), DeviceArray(lib, 4 * NT), DeviceArray(lib, 4 * NT)
+    lib.nwBatchSrdStep(dP.p, dV.p, com.p, h.p, ids.p, dt, shifts.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                       collision)
+    bP, bV = dP.download(np.float32, P.shape), dV.download(np.float32, V.shape)
+    bC, bH, bI = com.download(np.float32, (C, 4)), h.download(np.uint32, (NT,)), ids.download(np.uint32, (NT,))
+    slot0 = 0
+    for k, x in enumerate(ins):
+        p, e, s = x[0], x[1], x[2]
+        N, Ns, Cs = len(x[3]), len(x[5]), int(co[k + 1] - co[k])
+        s.randOffset = abi.Float3(*(float(v) for v in shifts[3 * k:3 * k + 3]))
+        lib.setParameters(ctypes.byref(p))
+        lib.setSolventParameters(ctypes.byref(s))
+        lib.nwSetExtParameters(ctypes.byref(e))
+        lib.nwRngSetSeed(SEEDS[k][0], SEEDS[k][1] + collision * ((Cs + 3) // 4))
+        aP = DeviceArray.from_numpy(lib, np.concatenate([x[3], x[5]]))
+        aV = DeviceArray.from_numpy(lib, np.concatenate([x[4], x[6]]))
+        aC, aH, aI = DeviceArray(lib, 16 * Cs), DeviceArray(lib, 4 * (N + Ns)), DeviceArray(lib, 4 * (N + Ns))
+        lib.nwFusedSrdStep(aP.p, aV.p, aC.p, aH.p, aI.p, dt, N, Ns, 0)
+        rows = np.r_[bo[k]:bo[k + 1], so[k]:so[k + 1]]
+        assert np.array_equal(bP[rows].view(np.uint32), aP.download(np.float32, (N + Ns, 4)).view(np.uint32)), k
+        assert np.array_equal(bV[rows].view(np.uint32), aV.download(np.float32, (N + Ns, 4)).view(np.uint32)), k
+        assert np.array_equal(bC[co[k]:co[k + 1]].view(np.uint32), aC.download(np.float32, (Cs, 4)).view(np.uint32))
+        assert np.array_equal(bH[rows] - co[k], aH.download(np.uint32, (N + Ns,)))
+        g = bI[slot0:slot0 + N + Ns].astype(np.int64)
+        local = np.where(g < bo[-1], g - bo[k], N + g - so[k])
+        assert np.array_equal(local, aI.download(np.uint32, (N + Ns,)))
+        slot0 += N + Ns
