@@ -99,10 +99,10 @@ def measured_traffic(workload, family):
     return int(b) if b else None
 
 
-def build_workload(name, per_call=False):
+def build_workload(name, per_call=False, solvent_sort_every=0):
     from nwamd import abi, system
     if name == "worms_srd_800k":
-        cfg = system.config_srd_800k()
+        cfg = system.config_srd_800k(solvent_sort_every=solvent_sort_every)
     elif name == "worms_dry_4k":
         cfg = system.config_dry_4k()
     else:
@@ -354,6 +354,9 @@ def main():
     ap.add_argument("--batch-adjoint-steps", type=int, default=1000,
                     help="steps of the batched run differentiated through SRD collisions (0: skip)")
     ap.add_argument("--batch-adjoint-replicas", type=int, default=64)
+    ap.add_argument("--solvent-sort-every", type=int, default=20,
+                    help="collisions between re-sorts of the solvent by SRD cell on the device "
+                         "(the host driver's data layout; 0: the reference's particle order)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="at the end, one elementwise kernel reading and writing this many MiB "
                          "(a known-size reference for HBM byte counters)")
@@ -373,7 +376,7 @@ def main():
     if not lib.nwDeviceAvailable():
         raise SystemExit("bench.py: no HIP device")
     lib.cudaInit(0, None)
-    s, cfg, w = build_workload(args.workload)
+    s, cfg, w = build_workload(args.workload, solvent_sort_every=args.solvent_sort_every)
     dt = args.dt
     names = list(ALG_BYTES)
 
@@ -424,7 +427,7 @@ def main():
 
     per_call = None
     if args.per_call_steps > 0:
-        s2, _, _ = build_workload(args.workload, per_call=True)
+        s2, _, _ = build_workload(args.workload, per_call=True, solvent_sort_every=args.solvent_sort_every)
         s2.update(dt, args.warmup)
         s2.sync()
         t0 = time.perf_counter()
@@ -447,6 +450,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "beads": w["N"], "solvent": w["Ns"], "md_cells": w["C"],
                    "srd_cells": w["Cs"], "dt": dt,
+                   "solvent_sort_every": args.solvent_sort_every,
                    "form": "plain: one replica, ParticleSystem::update through the fused C-ABI entry points"},
         "roofline": roof, "cpu_baseline": cpu, "per_call_sequence": per_call, "batched": batched, "adjoint": adjoint,
         "batched_adjoint_srd": batch_adjoint,

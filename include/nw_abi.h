@@ -368,6 +368,16 @@ void nwBatchPolarSpeedGrad(const float *pos, const float *vel, float *dPdPos, fl
    Synchronises. */
 double nwBondOrderGrad(const float *pos, uint numFilaments, float *dSdPos);
 
+/* Reorders the numSolvent solvent particles that follow the numParticles
+   beads in pos / vel by their SRD cell (the grid of the last
+   setSolventParameters, unshifted), stably; solventId (device, numSolvent;
+   may be NULL) is permuted alike, so slot k keeps the original index of the
+   particle now in it.  SRD collisions are invariant under the order up to
+   the rounding of the cell sums; a cell-ordered solvent turns their random
+   accesses into near-sequential ones.  No counterpart in the reference (a
+   data-layout step of the host driver, include/nw_system.h). */
+void nwSortSolvent(float *pos, float *vel, uint *solventId, uint numParticles, uint numSolvent);
+
 /* Polar speed P = <v . t> of the worms (NWObservables.polarSpeed) and, when
    dPdPos is not NULL, dP/dpos and (dPdVel not NULL) dP/dvel (device float4
    per bead, w = 0).  Synchronises. */

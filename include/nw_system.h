@@ -35,7 +35,11 @@ typedef struct NWSystemConfig {
     NWExtParams ext;         /* physics variants; zero = reference */
     uint32_t perCallSequence;/* 1: issue the reference's per-call C-ABI sequence every step;
                                 0: the fused entry points (same results, fewer launches) */
-    uint32_t reserved[7];
+    uint32_t solventSortEvery;/* k > 0: keep the solvent in SRD-cell order on the device, re-sorted
+                                at reset and every k collisions (nwSortSolvent); arrays read or
+                                written through the system stay in original particle order.
+                                0: original order (the reference's, bit for bit) */
+    uint32_t reserved[6];
 } NWSystemConfig;
 
 /* initial conditions */
@@ -62,11 +66,13 @@ uint32_t nwSystemNumParticles(NWSystem *sys);
 uint32_t nwSystemNumSolvent(NWSystem *sys);
 uint32_t nwSystemNumCells(NWSystem *sys);     /* cells allocated for cellStart/cellEnd */
 uint64_t nwSystemStep(NWSystem *sys);
-/* host <-> device copies of whole arrays (float4 records):
-   0 pos (N+Ns), 1 vel (N+Ns), 2 tangent (N), 3 force (N), 4 forceOld (N) */
+/* host <-> device copies of whole arrays (float4 records), in original
+   particle order: 0 pos (N+Ns), 1 vel (N+Ns), 2 tangent (N), 3 force (N),
+   4 forceOld (N) */
 void nwSystemGetArray(NWSystem *sys, int which, float *host);
 void nwSystemSetArray(NWSystem *sys, int which, const float *host);
-/* device pointers of the same arrays */
+/* device pointers of the same arrays (with solventSortEvery > 0 the solvent
+   part is in the device's cell order) */
 float *nwSystemDeviceArray(NWSystem *sys, int which);
 /* RNG state (seed, generator offset) at this point of the run */
 void nwSystemRngState(NWSystem *sys, uint64_t *seed, uint64_t *offset);

@@ -109,6 +109,7 @@ ParticleSystem::ParticleSystem(const NWSystemConfig &cfg) {
     m_shiftCentred = cfg.srdShiftCentred != 0;
     m_langevinFdt = cfg.langevinFdt != 0;
     m_perCall = cfg.perCallSequence != 0;
+    m_solventSortEvery = cfg.srd ? cfg.solventSortEvery : 0u;
     _initSolventParams(&cfg);
     _initialize();
 }
@@ -142,6 +143,8 @@ void ParticleSystem::_initialize() {
     allocateArray((void **)&m_dCellStart, size_t(m_numCellsAlloc) * sizeof(uint));
     allocateArray((void **)&m_dCellEnd, size_t(m_numCellsAlloc) * sizeof(uint));
     if (m_solventEnabled) allocateArray((void **)&m_dSolventCellCOM, size_t(m_numSolventCells) * 16);
+    if (m_solventEnabled && m_numSolvent) allocateArray((void **)&m_dSolventId, size_t(m_numSolvent) * sizeof(uint));
+    _solventOrderReset();
 
     // the reference never clears force / forceOld before the first step
     _upload(m_dForce, m_hForce, N);
@@ -158,7 +161,7 @@ void ParticleSystem::_finalize() {
                    m_dSortedTangent, m_dSolventCellCOM};
     for (float *p : fa)
         if (p) freeArray(p);
-    uint *ua[] = {m_dGridParticleHash, m_dGridParticleIndex, m_dCellStart, m_dCellEnd};
+    uint *ua[] = {m_dGridParticleHash, m_dGridParticleIndex, m_dCellStart, m_dCellEnd, m_dSolventId};
     for (uint *p : ua)
         if (p) freeArray(p);
     m_bInitialized = false;
@@ -166,6 +169,28 @@ void ParticleSystem::_finalize() {
 
 void ParticleSystem::_upload(float *dev, const std::vector<float> &host, size_t count4) {
     if (count4) copyArrayToDevice(dev, host.data(), 0, int(count4 * 4 * sizeof(float)));
+}
+
+// The device solvent in original order again (after new positions and
+// velocities arrive in that order), then sorted when re-sorting is on.
+void ParticleSystem::_solventOrderReset() {
+    m_solventPermuted = false;
+    if (!m_dSolventId) return;
+    m_hSolventId.resize(m_numSolvent);
+    for (uint k = 0; k < m_numSolvent; ++k) m_hSolventId[k] = k;
+    copyArrayToDevice(m_dSolventId, m_hSolventId.data(), 0, int(m_numSolvent * sizeof(uint)));
+}
+
+void ParticleSystem::_sortSolvent() {
+    if (!m_dSolventId) return;
+    setParameters(&m_params);
+    setSolventParameters(&m_solvent);
+    nwSortSolvent(m_dPos, m_dVel, m_dSolventId, m_numParticles, m_numSolvent);
+    m_solventPermuted = true;
+}
+
+void ParticleSystem::_maybeSortSolvent() {
+    if (m_solventSortEvery && m_solventUpdateCount % m_solventSortEvery == 0) _sortSolvent();
 }
 
 // --------------------------------------------------------------------------
@@ -242,6 +267,7 @@ void ParticleSystem::_updateFused(float deltaTime) {
                        deltaTime * m_solventUpdateGap, m_numParticles, m_numSolvent,
                        m_ext.srdThermostat == NW_SRD_THERMO_ISOKINETIC ? 1 : 0);
         m_solventUpdateCount++;
+        _maybeSortSolvent();
     }
     m_updateCount++;
     uint64_t seed = 0;
@@ -276,6 +302,7 @@ void ParticleSystem::_solventUpdate(float dt) {
     collideSolvent(m_dPos, m_dVel, m_dSolventCellCOM, m_dRandom, m_numSolventCells, NT);
     integrateSolvent(m_dPos, m_dVel, dt, m_numParticles, m_numSolvent);
     m_solventUpdateCount++;
+    _maybeSortSolvent();
 }
 
 // --------------------------------------------------------------------------
@@ -344,6 +371,8 @@ void ParticleSystem::reset() {
     _upload(m_dPos, m_hPos, NT);
     _upload(m_dVel, m_hVel, NT);
     _upload(m_dTangent, m_hTangent, m_numParticles);
+    _solventOrderReset();
+    if (m_solventSortEvery) _sortSolvent();
 }
 
 void ParticleSystem::resetRandom(int mode, float kT) {
@@ -383,6 +412,8 @@ void ParticleSystem::resetRandom(int mode, float kT) {
     _upload(m_dTangent, m_hTangent, m_numParticles);
     _upload(m_dForce, m_hForce, m_numParticles);
     _upload(m_dForceOld, m_hForce, m_numParticles);
+    _solventOrderReset();
+    if (m_solventSortEvery) _sortSolvent();
 }
 
 // --------------------------------------------------------------------------
@@ -400,15 +431,37 @@ float *ParticleSystem::deviceArray(ParticleArray a) {
 }
 
 float *ParticleSystem::getArray(ParticleArray a) {
-    const size_t n = (a == POSITION || a == VELOCITY) ? size_t(m_numParticles) + m_numSolvent : m_numParticles;
+    const bool full = a == POSITION || a == VELOCITY;
+    const size_t n = full ? size_t(m_numParticles) + m_numSolvent : m_numParticles;
     std::vector<float> &h = a == POSITION ? m_hPos : a == VELOCITY ? m_hVel : a == TANGENT ? m_hTangent : m_hForce;
     if (h.size() < 4 * n) h.resize(4 * n);
     if (n) copyArrayFromDevice(h.data(), deviceArray(a), nullptr, int(n * 16));
+    if (full && m_solventPermuted) {
+        // slot k of the device solvent holds original particle m_hSolventId[k]
+        copyArrayFromDevice(m_hSolventId.data(), m_dSolventId, nullptr, int(m_numSolvent * sizeof(uint)));
+        const size_t N = m_numParticles;
+        std::vector<float> tmp(h.begin() + 4 * N, h.begin() + 4 * (N + m_numSolvent));
+        for (size_t k = 0; k < m_numSolvent; ++k)
+            std::copy(&tmp[4 * k], &tmp[4 * k] + 4, &h[4 * (N + m_hSolventId[k])]);
+    }
     return h.data();
 }
 
 void ParticleSystem::setArray(ParticleArray a, const float *data, int start, int count) {
     if (count <= 0) return;
+    const bool full = a == POSITION || a == VELOCITY;
+    if (full && m_solventPermuted && size_t(start) + size_t(count) > m_numParticles) {
+        // the whole array in original order, the new values in, then back in slot order
+        const float *g = getArray(a);
+        std::vector<float> cur(g, g + 4 * (size_t(m_numParticles) + m_numSolvent));
+        std::copy(data, data + 4 * size_t(count), cur.begin() + 4 * size_t(start));
+        const size_t N = m_numParticles;
+        std::vector<float> dev(cur.begin(), cur.end());
+        for (size_t k = 0; k < m_numSolvent; ++k)
+            std::copy(&cur[4 * (N + m_hSolventId[k])], &cur[4 * (N + m_hSolventId[k])] + 4, &dev[4 * (N + k)]);
+        copyArrayToDevice(deviceArray(a), dev.data(), 0, int(dev.size() * sizeof(float)));
+        return;
+    }
     copyArrayToDevice(deviceArray(a), data, start * 4 * int(sizeof(float)), count * 4 * int(sizeof(float)));
 }
 

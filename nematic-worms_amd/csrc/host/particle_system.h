@@ -87,6 +87,11 @@ private:
     void _finalize();
     float _frand();  // host uniform in [0, 1] (stands in for rand()/RAND_MAX)
     void _upload(float *dev, const std::vector<float> &host, size_t count4);
+    // the solvent kept in SRD-cell order (every m_solventSortEvery collisions);
+    // m_dSolventId maps each solvent slot to its original index
+    void _solventOrderReset();
+    void _maybeSortSolvent();
+    void _sortSolvent();
 
     bool m_bInitialized = false;
     uint64_t m_updateCount = 0;
@@ -113,6 +118,10 @@ private:
     bool m_shiftCentred = false;
     bool m_langevinFdt = false;
     bool m_perCall = false;     // reference per-call sequence instead of the fused entry points
+    uint m_solventSortEvery = 0;    // collisions between solvent re-sorts (0: never)
+    bool m_solventPermuted = false; // the device solvent is not in original order
+    uint *m_dSolventId = nullptr;
+    std::vector<uint> m_hSolventId;
 
     uint64_t m_seed = 0;
     uint64_t m_rngOffset = 0;   // device generator offset owned by this system

@@ -1,4 +1,5 @@
 // system_capi.cpp -- C entry points of libnwhost.so (include/nw_system.h).
+#include <cstring>
 #include <cstdio>
 #include <new>
 #include <stdexcept>
@@ -57,12 +58,13 @@ uint32_t nwSystemNumSolvent(NWSystem *sys) { return sys->ps->getNumSolvent(); }
 uint32_t nwSystemNumCells(NWSystem *sys) { return sys->ps->getNumCellsAlloc(); }
 uint64_t nwSystemStep(NWSystem *sys) { return sys->ps->getStep(); }
 
+// through the driver's accessors: arrays in original particle order
 void nwSystemGetArray(NWSystem *sys, int which, float *host) {
     const auto a = static_cast<ParticleSystem::ParticleArray>(which);
     const size_t n = (a == ParticleSystem::POSITION || a == ParticleSystem::VELOCITY)
                          ? size_t(sys->ps->getNumParticles()) + sys->ps->getNumSolvent()
                          : size_t(sys->ps->getNumParticles());
-    if (n) copyArrayFromDevice(host, sys->ps->deviceArray(a), nullptr, int(n * 16));
+    if (n) std::memcpy(host, sys->ps->getArray(a), n * 16);
 }
 
 void nwSystemSetArray(NWSystem *sys, int which, const float *host) {
@@ -70,7 +72,7 @@ void nwSystemSetArray(NWSystem *sys, int which, const float *host) {
     const size_t n = (a == ParticleSystem::POSITION || a == ParticleSystem::VELOCITY)
                          ? size_t(sys->ps->getNumParticles()) + sys->ps->getNumSolvent()
                          : size_t(sys->ps->getNumParticles());
-    if (n) copyArrayToDevice(sys->ps->deviceArray(a), host, 0, int(n * 16));
+    if (n) sys->ps->setArray(a, host, 0, int(n));
 }
 
 float *nwSystemDeviceArray(NWSystem *sys, int which) {

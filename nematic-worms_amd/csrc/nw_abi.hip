@@ -128,6 +128,10 @@ struct State {
                  capWPos = 0, capWVel = 0, capCollCom = 0, capCellSum = 0, capIds = 0, capOff = 0, capHash = 0,
                  capSCell = 0, capCollHash = 0, capCollIds = 0, capCollOff = 0, capPartial = 0, capGrad = 0;
     } badj;
+    // nwSortSolvent: keys, order, and the reordered solvent before it is copied back
+    uint32_t *solKey = nullptr, *solIdx = nullptr, *solId = nullptr;
+    float4 *solPos = nullptr, *solVel = nullptr;
+    uint64_t capSolKey = 0, capSolIdx = 0, capSolId = 0, capSolPos = 0, capSolVel = 0;
     std::map<std::string, Timer> timers;
 };
 
@@ -1668,6 +1672,36 @@ double nwBondOrderGrad(const float *pos, uint numFilaments, float *dSdPos) {
         NW_HIP_R(hipStreamSynchronize(s.stream), S);
     }
     return S;
+}
+
+void nwSortSolvent(float *pos, float *vel, uint *solventId, uint numParticles, uint numSolvent) {
+    NW_INIT();
+    State &s = S();
+    NW_REQUIRE(pos && vel, "nwSortSolvent: NULL positions or velocities");
+    if (numSolvent == 0) return;
+    const KP kp = make_kp();
+    const uint64_t cells = uint64_t(kp.s.gridSize.x) * kp.s.gridSize.y * kp.s.gridSize.z;
+    NW_REQUIRE(cells > 0 && cells == kp.s.numCells, "nwSortSolvent: SRD grid does not match numCells");
+    NW_REQUIRE(uint64_t(numParticles) + numSolvent < (1ull << 31), "nwSortSolvent: too many particles");
+    const uint32_t N = numParticles, Ns = numSolvent;
+    NW_REQUIRE(grow(s.solKey, s.capSolKey, Ns) && grow(s.solIdx, s.capSolIdx, Ns) && grow(s.solId, s.capSolId, Ns) &&
+                   grow(s.solPos, s.capSolPos, Ns) && grow(s.solVel, s.capSolVel, Ns),
+               "nwSortSolvent: out of device memory");
+    float4 *P = reinterpret_cast<float4 *>(pos), *V = reinterpret_cast<float4 *>(vel);
+    NW_HIP(launch_solvent_keys(P, N, Ns, s.solKey, s.solIdx, kp, s.stream));
+    // stable by cell: solIdx becomes the new order
+    const hipError_t e = bucket_sort_pairs(s.solKey, s.solIdx, Ns, uint32_t(cells), s.bucket, s.stream);
+    if (e == hipErrorNotSupported)
+        NW_HIP(counting_sort_pairs(s.solKey, s.solIdx, Ns, uint32_t(cells), s.sort, s.stream));
+    else
+        NW_HIP(e);
+    NW_HIP(launch_solvent_gather(P, V, solventId, s.solIdx, N, Ns, s.solPos, s.solVel, s.solId, s.stream));
+    const size_t b4 = size_t(Ns) * sizeof(float4);
+    NW_HIP(hipMemcpyAsync(P + N, s.solPos, b4, hipMemcpyDeviceToDevice, s.stream));
+    NW_HIP(hipMemcpyAsync(V + N, s.solVel, b4, hipMemcpyDeviceToDevice, s.stream));
+    if (solventId)
+        NW_HIP(hipMemcpyAsync(solventId, s.solId, size_t(Ns) * 4, hipMemcpyDeviceToDevice, s.stream));
+    s.lastHashPtr = nullptr;
 }
 
 double nwPolarSpeedGrad(const float *pos, const float *vel, uint numFilaments, float *dPdPos, float *dPdVel) {

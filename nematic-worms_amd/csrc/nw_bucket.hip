@@ -890,6 +890,44 @@ hipError_t launch_md_cells_batch(const SortRep *reps, uint32_t R, const BatchSor
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(kBlock) void k_solvent_keys(const float4 *__restrict__ pos, uint32_t N, uint32_t Ns,
+                                                         uint32_t *__restrict__ key, uint32_t *__restrict__ idx,
+                                                         KP kp) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Ns) return;
+    key[i] = srd_hash_at(kp.p, kp.s, mk(0.f, 0.f, 0.f), mk(pos[N + i]));
+    idx[i] = i;
+}
+
+__global__ __launch_bounds__(kBlock) void k_solvent_gather(const float4 *__restrict__ pos,
+                                                           const float4 *__restrict__ vel,
+                                                           const uint32_t *__restrict__ id,
+                                                           const uint32_t *__restrict__ perm, uint32_t N, uint32_t Ns,
+                                                           float4 *__restrict__ tp, float4 *__restrict__ tv,
+                                                           uint32_t *__restrict__ tid) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= Ns) return;
+    const uint32_t j = perm[k];
+    tp[k] = pos[N + j];
+    tv[k] = vel[N + j];
+    tid[k] = id ? id[j] : j;
+}
+
+hipError_t launch_solvent_keys(const float4 *pos, uint32_t N, uint32_t Ns, uint32_t *key, uint32_t *idx,
+                               const KP &kp, hipStream_t st) {
+    if (Ns == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_solvent_keys, dim3(grid_for(Ns)), dim3(kBlock), 0, st, pos, N, Ns, key, idx, kp);
+    return hipGetLastError();
+}
+
+hipError_t launch_solvent_gather(const float4 *pos, const float4 *vel, const uint32_t *id, const uint32_t *perm,
+                                 uint32_t N, uint32_t Ns, float4 *tp, float4 *tv, uint32_t *tid, hipStream_t st) {
+    if (Ns == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_solvent_gather, dim3(grid_for(Ns)), dim3(kBlock), 0, st, pos, vel, id, perm, N, Ns, tp, tv,
+                       tid);
+    return hipGetLastError();
+}
+
 // keys[k] = cell of slot k, vals[k] = the value of the particle in slot k
 __global__ __launch_bounds__(kBlock) void k_sort_finish(uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                                         const uint32_t *__restrict__ pos,

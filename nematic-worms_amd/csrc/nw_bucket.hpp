@@ -81,6 +81,14 @@ hipError_t bucket_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t n, uint32_
                              hipStream_t st);
 // level 2 for MD: ids in stable cell order, CSR offsets, cell of every
 // slot, gathered records (sVel / sTan may be NULL)
+// the solvent's SRD cells on the unshifted grid (key) and indices (idx) of
+// the Ns solvent particles after the N beads
+hipError_t launch_solvent_keys(const float4 *pos, uint32_t N, uint32_t Ns, uint32_t *key, uint32_t *idx,
+                               const KP &kp, hipStream_t st);
+// tp / tv / tid[k] = the solvent particle perm[k]'s position, velocity and
+// id (id NULL: perm[k] itself)
+hipError_t launch_solvent_gather(const float4 *pos, const float4 *vel, const uint32_t *id, const uint32_t *perm,
+                                 uint32_t N, uint32_t Ns, float4 *tp, float4 *tv, uint32_t *tid, hipStream_t st);
 hipError_t launch_md_cells(const BucketPlan &bp, BucketScratch &s, uint32_t *ids, const float4 *pos,
                            const float4 *vel, const float4 *tan, float4 *sPos, float4 *sVel, float4 *sTan,
                            hipStream_t st);

@@ -156,6 +156,7 @@ PROTOTYPES = {
     "nwBatchAdjointBackward": (None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "nwBatchPolarSpeedGrad": (None, [ctypes.c_void_p] * 4 + [ctypes.POINTER(ctypes.c_double)]),
     "nwBondOrderGrad": (ctypes.c_double, [ctypes.c_void_p, c_uint, ctypes.c_void_p]),
+    "nwSortSolvent": (None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_uint, c_uint]),
     "nwPolarSpeedGrad": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_void_p, c_uint, ctypes.c_void_p,
                                            ctypes.c_void_p]),
     "nwSymEig3Max": (ctypes.c_double, [ctypes.POINTER(ctypes.c_double)]),

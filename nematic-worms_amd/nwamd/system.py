@@ -22,7 +22,8 @@ class NWSystemConfig(ctypes.Structure):
         ("srdGridSize", abi.UInt3), ("srdCellSize", ctypes.c_float), ("srdDensity", ctypes.c_float),
         ("srdGap", ctypes.c_uint32), ("srdAlpha", ctypes.c_float), ("srdShiftCentred", ctypes.c_uint32),
         ("langevinFdt", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("ext", abi.NWExtParams),
-        ("perCallSequence", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 7),
+        ("perCallSequence", ctypes.c_uint32), ("solventSortEvery", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 6),
     ]
 
 
@@ -206,12 +207,13 @@ def config_dry_4k(seed=1337, cells=64):
     return c
 
 
-def config_srd_800k(seed=1337, md_cells=890):
+def config_srd_800k(seed=1337, md_cells=890, solvent_sort_every=0):
     """40,000 worms x 20 beads in 2D L = 1000 with 5,000,000 SRD particles
     (5 per unit cell, 1000^2 cells of size 1), collision every 10 steps,
     rotation 90 deg with random sign per cell, grid shift in [-0.5, 0.5),
     beads in the collision with mass 1; MD grid md_cells^2 (cell 1000/md_cells
-    >= r_c = 2^(1/6))."""
+    >= r_c = 2^(1/6)); the solvent kept in cell order on the device
+    (re-sorted every solvent_sort_every collisions) when that is > 0."""
     assert 1000.0 / md_cells >= 2.0 ** (1.0 / 6.0)
     c = NWSystemConfig()
     c.numFilaments, c.filamentSize = 40_000, 20
@@ -226,4 +228,5 @@ def config_srd_800k(seed=1337, md_cells=890):
     c.srdShiftCentred = 1
     c.seed = seed
     c.ext = wca_ext(stencil=abi.NW_STENCIL_2D, srd_beads=1)
+    c.solventSortEvery = solvent_sort_every
     return c

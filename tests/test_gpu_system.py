@@ -89,11 +89,13 @@ def test_reference_2d_grid_with_srd(lib, per_call):
     compare(s, cpu, 1e-4, 2e-3)
 
 
-@pytest.mark.parametrize("per_call", [0, 1])
-def test_worm_srd_workload_small_matches_oracle(lib, per_call):
+@pytest.mark.parametrize("per_call,sort_every", [(0, 0), (1, 0), (0, 1), (1, 2)])
+def test_worm_srd_workload_small_matches_oracle(lib, per_call, sort_every):
     """The benchmark physics (WCA, chain bonds, bending, drive, SRD with beads,
-    90 deg, centred shift) on a 100 x 100 box with a non-power-of-two grid."""
-    cfg = system.config_srd_800k()
+    90 deg, centred shift) on a 100 x 100 box with a non-power-of-two grid;
+    sort_every > 0: the solvent kept in cell order on the device (the oracle
+    keeps the original order; only the cell sums' rounding differs)."""
+    cfg = system.config_srd_800k(solvent_sort_every=sort_every)
     cfg.numFilaments, cfg.filamentSize = 400, 20
     cfg.gridSize = abi.UInt3(89, 89, 1)
     cfg.boxSize = abi.Float3(100.0, 100.0, 100.0 / 89)
@@ -153,3 +155,60 @@ def test_cli_runs(tmp_path):
     assert "Throughput" in r.stdout
     frames = [ln for ln in open(out).read().splitlines() if ln == "500"]
     assert len(frames) == 3
+
+
+def test_sort_solvent_abi(lib):
+    """nwSortSolvent: the solvent in stable SRD-cell order (unshifted grid),
+    beads untouched, the id map consistent with the moved particles."""
+    import ctypes
+    from nwamd.abi import DeviceArray
+    from nwtest import sim_params, solvent_params
+    rng = np.random.default_rng(3)
+    p = sim_params(grid=(20, 20, 1), cell=1.0, W=10, n=5)
+    s = solvent_params(p, (20, 20, 1), (1.0, 1.0, 1.0), offset=(0.3, -0.2, 0.0), ns=4000)
+    lib.setParameters(ctypes.byref(p))
+    lib.setSolventParameters(ctypes.byref(s))
+    N, Ns = 50, 4000
+    P = np.zeros((N + Ns, 4), np.float32)
+    P[:, :2] = rng.random((N + Ns, 2)) * 20 - 10
+    V = rng.normal(0, 1, (N + Ns, 4)).astype(np.float32)
+    dP, dV = DeviceArray.from_numpy(lib, P), DeviceArray.from_numpy(lib, V)
+    ids = DeviceArray.from_numpy(lib, np.arange(Ns, dtype=np.uint32)[::-1].copy())  # an earlier permutation
+    lib.nwSortSolvent(dP.p, dV.p, ids.p, N, Ns)
+    gP, gV, gI = dP.download(np.float32, P.shape), dV.download(np.float32, V.shape), ids.download(np.uint32, (Ns,))
+    assert np.array_equal(gP[:N], P[:N]) and np.array_equal(gV[:N], V[:N])
+    s0 = solvent_params(p, (20, 20, 1), (1.0, 1.0, 1.0), ns=Ns)  # cells on the unshifted grid
+    h, _ = orc.calc_hash(p, s0, P[N:], srd=True)
+    order = np.argsort(h, kind="stable")
+    assert np.array_equal(gP[N:], P[N:][order]) and np.array_equal(gV[N:], V[N:][order])
+    assert np.array_equal(gI, (np.arange(Ns, dtype=np.uint32)[::-1])[order])
+
+
+def test_sorted_solvent_arrays_in_original_order(lib):
+    """With the solvent kept in cell order, get/set through the system still
+    use the original particle order."""
+    cfg = system.config_srd_800k(solvent_sort_every=1)
+    cfg.numFilaments, cfg.filamentSize = 40, 10
+    cfg.gridSize = abi.UInt3(26, 26, 1)
+    cfg.boxSize = abi.Float3(30.0, 30.0, 30.0 / 26)
+    cfg.srdGridSize = abi.UInt3(30, 30, 1)
+    s = system.System(cfg)
+    system.apply_worm_physics(s, kappa=5.0, gamma=2.0, kT=0.1)
+    s.reset(system.NW_INIT_LATTICE_RANDOM, 0.1)
+    P = s.get(system.POSITION)
+    V = s.get(system.VELOCITY)
+    N = s.n
+    # original order: the solvent in the order the reset drew it, not sorted by cell
+    cells = np.floor(P[N:, 1] + 15) * 30 + np.floor(P[N:, 0] + 15)
+    assert not np.all(np.diff(cells) >= 0)
+    P2 = P.copy()
+    P2[N + 7, :2] = (1.25, -3.5)
+    s.set(system.POSITION, P2)
+    assert np.array_equal(s.get(system.POSITION), P2)
+    assert np.array_equal(s.get(system.VELOCITY), V)
+    s.update(1e-3, 25)  # collisions (and re-sorts) at steps 0, 10, 20
+    P3 = s.get(system.POSITION)
+    assert np.all(np.isfinite(P3))
+    # beads keep their slots; the solvent's identities follow their particles
+    assert np.abs(P3[N + 7, :2] - P2[N + 7, :2]).max() < 0.5
+    s.close()
