@@ -71,6 +71,8 @@ def bucket_plan(n, cells, tile):
         want, tile = n * (B + 1) / float(1 << 21), 4096
         while tile < (1 << 17) and tile < want:
             tile <<= 1
+        if tile == 16384:  # two workgroups per CU instead of one
+            tile = 8192
     return B, (n + tile - 1) // tile
 
 
@@ -83,7 +85,7 @@ SINGLE_KERNEL = {"integrate", "langevin", "filament", "hash", "collide", "srd_co
 # kernel behind each single-kernel family of the fused path (the names that
 # rocprofv3 reports), for the HBM traffic measured by its counters
 FAMILY_KERNEL = {"md_front": "k_md_front", "md_tiles": "k_tiles_staged<false, 4>", "md_sort": "k_bucket_cells<false>",
-                 "md_collide": "k_collide_rows<1, false, true>", "srdf_tiles": "k_tiles_staged<true, 16>",
+                 "md_collide": "k_collide_rows<1, false, true>", "srdf_tiles": "k_tiles_staged<true, 8>",
                  "srdf_cells": "k_bucket_cells<true>", "srdf_rotate": "k_srd_rotate"}
 
 

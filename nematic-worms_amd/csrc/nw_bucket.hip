@@ -51,6 +51,9 @@ BucketPlan make_bucket_plan(uint64_t n, uint64_t cells, uint32_t tileItems, uint
         const double want = double(n) * double(b.buckets + 1) / double(1u << 21);
         uint32_t T = 4096;
         while (T < (1u << 17) && double(T) < want) T <<= 1;
+        // a 16384-particle tile stages 128 KiB of pairs: one workgroup per CU;
+        // 8192 fits two (SRD level 1 57 -> 41 us for a level 2 117 -> 123 us)
+        if (T == 16384) T = 8192;
         tileItems = T;
     }
     b.tileItems = tileItems;
@@ -749,6 +752,9 @@ static hipError_t launch_tiles(bool fromPos, const float4 *pos, uint32_t *hash, 
         if (bp.tileItems == 4 * kTileBlock)
             return fromPos ? launch_staged<true, 4>(pos, hash, bp, s, kp, st)
                            : launch_staged<false, 4>(pos, hash, bp, s, kp, st);
+        if (bp.tileItems == 8 * kTileBlock)
+            return fromPos ? launch_staged<true, 8>(pos, hash, bp, s, kp, st)
+                           : launch_staged<false, 8>(pos, hash, bp, s, kp, st);
         if (bp.tileItems == 16 * kTileBlock)
             return fromPos ? launch_staged<true, 16>(pos, hash, bp, s, kp, st)
                            : launch_staged<false, 16>(pos, hash, bp, s, kp, st);
