@@ -218,8 +218,8 @@ def batch_adjoint_form(args):
     1,000 worms x 12 beads in periodic 120 x 120 boxes (WCA + bonds k 57 r0
     0.8 + bending kappa 5 + drive f_a uniform in [0.5, 5] per replica, dt
     1e-3, no Langevin thermostat: the solvent takes its place), each with SRD
-    solvent at 3 per unit cell (43,200 particles, Maxwellian kT 0.1, mass 1;
-    beads join the collisions), a collision every 10 steps (90 degrees,
+    solvent at 3 per unit cell (43,200 particles at uniform positions, ids in
+    cell order, Maxwellian kT 0.1, mass 1; beads join the collisions), a collision every 10 steps (90 degrees,
     random sign per cell, random grid shift per replica and collision).
     `steps` batched steps with checkpoints every 50, loss = sum over replicas
     of (polar speed - 0.5)^2, adjoint sweep with the collisions' cells and
@@ -266,6 +266,10 @@ def batch_adjoint_form(args):
         q[:, :2] = rng.random((Ns, 2)) * 120.0 + p.origin.x
         q[:, 2] = p.origin.z + 0.5 * p.boxSize.z
         q[:, 3] = 0.5
+        # ids in SRD-cell order (the order the host driver's solvent re-sort
+        # keeps); 1,000 steps move the solvent ~0.03 cells, so it stays so
+        cell = np.floor(q[:, 1] - p.origin.y) * 120 + np.floor(q[:, 0] - p.origin.x)
+        q = q[np.argsort(cell, kind="stable")]
         v = np.zeros((Ns, 4), np.float32)
         v[:, :2] = rng.normal(0.0, math.sqrt(kT), (Ns, 2))
         v[:, 3] = 1.0
