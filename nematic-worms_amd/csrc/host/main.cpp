@@ -2,7 +2,9 @@
 // same flags (-name=value, names matched case-insensitively), defaults, step
 // loop and .xyzv output.  Throughput is printed per elapsed iteration (the
 // reference divides by the total iteration count, main.cpp:48).  Extra flags:
-// -seed=N, -wca (WCA excluded volume + chain bonds + drive), -drive=F.
+// -seed=N, -wca (WCA excluded volume + chain bonds + drive), -drive=F,
+// -solventSort=K (keep the SRD solvent in cell order on the device, re-sorted
+// every K collisions; frames stay in original particle order).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -102,6 +104,7 @@ int main(int argc, char *argv[]) {
     cfg.gridSize = gridSize;
     cfg.srd = srdSolvent;
     cfg.seed = seed;
+    if (has_flag(argc, argv, "solventSort")) cfg.solventSortEvery = uint32_t(flag_int(argc, argv, "solventSort"));
     if (has_flag(argc, argv, "wca")) {
         cfg.ext.pairPotential = NW_PAIR_WCA;
         cfg.ext.bondedMode = NW_BONDED_CHAIN;

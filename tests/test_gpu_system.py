@@ -157,6 +157,24 @@ def test_cli_runs(tmp_path):
     assert len(frames) == 3
 
 
+def test_cli_solvent_sort_keeps_frame_order(tmp_path):
+    """-srd with -solventSort=1: the device solvent is re-sorted at reset and
+    every collision, yet the first frame equals the unsorted run's byte for
+    byte (frames are written in original particle order)."""
+    exe = os.path.join(ROOT, "nematic-worms_amd", "nwamd", "nw")
+    frames = []
+    for extra in ([], ["-solventSort=1"]):
+        out = str(tmp_path / f"srd{len(extra)}.xyzv")
+        r = subprocess.run([exe, "-iterations=10", "-numFilaments=20", "-filamentSize=10", "-gridX=32", "-gridY=32",
+                            "-srd", "-printRate=10", f"-o={out}"] + extra, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        lines = open(out).read().splitlines()
+        n = int(lines[0])
+        frames.append((lines[:n + 2], lines))
+    assert frames[0][0] == frames[1][0]
+    assert len(frames[0][1]) == len(frames[1][1])
+
+
 def test_sort_solvent_abi(lib):
     """nwSortSolvent: the solvent in stable SRD-cell order (unshifted grid),
     beads untouched, the id map consistent with the moved particles."""
