@@ -70,7 +70,7 @@ const char *bucket_plan_problem(const BucketPlan &bp) {
 
 // level-2 LDS layout (bytes), shared by host (size) and device (carving)
 struct CellsLayout {
-    uint32_t wsum, cnt, cur, lid, lcell, segStart, segBase, tmpId, tmpCell, lv, total;
+    uint32_t wsum, cnt, cur, lid, lcell, segStart, segBase, tmpId, tmpCell, tileOf, lv, total;
 };
 
 __host__ __device__ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
@@ -91,6 +91,7 @@ __host__ __device__ inline CellsLayout cells_layout(const BucketPlan &bp, bool s
     L.segBase = o;  o += align16(4u * bp.tiles);
     L.tmpId = o;    o += align16(4u * cap);
     L.tmpCell = o;  o += align16(2u * cap);
+    L.tileOf = L.lid;  // phase A only: lid is first written after it
     L.lv = a0;
     if (srd && o < a0 + 16u * cap) o = a0 + 16u * cap;
     L.total = o;
@@ -354,6 +355,7 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
     uint32_t *segBase = reinterpret_cast<uint32_t *>(smem + L.segBase);
     uint32_t *tmpId = reinterpret_cast<uint32_t *>(smem + L.tmpId);
     uint16_t *tmpCell = reinterpret_cast<uint16_t *>(smem + L.tmpCell);
+    uint16_t *tileOf = reinterpret_cast<uint16_t *>(smem + L.tileOf);
     float4 *lv = reinterpret_cast<float4 *>(smem + L.lv);
 
     const uint32_t B1 = bp.buckets + 1;
@@ -383,13 +385,17 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
     const bool staged = m <= uint32_t(CAP);
     const uint64_t T = bp.tileItems;
     if (staged && m) {
+        // the tile of every member, one tile's segment per thread at a time
+        for (uint32_t t = tid; t < bp.tiles; t += bd)
+            for (uint32_t k = segStart[t]; k < segStart[t + 1]; ++k) tileOf[k] = uint16_t(t);
+        __syncthreads();
         // every load of the thread's members in flight before the first use
         // (indices past the last member load the last member again, unused)
         uint2 q[PER];
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
             const uint32_t k = min(tid + uint32_t(j) * BLOCK, m - 1u);
-            const uint32_t t = seg_of(segStart, bp.tiles, k);
+            const uint32_t t = tileOf[k];
             q[j] = o.pairs[uint64_t(t) * T + segBase[t] + (k - segStart[t])];
         }
 #pragma unroll
