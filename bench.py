@@ -48,7 +48,9 @@ ALG_BYTES = {
     "srd_stream": lambda w: 64 * w["Ns"],                      # RMW pos vel of the solvent
     # fused path (nwFusedMdStep / nwFusedSrdStep); Pm / Ps = bytes of the
     # sort's tile x bucket prefix matrix (written by the tile pass, read once)
-    "md_front": lambda w: 148 * w["N"],                        # R pos vel f f_old, W + tangent, W cell
+    # R pos vel f f_old, W pos vel f (+ f_old unless the host driver swaps
+    # the force buffers: no reversal), W tangent, W cell
+    "md_front": lambda w: (132 if w["swap"] else 148) * w["N"],
     "md_tiles": lambda w: 12 * w["N"] + w["Pm"],               # R cell, W pair
     "md_sort": lambda w: 48 * w["N"] + 4 * w["C"] + w["Pm"],   # R pairs, W ids cells, gather pos, W offsets
     "md_collide": lambda w: (56 if w["wca"] else 88) * w["N"] + 4 * w["C"],  # R sPos sIdx sCell (+sVel sTan), RMW f
@@ -115,7 +117,7 @@ def build_workload(name, per_call=False, solvent_sort_every=0):
     s.reset(system.NW_INIT_LATTICE_RANDOM if cfg.srd else system.NW_INIT_RANDOM, 0.1)
     p = s.params
     w = dict(N=s.n, Ns=s.ns, NT=s.n + s.ns, C=p.numCells, Cs=s.solvent.numCells if cfg.srd else 0,
-             wca=cfg.ext.pairPotential == 1)
+             wca=cfg.ext.pairPotential == 1, swap=p.reverseProbability == 0.0)
     rot_beads = bool(cfg.ext.srdCollideFilaments) or cfg.ext.srdThermostat == abi.NW_SRD_THERMO_ISOKINETIC
     w["Nrot"] = w["NT"] if rot_beads else w["Ns"]
     B, tiles = bucket_plan(w["N"], w["C"], 0)

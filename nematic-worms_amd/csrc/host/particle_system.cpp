@@ -4,6 +4,7 @@
 #include "worm_init.h"
 
 #include <algorithm>
+#include <utility>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -258,9 +259,19 @@ void ParticleSystem::_updateFused(float deltaTime) {
     const float kT = m_langevinFdt ? m_params.kbT / deltaTime : m_params.kbT;
     const bool needVel = m_ext.pairPotential == NW_PAIR_REF_DEM;
     const bool needTan = needVel || m_params.activity != 0.0f;
-    nwFusedMdStep(m_dPos, m_dVel, m_dForce, m_dForceOld, m_dTangent, m_dGridParticleHash, m_dGridParticleIndex,
-                  m_dSortedPos, needVel ? m_dSortedVel : nullptr, needTan ? m_dSortedTangent : nullptr, deltaTime,
-                  m_params.gamma, kT, m_solventEnabled ? 0 : 1, m_numFilaments);
+    if (m_params.reverseProbability == 0.0f) {
+        // double-buffered forces: the new force lands in the old force's
+        // buffer, then the two swap roles (both start cleared, w = 0)
+        nwFusedMdStepSwapForces(m_dPos, m_dVel, m_dForce, m_dForceOld, m_dTangent, m_dGridParticleHash,
+                                m_dGridParticleIndex, m_dSortedPos, needVel ? m_dSortedVel : nullptr,
+                                needTan ? m_dSortedTangent : nullptr, deltaTime, m_params.gamma, kT,
+                                m_solventEnabled ? 0 : 1, m_numFilaments);
+        std::swap(m_dForce, m_dForceOld);
+    } else {
+        nwFusedMdStep(m_dPos, m_dVel, m_dForce, m_dForceOld, m_dTangent, m_dGridParticleHash, m_dGridParticleIndex,
+                      m_dSortedPos, needVel ? m_dSortedVel : nullptr, needTan ? m_dSortedTangent : nullptr,
+                      deltaTime, m_params.gamma, kT, m_solventEnabled ? 0 : 1, m_numFilaments);
+    }
     if (m_solventEnabled && (m_updateCount % m_solventUpdateGap) == 0) {
         _srdShift();
         nwFusedSrdStep(m_dPos, m_dVel, m_dSolventCellCOM, m_dGridParticleHash, m_dGridParticleIndex,

@@ -575,7 +575,7 @@ static bool fused_md(const KP &kp, float *pos, float *vel, float *force, float *
                      uint *gridParticleHash, uint *gridParticleIndex, float *sortedPos, float *sortedVel,
                      float *sortedTangent, float deltaTime, float langevinGamma, float langevinKbT, int langevin,
                      uint numFilaments, uint32_t *offOut, uint32_t *cellOut, const float *posIn = nullptr,
-                     const float *velIn = nullptr) {
+                     const float *velIn = nullptr, bool swapForces = false) {
     State &s = S();
     const uint32_t n = kp.p.filamentSize;
     NW_REQUIRE_R(n >= 1 && n <= 64, false, "nwFusedMdStep: worms of %u beads (1..64 supported)", n);
@@ -595,7 +595,10 @@ static bool fused_md(const KP &kp, float *pos, float *vel, float *force, float *
     if (offOut) sc.off = offOut;
     if (cellOut) sc.sortedCell = cellOut;
 
+    NW_REQUIRE_R(!swapForces || kp.p.reverseProbability == 0.0f, false,
+                 "nwFusedMdStepSwapForces: no polarity reversal (it moves the old force)");
     MdFrontArgs a;
+    a.swapForces = swapForces ? 1 : 0;
     a.pos = reinterpret_cast<float4 *>(pos);
     a.vel = reinterpret_cast<float4 *>(vel);
     a.force = reinterpret_cast<float4 *>(force);
@@ -630,7 +633,7 @@ static bool fused_md(const KP &kp, float *pos, float *vel, float *force, float *
     }
     {
         TimedScope ts("md_collide", s.stream, true);
-        NW_HIP_R(launch_collide_off(a.force, reinterpret_cast<const float4 *>(sortedPos),
+        NW_HIP_R(launch_collide_off(swapForces ? a.forceOld : a.force, reinterpret_cast<const float4 *>(sortedPos),
                                   reinterpret_cast<const float4 *>(sortedVel),
                                   reinterpret_cast<const float4 *>(sortedTangent), gridParticleIndex,
                                   sc.off, sc.sortedCell, uint32_t(N), kp, s.stream), false);
@@ -646,6 +649,16 @@ void nwFusedMdStep(float *pos, float *vel, float *force, float *forceOld, float 
     (void)fused_md(make_kp(), pos, vel, force, forceOld, tangent, gridParticleHash, gridParticleIndex, sortedPos,
                    sortedVel, sortedTangent, deltaTime, langevinGamma, langevinKbT, langevin, numFilaments, nullptr,
                    nullptr);
+}
+
+void nwFusedMdStepSwapForces(float *pos, float *vel, float *force, float *forceOld, float *tangent,
+                             uint *gridParticleHash, uint *gridParticleIndex, float *sortedPos, float *sortedVel,
+                             float *sortedTangent, float deltaTime, float langevinGamma, float langevinKbT,
+                             int langevin, uint numFilaments) {
+    NW_INIT();
+    (void)fused_md(make_kp(), pos, vel, force, forceOld, tangent, gridParticleHash, gridParticleIndex, sortedPos,
+                   sortedVel, sortedTangent, deltaTime, langevinGamma, langevinKbT, langevin, numFilaments, nullptr,
+                   nullptr, nullptr, nullptr, true);
 }
 
 // The fused SRD collision with the parameters kp; offOut (may be NULL: the

@@ -57,6 +57,9 @@ struct MdFrontArgs {
     uint32_t *hash;               // out: the cell of every bead
     float dt, lgvStd;
     int langevin;
+    // nonzero: the new force goes to forceOld's buffer and force's buffer,
+    // left as it is, becomes the old force (no reversal; the caller swaps)
+    int swapForces;
     uint32_t W;
     uint64_t seed, offRev, offLgv;
 };

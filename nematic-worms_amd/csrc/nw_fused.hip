@@ -102,16 +102,22 @@ __device__ __forceinline__ FrontLane md_front_lane(const KP &kp, const WaveWorm 
     return o;
 }
 
-// stores of an active lane; returns the bead's cell (calcHashD)
+// stores of an active lane; returns the bead's cell (calcHashD).  swap: the
+// new force into forceOld's buffer; force's buffer already holds the old
+// force (no reversal moved it)
 __device__ __forceinline__ uint32_t md_front_store(const KP &kp, const WaveWorm &m, const FrontLane &o,
                                                    float4 *__restrict__ pos, float4 *__restrict__ vel,
                                                    float4 *__restrict__ force, float4 *__restrict__ forceOld,
-                                                   float4 *__restrict__ tangent) {
+                                                   float4 *__restrict__ tangent, bool swap = false) {
     const SimParams &p = kp.p;
     pos[m.bead] = f4(o.r, o.pw);
     vel[m.bead] = f4(o.v, o.vw);
-    force[m.bead] = f4(o.acc, 0.0f);
-    forceOld[m.bead] = f4(o.fold, o.fow);
+    if (swap) {
+        forceOld[m.bead] = f4(o.acc, 0.0f);
+    } else {
+        force[m.bead] = f4(o.acc, 0.0f);
+        forceOld[m.bead] = f4(o.fold, o.fow);
+    }
     if (o.writeTan) tangent[m.bead] = f4(o.tg, 0.0f);
     return md_hash_wrapped(p, cell_coord(o.r.x, p.origin.x, p.cellSize.x), cell_coord(o.r.y, p.origin.y, p.cellSize.y),
                            cell_coord(o.r.z, p.origin.z, p.cellSize.z));
@@ -121,7 +127,8 @@ __global__ __launch_bounds__(kBlock) void k_md_front(MdFrontArgs a, KP kp) {
     const WaveWorm m = wave_worm(kp.p.filamentSize, a.W);
     const FrontLane o = md_front_lane(kp, m, a.posIn ? a.posIn : a.pos, a.velIn ? a.velIn : a.vel, a.force,
                                       a.forceOld, a.dt, a.lgvStd, a.langevin, a.seed, a.offRev, a.offLgv);
-    if (m.active) a.hash[m.bead] = md_front_store(kp, m, o, a.pos, a.vel, a.force, a.forceOld, a.tangent);
+    if (m.active)
+        a.hash[m.bead] = md_front_store(kp, m, o, a.pos, a.vel, a.force, a.forceOld, a.tangent, a.swapForces != 0);
 }
 
 // ===========================================================================

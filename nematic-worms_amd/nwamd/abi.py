@@ -137,6 +137,7 @@ PROTOTYPES = {
     "nwSortParticlesBounded": (None, [P, P, c_uint, c_uint]),
     "nwRadixSortPairs": (None, [P, P, c_uint, c_uint]),
     "nwFusedMdStep": (None, [P, P, P, P, P, P, P, P, P, P, c_float, c_float, c_float, c_int, c_uint]),
+    "nwFusedMdStepSwapForces": (None, [P, P, P, P, P, P, P, P, P, P, c_float, c_float, c_float, c_int, c_uint]),
     "nwFusedSrdStep": (None, [P, P, P, P, P, c_float, c_uint, c_uint, c_int]),
     "nwFusedPlanFits": (c_int, [c_uint, c_uint, c_uint]),
     "nwObservables": (None, [ctypes.c_void_p, ctypes.c_void_p, c_uint, ctypes.POINTER(NWObservables)]),

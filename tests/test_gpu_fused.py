@@ -243,3 +243,42 @@ def test_lds_and_global_pair_kernels_agree_bitwise(nwlib, mode, activity):
         res.append([down(b, k, p.numParticles) for k in ("pos", "vel", "f")])
     for x, y in zip(*res):
         assert np.array_equal(bits(x), bits(y))
+
+
+@pytest.mark.parametrize("mode,lgv", [("wca", 0), ("wca", 1), ("ref", 1)])
+def test_swap_forces_step_matches_fused_step(nwlib, mode, lgv):
+    """nwFusedMdStepSwapForces with the caller swapping force / forceOld after
+    every call against nwFusedMdStep: positions, velocities, tangents and the
+    (logical) force and old force bit for bit over 3 steps."""
+    rng = np.random.default_rng(17 + lgv)
+    lib = nwlib
+    wca = mode == "wca"
+    p = sim_params(grid=(64, 64, 1), cell=1.1236 if wca else 1.0, W=300, n=10, bondSpringL=0.8 if wca else 0.4,
+                   activity=0.0 if wca else 0.2)
+    pos, vel, tan = random_worms(p, rng, twod=True)
+    N = p.numParticles
+    e = ext_params(pairPotential=abi.NW_PAIR_WCA if wca else abi.NW_PAIR_REF_DEM,
+                   bondedMode=abi.NW_BONDED_CHAIN if wca else abi.NW_BONDED_REF, driveForce=0.7 if wca else 0.0,
+                   stencil=abi.NW_STENCIL_2D)
+    lib.setParameters(ctypes.byref(p))
+    lib.nwSetExtParameters(ctypes.byref(e))
+    f0 = np.zeros((N, 4), np.float32)
+    res = []
+    for swap in (False, True):
+        d = dict(pos=dev(lib, pos), vel=dev(lib, vel), tan=dev(lib, tan))
+        fa, fb = dev(lib, f0), dev(lib, f0)
+        h, i = DeviceArray(lib, 4 * N), DeviceArray(lib, 4 * N)
+        sP, sV, sT = (DeviceArray(lib, 16 * N) for _ in range(3))
+        lib.nwRngSetSeed(4, 40)
+        for _ in range(3):
+            args = (d["pos"].p, d["vel"].p, fa.p, fb.p, d["tan"].p, h.p, i.p, sP.p, None if wca else sV.p,
+                    None if wca else sT.p, 1e-3, p.gamma, 0.05, lgv, p.numFilaments)
+            if swap:
+                lib.nwFusedMdStepSwapForces(*args)
+                fa, fb = fb, fa
+            else:
+                lib.nwFusedMdStep(*args)
+        lib.threadSync()
+        res.append([x.download(np.float32, (N, 4)) for x in (d["pos"], d["vel"], d["tan"], fa, fb)])
+    for a, b in zip(*res):
+        assert np.array_equal(bits(a), bits(b))
