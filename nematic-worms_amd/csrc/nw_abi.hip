@@ -64,9 +64,12 @@ struct State {
         std::vector<SortRep> sort;  // every replica's own sort plan
         SortRep *sortDev = nullptr;
         BatchSortPlan sortPlan;
-        hipGraphExec_t exec = nullptr;
-        hipGraph_t graph = nullptr;
-        std::vector<uintptr_t> key;  // arguments the graph was captured with
+        // the step as graphs: [0] force -> forceOld, [1] the buffers' roles
+        // exchanged (double-buffered forces, when no replica reverses)
+        hipGraphExec_t exec[2] = {nullptr, nullptr};
+        hipGraph_t graph[2] = {nullptr, nullptr};
+        std::vector<uintptr_t> key;  // arguments the graphs were captured with
+        bool swap = false;
     } batch;
     // adjoint of the fused MD step: checkpoints of the last nwAdjointForward
     struct Adjoint {
@@ -836,10 +839,12 @@ static void observe_kp(const KP &kp, const float *pos, const float *vel, uint nu
 // batched replicas
 // ---------------------------------------------------------------------------
 static void batch_drop_graph(State::Batch &b) {
-    if (b.exec) (void)hipGraphExecDestroy(b.exec);
-    if (b.graph) (void)hipGraphDestroy(b.graph);
-    b.exec = nullptr;
-    b.graph = nullptr;
+    for (int k = 0; k < 2; ++k) {
+        if (b.exec[k]) (void)hipGraphExecDestroy(b.exec[k]);
+        if (b.graph[k]) (void)hipGraphDestroy(b.graph[k]);
+        b.exec[k] = nullptr;
+        b.graph[k] = nullptr;
+    }
     b.key.clear();
 }
 
@@ -894,6 +899,8 @@ void nwBatchSetReplicas(const NWReplicaParams *reps, uint numReplicas) {
     }
     b.R = numReplicas;
     b.N = uint32_t(N);
+    b.swap = true;
+    for (uint r = 0; r < numReplicas; ++r) b.swap = b.swap && b.reps[r].revBlocks == 0;
     b.cells = uint32_t(C);
     b.waves = uint32_t(waves);
     b.blocks = uint32_t(blocks);
@@ -959,12 +966,13 @@ void nwBatchMdStep(float *pos, float *vel, float *force, float *forceOld, float 
                "nwBatchMdStep: NULL array");
     NW_REQUIRE(b.mode == 1 || sortedVel != nullptr, "nwBatchMdStep: sortedVel is NULL");
     NW_REQUIRE(!b.tan || sortedTangent != nullptr, "nwBatchMdStep: sortedTangent is NULL");
-    BatchArrays a;
+    BatchArrays a{};
     a.pos = reinterpret_cast<float4 *>(pos);
     a.vel = reinterpret_cast<float4 *>(vel);
     a.force = reinterpret_cast<float4 *>(force);
     a.forceOld = reinterpret_cast<float4 *>(forceOld);
     a.tangent = reinterpret_cast<float4 *>(tangent);
+    a.swapForces = b.swap ? 1 : 0;
     a.hash = b.scratch.hash;
     a.ids = gridParticleIndex;
     a.off = b.scratch.off;
@@ -979,34 +987,47 @@ void nwBatchMdStep(float *pos, float *vel, float *force, float *forceOld, float 
     const std::vector<uintptr_t> key = {uintptr_t(pos), uintptr_t(vel), uintptr_t(force), uintptr_t(forceOld),
                                         uintptr_t(tangent), uintptr_t(gridParticleIndex), uintptr_t(sortedPos),
                                         uintptr_t(sortedVel), uintptr_t(sortedTangent), uintptr_t(dtBits)};
+    // double-buffered forces: phase 0 reads force and writes the new force
+    // into forceOld's buffer, phase 1 the other way round; after an odd
+    // number of steps the two buffers exchange contents once
+    BatchArrays ar[2] = {a, a};
+    ar[1].force = a.forceOld;
+    ar[1].forceOld = a.force;
+    int ph = 0;
     uint done = 0;
-    if (useGraph && (!b.exec || b.key != key)) {
+    if (useGraph && (!b.exec[0] || b.key != key)) {
         // one direct step first (scratch and kernel attributes are set up
-        // outside the capture), then capture a step
+        // outside the capture), then capture the step (both phases)
         batch_drop_graph(b);
         {
             TimedScope ts("batch_step", s.stream);
-            NW_HIP(batch_step(b, a, gv, gt, a.sVel, a.sTan, deltaTime, s.stream));
+            NW_HIP(batch_step(b, ar[0], gv, gt, a.sVel, a.sTan, deltaTime, s.stream));
         }
         ++done;
         ++b.steps;
-        if (done == steps) return;
-        NW_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
-        const hipError_t e1 = batch_step(b, a, gv, gt, a.sVel, a.sTan, deltaTime, s.stream);
-        hipGraph_t g = nullptr;
-        const hipError_t e2 = hipStreamEndCapture(s.stream, &g);
-        NW_HIP(e1);
-        NW_HIP(e2);
-        b.graph = g;
-        NW_HIP(hipGraphInstantiate(&b.exec, g, nullptr, nullptr, 0));
-        b.key = key;
+        if (b.swap) ph = 1;
+        if (done < steps) {
+            for (int k = 0; k < (b.swap ? 2 : 1); ++k) {
+                NW_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+                const hipError_t e1 = batch_step(b, ar[k], gv, gt, a.sVel, a.sTan, deltaTime, s.stream);
+                hipGraph_t g = nullptr;
+                const hipError_t e2 = hipStreamEndCapture(s.stream, &g);
+                NW_HIP(e1);
+                NW_HIP(e2);
+                b.graph[k] = g;
+                NW_HIP(hipGraphInstantiate(&b.exec[k], g, nullptr, nullptr, 0));
+            }
+            b.key = key;
+        }
     }
     for (; done < steps; ++done) {
         TimedScope ts("batch_step", s.stream);
-        if (useGraph) NW_HIP(hipGraphLaunch(b.exec, s.stream));
-        else NW_HIP(batch_step(b, a, gv, gt, a.sVel, a.sTan, deltaTime, s.stream));
+        if (useGraph) NW_HIP(hipGraphLaunch(b.exec[ph], s.stream));
+        else NW_HIP(batch_step(b, ar[ph], gv, gt, a.sVel, a.sTan, deltaTime, s.stream));
         ++b.steps;
+        if (b.swap) ph ^= 1;
     }
+    if (ph == 1) NW_HIP(launch_swap4(a.force, a.forceOld, b.N, s.stream));
 }
 
 // ---------------------------------------------------------------------------

@@ -208,6 +208,9 @@ hipError_t launch_srd_rotate_batch(const uint32_t *hash, const float4 *com, floa
 
 struct BatchArrays {
     float4 *pos, *vel, *force, *forceOld, *tangent;
+    // nonzero: the new force goes to forceOld's buffer (pairs add there too)
+    // and force's buffer is left as the old force (no replica reverses)
+    int swapForces;
     uint32_t *hash;   // global cell of every bead
     uint32_t *ids;    // bead ids in cell order
     uint32_t *off;    // CSR offsets over the concatenated cells
@@ -215,6 +218,8 @@ struct BatchArrays {
     float4 *sPos, *sVel, *sTan;
 };
 
+// exchanges the contents of two float4 arrays of n records
+hipError_t launch_swap4(float4 *a, float4 *b, uint64_t n, hipStream_t st);
 // sets the device step counter of a batch (replaying a stretch of steps)
 hipError_t launch_set_step(uint64_t *ctr, uint64_t v, hipStream_t st);
 hipError_t launch_md_front_batch(const BatchArrays &a, const BatchRep *reps, uint32_t R, uint32_t waves,

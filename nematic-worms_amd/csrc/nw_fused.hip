@@ -577,7 +577,8 @@ __global__ __launch_bounds__(kBlock) void k_md_front_batch(BatchArrays a, const 
     const FrontLane o = md_front_lane(kp, m, pos, vel, force, forceOld, dt, rep.lgvStd, rep.langevin, rep.seed,
                                       offRev, offLgv);
     if (m.active) {
-        const uint32_t h = md_front_store(kp, m, o, pos, vel, force, forceOld, a.tangent + rep.beadOffset);
+        const uint32_t h =
+            md_front_store(kp, m, o, pos, vel, force, forceOld, a.tangent + rep.beadOffset, a.swapForces != 0);
         a.hash[rep.beadOffset + m.bead] = rep.cellOffset + h;
     }
 }
@@ -594,7 +595,8 @@ __global__ __launch_bounds__(kPairBlock) void k_collide_rows_batch(BatchArrays a
     const BatchRep &rep = reps[r];
     const uint32_t i0 = rep.beadOffset + (tile - rep.blockOffset) * blockDim.x;
     const uint32_t iEnd = rep.beadOffset + rep.kp.p.numFilaments * rep.kp.p.filamentSize;
-    collide_rows_tile<MODE, TAN, UNIT>(rep.kp, i0, iEnd, rep.beadOffset, rep.cellOffset, a.force, a.sPos, a.sVel,
+    collide_rows_tile<MODE, TAN, UNIT>(rep.kp, i0, iEnd, rep.beadOffset, rep.cellOffset,
+                                       a.swapForces ? a.forceOld : a.force, a.sPos, a.sVel,
                                        a.sTan, a.ids, a.off + rep.cellOffset, a.sCell, L);
 }
 
@@ -638,6 +640,20 @@ hipError_t launch_collide_off(float4 *force, const float4 *sPos, const float4 *s
         else NW_COLLIDE(k_collide_csr, 0, true);
     }
 #undef NW_COLLIDE
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_swap4(float4 *__restrict__ a, float4 *__restrict__ b, uint64_t n) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 x = a[i], y = b[i];
+    a[i] = y;
+    b[i] = x;
+}
+
+hipError_t launch_swap4(float4 *a, float4 *b, uint64_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_swap4, dim3(grid_for(n)), dim3(kBlock), 0, st, a, b, n);
     return hipGetLastError();
 }
 
