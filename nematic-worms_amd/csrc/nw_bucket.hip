@@ -70,7 +70,7 @@ const char *bucket_plan_problem(const BucketPlan &bp) {
 
 // level-2 LDS layout (bytes), shared by host (size) and device (carving)
 struct CellsLayout {
-    uint32_t wsum, cnt, cur, lid, lcell, segStart, segBase, tmpId, tmpCell, tileOf, lv, total;
+    uint32_t wsum, cnt, cur, lid, lcell, segStart, segBase, tmpId, tmpCell, tileOf, total;
 };
 
 __host__ __device__ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
@@ -85,15 +85,13 @@ __host__ __device__ inline CellsLayout cells_layout(const BucketPlan &bp, bool s
     L.cur = o;  o += align16(4u * nc);
     L.lid = o;  o += align16(4u * cap);
     L.lcell = o; if (!srd) o += align16(2u * cap);
-    // region A (segment tables, staged pairs) -- SRD velocities alias it
+    // region A (segment tables, staged pairs)
     const uint32_t a0 = o;
     L.segStart = o; o += align16(4u * (bp.tiles + 1));
     L.segBase = o;  o += align16(4u * bp.tiles);
     L.tmpId = o;    o += align16(4u * cap);
     L.tmpCell = o;  o += align16(2u * cap);
     L.tileOf = L.lid;  // phase A only: lid is first written after it
-    L.lv = a0;
-    if (srd && o < a0 + 16u * cap) o = a0 + 16u * cap;
     L.total = o;
     return L;
 }
@@ -356,7 +354,6 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
     uint32_t *tmpId = reinterpret_cast<uint32_t *>(smem + L.tmpId);
     uint16_t *tmpCell = reinterpret_cast<uint16_t *>(smem + L.tmpCell);
     uint16_t *tileOf = reinterpret_cast<uint16_t *>(smem + L.tileOf);
-    float4 *lv = reinterpret_cast<float4 *>(smem + L.lv);
 
     const uint32_t B1 = bp.buckets + 1;
     const uint32_t tid = threadIdx.x, bd = blockDim.x;
@@ -475,24 +472,11 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
         return;
     }
 
-    // SRD: stage the members' velocities (region A is dead now)
-    if (staged && m) {
-        float4 v[PER];
-        uint32_t id[PER];
-#pragma unroll
-        for (int j = 0; j < PER; ++j) id[j] = lid[min(tid + uint32_t(j) * BLOCK, m - 1u)];
-#pragma unroll
-        for (int j = 0; j < PER; ++j) v[j] = o.srdVel[id[j]];
-        // lv holds CAP entries: slots past the last member take
-        // copies, never read
-#pragma unroll
-        for (int j = 0; j < PER; ++j) lv[tid + uint32_t(j) * BLOCK] = v[j];
-#pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            const uint32_t k = tid + uint32_t(j) * BLOCK;
-            if (k < m) out[k] = id[j];
-        }
-    }
+    // SRD: the sorted ids out (the cell sums below read the velocities from
+    // memory by id: with the solvent in cell order those reads are near
+    // sequential, and without an LDS stage of them more buckets fit per CU)
+    if (staged && m)
+        for (uint32_t k = tid; k < m; k += bd) out[k] = lid[k];
     __syncthreads();
     // one thread per cell, sums in sorted order: the arithmetic of
     // isokineticThermostatSRD and cellCenterOfMomentumSRD
@@ -505,7 +489,7 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
             if (o.iso) {
                 float E = 0.0f;
                 for (uint32_t u = a; u < z; ++u) {
-                    const float4 V = staged ? lv[u] : o.srdVel[out[u]];
+                    const float4 V = o.srdVel[staged ? lid[u] : out[u]];
                     const f3 v = mk(V);
                     E += 0.5f * V.w * dot(v, v);
                 }
@@ -513,7 +497,7 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
             }
             float M = 0.0f;
             for (uint32_t u = a; u < z; ++u) {
-                float4 V = staged ? lv[u] : o.srdVel[out[u]];
+                float4 V = o.srdVel[staged ? lid[u] : out[u]];
                 if (o.iso) V = f4(lambda * mk(V), V.w);
                 mom += V.w * mk(V);
                 M += V.w;
