@@ -70,7 +70,7 @@ const char *bucket_plan_problem(const BucketPlan &bp) {
 
 // level-2 LDS layout (bytes), shared by host (size) and device (carving)
 struct CellsLayout {
-    uint32_t wsum, cnt, cur, lid, lcell, segStart, segBase, tmpId, tmpCell, tileOf, total;
+    uint32_t wsum, cnt, cur, lid, lcell, segStart, segBase, tileOf, total;
 };
 
 __host__ __device__ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
@@ -85,12 +85,9 @@ __host__ __device__ inline CellsLayout cells_layout(const BucketPlan &bp, bool s
     L.cur = o;  o += align16(4u * nc);
     L.lid = o;  o += align16(4u * cap);
     L.lcell = o; if (!srd) o += align16(2u * cap);
-    // region A (segment tables, staged pairs)
-    const uint32_t a0 = o;
+    // segment tables (the staged members themselves wait in registers)
     L.segStart = o; o += align16(4u * (bp.tiles + 1));
     L.segBase = o;  o += align16(4u * bp.tiles);
-    L.tmpId = o;    o += align16(4u * cap);
-    L.tmpCell = o;  o += align16(2u * cap);
     L.tileOf = L.lid;  // phase A only: lid is first written after it
     L.total = o;
     return L;
@@ -351,8 +348,6 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
     uint16_t *lcell = reinterpret_cast<uint16_t *>(smem + L.lcell);
     uint32_t *segStart = reinterpret_cast<uint32_t *>(smem + L.segStart);
     uint32_t *segBase = reinterpret_cast<uint32_t *>(smem + L.segBase);
-    uint32_t *tmpId = reinterpret_cast<uint32_t *>(smem + L.tmpId);
-    uint16_t *tmpCell = reinterpret_cast<uint16_t *>(smem + L.tmpCell);
     uint16_t *tileOf = reinterpret_cast<uint16_t *>(smem + L.tileOf);
 
     const uint32_t B1 = bp.buckets + 1;
@@ -381,6 +376,9 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
 
     const bool staged = m <= uint32_t(CAP);
     const uint64_t T = bp.tileItems;
+    // a staged bucket's members, held by their threads from the count to the
+    // placement (member k = tid + j * BLOCK)
+    uint2 q[PER];
     if (staged && m) {
         // the tile of every member, one tile's segment per thread at a time
         for (uint32_t t = tid; t < bp.tiles; t += bd)
@@ -388,7 +386,6 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
         __syncthreads();
         // every load of the thread's members in flight before the first use
         // (indices past the last member load the last member again, unused)
-        uint2 q[PER];
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
             const uint32_t k = min(tid + uint32_t(j) * BLOCK, m - 1u);
@@ -398,12 +395,7 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
             const uint32_t k = tid + uint32_t(j) * BLOCK;
-            if (k < m) {
-                const uint32_t lc = q[j].x - cell0;
-                tmpId[k] = q[j].y;
-                tmpCell[k] = uint16_t(lc);
-                atomicAdd(&cnt[lc], 1u);
-            }
+            if (k < m) atomicAdd(&cnt[q[j].x - cell0], 1u);
         }
     } else if (!staged) {
         for (uint32_t k = tid; k < m; k += bd) {
@@ -419,11 +411,15 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
 
     uint32_t *out = o.ids + sb;
     if (staged) {
-        for (uint32_t k = tid; k < m; k += bd) {
-            const uint32_t lc = tmpCell[k];
-            const uint32_t slot = atomicAdd(&cur[lc], 1u);
-            lid[slot] = tmpId[k];
-            if (!SRD) lcell[slot] = uint16_t(lc);
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t k = tid + uint32_t(j) * BLOCK;
+            if (k < m) {
+                const uint32_t lc = q[j].x - cell0;
+                const uint32_t slot = atomicAdd(&cur[lc], 1u);
+                lid[slot] = q[j].y;
+                if (!SRD) lcell[slot] = uint16_t(lc);
+            }
         }
     } else {
         for (uint32_t k = tid; k < m; k += bd) {
@@ -520,7 +516,7 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
 }
 
 template <bool SRD, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_bucket_cells(CellsArgs o, BucketPlan bp) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_bucket_cells(CellsArgs o, BucketPlan bp) {
     extern __shared__ __align__(16) unsigned char smem[];
     bucket_cells_body<SRD, BLOCK>(o, bp, xcd_contiguous(blockIdx.x, gridDim.x), smem);
 }
@@ -529,7 +525,7 @@ __global__ __launch_bounds__(BLOCK) void k_bucket_cells(CellsArgs o, BucketPlan 
 // arrays are offset to the replica's first bead and cell, the bead ids in the
 // pairs are global
 template <int BLOCK, int CAP>
-__global__ __launch_bounds__(BLOCK) void k_bucket_cells_batch(CellsArgs a, const SortRep *__restrict__ reps,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_bucket_cells_batch(CellsArgs a, const SortRep *__restrict__ reps,
                                                               uint32_t R) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t v = xcd_contiguous(blockIdx.x, gridDim.x);
