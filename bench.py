@@ -51,11 +51,11 @@ ALG_BYTES = {
     # R pos vel f f_old, W pos vel f (+ f_old unless the host driver swaps
     # the force buffers: no reversal), W tangent, W cell
     "md_front": lambda w: (132 if w["swap"] else 148) * w["N"],
-    "md_tiles": lambda w: 12 * w["N"] + w["Pm"],               # R cell, W pair
-    "md_sort": lambda w: 48 * w["N"] + 4 * w["C"] + w["Pm"],   # R pairs, W ids cells, gather pos, W offsets
+    "md_tiles": lambda w: 8 * w["N"] + w["Pm"],                # R cell, W pair (4 B packed)
+    "md_sort": lambda w: 44 * w["N"] + 4 * w["C"] + w["Pm"],   # R pairs, W ids cells, gather pos, W offsets
     "md_collide": lambda w: (56 if w["wca"] else 88) * w["N"] + 4 * w["C"],  # R sPos sIdx sCell (+sVel sTan), RMW f
-    "srdf_tiles": lambda w: 28 * w["NT"] + w["Ps"],            # R pos, W cell, W pair
-    "srdf_cells": lambda w: 28 * w["NT"] + 20 * w["Cs"] + w["Ps"],  # R pairs, W ids, gather vel, W offsets com
+    "srdf_tiles": lambda w: 24 * w["NT"] + w["Ps"],            # R pos, W cell, W pair
+    "srdf_cells": lambda w: 24 * w["NT"] + 20 * w["Cs"] + w["Ps"],  # R pairs, W ids, gather vel, W offsets com
     # R cell, RMW vel (+pos), R com; beads only when rotated or rescaled
     "srdf_rotate": lambda w: 36 * w["Nrot"] + 32 * w["Ns"] + 16 * w["Cs"],
 }

@@ -51,8 +51,8 @@ BucketPlan make_bucket_plan(uint64_t n, uint64_t cells, uint32_t tileItems, uint
         const double want = double(n) * double(b.buckets + 1) / double(1u << 21);
         uint32_t T = 4096;
         while (T < (1u << 17) && double(T) < want) T <<= 1;
-        // a 16384-particle tile stages 128 KiB of pairs: one workgroup per CU;
-        // 8192 fits two (SRD level 1 57 -> 41 us for a level 2 117 -> 123 us)
+        // 8192 rather than 16384: the SRD workload's level 1 takes 35 us
+        // against 50 us, for a level 2 of 88 us against 84 us
         if (T == 16384) T = 8192;
         tileItems = T;
     }
@@ -65,6 +65,7 @@ const char *bucket_plan_problem(const BucketPlan &bp) {
     if (bp.buckets > kMaxBuckets) return "grid has too many cells for the fused sort";
     if (bp.tiles > kMaxTiles) return "too many particles for the fused sort";
     if (uint64_t(bp.tiles) * (bp.buckets + 1) > kMaxMatrix) return "particles x cells too large for the fused sort";
+    if (bp.tileItems > (1u << kPairIdBits)) return "level-1 tile too large for the packed sort pairs";
     return "";
 }
 
@@ -101,7 +102,7 @@ __host__ __device__ inline CellsLayout cells_layout(const BucketPlan &bp, bool s
 // ---------------------------------------------------------------------------
 template <bool FROM_POS>
 __global__ __launch_bounds__(kTileBlock) void k_tiles(const float4 *__restrict__ pos, uint32_t *__restrict__ hash,
-                                                      uint2 *__restrict__ pairs, uint32_t *__restrict__ P,
+                                                      uint32_t *__restrict__ pairs, uint32_t *__restrict__ P,
                                                       BucketPlan bp, KP kp) {
     extern __shared__ uint32_t lh[];
     uint32_t *wsum = lh + bp.buckets + 1;
@@ -117,7 +118,7 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(const float4 *__restrict__
             h = srd_hash(kp.p, kp.s, mk(pos[i]));
             hash[i] = h;
         } else {
-            h = hash[i];
+            h = min(hash[i], bp.cells - 1u);  // a key outside the grid sorts as its last cell
         }
         atomicAdd(&lh[h >> bp.shift], 1u);
     }
@@ -127,10 +128,11 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(const float4 *__restrict__
     for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) row[b] = lh[b];
     __syncthreads();
     // second pass over the same particles (each thread re-reads its own cells)
+    const uint32_t cellMask = (1u << bp.shift) - 1u;
     for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        const uint32_t h = hash[i];
+        const uint32_t h = min(hash[i], bp.cells - 1u);
         const uint32_t slot = atomicAdd(&lh[h >> bp.shift], 1u);
-        pairs[i0 + slot] = make_uint2(h, i);
+        pairs[i0 + slot] = pack_pair(h & cellMask, i - i0);
     }
 }
 
@@ -139,17 +141,16 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(const float4 *__restrict__
 // bucket order, then written out with coalesced stores (tiles of
 // 1024 x ITEMS particles).
 // Tile t of one system: hash / pairs / P start at the system's own first
-// entries, its cells start at cellBase in the hash values and its bead ids at
-// idBase (zero for a single system); pairs carry the system's own cell and the
-// global id.
+// entries and its cells start at cellBase in the hash values (zero for a
+// single system).
 template <bool FROM_POS, int ITEMS, int TB = kTileBlock>
 __device__ __forceinline__ void tiles_staged_body(const float4 *__restrict__ pos, uint32_t *__restrict__ hash,
-                                                  uint2 *__restrict__ pairs, uint32_t *__restrict__ P,
+                                                  uint32_t *__restrict__ pairs, uint32_t *__restrict__ P,
                                                   const BucketPlan &bp, const KP &kp, uint32_t t, uint32_t cellBase,
-                                                  uint32_t idBase, uint32_t *lh) {
+                                                  uint32_t *lh) {
     const uint32_t B1 = bp.buckets + 1;
     uint32_t *wsum = lh + B1;
-    uint2 *stage = reinterpret_cast<uint2 *>(lh + ((B1 + TB / 64 + 3) & ~3u));
+    uint32_t *stage = lh + ((B1 + TB / 64 + 3) & ~3u);
     for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) lh[b] = 0u;
     __syncthreads();
     const uint32_t i0 = t * bp.tileItems;
@@ -178,7 +179,7 @@ __device__ __forceinline__ void tiles_staged_body(const float4 *__restrict__ pos
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         const uint32_t i = i0 + uint32_t(k) * TB + threadIdx.x;
-        if (i < i1) stage[lh[h[k] >> bp.shift] + rk[k]] = make_uint2(h[k], idBase + i);
+        if (i < i1) stage[lh[h[k] >> bp.shift] + rk[k]] = pack_pair(h[k] & ((1u << bp.shift) - 1u), i - i0);
     }
     __syncthreads();
     const uint32_t cnt = i1 - i0;
@@ -188,10 +189,10 @@ __device__ __forceinline__ void tiles_staged_body(const float4 *__restrict__ pos
 template <bool FROM_POS, int ITEMS>
 __global__ __launch_bounds__(kTileBlock) void k_tiles_staged(const float4 *__restrict__ pos,
                                                              uint32_t *__restrict__ hash,
-                                                             uint2 *__restrict__ pairs, uint32_t *__restrict__ P,
+                                                             uint32_t *__restrict__ pairs, uint32_t *__restrict__ P,
                                                              BucketPlan bp, KP kp) {
     extern __shared__ __align__(16) uint32_t lh[];
-    tiles_staged_body<FROM_POS, ITEMS>(pos, hash, pairs, P, bp, kp, blockIdx.x, 0u, 0u, lh);
+    tiles_staged_body<FROM_POS, ITEMS>(pos, hash, pairs, P, bp, kp, blockIdx.x, 0u, lh);
 }
 
 // the replica that owns item v of a launch over all replicas (offsets
@@ -224,27 +225,30 @@ __device__ __forceinline__ BucketPlan plan_of(const SortRep &r) {
 constexpr int kBatchTileBlock = 256;
 
 __global__ __launch_bounds__(kBatchTileBlock) void k_tiles_batch(const uint32_t *__restrict__ hash,
-                                                                 uint2 *__restrict__ pairs, uint32_t *__restrict__ P,
+                                                                 uint32_t *__restrict__ pairs,
+                                                                 uint32_t *__restrict__ P,
                                                                  const SortRep *__restrict__ reps, uint32_t R) {
     extern __shared__ __align__(16) uint32_t lh[];
     const SortRep &r = reps[sort_rep_of(reps, R, blockIdx.x, true)];
     const BucketPlan bp = plan_of(r);
     tiles_staged_body<false, kBatchTileItems / kBatchTileBlock, kBatchTileBlock>(
-        nullptr, const_cast<uint32_t *>(hash) + r.beadOffset,
-                                                            pairs + r.beadOffset, P + r.matOff, bp, KP{},
-                                                            blockIdx.x - r.tileOff, r.cellOffset, r.beadOffset, lh);
+        nullptr, const_cast<uint32_t *>(hash) + r.beadOffset, pairs + r.beadOffset, P + r.matOff, bp, KP{},
+        blockIdx.x - r.tileOff, r.cellOffset, lh);
 }
 
 // ---------------------------------------------------------------------------
 // level 2
 // ---------------------------------------------------------------------------
 struct CellsArgs {
-    const uint2 *pairs;
+    const uint32_t *pairs;
     const uint32_t *P;
     uint32_t *ids, *off, *cellOf;
     // added to the cells written to cellOf and to the slots written to off
     // (a replica's first cell and first bead; zero for a single system)
     uint32_t cellBase, slotBase;
+    // the id of the system's first particle (a replica's first bead in the
+    // concatenated arrays; zero for a single system)
+    uint32_t idBase;
     // MD gathers (sPos NULL: none)
     const float4 *pos, *vel, *tan;
     float4 *sPos, *sVel, *sTan;
@@ -376,8 +380,8 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
 
     const bool staged = m <= uint32_t(CAP);
     const uint64_t T = bp.tileItems;
-    // a staged bucket's members, held by their threads from the count to the
-    // placement (member k = tid + j * BLOCK)
+    // a staged bucket's members (cell in the bucket, id), held by their
+    // threads from the count to the placement (member k = tid + j * BLOCK)
     uint2 q[PER];
     if (staged && m) {
         // the tile of every member, one tile's segment per thread at a time
@@ -390,18 +394,17 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
         for (int j = 0; j < PER; ++j) {
             const uint32_t k = min(tid + uint32_t(j) * BLOCK, m - 1u);
             const uint32_t t = tileOf[k];
-            q[j] = o.pairs[uint64_t(t) * T + segBase[t] + (k - segStart[t])];
+            q[j] = unpack_pair(o.pairs[uint64_t(t) * T + segBase[t] + (k - segStart[t])], o.idBase + t * uint32_t(T));
         }
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
             const uint32_t k = tid + uint32_t(j) * BLOCK;
-            if (k < m) atomicAdd(&cnt[q[j].x - cell0], 1u);
+            if (k < m) atomicAdd(&cnt[q[j].x], 1u);
         }
     } else if (!staged) {
         for (uint32_t k = tid; k < m; k += bd) {
             const uint32_t t = seg_of(segStart, bp.tiles, k);
-            const uint2 q = o.pairs[uint64_t(t) * T + segBase[t] + (k - segStart[t])];
-            atomicAdd(&cnt[q.x - cell0], 1u);
+            atomicAdd(&cnt[o.pairs[uint64_t(t) * T + segBase[t] + (k - segStart[t])] >> kPairIdBits], 1u);
         }
     }
     __syncthreads();
@@ -415,7 +418,7 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
         for (int j = 0; j < PER; ++j) {
             const uint32_t k = tid + uint32_t(j) * BLOCK;
             if (k < m) {
-                const uint32_t lc = q[j].x - cell0;
+                const uint32_t lc = q[j].x;
                 const uint32_t slot = atomicAdd(&cur[lc], 1u);
                 lid[slot] = q[j].y;
                 if (!SRD) lcell[slot] = uint16_t(lc);
@@ -424,8 +427,9 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
     } else {
         for (uint32_t k = tid; k < m; k += bd) {
             const uint32_t t = seg_of(segStart, bp.tiles, k);
-            const uint2 q = o.pairs[uint64_t(t) * T + segBase[t] + (k - segStart[t])];
-            out[atomicAdd(&cur[q.x - cell0], 1u)] = q.y;
+            const uint2 q =
+                unpack_pair(o.pairs[uint64_t(t) * T + segBase[t] + (k - segStart[t])], o.idBase + t * uint32_t(T));
+            out[atomicAdd(&cur[q.x], 1u)] = q.y;
         }
     }
     __syncthreads();
@@ -541,6 +545,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7, 8))) v
     if (o.sTan) o.sTan = a.sTan + r.beadOffset;
     o.cellBase = r.cellOffset;
     o.slotBase = r.beadOffset;
+    o.idBase = r.beadOffset;
     bucket_cells_body<false, BLOCK, CAP>(o, plan_of(r), v - r.bucketOff, smem);
 }
 
@@ -687,7 +692,7 @@ hipError_t ensure_bucket_scratch(BucketScratch &s, const BucketPlan &bp) {
         if ((e = hipMalloc(&s.aux, c * 4)) != hipSuccess) return e;
         s.cap_n = c;
     }
-    if ((e = regrow(s.pairs, s.cap_pairs, uint64_t(bp.tiles) * bp.tileItems, 8)) != hipSuccess) return e;
+    if ((e = regrow(s.pairs, s.cap_pairs, uint64_t(bp.tiles) * bp.tileItems, 4)) != hipSuccess) return e;
     if ((e = regrow(s.P, s.cap_mat, uint64_t(bp.tiles) * (bp.buckets + 1), 4)) != hipSuccess) return e;
     if (uint64_t(bp.cells) + 2 > s.cap_cells) {
         const uint64_t c = uint64_t(bp.cells) + bp.cells / 8 + 1024;
@@ -719,7 +724,7 @@ template <bool FROM_POS, int ITEMS>
 static hipError_t launch_staged(const float4 *pos, uint32_t *hash, const BucketPlan &bp, BucketScratch &s,
                                 const KP &kp, hipStream_t st) {
     const size_t lds = (((size_t(bp.buckets) + 1 + kTileBlock / 64 + 3) & ~size_t(3)) * sizeof(uint32_t)) +
-                       size_t(bp.tileItems) * sizeof(uint2);
+                       size_t(bp.tileItems) * sizeof(uint32_t);
     const void *fn = reinterpret_cast<const void *>(&k_tiles_staged<FROM_POS, ITEMS>);
     hipError_t e = grant_lds(fn, lds);
     if (e != hipSuccess) return e;
@@ -733,7 +738,7 @@ static hipError_t launch_staged(const float4 *pos, uint32_t *hash, const BucketP
 static hipError_t launch_tiles(bool fromPos, const float4 *pos, uint32_t *hash, const BucketPlan &bp,
                                BucketScratch &s, const KP &kp, hipStream_t st) {
     if (bp.n == 0) return hipSuccess;
-    const size_t stagedLds = (size_t(bp.buckets) + 1 + kTileBlock / 64 + 3) * 4 + size_t(bp.tileItems) * 8;
+    const size_t stagedLds = (size_t(bp.buckets) + 1 + kTileBlock / 64 + 3) * 4 + size_t(bp.tileItems) * 4;
     if (stagedLds <= 160 * 1024) {
         if (bp.tileItems == 4 * kTileBlock)
             return fromPos ? launch_staged<true, 4>(pos, hash, bp, s, kp, st)
@@ -772,7 +777,7 @@ hipError_t launch_hash_tiles(const uint32_t *hash, const BucketPlan &bp, BucketS
 // ---------------------------------------------------------------------------
 static size_t batch_tiles_lds(uint32_t maxBuckets) {
     return ((size_t(maxBuckets) + 1 + kBatchTileBlock / 64 + 3) & ~size_t(3)) * sizeof(uint32_t) +
-           size_t(kBatchTileItems) * sizeof(uint2);
+           size_t(kBatchTileItems) * sizeof(uint32_t);
 }
 
 static CellsLayout batch_cells_layout(const BatchSortPlan &tot) {
@@ -831,7 +836,7 @@ hipError_t ensure_batch_sort_scratch(BucketScratch &s, const BatchSortPlan &tot)
         if ((e = hipMalloc(&s.aux, c * 4)) != hipSuccess) return e;
         s.cap_n = c;
     }
-    if ((e = regrow(s.pairs, s.cap_pairs, tot.n, 8)) != hipSuccess) return e;
+    if ((e = regrow(s.pairs, s.cap_pairs, tot.n, 4)) != hipSuccess) return e;
     if ((e = regrow(s.P, s.cap_mat, tot.matrix, 4)) != hipSuccess) return e;
     if (tot.cells + 2 > s.cap_cells) {
         const uint64_t c = tot.cells + tot.cells / 8 + 1024;

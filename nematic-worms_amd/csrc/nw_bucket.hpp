@@ -26,6 +26,18 @@ constexpr uint32_t kBucketCap = 2048;     // bucket members staged in LDS (more:
 constexpr uint32_t kMaxTiles = 4096;      // level-2 segment tables in LDS
 constexpr uint32_t kMaxBuckets = 32767;   // level-1 LDS histogram (128 KiB of the 160 KiB)
 constexpr uint64_t kMaxMatrix = 1ull << 24;  // entries of P
+
+// A sort pair in 4 bytes: the particle's cell within its bucket (below
+// 2^11, the largest bucket) in the high bits and its index within its
+// level-1 tile (below 2^kPairIdBits) in the low bits; level 2 knows the
+// bucket and the tile of every member and restores (cell in bucket, id).
+constexpr uint32_t kPairIdBits = 21;
+__host__ __device__ inline uint32_t pack_pair(uint32_t cellInBucket, uint32_t idInTile) {
+    return (cellInBucket << kPairIdBits) | idInTile;
+}
+__host__ __device__ inline uint2 unpack_pair(uint32_t p, uint32_t tileFirstId) {
+    return make_uint2(p >> kPairIdBits, tileFirstId + (p & ((1u << kPairIdBits) - 1u)));
+}
 constexpr int kMdCellsBlock = 256;        // level-2 workgroup, MD sort
 constexpr int kSrdCellsBlock = 256;       // level-2 workgroup, SRD sort
 constexpr int kCellsBlockMax = 1024;
@@ -40,7 +52,7 @@ const char *bucket_plan_problem(const BucketPlan &bp);
 
 struct BucketScratch {
     uint32_t *hash = nullptr;        // cell per particle, particle order
-    uint2 *pairs = nullptr;          // (cell, id), per tile grouped by bucket
+    uint32_t *pairs = nullptr;       // packed (cell in bucket, index in tile), per tile grouped by bucket
     uint32_t *P = nullptr;           // tiles x (buckets + 1)
     uint32_t *sortedCell = nullptr;  // cell of each sorted slot
     uint32_t *off = nullptr;         // CSR cell offsets (cells + 1)
