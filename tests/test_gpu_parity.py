@@ -513,3 +513,30 @@ def test_full_size_hash_sort_reorder_800k(nwlib):
     assert np.array_equal(downu(cs, C), ocs)
     assert np.array_equal(downu(ce, C), oce)
     assert np.array_equal(down4(sp, n).view(np.uint32), osp.view(np.uint32))
+
+
+def test_hash_sort_sparse_grid_two_pass_level1(nwlib):
+    """4,000,000 particles on an 8000 x 7000 grid (56M cells, 0.07 per cell):
+    the bucket sort's plan has 27,344 buckets of 2048 cells and level-1 tiles
+    of 65,536 particles, more than the LDS stage holds, so level 1 runs as its
+    two-pass kernel (packed pairs written straight to memory); hash and sort
+    bit-exact."""
+    rng = np.random.default_rng(71)
+    gx, gy = 8000, 7000
+    p = sim_params(grid=(gx, gy, 1), cell=1.0, W=200_000, n=20, box=(float(gx), float(gy), 1.0))
+    n = 4_000_000
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, 0] = (rng.random(n) * gx - 0.5 * gx).astype(np.float32)
+    pos[:, 1] = (rng.random(n) * gy - 0.5 * gy).astype(np.float32)
+    pos[:, 2] = np.float32(p.origin.z + 0.5 * p.cellSize.z)
+    set_params(nwlib, p)
+    assert nwlib.nwFusedPlanFits(n, gx * gy, 20) == 1  # the bucket sort, not the counting-sort fallback
+    dpos = dev(nwlib, pos)
+    dh, di = DeviceArray(nwlib, 4 * n), DeviceArray(nwlib, 4 * n)
+    nwlib.calcHash(dh.p, di.p, dpos.p, n, False)
+    nwlib.sortParticles(dh.p, di.p, n)
+    oh, oi = orc.calc_hash(p, None, pos)
+    assert oh.max() >= gx * gy - 10_000  # the grid is covered to its last rows
+    ok, ov = orc.sort_pairs(oh, oi)
+    assert np.array_equal(downu(dh, n), ok)
+    assert np.array_equal(downu(di, n), ov)
