@@ -86,9 +86,10 @@ SINGLE_KERNEL = {"integrate", "langevin", "filament", "hash", "collide", "srd_co
 
 # kernel behind each single-kernel family of the fused path (the names that
 # rocprofv3 reports), for the HBM traffic measured by its counters
-FAMILY_KERNEL = {"md_front": "k_md_front", "md_tiles": "k_tiles_staged<false, 4>", "md_sort": "k_bucket_cells<false>",
-                 "md_collide": "k_collide_rows<1, false, true>", "srdf_tiles": "k_tiles_staged<true, 8>",
-                 "srdf_cells": "k_bucket_cells<true>", "srdf_rotate": "k_srd_rotate"}
+FAMILY_KERNEL = {"md_front": "k_md_front", "md_tiles": "k_tiles_staged<false, 4, 1024>",
+                 "md_sort": "k_bucket_cells<false, 256>",
+                 "md_collide": "k_collide_rows<1, false, true>", "srdf_tiles": "k_tiles_staged<true, 16, 512>",
+                 "srdf_cells": "k_bucket_cells<true, 256>", "srdf_rotate": "k_srd_rotate"}
 
 
 def measured_traffic(workload, family):

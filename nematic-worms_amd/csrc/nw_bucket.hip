@@ -186,13 +186,13 @@ __device__ __forceinline__ void tiles_staged_body(const float4 *__restrict__ pos
     for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) pairs[i0 + k] = stage[k];
 }
 
-template <bool FROM_POS, int ITEMS>
-__global__ __launch_bounds__(kTileBlock) void k_tiles_staged(const float4 *__restrict__ pos,
+template <bool FROM_POS, int ITEMS, int TB = kTileBlock>
+__global__ __launch_bounds__(TB) void k_tiles_staged(const float4 *__restrict__ pos,
                                                              uint32_t *__restrict__ hash,
                                                              uint32_t *__restrict__ pairs, uint32_t *__restrict__ P,
                                                              BucketPlan bp, KP kp) {
     extern __shared__ __align__(16) uint32_t lh[];
-    tiles_staged_body<FROM_POS, ITEMS>(pos, hash, pairs, P, bp, kp, blockIdx.x, 0u, lh);
+    tiles_staged_body<FROM_POS, ITEMS, TB>(pos, hash, pairs, P, bp, kp, blockIdx.x, 0u, lh);
 }
 
 // the replica that owns item v of a launch over all replicas (offsets
@@ -720,16 +720,17 @@ static hipError_t grant_lds(const void *fn, size_t lds) {
     return e;
 }
 
-template <bool FROM_POS, int ITEMS>
+template <bool FROM_POS, int ITEMS, int TB = kTileBlock>
 static hipError_t launch_staged(const float4 *pos, uint32_t *hash, const BucketPlan &bp, BucketScratch &s,
                                 const KP &kp, hipStream_t st) {
-    const size_t lds = (((size_t(bp.buckets) + 1 + kTileBlock / 64 + 3) & ~size_t(3)) * sizeof(uint32_t)) +
+    static_assert(ITEMS * TB == 4096 || ITEMS * TB == 8192 || ITEMS * TB == 16384, "tile size");
+    const size_t lds = (((size_t(bp.buckets) + 1 + TB / 64 + 3) & ~size_t(3)) * sizeof(uint32_t)) +
                        size_t(bp.tileItems) * sizeof(uint32_t);
-    const void *fn = reinterpret_cast<const void *>(&k_tiles_staged<FROM_POS, ITEMS>);
+    const void *fn = reinterpret_cast<const void *>(&k_tiles_staged<FROM_POS, ITEMS, TB>);
     hipError_t e = grant_lds(fn, lds);
     if (e != hipSuccess) return e;
-    NW_LAUNCH((k_tiles_staged<FROM_POS, ITEMS>), dim3(bp.tiles), dim3(kTileBlock), lds, st, pos, hash,
-                       s.pairs, s.P, bp, kp);
+    NW_LAUNCH((k_tiles_staged<FROM_POS, ITEMS, TB>), dim3(bp.tiles), dim3(TB), lds, st, pos, hash, s.pairs, s.P,
+              bp, kp);
     return hipGetLastError();
 }
 
@@ -743,8 +744,11 @@ static hipError_t launch_tiles(bool fromPos, const float4 *pos, uint32_t *hash, 
         if (bp.tileItems == 4 * kTileBlock)
             return fromPos ? launch_staged<true, 4>(pos, hash, bp, s, kp, st)
                            : launch_staged<false, 4>(pos, hash, bp, s, kp, st);
+        // the SRD tiles of 8192 as 512 threads x 16: three workgroups per CU
+        // by LDS and registers against two of 1024 threads (SRD workload:
+        // 708 tiles in one round, 35 -> 33 us)
         if (bp.tileItems == 8 * kTileBlock)
-            return fromPos ? launch_staged<true, 8>(pos, hash, bp, s, kp, st)
+            return fromPos ? launch_staged<true, 16, 512>(pos, hash, bp, s, kp, st)
                            : launch_staged<false, 8>(pos, hash, bp, s, kp, st);
         if (bp.tileItems == 16 * kTileBlock)
             return fromPos ? launch_staged<true, 16>(pos, hash, bp, s, kp, st)
