@@ -51,7 +51,16 @@ void randomizeUniform(float *device, uint n);
 void randomizeGaussian(float *device, uint n, float mean, float stddev);
 
 /* particleSystem.cuh:18 declares int, particleSystem_cuda.cu:66 defines size_t;
-   size_t is used here (int callers convert implicitly). */
+   size_t is used here, as in the reference's definition.  A host compiled
+   against the reference header passes an int in the 64-bit size register,
+   whose upper half the x86-64 SysV ABI leaves unspecified -- the same
+   mismatch the reference has between its header and its definition.  GCC
+   and Clang form an int argument with 32-bit instructions (which clear the
+   upper half) when it comes from int arithmetic, as every size in
+   particleSystem.cpp does; an int narrowed from a 64-bit value may keep the
+   old upper half, and the allocation is then larger than asked (never
+   smaller) or fails with an error.  tests/test_gpu_int_callers.py calls
+   this entry point from C through the reference's int prototype. */
 void allocateArray(void **devPtr, size_t size);
 /* particleSystem.cuh:19 / particleSystem_cuda.cu:71-74 */
 void freeArray(void *devPtr);
@@ -252,6 +261,11 @@ void nwFusedSrdStep(float *pos, float *vel, float *solventCellCOM, uint *gridPar
    numCells cells (filamentSize 1..64: the MD step's tiles of whole worms;
    0: the SRD step), else 0.  No device work. */
 int nwFusedPlanFits(uint numParticles, uint numCells, uint filamentSize);
+/* The plan of that cell sort (no counterpart in the reference; no device
+   work): out[0] log2 of the cells per bucket, out[1] buckets, out[2]
+   particles per first-level tile, out[3] tiles.  Returns what
+   nwFusedPlanFits returns. */
+int nwFusedPlanInfo(uint numParticles, uint numCells, uint filamentSize, uint out[4]);
 
 /* ------------------------------------------------------------------------
    Batched replicas: many independent worm systems of uneven size stepped by

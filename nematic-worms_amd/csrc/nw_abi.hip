@@ -422,7 +422,11 @@ void calcHash(uint *gridParticleHash, uint *gridParticleIndex, float *pos, int n
     TimedScope ts(srd ? "srd_hash" : "hash", s.stream);
     NW_HIP(launch_calc_hash(gridParticleHash, gridParticleIndex, reinterpret_cast<const float4 *>(pos),
                             uint32_t(numParticles), srd, make_kp(), s.stream));
-    s.lastHashBound = srd ? s.solvent.numCells : s.params.numCells;
+    // every key calcHash writes lies below the product of the grid's sides
+    // (numCells may disagree with it; the sort must never clamp a key)
+    const nw_uint3 g = srd ? s.solvent.gridSize : s.params.gridSize;
+    const uint64_t bound = uint64_t(g.x) * g.y * g.z;
+    s.lastHashBound = bound > 0 && bound <= 0xffffffffull ? uint32_t(bound) : 0u;
     s.lastHashPtr = gridParticleHash;
     s.lastHashSrd = srd;
 }
@@ -1007,23 +1011,45 @@ void nwBatchMdStep(float *pos, float *vel, float *force, float *forceOld, float 
         ++b.steps;
         if (b.swap) ph = 1;
         if (done < steps) {
-            for (int k = 0; k < (b.swap ? 2 : 1); ++k) {
-                NW_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipSuccess;
+            for (int k = 0; k < (b.swap ? 2 : 1) && e == hipSuccess; ++k) {
+                if ((e = hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) break;
                 const hipError_t e1 = batch_step(b, ar[k], gv, gt, a.sVel, a.sTan, deltaTime, s.stream);
                 hipGraph_t g = nullptr;
                 const hipError_t e2 = hipStreamEndCapture(s.stream, &g);
-                NW_HIP(e1);
-                NW_HIP(e2);
+                e = e1 != hipSuccess ? e1 : e2;
+                if (e != hipSuccess) break;
                 b.graph[k] = g;
-                NW_HIP(hipGraphInstantiate(&b.exec[k], g, nullptr, nullptr, 0));
+                e = hipGraphInstantiate(&b.exec[k], g, nullptr, nullptr, 0);
+            }
+            if (e != hipSuccess) {
+                // as in the replay loop below: the contents go back to the
+                // caller's roles before the error is reported
+                batch_drop_graph(b);
+                if (ph == 1) (void)launch_swap4(a.force, a.forceOld, b.N, s.stream);
+                fail(__FILE__, __LINE__, int(e), "nwBatchMdStep: graph capture failed: HIP error %d (%s)", int(e),
+                     hipGetErrorString(e));
+                return;
             }
             b.key = key;
         }
     }
     for (; done < steps; ++done) {
-        TimedScope ts("batch_step", s.stream);
-        if (useGraph) NW_HIP(hipGraphLaunch(b.exec[ph], s.stream));
-        else NW_HIP(batch_step(b, ar[ph], gv, gt, a.sVel, a.sTan, deltaTime, s.stream));
+        hipError_t e = hipSuccess;
+        {
+            TimedScope ts("batch_step", s.stream);
+            e = useGraph ? hipGraphLaunch(b.exec[ph], s.stream)
+                         : batch_step(b, ar[ph], gv, gt, a.sVel, a.sTan, deltaTime, s.stream);
+        }
+        if (e != hipSuccess) {
+            // the buffers' roles are exchanged after an odd number of steps:
+            // exchange their contents back so force / forceOld keep their
+            // meaning for the caller, then report the failing step
+            if (ph == 1) (void)launch_swap4(a.force, a.forceOld, b.N, s.stream);
+            fail(__FILE__, __LINE__, int(e), "nwBatchMdStep: step %u of %u failed: HIP error %d (%s)", done, steps,
+                 int(e), hipGetErrorString(e));
+            return;
+        }
         ++b.steps;
         if (b.swap) ph ^= 1;
     }
@@ -1752,6 +1778,18 @@ double nwPolarSpeedGrad(const float *pos, const float *vel, uint numFilaments, f
                  o.polarSpeed);
     NW_HIP_R(hipStreamSynchronize(s.stream), o.polarSpeed);
     return o.polarSpeed;
+}
+
+int nwFusedPlanInfo(uint numParticles, uint numCells, uint filamentSize, uint out[4]) {
+    const BucketPlan bp = make_bucket_plan(numParticles, numCells, 0);
+    if (out) {
+        out[0] = bp.shift;
+        out[1] = bp.buckets;
+        out[2] = bp.tileItems;
+        out[3] = bp.tiles;
+    }
+    if (filamentSize > 64 || numCells == 0) return 0;
+    return *bucket_plan_problem(bp) ? 0 : 1;
 }
 
 int nwFusedPlanFits(uint numParticles, uint numCells, uint filamentSize) {

@@ -140,6 +140,7 @@ PROTOTYPES = {
     "nwFusedMdStepSwapForces": (None, [P, P, P, P, P, P, P, P, P, P, c_float, c_float, c_float, c_int, c_uint]),
     "nwFusedSrdStep": (None, [P, P, P, P, P, c_float, c_uint, c_uint, c_int]),
     "nwFusedPlanFits": (c_int, [c_uint, c_uint, c_uint]),
+    "nwFusedPlanInfo": (c_int, [c_uint, c_uint, c_uint, P]),
     "nwObservables": (None, [ctypes.c_void_p, ctypes.c_void_p, c_uint, ctypes.POINTER(NWObservables)]),
     "nwBatchSetReplicas": (None, [ctypes.POINTER(NWReplicaParams), c_uint]),
     "nwBatchLayout": (None, [ctypes.c_void_p, ctypes.c_void_p]),

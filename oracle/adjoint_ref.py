@@ -124,8 +124,10 @@ class Model:
             Fw[:, :-2] += fi
             Fw[:, 1:-1] += -fi - fk
             Fw[:, 2:] += fk
-        # WCA over all pairs of beads of different worms
-        i, j = np.triu_indices(self.N, 1)
+        # WCA between beads of different worms: all pairs under a complex
+        # step, else the pairs within the cutoff from a periodic k-d tree
+        # (the same pairs and terms, summed in another order)
+        i, j = self.pairs(r) if not cplx else np.triu_indices(self.N, 1)
         keep = self.worm[i] != self.worm[j]
         i, j = i[keep], j[keep]
         d = self.mi(r[j] - r[i])
@@ -138,6 +140,18 @@ class Model:
         np.add.at(F, i, -f * d)
         np.add.at(F, j, f * d)
         return F
+
+    def pairs(self, r):
+        """index pairs i < j closer than the cutoff under the minimum image,
+        with some margin (the caller applies the exact cutoff)"""
+        from scipy.spatial import cKDTree
+        x = np.mod(np.asarray(r, np.float64) + self.L / 2, self.L)
+        x = np.minimum(x, np.nextafter(self.L, 0))  # [0, L) for the tree
+        ij = cKDTree(x, boxsize=self.L).query_pairs(self.rc * (1 + 1e-9), output_type="ndarray")
+        if len(ij) == 0:
+            return np.zeros(0, np.int64), np.zeros(0, np.int64)
+        a, b = ij[:, 0].astype(np.int64), ij[:, 1].astype(np.int64)
+        return np.minimum(a, b), np.maximum(a, b)
 
     def noise(self, off):
         return orc.random_gaussian(self.seed, off, 4 * self.N).reshape(self.N, 4)[:, :3]

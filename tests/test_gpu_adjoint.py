@@ -248,3 +248,49 @@ def test_polar_speed_gradient_matches_differences(nwlib):
             fdv = (_polar_speed(p, base_r, vp, p.numFilaments) - _polar_speed(p, base_r, vm, p.numFilaments)) / (2 * h)
             assert gv[i, a] == pytest.approx(fdv, rel=2e-3, abs=2e-7), (i, a)
     assert np.all(g[:, 3] == 0) and np.all(gv[:, 3] == 0)
+
+
+# ---- the dry differentiation workload's shape ----------------------------------
+def test_config3_shaped_gradient_matches_differences(nwlib):
+    """50 worms x 16 beads in a 32 x 32 box (the bead density of 2,000 x 16
+    beads in 200 x 200), rows of straight worms with random polarity, 200
+    steps with checkpoints every 20, loss = the final bond-tangent nematic
+    order S: d S / d (f_a, kappa, epsilon, gamma) of the FP32 adjoint against
+    FP64 central differences of the FP64 forward (h = 1e-4 relative) at
+    1e-3."""
+    import config_cases as cc
+    lib = nwlib
+    p, e = cc.config3_system()
+    pos, vel = cc.config3_state()
+    W, N = p.numFilaments, len(pos)
+    steps, K = 200, 20
+    theta = cc.C3_THETA
+    lib.setParameters(ctypes.byref(p))
+    lib.nwSetExtParameters(ctypes.byref(e))
+    lib.nwRngSetSeed(cc.C3_SEED, cc.C3_OFFSET)
+    zero = np.zeros((N, 4), np.float32)
+    d = [DeviceArray.from_numpy(lib, a) for a in (pos, vel, zero, zero, zero)]
+    lib.nwAdjointForward(d[0].p, d[1].p, d[2].p, d[3].p, d[4].p, W, cc.DT, theta[3], cc.KT / cc.DT, 1, steps, K)
+    dS = DeviceArray(lib, 16 * N)
+    S = lib.nwBondOrderGrad(d[0].p, W, dS.p)
+    g = (ctypes.c_double * 4)()
+    lib.nwAdjointBackward(dS.p, None, g)
+    g = np.array(g)
+
+    def S64(t):
+        m = Model(p, e, W, cc.DT, cc.KT / cc.DT, True, cc.C3_SEED, cc.C3_OFFSET)
+        rf, *_ = m.run(pos, vel, zero, zero, t, steps)
+        return cc.bond_order_S(p, rf, W)
+
+    assert abs(S - S64(theta)) < 1e-5
+    fd = np.zeros(4)
+    for k in range(4):
+        h = 1e-4 * abs(theta[k])
+        tp, tm = list(theta), list(theta)
+        tp[k] += h
+        tm[k] -= h
+        fd[k] = (S64(tp) - S64(tm)) / (2 * h)
+    rel = np.abs(g - fd) / np.abs(fd)
+    print(f"config-3 shaped: S {S:.6f} grad {g} fd {fd} rel err {rel}", flush=True)
+    assert np.all(np.abs(fd) > 0)
+    np.testing.assert_allclose(g, fd, rtol=1e-3, atol=1e-4 * np.abs(fd).max())

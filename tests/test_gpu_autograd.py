@@ -51,7 +51,8 @@ def test_bond_order_objective_gradient(nwlib):
             rf, *_ = fp64_model(t).run(pos, vel, zero, zero, t, 40)
             vals.append(_bond_order_S(p, rf, p.numFilaments))
         fd[k] = (vals[0] - vals[1]) / (2 * h)
-    np.testing.assert_allclose(g, fd, rtol=5e-3, atol=5e-3 * np.abs(fd).max())
+    print(f"S-loss gradient {g} fd {fd} rel err {np.abs(g - fd) / np.abs(fd)}", flush=True)
+    np.testing.assert_allclose(g, fd, rtol=1e-3, atol=1e-4 * np.abs(fd).max())
 
 
 def _replica_srd_setup():

@@ -282,3 +282,66 @@ def test_swap_forces_step_matches_fused_step(nwlib, mode, lgv):
         res.append([x.download(np.float32, (N, 4)) for x in (d["pos"], d["vel"], d["tan"], fa, fb)])
     for a, b in zip(*res):
         assert np.array_equal(bits(a), bits(b))
+
+
+def test_fused_srd_step_8192_particle_tiles(nwlib):
+    """3,003,000 particles at 7 per SRD cell (655 x 655 cells): the sort's plan
+    takes first-level tiles of 8192 particles -- the SRD tile kernel of 512
+    threads x 16 particles in nwFusedSrdStep, the 1024 x 8 kernel in the
+    per-call sortParticles.  calcHash + sortParticles against the oracle's
+    stable sort bit for bit; nwFusedSrdStep against the per-call SRD sequence
+    bit for bit."""
+    rng = np.random.default_rng(8192)
+    lib = nwlib
+    G = 655
+    p = sim_params(grid=(G, G, 1), cell=1.0, W=100, n=10)
+    pos, vel, _ = random_worms(p, rng, twod=True)
+    N = p.numParticles
+    Ns = 7 * G * G - N
+    NT, Cs = N + Ns, G * G
+    plan = (ctypes.c_uint * 4)()
+    assert lib.nwFusedPlanInfo(NT, Cs, 0, plan) == 1
+    assert plan[2] == 8192, list(plan)  # particles per first-level tile
+    sp = np.zeros((Ns, 4), np.float32)
+    sp[:, :2] = (rng.random((Ns, 2)) * G - G / 2).astype(np.float32)
+    sp[:, 2] = p.origin.z + 0.5
+    sv = np.zeros((Ns, 4), np.float32)
+    sv[:, :2] = rng.normal(0, 1, (Ns, 2))
+    sv[:, 3] = 1.0
+    P, V = np.concatenate([pos, sp]), np.concatenate([vel, sv])
+    s = solvent_params(p, (G, G, 1), (1.0, 1.0, 1.0), alpha=np.pi / 2, kbT=0.3, offset=(0.21, -0.37, 0.0), ns=Ns)
+    e = ext_params(srdCollideFilaments=1)
+    lib.setParameters(ctypes.byref(p))
+    lib.setSolventParameters(ctypes.byref(s))
+    lib.nwSetExtParameters(ctypes.byref(e))
+    dt = 0.05
+    # the per-call hash and sort against the oracle
+    oh, oi = orc.calc_hash(p, s, P, srd=True)
+    oh, oi = orc.sort_pairs(oh, oi)
+    res = []
+    for fused in (False, True):
+        dP, dV = dev(lib, P), dev(lib, V)
+        h, i = DeviceArray(lib, 4 * NT), DeviceArray(lib, 4 * NT)
+        com = DeviceArray(lib, 16 * Cs)
+        lib.nwRngSetSeed(5, 11)
+        if fused:
+            lib.nwFusedSrdStep(dP.p, dV.p, com.p, h.p, i.p, dt, N, Ns, 0)
+        else:
+            cs, ce = DeviceArray(lib, 4 * Cs), DeviceArray(lib, 4 * Cs)
+            sP, sV = DeviceArray(lib, 16 * NT), DeviceArray(lib, 16 * NT)
+            lib.calcHash(h.p, i.p, dP.p, NT, True)
+            lib.sortParticles(h.p, i.p, NT)
+            assert np.array_equal(h.download(np.uint32, (NT,)), oh)
+            assert np.array_equal(i.download(np.uint32, (NT,)), oi)
+            lib.reorderDataAndFindCellStart(cs.p, ce.p, sP.p, sV.p, None, h.p, i.p, dP.p, dV.p, None, NT, Cs)
+            lib.solventCellCentorOfMomentum(com.p, sV.p, cs.p, ce.p, i.p, Cs)
+            lib.collideSolvent(dP.p, dV.p, com.p, None, Cs, NT)
+            lib.integrateSolvent(dP.p, dV.p, dt, N, Ns)
+        res.append((dP.download(np.float32, (NT, 4)), dV.download(np.float32, (NT, 4)),
+                    com.download(np.float32, (Cs, 4)), i.download(np.uint32, (NT,)), h.download(np.uint32, (NT,))))
+    (pa, va, ca, ia, ha), (pb, vb, cb, ib, hb) = res
+    assert np.array_equal(ib, oi)  # the fused step's stable cell order
+    assert np.array_equal(hb[ib], ha)
+    assert np.array_equal(bits(pa), bits(pb))
+    assert np.array_equal(bits(va), bits(vb))
+    assert np.array_equal(bits(ca[:, :3]), bits(cb[:, :3]))

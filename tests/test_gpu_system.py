@@ -230,3 +230,28 @@ def test_sorted_solvent_arrays_in_original_order(lib):
     # beads keep their slots; the solvent's identities follow their particles
     assert np.abs(P3[N + 7, :2] - P2[N + 7, :2]).max() < 0.5
     s.close()
+
+
+def test_benchmark_workload_full_size_matches_oracle(lib):
+    """The benchmark's workload at full size -- 40,000 worms x 20 beads,
+    5,000,000 SRD solvent particles, 890^2 MD cells, 1000^2 SRD cells, the
+    solvent kept in cell order, the force buffers swapped -- against the
+    oracle's FP32 step from the same state: after the first step (with its
+    collision) the forces per bead within 1e-5, then positions and velocities
+    after that step and after a second collision within the system bars."""
+    from nwtest import force_tol_check
+    cfg = system.config_srd_800k(solvent_sort_every=20)
+    s = system.System(cfg)
+    system.apply_worm_physics(s, kappa=5.0, gamma=2.0, kT=0.1)
+    s.reset(system.NW_INIT_LATTICE_RANDOM, 0.1)
+    assert (s.n, s.ns) == (800_000, 5_000_000)
+    cpu = make_cpu(s, cfg, 1e-3)
+    s.update(1e-3, 1)
+    cpu.step(1)
+    force_tol_check(s.get(system.FORCE), cpu.get("force"), name="800k forces after one step")
+    compare(s, cpu, 1e-4, 1e-3)
+    s.update(1e-3, 10)
+    cpu.step(10)
+    dp, dv = compare(s, cpu, 1e-4, 1e-3)
+    print(f"800k + 5M after 11 steps: max |dpos| {dp:.2e} max |dvel| {dv:.2e}", flush=True)
+    s.close()
