@@ -6,7 +6,12 @@ cell), WCA excluded volume, harmonic bonds k = 57 r0 = 0.8, bending kappa = 5,
 drive f_a = 1 along the bond tangent, SRD collision every 10 steps (90 deg,
 random sign per cell, grid shift in [-0.5, 0.5), beads included), dt = 1e-3.
 A step is one ParticleSystem::update() through the C-ABI; inputs are resident
-in HBM.  value = beads x steps / wall time of the timed region.
+in HBM.  The system first relaxes from its lattice start for --relax-steps
+untimed steps (the order parameters of the start and of the timed state are
+reported).  value = beads x steps / wall time of the timed region, the
+window charged its exact share of the solvent re-sort that the host driver
+runs once every 20 collisions (the re-sorts the window held replaced by
+steps / 200 re-sorts at the average re-sort time).
 
 Also reported: per-kernel average durations (HIP events on the library
 stream), the roofline of the dominant kernel (algorithmic bytes per launch /
@@ -58,24 +63,18 @@ ALG_BYTES = {
     "srdf_cells": lambda w: 24 * w["NT"] + 20 * w["Cs"] + w["Ps"],  # R pairs, W ids, gather vel, W offsets com
     # R cell, RMW vel (+pos), R com; beads only when rotated or rescaled
     "srdf_rotate": lambda w: 36 * w["Nrot"] + 32 * w["Ns"] + 16 * w["Cs"],
+    # batched step over all replicas (WCA, no tangent gathers, swapped force
+    # buffers): the fused kernels' bytes over the concatenated beads and cells
+    "batch_front": lambda w: 132 * w["N"],
+    "batch_tiles": lambda w: 8 * w["N"] + w["Pm"],
+    "batch_sort": lambda w: 44 * w["N"] + 4 * w["C"] + w["Pm"],
+    "batch_collide": lambda w: 56 * w["N"] + 4 * w["C"],
+    # one adjoint step: R r', own lF, sorted r' and ids of the cell lists,
+    # RMW lr lv lG, W lF of the previous force (+ R v' with the Langevin
+    # thermostat), R the CSR offsets
+    "adj_step": lambda w: (180 if w["lgv"] else 164) * w["N"] + 4 * w["C"],
+    "badj_step": lambda w: (180 if w["lgv"] else 164) * w["N"] + 4 * w["C"],
 }
-
-
-def bucket_plan(n, cells, tile):
-    """The fused sort's plan (make_bucket_plan): buckets of 2^S cells with at
-    most ~1536 members on average; returns (buckets, tiles)."""
-    per = n / cells
-    S = 0
-    while S < 11 and (1 << (S + 1)) * per <= 1536.0:
-        S += 1
-    B = (cells + (1 << S) - 1) >> S
-    if tile == 0:
-        want, tile = n * (B + 1) / float(1 << 21), 4096
-        while tile < (1 << 17) and tile < want:
-            tile <<= 1
-        if tile == 16384:  # two workgroups per CU instead of one
-            tile = 8192
-    return B, (n + tile - 1) // tile
 
 
 # families that are one kernel (the roofline is quoted for a single kernel)
@@ -86,22 +85,43 @@ SINGLE_KERNEL = {"integrate", "langevin", "filament", "hash", "collide", "srd_co
 
 # kernel behind each single-kernel family of the fused path (the names that
 # rocprofv3 reports), for the HBM traffic measured by its counters
-FAMILY_KERNEL = {"md_front": "k_md_front", "md_tiles": "k_tiles_staged<false, 4, 1024>",
-                 "md_sort": "k_bucket_cells<false, 256>",
-                 "md_collide": "k_collide_rows<1, false, true>", "srdf_tiles": "k_tiles_staged<true, 16, 512>",
-                 "srdf_cells": "k_bucket_cells<true, 256>", "srdf_rotate": "k_srd_rotate"}
+FAMILY_KERNEL = {"md_sort": "k_bucket_cells<false, 256>", "srdf_cells": "k_bucket_cells<true, 256>",
+                 "srdf_rotate": "k_srd_rotate"}
 
 
-def measured_traffic(workload, family):
+def dispatched_kernel(family, w):
+    """The kernel this run dispatches for a single-kernel family of the fused
+    path (the tile kernels follow the sort's plan of this run)."""
+    tun = w.get("tuning") or {}
+    if family == "md_front":
+        return f"k_md_front<{tun.get('frontBlock') or 256}>"
+    if family == "md_collide":
+        return f"k_collide_rows<1, false, true, {tun.get('pairBlock') or 256}>"
+    if family == "md_tiles":
+        items = w["md_plan"][2]
+        shapes = {1024: "4, 256", 2048: "8, 256", 4096: "4, 1024", 8192: "8, 1024", 16384: "16, 1024"}
+        return f"k_tiles_staged<false, {shapes[items]}>" if items in shapes else None
+    if family == "srdf_tiles":
+        items = w["srd_plan"][2]
+        shapes = {4096: "4, 1024", 8192: "16, 512", 16384: "16, 1024"}
+        return f"k_tiles_staged<true, {shapes[items]}>" if items in shapes else None
+    return FAMILY_KERNEL.get(family)
+
+
+def measured_traffic(workload, family, w):
     """HBM bytes per dispatch of the family's kernel from the committed
     counter summary of this workload (profiles/<workload>_traffic.json,
-    written by python -m nwamd.profsum traffic), or None."""
+    written by python -m nwamd.profsum traffic) -- taken from that file, not
+    measured in this run -- and the file it came from; (None, None) when the
+    file does not list the kernel this run dispatches."""
     path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
-    kern = FAMILY_KERNEL.get(family)
+    kern = dispatched_kernel(family, w)
     if not kern or not os.path.exists(path):
-        return None
+        return None, None
     b = json.load(open(path)).get("bytes_per_dispatch", {}).get(kern)
-    return int(b) if b else None
+    if not b:
+        return None, None
+    return int(b), f"profiles/{workload}_traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; committed)"
 
 
 def build_workload(name, per_call=False, solvent_sort_every=0):
@@ -121,11 +141,21 @@ def build_workload(name, per_call=False, solvent_sort_every=0):
              wca=cfg.ext.pairPotential == 1, swap=p.reverseProbability == 0.0)
     rot_beads = bool(cfg.ext.srdCollideFilaments) or cfg.ext.srdThermostat == abi.NW_SRD_THERMO_ISOKINETIC
     w["Nrot"] = w["NT"] if rot_beads else w["Ns"]
-    B, tiles = bucket_plan(w["N"], w["C"], 0)
-    w["Pm"] = 4 * tiles * (B + 1)
-    B, tiles = bucket_plan(w["NT"], w["Cs"], 0) if cfg.srd else (0, 0)
-    w["Ps"] = 4 * tiles * (B + 1)
+    w["md_plan"] = fused_plan(w["N"], w["C"], cfg.filamentSize)
+    w["Pm"] = 4 * w["md_plan"][3] * (w["md_plan"][1] + 1)
+    w["srd_plan"] = fused_plan(w["NT"], w["Cs"], 0) if cfg.srd else (0, 0, 0, 0)
+    w["Ps"] = 4 * w["srd_plan"][3] * (w["srd_plan"][1] + 1)
     return s, cfg, w
+
+
+def fused_plan(n, cells, filament_size):
+    """(log2 cells per bucket, buckets, particles per tile, tiles) of the
+    fused cell sort for n particles over `cells` (nwFusedPlanInfo)"""
+    import ctypes
+    from nwamd import abi
+    out = (ctypes.c_uint * 4)()
+    abi.load().nwFusedPlanInfo(n, cells, filament_size, out)
+    return tuple(out)
 
 
 def batched_form(args):
@@ -153,6 +183,29 @@ def batched_form(args):
                         ms_per_step=round(el / steps * 1e3, 4))
     if "graph" in out and "one_per_launch" in out:
         out["speedup"] = round(out["graph"]["value"] / out["one_per_launch"]["value"], 2)
+    # the kernels the graph replays, launched directly for a few steps with
+    # events on their dispatch packets: per-kernel durations, the roofline of
+    # the longest
+    import ctypes
+
+    import numpy as np
+    from nwamd import abi
+    lib = abi.load()
+    fams = ["batch_front", "batch_tiles", "batch_sort", "batch_collide"]
+    timers(lib, fams, True)
+    rs.update(args.dt, 10, replicas.NW_REPLICAS_LAUNCHES)
+    rs.sync()
+    R = rs.R
+    bo, co = np.zeros(R + 1, np.uint32), np.zeros(R + 1, np.uint32)
+    lib.nwBatchLayout(bo.ctypes.data_as(ctypes.c_void_p), co.ctypes.data_as(ctypes.c_void_p))
+    wb = dict(N=N, C=int(co[-1]), Pm=batch_matrix_bytes(bo, co))
+    per = {f: abi.timer_read(lib, f) for f in fams}
+    out["kernels"] = {f: dict(avg_us=round(ms / max(c, 1) * 1e3, 2), launches=c,
+                              gbs=round(ALG_BYTES[f](wb) / (ms / max(c, 1) * 1e-3) / 1e9, 1) if c else None)
+                      for f, (ms, c) in per.items()}
+    dom = max(fams, key=lambda f: per[f][0])
+    out["roofline"] = roofline_of(lib, dom, wb, 1)
+    timers(lib, fams, False)
     o = rs.observables(0)
     out["replica0_S"] = round(o["nematicS"], 4)
     rs.close()
@@ -201,19 +254,22 @@ def adjoint_form(args):
     dS = DeviceArray(lib, 16 * N)
     g = (ctypes.c_double * 4)()
     lib.threadSync()
+    lib.nwTimerEnable(b"adj_step", 10)
     t0 = time.perf_counter()
     lib.nwAdjointForward(*arr, W, dt, p.gamma, p.kbT / dt, 1, steps, K)
     S = lib.nwBondOrderGrad(arr[0], W, dS.p)
     t1 = time.perf_counter()
     lib.nwAdjointBackward(dS.p, None, g)
     t2 = time.perf_counter()
+    roof = roofline_of(lib, "adj_step", dict(N=N, C=G * G, lgv=True), 10)
+    lib.nwTimerEnable(b"adj_step", 0)
     s.close()
     return dict(workload="2000 worms x 16 beads, L 200, dry", beads=N, steps=steps, checkpoint_every=K,
                 forward_ms=round(t_fwd * 1e3, 2), forward_value=round(N * steps / t_fwd, 1),
                 checkpointed_forward_ms=round((t1 - t0) * 1e3, 2), backward_ms=round((t2 - t1) * 1e3, 2),
                 adjoint_over_forward=round((t2 - t0) / t_fwd, 3), backward_over_forward=round((t2 - t1) / t_fwd, 3),
                 value=round(N * steps / (t2 - t0), 1), unit="bead-steps/s (forward + adjoint)",
-                loss_S=round(S, 6), grad=dict(drive=g[0], kappa=g[1], epsilon=g[2], gamma=g[3]))
+                loss_S=round(S, 6), grad=dict(drive=g[0], kappa=g[1], epsilon=g[2], gamma=g[3]), roofline=roof)
 
 
 def batch_adjoint_form(args):
@@ -300,6 +356,7 @@ def batch_adjoint_form(args):
     lib.nwBatchPolarSpeedGrad(P.p, V.p, dP.p, dV.p, Pr)
     lib.nwBatchAdjointBackward(dP.p, dV.p, g)
     lib.threadSync()
+    lib.nwTimerEnable(b"badj_step", 10)
     t0 = time.perf_counter()
     lib.nwBatchAdjointForward(P.p, V.p, F.p, G.p, T.p, dt, gap, shp, steps, K)
     lib.threadSync()
@@ -307,6 +364,10 @@ def batch_adjoint_form(args):
     lib.nwBatchPolarSpeedGrad(P.p, V.p, dP.p, dV.p, Pr)
     lib.nwBatchAdjointBackward(dP.p, dV.p, g)
     t2 = time.perf_counter()
+    co = np.zeros(R + 1, np.uint32)
+    lib.nwBatchLayout(None, co.ctypes.data_as(ctypes.c_void_p))
+    roof = roofline_of(lib, "badj_step", dict(N=N, C=int(co[-1]), lgv=False), 10)
+    lib.nwTimerEnable(b"badj_step", 0)
     Pv = np.array(Pr[:R])
     dfa = 2.0 * (Pv - 0.5) * np.array(g[:])[0::4]  # d loss / d f_a of every replica
     return dict(workload=f"{R} replicas x 1000 worms x 12 beads, L 120, SRD 3 per cell", replicas=R, beads=N,
@@ -317,7 +378,34 @@ def batch_adjoint_form(args):
                 value=round(N * steps / (t2 - t0), 1), unit="bead-steps/s (forward + adjoint)",
                 loss=round(float(((Pv - 0.5) ** 2).sum()), 6),
                 dloss_dfa=dict(replica0=float(dfa[0]), replica1=float(dfa[1]), mean=float(dfa.mean()),
-                               min=float(dfa.min()), max=float(dfa.max())))
+                               min=float(dfa.min()), max=float(dfa.max())), roofline=roof)
+
+
+def roofline_of(lib, family, w, stride):
+    """roofline of one timed single-kernel family: its algorithmic bytes per
+    launch over its average duration (HIP events on its dispatch packets)"""
+    from nwamd import abi
+    ms, cnt = abi.timer_read(lib, family)
+    avg_s = ms / max(cnt, 1) * 1e-3
+    alg = ALG_BYTES[family](w)
+    achieved = alg / avg_s / 1e9 if avg_s > 0 else 0.0
+    return dict(bound="hbm", kernel=family, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, alg_bytes_per_launch=int(alg),
+                avg_launch_us=round(avg_s * 1e6, 2), launches_timed=cnt, timed_every=stride)
+
+
+def batch_matrix_bytes(bead_off, cell_off):
+    """bytes of the batched sort's prefix matrices: every replica sorted
+    against its own plan (tiles of 4096 beads, buckets of 2^S cells aiming
+    at 512 members; make_sort_reps in the library)"""
+    tot = 0
+    for r in range(len(bead_off) - 1):
+        n, c = int(bead_off[r + 1] - bead_off[r]), int(cell_off[r + 1] - cell_off[r])
+        S = 0
+        while S < 11 and (1 << (S + 1)) * n / c <= 512.0:
+            S += 1
+        tot += 4 * ((n + 4095) // 4096) * (((c + (1 << S) - 1) >> S) + 1)
+    return tot
 
 
 def timers(lib, names, enable):
@@ -371,6 +459,14 @@ def main():
                          "(a known-size reference for HBM byte counters)")
     ap.add_argument("--profile-steps", type=int, default=20,
                     help="untimed steps after the warm-up that rank the kernel families")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in launch shapes (default: the library's autotuner picks them on a copy "
+                         "of the relaxed state before the warm-up)")
+    ap.add_argument("--relax-steps", type=int, default=20000,
+                    help="untimed steps from the lattice start before the warm-up, so the timed state is a "
+                         "relaxed one (its order parameters are reported)")
+    ap.add_argument("--shuffled-steps", type=int, default=200,
+                    help="timed steps of the same state with its worms permuted in memory (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--per-call-steps", type=int, default=100,
                     help="steps of the reference per-call sequence timed beside (0: skip)")
@@ -388,6 +484,40 @@ def main():
     s, cfg, w = build_workload(args.workload, solvent_sort_every=args.solvent_sort_every)
     dt = args.dt
     names = list(ALG_BYTES)
+
+    # relaxation from the lattice start (untimed), timing every solvent
+    # re-sort it contains
+    obs0 = s.observables()
+    resort_every = (cfg.srdGap or 5) * args.solvent_sort_every if cfg.srd and args.solvent_sort_every else 0
+    if resort_every:
+        lib.nwTimerEnable(b"srd_resort", 1)
+    t0 = time.perf_counter()
+    s.update(dt, args.relax_steps)
+    s.sync()
+    relax_s = time.perf_counter() - t0
+    resort_ms, resorts = abi.timer_read(lib, "srd_resort") if resort_every else (0.0, 0)
+    lib.nwTimerEnable(b"srd_resort", 0)
+    obs1 = s.observables()
+    state = dict(relax_steps=args.relax_steps, relax_seconds=round(relax_s, 2),
+                 start=dict(nematicS=round(obs0["nematicS"], 4), polarP=round(obs0["polarP"], 4),
+                            polarSpeed=round(obs0["polarSpeed"], 4)),
+                 timed=dict(nematicS=round(obs1["nematicS"], 4), polarP=round(obs1["polarP"], 4),
+                            polarSpeed=round(obs1["polarSpeed"], 4)))
+
+    # launch shapes of the fused kernels: the library's autotuner, timed on a
+    # copy of the relaxed state (or from its cache file)
+    tuning, tuned = None, "built-in"
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        rc, tuning = s.autotune(dt)
+        tuned = {1: "timed", 2: "cache"}.get(rc, "failed")
+        tuning["tuner_seconds"] = round(time.perf_counter() - t0, 3)
+        w["tuning"] = tuning
+        w["md_plan"] = fused_plan(w["N"], w["C"], cfg.filamentSize)
+        w["Pm"] = 4 * w["md_plan"][3] * (w["md_plan"][1] + 1)
+        if cfg.srd:
+            w["srd_plan"] = fused_plan(w["NT"], w["Cs"], 0)
+            w["Ps"] = 4 * w["srd_plan"][3] * (w["srd_plan"][1] + 1)
 
     # untimed warm-up, then (also untimed) every kernel family timed over a
     # few steps to find the dominant one
@@ -409,6 +539,8 @@ def main():
     # launch when it runs every step (an event pair stalls the stream ~5 us)
     stride = 10 if per.get(dominant, {}).get("launches", 0) >= args.profile_steps else 1
     lib.nwTimerEnable(dominant.encode(), stride)
+    if resort_every:
+        lib.nwTimerEnable(b"srd_resort", 1)  # counts the re-sorts inside the window
     s.sync()
     t0 = time.perf_counter()
     s.update(dt, args.steps)
@@ -416,18 +548,52 @@ def main():
     el = time.perf_counter() - t0
     dom_ms, dom_n = abi.timer_read(lib, dominant)
     lib.nwTimerEnable(dominant.encode(), 0)
+    in_window = abi.timer_read(lib, "srd_resort")[1] if resort_every else 0
+    lib.nwTimerEnable(b"srd_resort", 0)
 
-    value = w["N"] * args.steps / el
+    # the solvent re-sort runs once every resort_every steps: the window is
+    # charged its share exactly (the re-sorts it held replaced by the
+    # expected number, at the average re-sort time of the relaxation)
+    per_resort_s = resort_ms / resorts * 1e-3 if resorts else 0.0
+    el_charged = el + ((args.steps / resort_every - in_window) * per_resort_s if resort_every else 0.0)
+    resort = (dict(every_steps=resort_every, avg_ms=round(per_resort_s * 1e3, 3), in_window=in_window,
+                   expected_in_window=round(args.steps / resort_every, 3), window_wall_s=round(el, 6))
+              if resort_every else None)
+    value = w["N"] * args.steps / el_charged
     alg = ALG_BYTES[dominant](w)
     avg_s = dom_ms / max(dom_n, 1) * 1e-3
     achieved = alg / avg_s / 1e9 if avg_s > 0 else 0.0
+    traffic, traffic_source = measured_traffic(args.workload, dominant, w)
     roof = dict(bound="hbm", kernel=dominant, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(args.workload, dominant),
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=traffic_source,
                 alg_bytes_per_launch=int(alg),
                 avg_launch_us=round(avg_s * 1e6, 2), launches_timed=dom_n, timed_every=stride)
     kernels = {k: dict(avg_us=round(v["ms_per_launch"] * 1e3, 2), launches=v["launches"],
                        gbs=round(ALG_BYTES[k](w) / (v["ms_per_launch"] * 1e-3) / 1e9, 1))
                for k, v in per.items()}
+
+    # the same system with its worms in a random order in memory (the
+    # physics is unchanged; bead order no longer follows position, so the
+    # sort's gathers and the front's stores lose their locality)
+    shuffled = None
+    if args.shuffled_steps > 0:
+        import numpy as np
+        from nwamd import system as nsys
+        W, n = cfg.numFilaments, cfg.filamentSize
+        perm = np.random.default_rng(7).permutation(W)
+        for which in (nsys.POSITION, nsys.VELOCITY, nsys.TANGENT, nsys.FORCE, nsys.FORCE_OLD):
+            a = s.get(which)
+            beads = a[:W * n].reshape(W, n, 4)[perm].reshape(W * n, 4)
+            s.set(which, np.concatenate([beads, a[W * n:]]))
+        s.update(dt, args.warmup)
+        s.sync()
+        t0 = time.perf_counter()
+        s.update(dt, args.shuffled_steps)
+        s.sync()
+        el3 = time.perf_counter() - t0
+        shuffled = dict(value=round(w["N"] * args.shuffled_steps / el3, 1), unit="bead-steps/s",
+                        steps=args.shuffled_steps, ms_per_step=round(el3 / args.shuffled_steps * 1e3, 4),
+                        form="the timed state with its worms permuted at random in memory (raw window)")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -454,13 +620,16 @@ def main():
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "bead-steps/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 4),
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el_charged / args.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": args.workload, "beads": w["N"], "solvent": w["Ns"], "md_cells": w["C"],
                    "srd_cells": w["Cs"], "dt": dt,
                    "solvent_sort_every": args.solvent_sort_every,
-                   "form": "plain: one replica, ParticleSystem::update through the fused C-ABI entry points"},
+                   "form": "plain: one replica, ParticleSystem::update through the fused C-ABI entry points",
+                   "md_plan": list(w["md_plan"]), "srd_plan": list(w["srd_plan"]),
+                   "tuning": tuning, "tuning_source": tuned},
+        "state": state, "solvent_resort": resort, "shuffled_worm_order": shuffled,
         "roofline": roof, "cpu_baseline": cpu, "per_call_sequence": per_call, "batched": batched, "adjoint": adjoint,
         "batched_adjoint_srd": batch_adjoint,
         "kernels": kernels,

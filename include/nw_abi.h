@@ -268,6 +268,49 @@ int nwFusedPlanFits(uint numParticles, uint numCells, uint filamentSize);
 int nwFusedPlanInfo(uint numParticles, uint numCells, uint filamentSize, uint out[4]);
 
 /* ------------------------------------------------------------------------
+   Launch shapes and the autotuner (no counterpart in the reference, which
+   launches every kernel with computeGridSize(n, 256),
+   simulation/particleSystem_cuda.cu:111-115).  A zero field takes the
+   built-in choice.  Every setting gives the same results bit for bit --
+   the sort is stable, every pair sum runs in stencil order -- only the
+   time differs.
+   ------------------------------------------------------------------------ */
+typedef struct NWTuning {
+    uint32_t mdTileItems;   /* MD cell sort, first level: beads per tile (1024, 2048, 4096, 8192, 16384) */
+    uint32_t mdMembers;     /* MD cell sort, second level: bucket population aimed at (64..2048) */
+    uint32_t srdTileItems;  /* SRD cell sort, first level: particles per tile (4096, 8192, 16384) */
+    uint32_t srdMembers;    /* SRD cell sort, second level (64..2048) */
+    uint32_t pairBlock;     /* sorted beads per pair-kernel workgroup (128, 256; 512 for the WCA term) */
+    uint32_t frontBlock;    /* threads per workgroup of the fused front (64, 128, 256, 512, 1024) */
+    uint32_t reserved[10];
+} NWTuning;
+/* Applies to the fused single-system steps (nwFusedMdStep and its variants,
+   nwFusedSrdStep) and to nwFusedPlanInfo; values outside the lists above
+   fail with NW error -22 and leave the tuning unchanged.  A tile / bucket
+   pair whose plan does not fit a system falls back to the built-in plan. */
+void nwSetTuning(const NWTuning *tuning);
+void nwGetTuning(NWTuning *tuning);
+
+/* The autotuner: times the candidate launch shapes of nwFusedMdStep on a
+   copy of the given state (pos / vel hold the numFilaments worms' beads,
+   followed by numSolvent solvent particles) -- the front's workgroup size,
+   the MD sort's tile and bucket sizes jointly, the pair kernel's workgroup
+   size, each over `reps` steps with HIP events on the kernels' dispatch
+   packets -- and, when numSolvent > 0, those of nwFusedSrdStep (the SRD
+   sort's tile and bucket sizes) with the SolventParams of the last
+   setSolventParameters.  The fastest choice of each is set (nwSetTuning)
+   and written to out.  The state itself is not touched; the generator is
+   left where it was.  cachePath (NULL: $NW_TUNE_CACHE, else
+   $HOME/.cache/nwamd/tuning.txt; "": no cache) holds one line per device
+   and system size (beads, MD cells, beads per worm, solvent, SRD cells):
+   with useCache a recorded line is applied without timing.  Returns 2 when
+   the choice came from the cache, 1 when it was timed, 0 on an error. */
+int nwAutotuneFused(float *pos, float *vel, float *force, float *forceOld, float *tangent, uint numFilaments,
+                    uint numSolvent, float deltaTime, float langevinGamma, float langevinKbT, int langevin,
+                    float srdDeltaTime, int isokinetic, uint reps, const char *cachePath, int useCache,
+                    NWTuning *out);
+
+/* ------------------------------------------------------------------------
    Batched replicas: many independent worm systems of uneven size stepped by
    one launch of each fused kernel per step (no counterpart in the
    reference).  Replica r's beads occupy [beadOffset[r], beadOffset[r+1]) of
@@ -453,7 +496,11 @@ uint nwGetDeviceFlags(int clear);
    "srd_hash", "srd_sort", "srd_reorder", "srd_com", "srd_thermo",
    "srd_collide", "srd_stream", "reverse"; of the fused steps: "md_front",
    "md_tiles" (the sort's first level), "md_sort", "md_collide",
-   "srdf_tiles", "srdf_cells", "srdf_rotate". */
+   "srdf_tiles", "srdf_cells", "srdf_rotate"; of the batched step's kernels
+   (launched directly, not from a replayed graph): "batch_front",
+   "batch_tiles", "batch_sort", "batch_collide", and "batch_step" around a
+   graph replay; of the adjoint sweeps: "adj_step", "badj_step"; the solvent
+   re-sort nwSortSolvent: "srd_resort". */
 void nwTimerEnable(const char *kernel, int enable);
 /* Synchronises; returns total milliseconds and launch count since enable. */
 void nwTimerRead(const char *kernel, double *totalMs, uint64_t *launches);

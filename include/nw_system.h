@@ -128,6 +128,12 @@ void nwReplicaSetObservables(NWReplicaSet *set, int replica, NWObservables *out)
 void nwSystemObservables(NWSystem *sys, NWObservables *out);
 /* appends one .xyzv frame (particleSystem.cpp:481-532) */
 void nwSystemWriteOutputs(NWSystem *sys, const char *fileName, int iteration, float dt);
+/* runs the library's autotuner (nwAutotuneFused) on a copy of the system's
+   state with its parameters, for steps of dt: the launch shapes of the
+   fused steps it picks stay set for every later update.  cachePath / useCache
+   as for nwAutotuneFused.  Returns what nwAutotuneFused returns (0 when the
+   system does not take the fused steps). */
+int nwSystemAutotune(NWSystem *sys, float dt, const char *cachePath, int useCache, NWTuning *out);
 
 #ifdef __cplusplus
 }

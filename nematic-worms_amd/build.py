@@ -37,7 +37,8 @@ HIP_FLAGS = [
     f"-I{CSRC}",
 ]
 
-LIB_SOURCES = ["nw_kernels.hip", "nw_sort.hip", "nw_fused.hip", "nw_bucket.hip", "nw_observe.hip", "nw_adjoint.hip", "nw_abi.hip"]
+LIB_SOURCES = ["nw_kernels.hip", "nw_sort.hip", "nw_fused.hip", "nw_bucket.hip", "nw_observe.hip", "nw_adjoint.hip",
+               "nw_abi.hip", "nw_tune.hip"]
 HOST_SOURCES = ["host/particle_system.cpp", "host/worm_init.cpp", "host/replica_set.cpp", "host/system_capi.cpp"]
 CLI_SOURCES = ["host/main.cpp"]
 

@@ -202,6 +202,20 @@ void ParticleSystem::observables(NWObservables *out) {
     nwObservables(m_dPos, m_dVel, m_numFilaments, out);
 }
 
+// the launch shapes of the fused steps for this system's sizes, timed on a
+// copy of its state with the arguments _updateFused passes
+int ParticleSystem::autotune(float deltaTime, const char *cachePath, int useCache, NWTuning *out) {
+    if (!fusedActive()) return 0;
+    setParameters(&m_params);
+    nwSetExtParameters(&m_ext);
+    if (m_solventEnabled) setSolventParameters(&m_solvent);
+    const float kT = m_langevinFdt ? m_params.kbT / deltaTime : m_params.kbT;
+    return nwAutotuneFused(m_dPos, m_dVel, m_dForce, m_dForceOld, m_dTangent, m_numFilaments,
+                           m_solventEnabled ? m_numSolvent : 0u, deltaTime, m_params.gamma, kT,
+                           m_solventEnabled ? 0 : 1, deltaTime * float(m_solventUpdateGap),
+                           m_ext.srdThermostat == NW_SRD_THERMO_ISOKINETIC ? 1 : 0, 0, cachePath, useCache, out);
+}
+
 bool ParticleSystem::fusedActive() const {
     if (m_perCall || m_filamentSize < 1 || m_filamentSize > 64) return false;
     const uint mdCells = m_params.gridSize.x * m_params.gridSize.y * m_params.gridSize.z;
@@ -327,55 +341,81 @@ float ParticleSystem::_frand() {
     return u01_open(x);
 }
 
-// particleSystem.cpp:583-678
+// The reference's start (particleSystem.cpp:583-678), restated: every worm
+// gets a slot of n bond lengths plus one bead diameter along x; slots fill a
+// row of the box (as many as fit), rows sit one diameter apart in y and
+// layers one diameter apart in z, the first row and layer one diameter in.
+// A worm lies straight in its slot, head at the slot's left end; then every
+// worm, in order, draws its heading, and a reversed worm (draw >= 1/2) takes
+// its lattice points from tail to head -- except an odd worm's middle bead,
+// which keeps its point and its +x tangent.  Velocities are uniform in
+// [-0.1, 0.1) per component.  Host draws come in the reference's order:
+// three per bead, one per worm, then the solvent's six per particle.
+namespace {
+struct WormLattice {
+    float slot, radius, bond;
+    uint perRow, perLayer;
+    // lattice point k (0 .. n-1 along the slot) of worm w, relative to the
+    // origin, in the reference's operation order
+    void point(uint w, uint k, float out[3]) const {
+        const uint col = w % perRow, row = (w / perRow) % (perLayer / perRow), layer = w / perLayer;
+        const float headX = float(col) * slot + radius;
+        out[0] = headX + float(k) * bond;
+        out[1] = 2.f * radius * float(row + 1);
+        out[2] = 2.f * radius * float(layer + 1);
+    }
+};
+}  // namespace
+
 void ParticleSystem::reset() {
     const SimParams &p = m_params;
-    const float filamentLength = p.filamentSize * p.bondSpringL + 2.0f * p.particleRadius;
-    const int xdim_max = std::max(1, int(std::floor(p.boxSize.x / filamentLength)));
-    const int ydim_max = std::max(1, int(std::floor(p.boxSize.y / (2 * p.particleRadius))));
-    const int xyplane_max = xdim_max * ydim_max;
-    for (uint i = 0; i < m_numParticles; i++) {
-        const uint w = i / p.filamentSize;
-        const float x_head = (w % xdim_max) * filamentLength + p.particleRadius;
-        const float x = x_head + (i % p.filamentSize) * p.bondSpringL;
-        const float y = 2.f * p.particleRadius * ((w / xdim_max) % ydim_max + 1);
-        const float z = 2.f * p.particleRadius * (w / xyplane_max + 1);
-        float *P = &m_hPos[4 * size_t(i)], *V = &m_hVel[4 * size_t(i)], *T = &m_hTangent[4 * size_t(i)];
-        P[0] = x + p.origin.x;
-        P[1] = y + p.origin.y;
-        P[2] = z + p.origin.z;
-        P[3] = p.particleRadius;
-        V[0] = 0.1f * (_frand() * 2.0f - 1.0f);
-        V[1] = 0.1f * (_frand() * 2.0f - 1.0f);
-        V[2] = 0.1f * (_frand() * 2.0f - 1.0f);
+    const uint n = p.filamentSize;
+    WormLattice lat;
+    lat.radius = p.particleRadius;
+    lat.bond = p.bondSpringL;
+    lat.slot = float(n) * lat.bond + 2.0f * lat.radius;
+    lat.perRow = uint(std::max(1, int(std::floor(p.boxSize.x / lat.slot))));
+    lat.perLayer = lat.perRow * uint(std::max(1, int(std::floor(p.boxSize.y / (2 * lat.radius)))));
+
+    auto symmetric = [this](float amp) { return amp * (_frand() * 2.0f - 1.0f); };
+    for (uint i = 0; i < m_numParticles; ++i) {
+        float *V = &m_hVel[4 * size_t(i)];
+        V[0] = symmetric(0.1f);
+        V[1] = symmetric(0.1f);
+        V[2] = symmetric(0.1f);
         V[3] = p.particleMass;
-        T[0] = 1.0f;
-        T[1] = T[2] = T[3] = 0.0f;
     }
-    // flip half the worms
-    for (uint w = 0; w < p.numFilaments; w++) {
-        if (_frand() < 0.5f) continue;
-        for (uint q = 0; q < p.filamentSize / 2; q++) {
-            const size_t i = size_t(w) * p.filamentSize + q, j = size_t(w + 1) * p.filamentSize - 1 - q;
-            for (int c = 0; c < 4; ++c) {
-                std::swap(m_hPos[4 * i + c], m_hPos[4 * j + c]);
-                m_hTangent[4 * i + c] = -m_hTangent[4 * i + c];
-                m_hTangent[4 * j + c] = -m_hTangent[4 * j + c];
-            }
+    for (uint w = 0; w < p.numFilaments; ++w) {
+        const bool reversed = !(_frand() < 0.5f);
+        for (uint q = 0; q < n; ++q) {
+            // a reversed worm's middle bead (odd n) keeps its place and tangent
+            const bool moved = reversed && 2 * q + 1 != n;
+            const uint k = moved ? n - 1 - q : q;
+            float r[3];
+            lat.point(w, k, r);
+            const size_t i = size_t(w) * n + q;
+            float *P = &m_hPos[4 * i], *T = &m_hTangent[4 * i];
+            P[0] = r[0] + p.origin.x;
+            P[1] = r[1] + p.origin.y;
+            P[2] = r[2] + p.origin.z;
+            P[3] = p.particleRadius;
+            const float sgn = moved ? -1.0f : 1.0f;
+            T[0] = sgn;
+            T[1] = T[2] = T[3] = sgn * 0.0f;
         }
     }
     if (m_solventEnabled) {
-        const float v0 = 5.0f;
+        const float v0 = 5.0f;  // the reference's solvent speed scale
         for (uint s = 0; s < m_numSolvent; s++) {
-            const size_t i = size_t(m_numParticles) + s;
-            m_hPos[4 * i + 0] = p.boxSize.x * _frand() + p.origin.x;
-            m_hPos[4 * i + 1] = p.boxSize.y * _frand() + p.origin.y;
-            m_hPos[4 * i + 2] = p.boxSize.z * _frand() + p.origin.z;
-            m_hPos[4 * i + 3] = 0.0f;
-            m_hVel[4 * i + 0] = v0 * (_frand() * 2.0f - 1.0f);
-            m_hVel[4 * i + 1] = v0 * (_frand() * 2.0f - 1.0f);
-            m_hVel[4 * i + 2] = v0 * (_frand() * 2.0f - 1.0f);
-            m_hVel[4 * i + 3] = m_solvent.particleMass;
+            float *P = &m_hPos[4 * (size_t(m_numParticles) + s)], *V = &m_hVel[4 * (size_t(m_numParticles) + s)];
+            P[0] = p.boxSize.x * _frand() + p.origin.x;
+            P[1] = p.boxSize.y * _frand() + p.origin.y;
+            P[2] = p.boxSize.z * _frand() + p.origin.z;
+            P[3] = 0.0f;
+            V[0] = symmetric(v0);
+            V[1] = symmetric(v0);
+            V[2] = symmetric(v0);
+            V[3] = m_solvent.particleMass;
         }
     }
     const size_t NT = size_t(m_numParticles) + m_numSolvent;

@@ -43,6 +43,8 @@ public:
     void writeOutputs(const std::string &fileName, int iteration, float timeDelta);
     // nematic / polar order and speed along the tangent of the worms (on the device)
     void observables(NWObservables *out);
+    // the library's autotuner on a copy of this system's state (fused steps only)
+    int autotune(float deltaTime, const char *cachePath, int useCache, NWTuning *out);
 
     // setters / getters (particleSystem.h:53-138)
     void setDamping(float x) { m_params.gamma = x; }

@@ -92,6 +92,10 @@ void nwSystemWriteOutputs(NWSystem *sys, const char *fileName, int iteration, fl
     sys->ps->writeOutputs(fileName, iteration, dt);
 }
 
+int nwSystemAutotune(NWSystem *sys, float dt, const char *cachePath, int useCache, NWTuning *out) {
+    return sys->ps->autotune(dt, cachePath, useCache, out);
+}
+
 // ---- replica sets ----------------------------------------------------------
 NWReplicaSet *nwReplicaSetCreate(const NWReplicaSpec *specs, uint32_t numReplicas, const NWReplicaPhysics *phys) {
     if ((numReplicas && !specs) || !phys) {

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <map>
 #include <string>
 #include <vector>
@@ -307,6 +308,7 @@ LaunchTiming &launch_timing() {
     static thread_local LaunchTiming lt;
     return lt;
 }
+KP current_kp() { return make_kp(); }
 }  // namespace nw
 
 extern "C" {
@@ -595,7 +597,7 @@ static bool fused_md(const KP &kp, float *pos, float *vel, float *force, float *
     NW_REQUIRE_R(!needVel || sortedVel != nullptr, false, "nwFusedMdStep: sortedVel is NULL");
     NW_REQUIRE_R(!needTan || sortedTangent != nullptr, false, "nwFusedMdStep: sortedTangent is NULL");
     if (N == 0) return true;
-    const BucketPlan bp = make_bucket_plan(N, cells, 0);
+    const BucketPlan bp = fused_plan(N, cells, false);
     NW_REQUIRE_R(!*bucket_plan_problem(bp), false, "nwFusedMdStep: %s", bucket_plan_problem(bp));
     NW_HIP_R(ensure_bucket_scratch(s.bucket, bp), false);
     BucketScratch sc = s.bucket;
@@ -681,7 +683,7 @@ static bool fused_srd(const KP &kp, float *pos, float *vel, float *solventCellCO
     NW_REQUIRE_R(gridParticleIndex != nullptr && solventCellCOM != nullptr, false,
                  "nwFusedSrdStep: NULL output buffer");
     if (NT == 0) return true;
-    const BucketPlan bp = make_bucket_plan(NT, cells, 0);
+    const BucketPlan bp = fused_plan(NT, cells, true);
     NW_REQUIRE_R(!*bucket_plan_problem(bp), false, "nwFusedSrdStep: %s", bucket_plan_problem(bp));
     NW_HIP_R(ensure_bucket_scratch(s.bucket, bp), false);
     BucketScratch sc = s.bucket;
@@ -948,14 +950,26 @@ void nwBatchRngState(uint replica, uint64_t *offset) {
 }
 
 // the launches of one batched step
+// (timed: each kernel may carry the events of its timer family -- never
+// while the stream is being captured into a graph)
 static hipError_t batch_step(State::Batch &b, const BatchArrays &a, const float4 *vel, const float4 *tan,
-                             float4 *sVel, float4 *sTan, float dt, hipStream_t st) {
+                             float4 *sVel, float4 *sTan, float dt, hipStream_t st, bool timed = false) {
     hipError_t e;
-    if ((e = launch_md_front_batch(a, b.dev, b.R, b.waves, b.stepCtr, dt, st)) != hipSuccess) return e;
-    if ((e = launch_hash_tiles_batch(a.hash, b.sortDev, b.R, b.sortPlan, b.scratch, st)) != hipSuccess) return e;
-    if ((e = launch_md_cells_batch(b.sortDev, b.R, b.sortPlan, b.scratch, a.ids, a.pos, vel, tan, a.sPos, sVel, sTan,
-                                   st)) != hipSuccess)
-        return e;
+    {
+        TimedScope ts(timed ? "batch_front" : "", st, true);
+        if ((e = launch_md_front_batch(a, b.dev, b.R, b.waves, b.stepCtr, dt, st)) != hipSuccess) return e;
+    }
+    {
+        TimedScope ts(timed ? "batch_tiles" : "", st, true);
+        if ((e = launch_hash_tiles_batch(a.hash, b.sortDev, b.R, b.sortPlan, b.scratch, st)) != hipSuccess) return e;
+    }
+    {
+        TimedScope ts(timed ? "batch_sort" : "", st, true);
+        if ((e = launch_md_cells_batch(b.sortDev, b.R, b.sortPlan, b.scratch, a.ids, a.pos, vel, tan, a.sPos, sVel,
+                                       sTan, st)) != hipSuccess)
+            return e;
+    }
+    TimedScope ts(timed ? "batch_collide" : "", st, true);
     return launch_collide_batch(a, b.dev, b.R, b.blocks, b.mode, b.tan, b.unit, b.stepCtr, st);
 }
 
@@ -1004,8 +1018,7 @@ void nwBatchMdStep(float *pos, float *vel, float *force, float *forceOld, float 
         // outside the capture), then capture the step (both phases)
         batch_drop_graph(b);
         {
-            TimedScope ts("batch_step", s.stream);
-            NW_HIP(batch_step(b, ar[0], gv, gt, a.sVel, a.sTan, deltaTime, s.stream));
+            NW_HIP(batch_step(b, ar[0], gv, gt, a.sVel, a.sTan, deltaTime, s.stream, true));
         }
         ++done;
         ++b.steps;
@@ -1036,10 +1049,11 @@ void nwBatchMdStep(float *pos, float *vel, float *force, float *forceOld, float 
     }
     for (; done < steps; ++done) {
         hipError_t e = hipSuccess;
-        {
+        if (useGraph) {
             TimedScope ts("batch_step", s.stream);
-            e = useGraph ? hipGraphLaunch(b.exec[ph], s.stream)
-                         : batch_step(b, ar[ph], gv, gt, a.sVel, a.sTan, deltaTime, s.stream);
+            e = hipGraphLaunch(b.exec[ph], s.stream);
+        } else {
+            e = batch_step(b, ar[ph], gv, gt, a.sVel, a.sTan, deltaTime, s.stream, true);
         }
         if (e != hipSuccess) {
             // the buffers' roles are exchanged after an odd number of steps:
@@ -1320,7 +1334,10 @@ void nwAdjointBackward(const float *dLdPos, const float *dLdVel, double grad[4])
             as.dt = A.dt;
             as.seed = A.seed;
             as.offLgv = lgvOff[j];
-            NW_HIP(launch_adj_step(as, A.kp, s.stream));
+            {
+                TimedScope ts("adj_step", s.stream, true);
+                NW_HIP(launch_adj_step(as, A.kp, s.stream));
+            }
             cur = 1 - cur;
         }
         NW_HIP(launch_adj_total(A.partial, uint64_t(len) * rowsPerStep, A.grad, s.stream));
@@ -1647,7 +1664,10 @@ void nwBatchAdjointBackward(const float *dLdPos, const float *dLdVel, double *gr
             as.lG = lG;
             as.partial = A.partial;
             as.dt = A.dt;
-            NW_HIP(launch_adj_step_batch(as, b.dev, b.R, A.waves, A.ctr0 + s0 + uint32_t(j), s.stream));
+            {
+                TimedScope ts("badj_step", s.stream, true);
+                NW_HIP(launch_adj_step_batch(as, b.dev, b.R, A.waves, A.ctr0 + s0 + uint32_t(j), s.stream));
+            }
             cur = 1 - cur;
         }
     }
@@ -1748,6 +1768,7 @@ void nwSortSolvent(float *pos, float *vel, uint *solventId, uint numParticles, u
                    grow(s.solPos, s.capSolPos, Ns) && grow(s.solVel, s.capSolVel, Ns),
                "nwSortSolvent: out of device memory");
     float4 *P = reinterpret_cast<float4 *>(pos), *V = reinterpret_cast<float4 *>(vel);
+    TimedScope ts("srd_resort", s.stream);
     NW_HIP(launch_solvent_keys(P, N, Ns, s.solKey, s.solIdx, kp, s.stream));
     // stable by cell: solIdx becomes the new order
     const hipError_t e = bucket_sort_pairs(s.solKey, s.solIdx, Ns, uint32_t(cells), s.bucket, s.stream);
@@ -1780,8 +1801,46 @@ double nwPolarSpeedGrad(const float *pos, const float *vel, uint numFilaments, f
     return o.polarSpeed;
 }
 
+void nwSetTuning(const NWTuning *t) {
+    NW_REQUIRE(t != nullptr, "nwSetTuning: NULL tuning");
+    auto oneOf = [](uint32_t v, std::initializer_list<uint32_t> ok) {
+        if (v == 0) return true;
+        for (uint32_t x : ok)
+            if (v == x) return true;
+        return false;
+    };
+    NW_REQUIRE(oneOf(t->mdTileItems, {1024, 2048, 4096, 8192, 16384}), "nwSetTuning: mdTileItems %u",
+               t->mdTileItems);
+    NW_REQUIRE(oneOf(t->srdTileItems, {4096, 8192, 16384}), "nwSetTuning: srdTileItems %u", t->srdTileItems);
+    NW_REQUIRE(t->mdMembers == 0 || (t->mdMembers >= 64 && t->mdMembers <= 2048), "nwSetTuning: mdMembers %u",
+               t->mdMembers);
+    NW_REQUIRE(t->srdMembers == 0 || (t->srdMembers >= 64 && t->srdMembers <= 2048), "nwSetTuning: srdMembers %u",
+               t->srdMembers);
+    NW_REQUIRE(oneOf(t->pairBlock, {128, 256, 512}), "nwSetTuning: pairBlock %u", t->pairBlock);
+    NW_REQUIRE(oneOf(t->frontBlock, {64, 128, 256, 512, 1024}), "nwSetTuning: frontBlock %u", t->frontBlock);
+    Tuning &g = tuning();
+    g.mdTileItems = t->mdTileItems;
+    g.mdMembers = t->mdMembers;
+    g.srdTileItems = t->srdTileItems;
+    g.srdMembers = t->srdMembers;
+    g.pairBlock = t->pairBlock;
+    g.frontBlock = t->frontBlock;
+}
+
+void nwGetTuning(NWTuning *t) {
+    if (!t) return;
+    *t = NWTuning{};
+    const Tuning &g = tuning();
+    t->mdTileItems = g.mdTileItems;
+    t->mdMembers = g.mdMembers;
+    t->srdTileItems = g.srdTileItems;
+    t->srdMembers = g.srdMembers;
+    t->pairBlock = g.pairBlock;
+    t->frontBlock = g.frontBlock;
+}
+
 int nwFusedPlanInfo(uint numParticles, uint numCells, uint filamentSize, uint out[4]) {
-    const BucketPlan bp = make_bucket_plan(numParticles, numCells, 0);
+    const BucketPlan bp = fused_plan(numParticles, numCells, filamentSize == 0);
     if (out) {
         out[0] = bp.shift;
         out[1] = bp.buckets;
@@ -1794,7 +1853,7 @@ int nwFusedPlanInfo(uint numParticles, uint numCells, uint filamentSize, uint ou
 
 int nwFusedPlanFits(uint numParticles, uint numCells, uint filamentSize) {
     if (filamentSize > 64 || numCells == 0) return 0;
-    return *bucket_plan_problem(make_bucket_plan(numParticles, numCells, 0)) ? 0 : 1;
+    return *bucket_plan_problem(fused_plan(numParticles, numCells, filamentSize == 0)) ? 0 : 1;
 }
 
 uint nwGetDeviceFlags(int clear) {

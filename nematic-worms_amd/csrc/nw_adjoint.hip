@@ -502,7 +502,7 @@ uint32_t adj_step_blocks(uint32_t W, uint32_t n) {
 hipError_t launch_adj_step(const AdjStepArgs &a, const KP &kp, hipStream_t st) {
     const uint32_t blocks = adj_step_blocks(a.W, kp.p.filamentSize);
     if (!blocks) return hipSuccess;
-    hipLaunchKernelGGL(k_adj_step, dim3(blocks), dim3(kBlock), 0, st, a, kp);
+    NW_LAUNCH(k_adj_step, dim3(blocks), dim3(kBlock), 0, st, a, kp);
     return hipGetLastError();
 }
 
@@ -538,7 +538,7 @@ hipError_t launch_polar_speed_grad(const float4 *pos, const float4 *vel, uint32_
 hipError_t launch_adj_step_batch(const AdjStepArgs &a, const BatchRep *reps, uint32_t R, uint32_t waves,
                                  uint64_t step, hipStream_t st) {
     if (waves == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_adj_step_batch, dim3(grid_for(uint64_t(waves) * 64u)), dim3(kBlock), 0, st, a, reps, R,
+    NW_LAUNCH(k_adj_step_batch, dim3(grid_for(uint64_t(waves) * 64u)), dim3(kBlock), 0, st, a, reps, R,
                        waves, step);
     return hipGetLastError();
 }

@@ -61,6 +61,20 @@ BucketPlan make_bucket_plan(uint64_t n, uint64_t cells, uint32_t tileItems, uint
     return b;
 }
 
+Tuning &tuning() {
+    static Tuning t;
+    return t;
+}
+
+BucketPlan fused_plan(uint64_t n, uint64_t cells, bool srd) {
+    const Tuning &t = tuning();
+    const uint32_t items = srd ? t.srdTileItems : t.mdTileItems;
+    const uint32_t members = srd ? t.srdMembers : t.mdMembers;
+    const BucketPlan bp = make_bucket_plan(n, cells, items, members ? members : 1536u);
+    if ((items || members) && *bucket_plan_problem(bp)) return make_bucket_plan(n, cells, 0);
+    return bp;
+}
+
 const char *bucket_plan_problem(const BucketPlan &bp) {
     if (bp.buckets > kMaxBuckets) return "grid has too many cells for the fused sort";
     if (bp.tiles > kMaxTiles) return "too many particles for the fused sort";
@@ -723,7 +737,9 @@ static hipError_t grant_lds(const void *fn, size_t lds) {
 template <bool FROM_POS, int ITEMS, int TB = kTileBlock>
 static hipError_t launch_staged(const float4 *pos, uint32_t *hash, const BucketPlan &bp, BucketScratch &s,
                                 const KP &kp, hipStream_t st) {
-    static_assert(ITEMS * TB == 4096 || ITEMS * TB == 8192 || ITEMS * TB == 16384, "tile size");
+    static_assert(ITEMS * TB == 1024 || ITEMS * TB == 2048 || ITEMS * TB == 4096 || ITEMS * TB == 8192 ||
+                      ITEMS * TB == 16384,
+                  "tile size");
     const size_t lds = (((size_t(bp.buckets) + 1 + TB / 64 + 3) & ~size_t(3)) * sizeof(uint32_t)) +
                        size_t(bp.tileItems) * sizeof(uint32_t);
     const void *fn = reinterpret_cast<const void *>(&k_tiles_staged<FROM_POS, ITEMS, TB>);
@@ -741,6 +757,9 @@ static hipError_t launch_tiles(bool fromPos, const float4 *pos, uint32_t *hash, 
     if (bp.n == 0) return hipSuccess;
     const size_t stagedLds = (size_t(bp.buckets) + 1 + kTileBlock / 64 + 3) * 4 + size_t(bp.tileItems) * 4;
     if (stagedLds <= 160 * 1024) {
+        // small tiles (more workgroups for a chip of 256 CUs) as 256 threads
+        if (!fromPos && bp.tileItems == 1024) return launch_staged<false, 4, 256>(pos, hash, bp, s, kp, st);
+        if (!fromPos && bp.tileItems == 2048) return launch_staged<false, 8, 256>(pos, hash, bp, s, kp, st);
         if (bp.tileItems == 4 * kTileBlock)
             return fromPos ? launch_staged<true, 4>(pos, hash, bp, s, kp, st)
                            : launch_staged<false, 4>(pos, hash, bp, s, kp, st);
@@ -862,7 +881,7 @@ hipError_t launch_hash_tiles_batch(const uint32_t *hash, const SortRep *reps, ui
     const size_t lds = batch_tiles_lds(tot.maxBuckets);
     const hipError_t e = grant_lds(reinterpret_cast<const void *>(&k_tiles_batch), lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_tiles_batch, dim3(tot.tiles), dim3(kBatchTileBlock), lds, st, hash, s.pairs, s.P, reps, R);
+    NW_LAUNCH(k_tiles_batch, dim3(tot.tiles), dim3(kBatchTileBlock), lds, st, hash, s.pairs, s.P, reps, R);
     return hipGetLastError();
 }
 
@@ -886,7 +905,7 @@ hipError_t launch_md_cells_batch(const SortRep *reps, uint32_t R, const BatchSor
     const void *fn = reinterpret_cast<const void *>(&k_bucket_cells_batch<kMdCellsBlock, int(kBatchBucketCap)>);
     const hipError_t e = grant_lds(fn, L.total);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_bucket_cells_batch<kMdCellsBlock, int(kBatchBucketCap)>), dim3(tot.buckets),
+    NW_LAUNCH((k_bucket_cells_batch<kMdCellsBlock, int(kBatchBucketCap)>), dim3(tot.buckets),
                        dim3(kMdCellsBlock), L.total, st, o, reps, R);
     return hipGetLastError();
 }

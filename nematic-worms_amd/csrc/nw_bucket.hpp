@@ -50,6 +50,21 @@ BucketPlan make_bucket_plan(uint64_t n, uint64_t cells, uint32_t tileItems, uint
 // empty string when the plan fits the limits above, else the reason
 const char *bucket_plan_problem(const BucketPlan &bp);
 
+// Launch shapes of the fused kernels (nwSetTuning); zero takes the built-in
+// choice.  Every setting gives the same results bit for bit.
+struct Tuning {
+    uint32_t mdTileItems = 0;   // MD sort first level: beads per tile
+    uint32_t mdMembers = 0;     // MD sort second level: bucket population aimed at
+    uint32_t srdTileItems = 0;  // SRD sort first level: particles per tile
+    uint32_t srdMembers = 0;    // SRD sort second level
+    uint32_t pairBlock = 0;     // sorted beads per pair-kernel workgroup (128, 256, 512)
+    uint32_t frontBlock = 0;    // threads per front workgroup (64 .. 1024)
+};
+Tuning &tuning();
+// the plan of the fused steps' cell sort (srd: the SRD collision's) with the
+// tuning applied; a tuned plan that does not fit falls back to the built-in
+BucketPlan fused_plan(uint64_t n, uint64_t cells, bool srd);
+
 struct BucketScratch {
     uint32_t *hash = nullptr;        // cell per particle, particle order
     uint32_t *pairs = nullptr;       // packed (cell in bucket, index in tile), per tile grouped by bucket

@@ -123,7 +123,8 @@ __device__ __forceinline__ uint32_t md_front_store(const KP &kp, const WaveWorm 
                            cell_coord(o.r.z, p.origin.z, p.cellSize.z));
 }
 
-__global__ __launch_bounds__(kBlock) void k_md_front(MdFrontArgs a, KP kp) {
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_md_front(MdFrontArgs a, KP kp) {
     const WaveWorm m = wave_worm(kp.p.filamentSize, a.W);
     const FrontLane o = md_front_lane(kp, m, a.posIn ? a.posIn : a.pos, a.velIn ? a.velIn : a.vel, a.force,
                                       a.forceOld, a.dt, a.lgvStd, a.langevin, a.seed, a.offRev, a.offLgv);
@@ -360,8 +361,8 @@ __global__ __launch_bounds__(kBlock) void k_collide_csr(float4 *__restrict__ for
 // pair forces with LDS row runs (stencils of one z layer, gridSize.x >= 3,
 // cells of the sorted beads given)
 //
-// A workgroup owns 256 consecutive sorted beads whose cells span the hash
-// interval [h0, h1].  Every neighbour cell of those beads that does not wrap
+// A workgroup owns BLOCK (128, 256 or 512; 256 unless tuned) consecutive
+// sorted beads whose cells span the hash interval [h0, h1].  Every neighbour cell of those beads that does not wrap
 // lies in one of three hash intervals [h0 - 1 + dy Gx, h1 + 1 + dy Gx],
 // dy = -1, 0, 1, i.e. in three contiguous runs of the sorted arrays, staged in
 // LDS with coalesced loads (position + worm id in one float4; velocity +
@@ -372,26 +373,31 @@ __global__ __launch_bounds__(kBlock) void k_collide_csr(float4 *__restrict__ for
 // run) takes the global-memory walk.  Same order and arithmetic as
 // k_collide_csr.
 // ===========================================================================
-constexpr int kRunCap = kPairBlock + 128;  // staged beads per row run
+// staged beads per row run of a workgroup of BLOCK sorted beads
+template <int BLOCK>
+constexpr int run_cap() {
+    return BLOCK + 128;
+}
 
 // LDS of one pair-kernel workgroup
-template <int MODE, bool TAN>
+template <int MODE, bool TAN, int CAP>
 struct RowsLds {
-    float4 pos[3 * kRunCap];
-    float4 vel[MODE == 0 ? 3 * kRunCap : 1];
-    float4 tan[TAN ? 3 * kRunCap : 1];
+    float4 pos[3 * CAP];
+    float4 vel[MODE == 0 ? 3 * CAP : 1];
+    float4 tan[TAN ? 3 * CAP : 1];
 };
 
 // The workgroup's sorted beads [i0, min(i0 + blockDim, iEnd)) of one system
 // whose beads and cells start at beadBase / cellBase in the shared arrays
 // (zero for a single system); offR = off + cellBase.
-template <int MODE, bool TAN, bool UNIT>
+template <int MODE, bool TAN, bool UNIT, int BLOCK, int CAP = run_cap<BLOCK>()>
 __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uint32_t iEnd, uint32_t beadBase,
                                                   uint32_t cellBase, float4 *__restrict__ force,
                                                   const float4 *__restrict__ sPos, const float4 *__restrict__ sVel,
                                                   const float4 *__restrict__ sTan, const uint32_t *__restrict__ sIdx,
                                                   const uint32_t *__restrict__ offR,
-                                                  const uint32_t *__restrict__ sCell, RowsLds<MODE, TAN> &L) {
+                                                  const uint32_t *__restrict__ sCell,
+                                                  RowsLds<MODE, TAN, CAP> &L) {
     // records move as plain 4-vectors (register values)
     typedef float v4 __attribute__((ext_vector_type(4)));
     const v4 *pos4 = reinterpret_cast<const v4 *>(sPos), *vel4 = reinterpret_cast<const v4 *>(sVel),
@@ -424,7 +430,7 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
             const uint32_t chi = uint32_t(hi < 0 ? 0 : (hi > int64_t(C) ? C : hi));
             const uint32_t s = offR[clo], e = offR[chi];
             rs[r] = s;
-            rl[r] = (e > s && e - s <= uint32_t(kRunCap)) ? e - s : 0u;
+            rl[r] = (e > s && e - s <= uint32_t(CAP)) ? e - s : 0u;
         }
     }
 
@@ -446,7 +452,7 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
             rb[yi] = offR[base + gx + 2];
         }
     }
-    constexpr int RH = (kRunCap + kPairBlock - 1) / kPairBlock;  // staged records per thread and row
+    constexpr int RH = (CAP + BLOCK - 1) / BLOCK;  // staged records per thread and row
     v4 SP[3][RH], SV[3][RH], ST[3][RH];
     uint32_t SI[3][RH];
 #pragma unroll
@@ -455,7 +461,7 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
 #pragma unroll
         for (int h = 0; h < RH; ++h) {
             // an empty run reads bead 0 (in bounds, unused)
-            const uint32_t j = l ? s + min(threadIdx.x + uint32_t(h) * kPairBlock, l - 1u) : 0u;
+            const uint32_t j = l ? s + min(threadIdx.x + uint32_t(h) * BLOCK, l - 1u) : 0u;
             SP[r][h] = pos4[j];
             SI[r][h] = sIdx[j];
             if (MODE == 0) SV[r][h] = vel4[j];
@@ -467,14 +473,14 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
         const uint32_t l = rl[r];
 #pragma unroll
         for (int h = 0; h < RH; ++h) {
-            const uint32_t k = threadIdx.x + uint32_t(h) * kPairBlock;
+            const uint32_t k = threadIdx.x + uint32_t(h) * BLOCK;
             if (k < l) {
                 uint32_t fil, ch;
                 worm_of(kp, SI[r][h] - beadBase, fil, ch);
-                L.pos[r * kRunCap + k] = make_float4(SP[r][h].x, SP[r][h].y, SP[r][h].z, __uint_as_float(fil));
+                L.pos[r * CAP + k] = make_float4(SP[r][h].x, SP[r][h].y, SP[r][h].z, __uint_as_float(fil));
                 if (MODE == 0)
-                    L.vel[r * kRunCap + k] = make_float4(SV[r][h].x, SV[r][h].y, SV[r][h].z, __uint_as_float(ch));
-                if (TAN) L.tan[r * kRunCap + k] = make_float4(ST[r][h].x, ST[r][h].y, ST[r][h].z, ST[r][h].w);
+                    L.vel[r * CAP + k] = make_float4(SV[r][h].x, SV[r][h].y, SV[r][h].z, __uint_as_float(ch));
+                if (TAN) L.tan[r * CAP + k] = make_float4(ST[r][h].x, ST[r][h].y, ST[r][h].z, ST[r][h].w);
             }
         }
     }
@@ -501,7 +507,7 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
                 if (!hit && rl[r] && a >= rs[r] && b <= rs[r] + rl[r]) {
-                    q[yi] = uint32_t(r) * kRunCap + (a - rs[r]);
+                    q[yi] = uint32_t(r) * CAP + (a - rs[r]);
                     hit = true;
                 }
             }
@@ -536,18 +542,18 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
     force[me] = make_float4(Fo.x + acc.x, Fo.y + acc.y, Fo.z + acc.z, Fo.w + 0.0f);
 }
 
-template <int MODE, bool TAN, bool UNIT>
-__global__ __launch_bounds__(kPairBlock) void k_collide_rows(float4 *__restrict__ force, const float4 *__restrict__ sPos,
+template <int MODE, bool TAN, bool UNIT, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_collide_rows(float4 *__restrict__ force, const float4 *__restrict__ sPos,
                                                          const float4 *__restrict__ sVel,
                                                          const float4 *__restrict__ sTan,
                                                          const uint32_t *__restrict__ sIdx,
                                                          const uint32_t *__restrict__ off,
                                                          const uint32_t *__restrict__ sCell, uint32_t n, KP kp) {
-    __shared__ RowsLds<MODE, TAN> L;
+    __shared__ RowsLds<MODE, TAN, run_cap<BLOCK>()> L;
     // neighbouring tiles share row runs: a contiguous range of tiles per XCD
     // keeps their re-reads in that XCD's L2
     const uint32_t tile = xcd_contiguous(blockIdx.x, gridDim.x);
-    collide_rows_tile<MODE, TAN, UNIT>(kp, tile * blockDim.x, n, 0u, 0u, force, sPos, sVel, sTan, sIdx, off, sCell,
+    collide_rows_tile<MODE, TAN, UNIT, BLOCK>(kp, tile * blockDim.x, n, 0u, 0u, force, sPos, sVel, sTan, sIdx, off, sCell,
                                        L);
 }
 
@@ -586,7 +592,7 @@ __global__ __launch_bounds__(kBlock) void k_md_front_batch(BatchArrays a, const 
 template <int MODE, bool TAN, bool UNIT>
 __global__ __launch_bounds__(kPairBlock) void k_collide_rows_batch(BatchArrays a, const BatchRep *__restrict__ reps,
                                                                uint32_t R, uint64_t *__restrict__ stepCtr) {
-    __shared__ RowsLds<MODE, TAN> L;
+    __shared__ RowsLds<MODE, TAN, run_cap<kPairBlock>()> L;
     // the step's last kernel advances the step counter the next front reads
     if (blockIdx.x == 0 && threadIdx.x == 0) stepCtr[0] = stepCtr[0] + 1u;
     // tiles in contiguous runs per XCD (neighbouring tiles share row runs)
@@ -595,7 +601,7 @@ __global__ __launch_bounds__(kPairBlock) void k_collide_rows_batch(BatchArrays a
     const BatchRep &rep = reps[r];
     const uint32_t i0 = rep.beadOffset + (tile - rep.blockOffset) * blockDim.x;
     const uint32_t iEnd = rep.beadOffset + rep.kp.p.numFilaments * rep.kp.p.filamentSize;
-    collide_rows_tile<MODE, TAN, UNIT>(rep.kp, i0, iEnd, rep.beadOffset, rep.cellOffset,
+    collide_rows_tile<MODE, TAN, UNIT, kPairBlock>(rep.kp, i0, iEnd, rep.beadOffset, rep.cellOffset,
                                        a.swapForces ? a.forceOld : a.force, a.sPos, a.sVel,
                                        a.sTan, a.ids, a.off + rep.cellOffset, a.sCell, L);
 }
@@ -607,7 +613,14 @@ hipError_t launch_md_front(const MdFrontArgs &a, const KP &kp, hipStream_t st) {
     const uint32_t n = kp.p.filamentSize;
     const uint32_t wpw = 64u / n;
     const uint64_t threads = ((uint64_t(a.W) + wpw - 1) / wpw) * 64u;
-    NW_LAUNCH(k_md_front, dim3(grid_for(threads)), dim3(kBlock), 0, st, a, kp);
+    // the workgroup size changes only how the wavefronts are grouped
+    switch (tuning().frontBlock) {
+        case 64: NW_LAUNCH(k_md_front<64>, dim3(grid_for(threads, 64)), dim3(64), 0, st, a, kp); break;
+        case 128: NW_LAUNCH(k_md_front<128>, dim3(grid_for(threads, 128)), dim3(128), 0, st, a, kp); break;
+        case 512: NW_LAUNCH(k_md_front<512>, dim3(grid_for(threads, 512)), dim3(512), 0, st, a, kp); break;
+        case 1024: NW_LAUNCH(k_md_front<1024>, dim3(grid_for(threads, 1024)), dim3(1024), 0, st, a, kp); break;
+        default: NW_LAUNCH(k_md_front<kBlock>, dim3(grid_for(threads)), dim3(kBlock), 0, st, a, kp); break;
+    }
     return hipGetLastError();
 }
 
@@ -620,20 +633,31 @@ hipError_t launch_collide_off(float4 *force, const float4 *sPos, const float4 *s
     const bool lds = ldsOk && kp.e.collideKernel != NW_COLLIDE_GLOBAL;
     const bool wca = kp.e.pairPotential == NW_PAIR_WCA;
     const bool tan = !wca || kp.p.activity != 0.0f;
-    const dim3 g(grid_for(n, lds ? kPairBlock : kBlock)), b(lds ? kPairBlock : kBlock);
+    // sorted beads per LDS workgroup: 128, 256 or 512 (512 only for the WCA
+    // pair term, whose stage fits the static LDS of a workgroup)
+    const uint32_t tb = tuning().pairBlock;
+    const int pb = tb == 128u ? 128 : (tb == 512u && wca ? 512 : kPairBlock);
+    const dim3 g(grid_for(n, lds ? uint32_t(pb) : uint32_t(kBlock))), b(lds ? pb : kBlock);
 #define NW_COLLIDE(K, ...) NW_LAUNCH((K<__VA_ARGS__>), g, b, 0, st, force, sPos, sVel, sTan, sIdx, off, sCell, n, kp)
+#define NW_COLLIDE_ROWS(M, T, U)                                                                         \
+    do {                                                                                                 \
+        if (pb == 128) NW_COLLIDE(k_collide_rows, M, T, U, 128);                                         \
+        else if (pb == 512) NW_COLLIDE(k_collide_rows, M, T, U, (M == 1 ? 512 : kPairBlock));            \
+        else NW_COLLIDE(k_collide_rows, M, T, U, kPairBlock);                                            \
+    } while (0)
     if (lds) {
         // every row weight is 1 for the one-layer stencil over >= 3 rows
         const bool unit = kp.e.stencil == NW_STENCIL_2D && kp.p.gridSize.y >= 3;
         if (unit) {
-            if (wca && !tan) NW_COLLIDE(k_collide_rows, 1, false, true);
-            else if (wca) NW_COLLIDE(k_collide_rows, 1, true, true);
-            else NW_COLLIDE(k_collide_rows, 0, true, true);
+            if (wca && !tan) NW_COLLIDE_ROWS(1, false, true);
+            else if (wca) NW_COLLIDE_ROWS(1, true, true);
+            else NW_COLLIDE_ROWS(0, true, true);
         } else {
-            if (wca && !tan) NW_COLLIDE(k_collide_rows, 1, false, false);
-            else if (wca) NW_COLLIDE(k_collide_rows, 1, true, false);
-            else NW_COLLIDE(k_collide_rows, 0, true, false);
+            if (wca && !tan) NW_COLLIDE_ROWS(1, false, false);
+            else if (wca) NW_COLLIDE_ROWS(1, true, false);
+            else NW_COLLIDE_ROWS(0, true, false);
         }
+#undef NW_COLLIDE_ROWS
     } else {
         if (wca && !tan) NW_COLLIDE(k_collide_csr, 1, false);
         else if (wca) NW_COLLIDE(k_collide_csr, 1, true);
@@ -670,7 +694,7 @@ hipError_t launch_set_step(uint64_t *ctr, uint64_t v, hipStream_t st) {
 hipError_t launch_md_front_batch(const BatchArrays &a, const BatchRep *reps, uint32_t R, uint32_t waves,
                                  const uint64_t *stepCtr, float dt, hipStream_t st) {
     if (waves == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_md_front_batch, dim3(grid_for(uint64_t(waves) * 64u)), dim3(kBlock), 0, st, a, reps, R,
+    NW_LAUNCH(k_md_front_batch, dim3(grid_for(uint64_t(waves) * 64u)), dim3(kBlock), 0, st, a, reps, R,
                        waves, stepCtr, dt);
     return hipGetLastError();
 }
@@ -679,7 +703,7 @@ hipError_t launch_collide_batch(const BatchArrays &a, const BatchRep *reps, uint
                                 bool tan, bool unit, uint64_t *stepCtr, hipStream_t st) {
     if (blocks == 0) return hipSuccess;
     const dim3 g(blocks), b(kPairBlock);
-#define NW_BATCH_COLLIDE(...) hipLaunchKernelGGL((k_collide_rows_batch<__VA_ARGS__>), g, b, 0, st, a, reps, R, stepCtr)
+#define NW_BATCH_COLLIDE(...) NW_LAUNCH((k_collide_rows_batch<__VA_ARGS__>), g, b, 0, st, a, reps, R, stepCtr)
     if (mode == 1 && !tan) {
         if (unit) NW_BATCH_COLLIDE(1, false, true);
         else NW_BATCH_COLLIDE(1, false, false);

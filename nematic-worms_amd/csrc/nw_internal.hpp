@@ -49,6 +49,10 @@ __device__ __forceinline__ uint32_t xcd_contiguous(uint32_t i, uint32_t G) {
 // device anomaly flags (bit 0: sort key >= bound, bit 1: hash >= numCells in reorder)
 uint32_t *device_flags();
 
+// the parameters of the last setParameters / setSolventParameters /
+// nwSetExtParameters, as the kernels receive them
+KP current_kp();
+
 // ---- nw_kernels.hip ----
 hipError_t launch_integrate_filaments(float4 *pos, float4 *vel, float4 *f, float4 *fold, float dt,
                                       uint32_t n, const KP &kp, hipStream_t st);

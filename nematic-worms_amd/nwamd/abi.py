@@ -72,6 +72,15 @@ class NWExtParams(ctypes.Structure):
 NW_COLLIDE_AUTO, NW_COLLIDE_GLOBAL, NW_COLLIDE_LDS = 0, 1, 2
 
 
+class NWTuning(ctypes.Structure):
+    """include/nw_abi.h NWTuning: launch shapes of the fused kernels (0: built-in)."""
+    _fields_ = [("mdTileItems", c_uint), ("mdMembers", c_uint), ("srdTileItems", c_uint), ("srdMembers", c_uint),
+                ("pairBlock", c_uint), ("frontBlock", c_uint), ("reserved", c_uint * 10)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 assert ctypes.sizeof(SimParams) == 144
 assert ctypes.sizeof(SolventParams) == 60
 assert ctypes.sizeof(NWExtParams) == 64
@@ -141,6 +150,10 @@ PROTOTYPES = {
     "nwFusedSrdStep": (None, [P, P, P, P, P, c_float, c_uint, c_uint, c_int]),
     "nwFusedPlanFits": (c_int, [c_uint, c_uint, c_uint]),
     "nwFusedPlanInfo": (c_int, [c_uint, c_uint, c_uint, P]),
+    "nwSetTuning": (None, [ctypes.c_void_p]),
+    "nwGetTuning": (None, [ctypes.c_void_p]),
+    "nwAutotuneFused": (c_int, [P, P, P, P, P, c_uint, c_uint, c_float, c_float, c_float, c_int, c_float, c_int,
+                                c_uint, ctypes.c_char_p, c_int, ctypes.c_void_p]),
     "nwObservables": (None, [ctypes.c_void_p, ctypes.c_void_p, c_uint, ctypes.POINTER(NWObservables)]),
     "nwBatchSetReplicas": (None, [ctypes.POINTER(NWReplicaParams), c_uint]),
     "nwBatchLayout": (None, [ctypes.c_void_p, ctypes.c_void_p]),

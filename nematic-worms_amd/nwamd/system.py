@@ -57,6 +57,7 @@ def load():
             "nwSystemSetRngState": (None, [P, u64]),
             "nwSystemWriteOutputs": (None, [P, ctypes.c_char_p, ctypes.c_int, f]),
             "nwSystemObservables": (None, [P, ctypes.POINTER(abi.NWObservables)]),
+            "nwSystemAutotune": (ctypes.c_int, [P, f, ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p]),
             "nwReplicaSetCreate": (P, [ctypes.c_void_p, u32, ctypes.c_void_p]),
             "nwReplicaSetDestroy": (None, [P]),
             "nwReplicaSetReset": (None, [P, ctypes.c_int, f]),
@@ -150,6 +151,15 @@ class System:
         self.lib.nwSystemObservables(self.h, ctypes.byref(o))
         return dict(nematicS=o.nematicS, polarP=o.polarP, bondS=o.bondS, polarSpeed=o.polarSpeed,
                     meanV2=o.meanV2, beads=o.beads, bonds=o.bonds, sums=list(o.sums))
+
+    def autotune(self, dt, cache_path=None, use_cache=True):
+        """The library's autotuner on a copy of this system's state: returns
+        (status, the chosen NWTuning as a dict); status 2 = from the cache,
+        1 = timed, 0 = not tuned."""
+        t = abi.NWTuning()
+        path = None if cache_path is None else cache_path.encode()
+        rc = self.lib.nwSystemAutotune(self.h, dt, path, 1 if use_cache else 0, ctypes.byref(t))
+        return rc, t.as_dict()
 
     def write_outputs(self, path, iteration, dt):
         self.lib.nwSystemWriteOutputs(self.h, path.encode(), iteration, dt)

@@ -144,3 +144,31 @@ def force_tol_check(gpu, ref, rtol=1e-5, floor=None, name="force"):
     assert np.all(err <= bound), (f"{name}: bead {worst} |err| {err[worst]:.3e} > bound {bound[worst]:.3e} "
                                   f"(gpu {g[worst]}, oracle {r[worst]})")
     return float(np.max(err / bound))
+
+
+def lattice_system_2d(W, n, G, rng, cell=1.1236):
+    """W worms of n beads in straight rows along x (random polarity, small
+    jitter) in a 2D periodic box of G x G cells of `cell` (>= the WCA
+    cutoff): SimParams with bonds k = 57, r0 = 0.8, kappa = 5, and the
+    pos / vel float4 arrays (w = radius 0.5 / mass 1)."""
+    L = G * cell
+    p = sim_params(grid=(G, G, 1), cell=cell, W=W, n=n, bondSpringL=0.8, activity=0.0)
+    p.bondSpringK, p.bondBendingK = 57.0, 5.0
+    per_row = max(1, int(L // (0.8 * n + 1.0)))
+    rows = -(-W // per_row)
+    assert rows * 1.0 <= L, "rows closer than one bead diameter"
+    pos = np.zeros((W * n, 4), np.float32)
+    w = np.arange(W)
+    row, col = w // per_row, w % per_row
+    flip = rng.random(W) < 0.5
+    q = np.arange(n)
+    qq = np.where(flip[:, None], n - 1 - q[None, :], q[None, :])
+    pos[:, 0] = (p.origin.x + 0.5 + (col * (L / per_row))[:, None] + 0.8 * qq).reshape(-1)
+    pos[:, 1] = np.repeat(p.origin.y + (row + 0.5) * (L / rows), n)
+    pos[:, :2] += rng.normal(0, 0.02, (W * n, 2))
+    pos[:, 2] = p.origin.z + 0.5 * cell
+    pos[:, 3] = 0.5
+    vel = np.zeros_like(pos)
+    vel[:, :2] = rng.normal(0, 0.3, (W * n, 2))
+    vel[:, 3] = 1.0
+    return p, pos, vel

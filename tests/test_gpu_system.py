@@ -255,3 +255,45 @@ def test_benchmark_workload_full_size_matches_oracle(lib):
     dp, dv = compare(s, cpu, 1e-4, 1e-3)
     print(f"800k + 5M after 11 steps: max |dpos| {dp:.2e} max |dvel| {dv:.2e}", flush=True)
     s.close()
+
+
+@pytest.mark.parametrize("n,grid", [(8, (32, 32, 4)), (7, (20, 16, 2)), (20, (64, 64, 1))])
+def test_reference_start_is_the_lattice(lib, n, grid):
+    """NW_INIT_REFERENCE places worms as simulation/particleSystem.cpp:583-610
+    does: slots of n bonds plus one bead diameter along x (as many per row as
+    fit the box), rows and layers one diameter apart, starting one diameter
+    in; bead q of worm w at the slot's head + R + q bond lengths; a reversed
+    worm (tangent -x) takes its points from tail to head, its middle bead
+    (odd n) keeping its own point and +x.  Velocities uniform in [-0.1, 0.1)."""
+    cfg = system.NWSystemConfig()
+    cfg.numFilaments, cfg.filamentSize = 300, n
+    cfg.gridSize = abi.UInt3(*grid)
+    cfg.seed = 3 + n
+    s = system.System(cfg)
+    s.reset(system.NW_INIT_REFERENCE)
+    p = s.params
+    P, T, V = s.get(system.POSITION), s.get(system.TANGENT), s.get(system.VELOCITY)
+    f = np.float32
+    R, b = f(p.particleRadius), f(p.bondSpringL)
+    slot = f(n) * b + f(2.0) * R
+    nx = max(1, int(np.floor(f(p.boxSize.x) / slot)))
+    ny = max(1, int(np.floor(f(p.boxSize.y) / (f(2) * R))))
+    o = (f(p.origin.x), f(p.origin.y), f(p.origin.z))
+    reversed_count = 0
+    for w in range(cfg.numFilaments):
+        col, row, layer = w % nx, (w // nx) % ny, w // (nx * ny)
+        rev = T[w * n, 0] < 0
+        reversed_count += int(rev)
+        for q in range(n):
+            moved = rev and 2 * q + 1 != n
+            k = n - 1 - q if moved else q
+            x = (f(col) * slot + R) + f(k) * b
+            y = f(2.0) * R * f(row + 1)
+            z = f(2.0) * R * f(layer + 1)
+            want = np.array([x + o[0], y + o[1], z + o[2], R], np.float32)
+            assert np.array_equal(P[w * n + q], want), (w, q, P[w * n + q], want)
+            assert T[w * n + q, 0] == (-1.0 if moved else 1.0), (w, q)
+            assert np.all(T[w * n + q, 1:] == 0.0)
+    assert 0.35 * cfg.numFilaments < reversed_count < 0.65 * cfg.numFilaments
+    assert np.all(np.abs(V[:, :3]) <= 0.1) and np.all(V[:, 3] == p.particleMass)
+    s.close()
