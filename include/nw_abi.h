@@ -244,6 +244,25 @@ void nwFusedMdStepSwapForces(float *pos, float *vel, float *force, float *forceO
                              float *sortedTangent, float deltaTime, float langevinGamma, float langevinKbT,
                              int langevin, uint numFilaments);
 
+/* nwFusedMdStep for callers that cannot allocate device memory -- code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+   No counterpart in the reference.  Same arguments and results as
+   nwFusedMdStep (bit for bit), but the cell sort's scratch (the cell of
+   every bead when gridParticleHash is NULL, the sort pairs, the tile x
+   bucket prefix matrix, the cell of every sorted slot, the CSR offsets)
+   lives in a fixed 32 MiB device array of the library, one per device, held
+   until the process ends; the call allocates nothing and records no timing
+   events.  Systems whose scratch exceeds the array -- about 1.5 million
+   beads at one bead per cell -- fail with NW error -22 and a message giving
+   the bytes needed, before any launch.  Every call on a device shares that
+   device's array, so calls that overlap on one device (two streams, two
+   threads) race; a call fully overwrites what it reads, so nothing carries
+   over from the previous call. */
+void nwFusedMdStepNoAlloc(float *pos, float *vel, float *force, float *forceOld, float *tangent,
+                          uint *gridParticleHash, uint *gridParticleIndex, float *sortedPos, float *sortedVel,
+                          float *sortedTangent, float deltaTime, float langevinGamma, float langevinKbT, int langevin,
+                          uint numFilaments);
+
 /* Fused SRD collision step: calcHash(srd) + sortParticles +
    [solventIsokineticThermostat] + solventCellCentorOfMomentum + collideSolvent
    + integrateSolvent (simulation/particleSystem.cpp:352-410) in three launches

@@ -197,6 +197,42 @@ bool fail(const char *file, int line, int code, const char *fmt, ...) {
         }                                                      \
     } while (0)
 
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free entry point (nwFusedMdStepNoAlloc): one
+// array per device, held until the process ends.  The host-side name of the
+// array is not a device address: host code uses only the address
+// hipGetSymbolAddress returns for the current device.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+void *g_noallocScratch[kNoAllocMaxDevices] = {};
+
+// the address of `scratch` on the current device, resolved once per device;
+// false (and *out NULL) when the device or the symbol cannot be resolved
+bool noalloc_scratch(void **out, int *device, hipError_t *err) {
+    *out = nullptr;
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess && (dev < 0 || dev >= kNoAllocMaxDevices)) e = hipErrorInvalidDevice;
+    if (e == hipSuccess && !g_noallocScratch[dev]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e == hipSuccess && ptr == nullptr) e = hipErrorInvalidSymbol;
+        if (e == hipSuccess) g_noallocScratch[dev] = ptr;
+    }
+    *device = dev;
+    *err = e;
+    if (e != hipSuccess) return false;
+    *out = g_noallocScratch[dev];
+    return true;
+}
+
 bool ensure_init() {
     State &s = S();
     if (s.initialized) return true;
@@ -217,6 +253,12 @@ bool ensure_init() {
     }
     s.ownStream = true;
     s.initialized = true;
+    // resolve the allocation-free entry point's scratch on this device now
+    // (set-up time); a failure is reported by that entry point when called
+    void *ptr = nullptr;
+    int dev = 0;
+    hipError_t se = hipSuccess;
+    (void)noalloc_scratch(&ptr, &dev, &se);
     return true;
 }
 
@@ -580,11 +622,13 @@ void collideSolvent(float *pos, float *vel, float *solventCellCOM, float *random
 // ---------------------------------------------------------------------------
 // The fused MD step of nwFusedMdStep; offOut / cellOut (NULL: the library's
 // scratch) receive the CSR cell offsets and the cell of every sorted slot.
+// fixed: the scratch to use as it is (no allocation, no timing events), else
+// the library's own, grown as needed.
 static bool fused_md(const KP &kp, float *pos, float *vel, float *force, float *forceOld, float *tangent,
                      uint *gridParticleHash, uint *gridParticleIndex, float *sortedPos, float *sortedVel,
                      float *sortedTangent, float deltaTime, float langevinGamma, float langevinKbT, int langevin,
                      uint numFilaments, uint32_t *offOut, uint32_t *cellOut, const float *posIn = nullptr,
-                     const float *velIn = nullptr, bool swapForces = false) {
+                     const float *velIn = nullptr, bool swapForces = false, const BucketScratch *fixed = nullptr) {
     State &s = S();
     const uint32_t n = kp.p.filamentSize;
     NW_REQUIRE_R(n >= 1 && n <= 64, false, "nwFusedMdStep: worms of %u beads (1..64 supported)", n);
@@ -599,10 +643,12 @@ static bool fused_md(const KP &kp, float *pos, float *vel, float *force, float *
     if (N == 0) return true;
     const BucketPlan bp = fused_plan(N, cells, false);
     NW_REQUIRE_R(!*bucket_plan_problem(bp), false, "nwFusedMdStep: %s", bucket_plan_problem(bp));
-    NW_HIP_R(ensure_bucket_scratch(s.bucket, bp), false);
-    BucketScratch sc = s.bucket;
+    if (!fixed) NW_HIP_R(ensure_bucket_scratch(s.bucket, bp), false);
+    BucketScratch base = fixed ? *fixed : s.bucket;
+    BucketScratch sc = base;
     if (offOut) sc.off = offOut;
     if (cellOut) sc.sortedCell = cellOut;
+    const char *const noTimer = "";  // no timer carries this name
 
     NW_REQUIRE_R(!swapForces || kp.p.reverseProbability == 0.0f, false,
                  "nwFusedMdStepSwapForces: no polarity reversal (it moves the old force)");
@@ -615,7 +661,7 @@ static bool fused_md(const KP &kp, float *pos, float *vel, float *force, float *
     a.tangent = reinterpret_cast<float4 *>(tangent);
     a.posIn = reinterpret_cast<const float4 *>(posIn);
     a.velIn = reinterpret_cast<const float4 *>(velIn);
-    a.hash = gridParticleHash ? gridParticleHash : s.bucket.hash;
+    a.hash = gridParticleHash ? gridParticleHash : base.hash;
     a.dt = deltaTime;
     a.W = numFilaments;
     a.seed = s.seed;
@@ -626,22 +672,22 @@ static bool fused_md(const KP &kp, float *pos, float *vel, float *force, float *
     a.lgvStd = sqrtf(2.0f * langevinGamma * langevinKbT);
     if (langevin) s.offset += N;
     {
-        TimedScope ts("md_front", s.stream, true);
+        TimedScope ts(fixed ? noTimer : "md_front", s.stream, true);
         NW_HIP_R(launch_md_front(a, kp, s.stream), false);
     }
     {
-        TimedScope ts("md_tiles", s.stream, true);
-        NW_HIP_R(launch_hash_tiles(a.hash, bp, s.bucket, s.stream), false);
+        TimedScope ts(fixed ? noTimer : "md_tiles", s.stream, true);
+        NW_HIP_R(launch_hash_tiles(a.hash, bp, base, s.stream), false);
     }
     {
-        TimedScope ts("md_sort", s.stream, true);
+        TimedScope ts(fixed ? noTimer : "md_sort", s.stream, true);
         NW_HIP_R(launch_md_cells(bp, sc, gridParticleIndex, a.pos, needVel ? a.vel : nullptr,
                                needTan ? a.tangent : nullptr, reinterpret_cast<float4 *>(sortedPos),
                                needVel ? reinterpret_cast<float4 *>(sortedVel) : nullptr,
                                needTan ? reinterpret_cast<float4 *>(sortedTangent) : nullptr, s.stream), false);
     }
     {
-        TimedScope ts("md_collide", s.stream, true);
+        TimedScope ts(fixed ? noTimer : "md_collide", s.stream, true);
         NW_HIP_R(launch_collide_off(swapForces ? a.forceOld : a.force, reinterpret_cast<const float4 *>(sortedPos),
                                   reinterpret_cast<const float4 *>(sortedVel),
                                   reinterpret_cast<const float4 *>(sortedTangent), gridParticleIndex,
@@ -668,6 +714,54 @@ void nwFusedMdStepSwapForces(float *pos, float *vel, float *force, float *forceO
     (void)fused_md(make_kp(), pos, vel, force, forceOld, tangent, gridParticleHash, gridParticleIndex, sortedPos,
                    sortedVel, sortedTangent, deltaTime, langevinGamma, langevinKbT, langevin, numFilaments, nullptr,
                    nullptr, nullptr, nullptr, true);
+}
+
+void nwFusedMdStepNoAlloc(float *pos, float *vel, float *force, float *forceOld, float *tangent,
+                          uint *gridParticleHash, uint *gridParticleIndex, float *sortedPos, float *sortedVel,
+                          float *sortedTangent, float deltaTime, float langevinGamma, float langevinKbT, int langevin,
+                          uint numFilaments) {
+    NW_INIT();
+    void *base = nullptr;
+    int dev = -1;
+    hipError_t e = hipSuccess;
+    if (!noalloc_scratch(&base, &dev, &e)) {
+        fail(__FILE__, __LINE__, int(e), "nwFusedMdStepNoAlloc: no address of the scratch array on device %d: %s", dev,
+             hipGetErrorString(e));
+        return;
+    }
+    const KP kp = make_kp();
+    const uint64_t N = uint64_t(numFilaments) * kp.p.filamentSize;
+    const uint64_t cells = uint64_t(kp.p.gridSize.x) * kp.p.gridSize.y * kp.p.gridSize.z;
+    NW_REQUIRE(N < (1ull << 31) && cells > 0 && cells < (1ull << 32),
+               "nwFusedMdStepNoAlloc: %llu beads over %llu cells", (unsigned long long)N, (unsigned long long)cells);
+    // the sort's arrays carved from the scratch, 256-byte aligned: the cell
+    // of every bead (unless the caller takes it), the packed pairs, the tile
+    // x bucket prefix matrix, the cell of every sorted slot, the CSR offsets
+    const BucketPlan bp = fused_plan(N, cells, false);
+    NW_REQUIRE(!*bucket_plan_problem(bp), "nwFusedMdStepNoAlloc: %s", bucket_plan_problem(bp));
+    auto words = [](uint64_t n) { return (n * 4 + 255) & ~uint64_t(255); };
+    const uint64_t need = (gridParticleHash ? 0 : words(N)) + words(uint64_t(bp.tiles) * bp.tileItems) +
+                          words(uint64_t(bp.tiles) * (bp.buckets + 1)) + words(N) + words(cells + 2);
+    NW_REQUIRE(need <= kNoAllocScratchBytes,
+               "nwFusedMdStepNoAlloc: %llu beads over %llu cells need %llu bytes of scratch; the fixed scratch "
+               "array holds %llu (use nwFusedMdStep)",
+               (unsigned long long)N, (unsigned long long)cells, (unsigned long long)need,
+               (unsigned long long)kNoAllocScratchBytes);
+    unsigned char *at = static_cast<unsigned char *>(base);
+    auto take = [&at, &words](uint64_t n) {
+        uint32_t *p = reinterpret_cast<uint32_t *>(at);
+        at += words(n);
+        return p;
+    };
+    BucketScratch sc{};
+    sc.hash = gridParticleHash ? nullptr : take(N);
+    sc.pairs = take(uint64_t(bp.tiles) * bp.tileItems);
+    sc.P = take(uint64_t(bp.tiles) * (bp.buckets + 1));
+    sc.sortedCell = take(N);
+    sc.off = take(cells + 2);
+    (void)fused_md(kp, pos, vel, force, forceOld, tangent, gridParticleHash, gridParticleIndex, sortedPos, sortedVel,
+                   sortedTangent, deltaTime, langevinGamma, langevinKbT, langevin, numFilaments, nullptr, nullptr,
+                   nullptr, nullptr, false, &sc);
 }
 
 // The fused SRD collision with the parameters kp; offOut (may be NULL: the

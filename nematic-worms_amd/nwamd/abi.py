@@ -147,6 +147,7 @@ PROTOTYPES = {
     "nwRadixSortPairs": (None, [P, P, c_uint, c_uint]),
     "nwFusedMdStep": (None, [P, P, P, P, P, P, P, P, P, P, c_float, c_float, c_float, c_int, c_uint]),
     "nwFusedMdStepSwapForces": (None, [P, P, P, P, P, P, P, P, P, P, c_float, c_float, c_float, c_int, c_uint]),
+    "nwFusedMdStepNoAlloc": (None, [P, P, P, P, P, P, P, P, P, P, c_float, c_float, c_float, c_int, c_uint]),
     "nwFusedSrdStep": (None, [P, P, P, P, P, c_float, c_uint, c_uint, c_int]),
     "nwFusedPlanFits": (c_int, [c_uint, c_uint, c_uint]),
     "nwFusedPlanInfo": (c_int, [c_uint, c_uint, c_uint, P]),
