@@ -54,8 +54,9 @@ ALG_BYTES = {
     # fused path (nwFusedMdStep / nwFusedSrdStep); Pm / Ps = bytes of the
     # sort's tile x bucket prefix matrix (written by the tile pass, read once)
     # R pos vel f f_old, W pos vel f (+ f_old unless the host driver swaps
-    # the force buffers: no reversal), W tangent, W cell
-    "md_front": lambda w: (132 if w["swap"] else 148) * w["N"],
+    # the force buffers: no reversal), W cell, W tangent when the pair term
+    # reads tangents (the host driver otherwise rebuilds them on demand)
+    "md_front": lambda w: ((132 if w["swap"] else 148) - (0 if w["tan_store"] else 16)) * w["N"],
     "md_tiles": lambda w: 8 * w["N"] + w["Pm"],                # R cell, W pair (4 B packed)
     "md_sort": lambda w: 44 * w["N"] + 4 * w["C"] + w["Pm"],   # R pairs, W ids cells, gather pos, W offsets
     "md_collide": lambda w: (56 if w["wca"] else 88) * w["N"] + 4 * w["C"],  # R sPos sIdx sCell (+sVel sTan), RMW f
@@ -138,7 +139,8 @@ def build_workload(name, per_call=False, solvent_sort_every=0):
     s.reset(system.NW_INIT_LATTICE_RANDOM if cfg.srd else system.NW_INIT_RANDOM, 0.1)
     p = s.params
     w = dict(N=s.n, Ns=s.ns, NT=s.n + s.ns, C=p.numCells, Cs=s.solvent.numCells if cfg.srd else 0,
-             wca=cfg.ext.pairPotential == 1, swap=p.reverseProbability == 0.0)
+             wca=cfg.ext.pairPotential == 1, swap=p.reverseProbability == 0.0,
+             tan_store=cfg.ext.pairPotential != 1 or p.activity != 0.0)
     rot_beads = bool(cfg.ext.srdCollideFilaments) or cfg.ext.srdThermostat == abi.NW_SRD_THERMO_ISOKINETIC
     w["Nrot"] = w["NT"] if rot_beads else w["Ns"]
     w["md_plan"] = fused_plan(w["N"], w["C"], cfg.filamentSize)
@@ -169,7 +171,8 @@ def batched_form(args):
     rs.reset(system.NW_INIT_LATTICE_RANDOM, 0.1)
     N = rs.n()
     out = dict(workload=f"{args.replicas} replicas", replicas=args.replicas, beads=N)
-    for key, form, steps in (("graph", replicas.NW_REPLICAS_GRAPH, args.batched_steps),
+    graph_form = replicas.NW_REPLICAS_LAUNCHES if args.batched_no_graph else replicas.NW_REPLICAS_GRAPH
+    for key, form, steps in (("graph", graph_form, args.batched_steps),
                              ("one_per_launch", replicas.NW_REPLICAS_ONE_PER_LAUNCH, args.one_per_launch_steps)):
         if steps <= 0:
             continue
@@ -444,6 +447,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--replicas", type=int, default=256, help="replicas of the batched form")
     ap.add_argument("--batched-steps", type=int, default=200, help="timed steps of the batched form (0: skip)")
+    ap.add_argument("--batched-no-graph", action="store_true",
+                    help="launch the batched step's kernels directly instead of replaying the captured graph "
+                         "(for counter passes of a profiler)")
     ap.add_argument("--one-per-launch-steps", type=int, default=20,
                     help="timed steps of the replicas one launch sequence each (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=2000,
@@ -457,6 +463,9 @@ def main():
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="at the end, one elementwise kernel reading and writing this many MiB "
                          "(a known-size reference for HBM byte counters)")
+    ap.add_argument("--gather-probe-mb", type=int, default=0,
+                    help="at the end, one gather of this many MiB of 16-byte rows, whole 128-byte lines in random "
+                         "order, each line read once (a known-size reference for counters of gather kernels)")
     ap.add_argument("--profile-steps", type=int, default=20,
                     help="untimed steps after the warm-up that rank the kernel families")
     ap.add_argument("--no-autotune", action="store_true",
@@ -644,6 +653,21 @@ def main():
         torch.cuda.synchronize()
         print(f"copy probe: neg over {a.numel()} floats, {a.numel() * 4} B read and {a.numel() * 4} B written",
               flush=True)
+    if args.gather_probe_mb > 0:
+        import torch
+        rows = args.gather_probe_mb * 2 ** 20 // 16
+        lines = rows // 8
+        table = torch.ones((lines * 8, 4), dtype=torch.float32, device="cuda")
+        g = torch.Generator(device="cuda")
+        g.manual_seed(5)
+        perm = torch.randperm(lines, device="cuda", generator=g)
+        idx = (perm[:, None] * 8 + torch.arange(8, device="cuda")[None, :]).reshape(-1)
+        torch.cuda.synchronize()
+        out = torch.index_select(table, 0, idx)
+        torch.cuda.synchronize()
+        print(f"gather probe: index_select of {idx.numel()} rows of 16 B in {lines} whole 128 B lines, random line "
+              f"order: {table.numel() * 4} B of rows + {idx.numel() * 8} B of indices read, {out.numel() * 4} B "
+              f"written", flush=True)
 
 
 if __name__ == "__main__":

@@ -223,7 +223,9 @@ void nwRadixSortPairs(uint *dGridParticleHash, uint *dGridParticleIndex,
    which may be NULL when the WCA pair term without activity does not read
    them) the gathered records, gridParticleHash (may be NULL) the cell of
    every bead in bead order; the reference's cellStart/cellEnd tables are not
-   produced.  Fails (NW error -22) when the grid has more than ~67M cells or
+   produced.  tangent may be NULL when the pair term reads no tangents (the
+   WCA term without activity): the step then stores none (16 B per bead
+   less), and nwFilamentTangents gives them later, bit for bit.  Fails (NW error -22) when the grid has more than ~67M cells or
    the beads x cells product exceeds the sort's tile x bucket table
    (nwFusedPlanFits). */
 void nwFusedMdStep(float *pos, float *vel, float *force, float *forceOld, float *tangent,
@@ -243,6 +245,14 @@ void nwFusedMdStepSwapForces(float *pos, float *vel, float *force, float *forceO
                              uint *gridParticleHash, uint *gridParticleIndex, float *sortedPos, float *sortedVel,
                              float *sortedTangent, float deltaTime, float langevinGamma, float langevinKbT,
                              int langevin, uint numFilaments);
+
+/* The bead tangents filamentForces writes (particles_kernel_impl.cuh:751-818:
+   unit(r[p+1] - r[p-1]), one-sided at the worm ends, none for worms of
+   fewer than 3 beads, no tail tangent for 3) from the positions alone,
+   without forces: equal bit for bit to what a fused step stores, for a
+   caller that passed no tangent array to its fused steps.  No counterpart
+   in the reference.  Worms of 1..64 beads (SimParams.filamentSize). */
+void nwFilamentTangents(float *tangent, const float *pos, uint numFilaments);
 
 /* nwFusedMdStep for callers that cannot allocate device memory -- code under
    HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
@@ -301,7 +311,10 @@ typedef struct NWTuning {
     uint32_t srdMembers;    /* SRD cell sort, second level (64..2048) */
     uint32_t pairBlock;     /* sorted beads per pair-kernel workgroup (128, 256; 512 for the WCA term) */
     uint32_t frontBlock;    /* threads per workgroup of the fused front (64, 128, 256, 512, 1024) */
-    uint32_t reserved[10];
+    uint32_t pairStage;     /* beads staged in LDS per neighbour row of a pair workgroup, on average (the
+                               three rows share one pool): pairBlock + 128 or 2 pairBlock where that fits
+                               the workgroup's LDS (0: pairBlock + 128) */
+    uint32_t reserved[9];
 } NWTuning;
 /* Applies to the fused single-system steps (nwFusedMdStep and its variants,
    nwFusedSrdStep) and to nwFusedPlanInfo; values outside the lists above

@@ -228,6 +228,7 @@ void ParticleSystem::update(float deltaTime) {
         _updateFused(deltaTime);
         return;
     }
+    _materializeTangents();
     nwRngSetSeed(m_seed, m_rngOffset);
     setParameters(&m_params);
     nwSetExtParameters(&m_ext);
@@ -273,16 +274,20 @@ void ParticleSystem::_updateFused(float deltaTime) {
     const float kT = m_langevinFdt ? m_params.kbT / deltaTime : m_params.kbT;
     const bool needVel = m_ext.pairPotential == NW_PAIR_REF_DEM;
     const bool needTan = needVel || m_params.activity != 0.0f;
+    // a step whose pair term reads no tangents stores none; they are rebuilt
+    // from the positions when the array is read (nwFilamentTangents)
+    float *tan = needTan ? m_dTangent : nullptr;
+    m_tangentStale = !needTan;
     if (m_params.reverseProbability == 0.0f) {
         // double-buffered forces: the new force lands in the old force's
         // buffer, then the two swap roles (both start cleared, w = 0)
-        nwFusedMdStepSwapForces(m_dPos, m_dVel, m_dForce, m_dForceOld, m_dTangent, m_dGridParticleHash,
+        nwFusedMdStepSwapForces(m_dPos, m_dVel, m_dForce, m_dForceOld, tan, m_dGridParticleHash,
                                 m_dGridParticleIndex, m_dSortedPos, needVel ? m_dSortedVel : nullptr,
                                 needTan ? m_dSortedTangent : nullptr, deltaTime, m_params.gamma, kT,
                                 m_solventEnabled ? 0 : 1, m_numFilaments);
         std::swap(m_dForce, m_dForceOld);
     } else {
-        nwFusedMdStep(m_dPos, m_dVel, m_dForce, m_dForceOld, m_dTangent, m_dGridParticleHash, m_dGridParticleIndex,
+        nwFusedMdStep(m_dPos, m_dVel, m_dForce, m_dForceOld, tan, m_dGridParticleHash, m_dGridParticleIndex,
                       m_dSortedPos, needVel ? m_dSortedVel : nullptr, needTan ? m_dSortedTangent : nullptr,
                       deltaTime, m_params.gamma, kT, m_solventEnabled ? 0 : 1, m_numFilaments);
     }
@@ -422,6 +427,7 @@ void ParticleSystem::reset() {
     _upload(m_dPos, m_hPos, NT);
     _upload(m_dVel, m_hVel, NT);
     _upload(m_dTangent, m_hTangent, m_numParticles);
+    m_tangentStale = false;
     _solventOrderReset();
     if (m_solventSortEvery) _sortSolvent();
 }
@@ -461,6 +467,7 @@ void ParticleSystem::resetRandom(int mode, float kT) {
     _upload(m_dPos, m_hPos, NT);
     _upload(m_dVel, m_hVel, NT);
     _upload(m_dTangent, m_hTangent, m_numParticles);
+    m_tangentStale = false;
     _upload(m_dForce, m_hForce, m_numParticles);
     _upload(m_dForceOld, m_hForce, m_numParticles);
     _solventOrderReset();
@@ -470,7 +477,15 @@ void ParticleSystem::resetRandom(int mode, float kT) {
 // --------------------------------------------------------------------------
 // arrays, dumps, output
 // --------------------------------------------------------------------------
+void ParticleSystem::_materializeTangents() {
+    if (!m_tangentStale) return;
+    setParameters(&m_params);
+    nwFilamentTangents(m_dTangent, m_dPos, m_numFilaments);
+    m_tangentStale = false;
+}
+
 float *ParticleSystem::deviceArray(ParticleArray a) {
+    if (a == TANGENT) _materializeTangents();
     switch (a) {
         case POSITION: return m_dPos;
         case VELOCITY: return m_dVel;
@@ -500,6 +515,7 @@ float *ParticleSystem::getArray(ParticleArray a) {
 
 void ParticleSystem::setArray(ParticleArray a, const float *data, int start, int count) {
     if (count <= 0) return;
+    if (a == TANGENT) _materializeTangents();  // a partial write keeps the rest current
     const bool full = a == POSITION || a == VELOCITY;
     if (full && m_solventPermuted && size_t(start) + size_t(count) > m_numParticles) {
         // the whole array in original order, the new values in, then back in slot order

@@ -122,6 +122,10 @@ private:
     bool m_perCall = false;     // reference per-call sequence instead of the fused entry points
     uint m_solventSortEvery = 0;    // collisions between solvent re-sorts (0: never)
     bool m_solventPermuted = false; // the device solvent is not in original order
+    // the fused steps stored no tangents (the pair term reads none): the
+    // device array is rebuilt from the positions when it is read
+    bool m_tangentStale = false;
+    void _materializeTangents();
     uint *m_dSolventId = nullptr;
     std::vector<uint> m_hSolventId;
 

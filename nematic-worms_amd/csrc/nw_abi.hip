@@ -267,18 +267,12 @@ bool ensure_init() {
         if (!ensure_init()) return; \
     } while (0)
 
-// fast division by a constant n: q = mulhi(i, m) with m = ceil(2^32 / n) is
-// exact for every i < 2^32 / e, e = m * n - 2^32 (Granlund & Montgomery
-// 1994); limit 0 sends every division to the hardware sequence
-static void fast_div_for(uint64_t n, uint32_t &magic, uint32_t &limit) {
-    magic = limit = 0;
-    if (n < 2) return;
+// the multiplier of div_exact for a divisor n: ceil(2^32 / n), below 2^32
+// for n >= 2 (Granlund & Montgomery 1994); 0 for n = 1 (and the invalid 0)
+static uint32_t div_magic_for(uint64_t n) {
+    if (n < 2) return 0u;
     const uint64_t two32 = 1ull << 32;
-    const uint64_t m = (two32 + n - 1) / n;
-    const uint64_t e = m * n - two32;
-    if (m >= two32) return;
-    magic = uint32_t(m);
-    limit = e == 0 ? 0xffffffffu : uint32_t(std::min<uint64_t>(0xffffffffull, (two32 - 1) / e));
+    return uint32_t((two32 + n - 1) / n);
 }
 
 KP make_kp_for(const SimParams &p, const SolventParams &sp, const NWExtParams &ext) {
@@ -286,9 +280,9 @@ KP make_kp_for(const SimParams &p, const SolventParams &sp, const NWExtParams &e
     kp.p = p;
     kp.s = sp;
     kp.e = ext;
-    fast_div_for(p.filamentSize, kp.fdivMagic, kp.fdivLimit);
-    fast_div_for(p.gridSize.x, kp.gxMagic, kp.gxLimit);
-    fast_div_for(uint64_t(p.gridSize.x) * p.gridSize.y, kp.layerMagic, kp.layerLimit);
+    kp.fdivMagic = div_magic_for(p.filamentSize);
+    kp.gxMagic = div_magic_for(p.gridSize.x);
+    kp.layerMagic = div_magic_for(uint64_t(p.gridSize.x) * p.gridSize.y);
     return kp;
 }
 
@@ -640,6 +634,8 @@ static bool fused_md(const KP &kp, float *pos, float *vel, float *force, float *
     const bool needTan = needVel || kp.p.activity != 0.0f;
     NW_REQUIRE_R(!needVel || sortedVel != nullptr, false, "nwFusedMdStep: sortedVel is NULL");
     NW_REQUIRE_R(!needTan || sortedTangent != nullptr, false, "nwFusedMdStep: sortedTangent is NULL");
+    NW_REQUIRE_R(!needTan || tangent != nullptr, false,
+                 "nwFusedMdStep: tangent is NULL but the pair term reads tangents");
     if (N == 0) return true;
     const BucketPlan bp = fused_plan(N, cells, false);
     NW_REQUIRE_R(!*bucket_plan_problem(bp), false, "nwFusedMdStep: %s", bucket_plan_problem(bp));
@@ -714,6 +710,17 @@ void nwFusedMdStepSwapForces(float *pos, float *vel, float *force, float *forceO
     (void)fused_md(make_kp(), pos, vel, force, forceOld, tangent, gridParticleHash, gridParticleIndex, sortedPos,
                    sortedVel, sortedTangent, deltaTime, langevinGamma, langevinKbT, langevin, numFilaments, nullptr,
                    nullptr, nullptr, nullptr, true);
+}
+
+void nwFilamentTangents(float *tangent, const float *pos, uint numFilaments) {
+    NW_INIT();
+    State &s = S();
+    NW_REQUIRE(tangent != nullptr && pos != nullptr, "nwFilamentTangents: NULL array");
+    const KP kp = make_kp();
+    NW_REQUIRE(kp.p.filamentSize >= 1 && kp.p.filamentSize <= 64, "nwFilamentTangents: worms of %u beads (1..64)",
+               kp.p.filamentSize);
+    NW_HIP(launch_filament_tangents(reinterpret_cast<const float4 *>(pos), reinterpret_cast<float4 *>(tangent),
+                                    numFilaments, kp, s.stream));
 }
 
 void nwFusedMdStepNoAlloc(float *pos, float *vel, float *force, float *forceOld, float *tangent,
@@ -1912,6 +1919,7 @@ void nwSetTuning(const NWTuning *t) {
                t->srdMembers);
     NW_REQUIRE(oneOf(t->pairBlock, {128, 256, 512}), "nwSetTuning: pairBlock %u", t->pairBlock);
     NW_REQUIRE(oneOf(t->frontBlock, {64, 128, 256, 512, 1024}), "nwSetTuning: frontBlock %u", t->frontBlock);
+    NW_REQUIRE(t->pairStage <= 2048, "nwSetTuning: pairStage %u", t->pairStage);
     Tuning &g = tuning();
     g.mdTileItems = t->mdTileItems;
     g.mdMembers = t->mdMembers;
@@ -1919,6 +1927,7 @@ void nwSetTuning(const NWTuning *t) {
     g.srdMembers = t->srdMembers;
     g.pairBlock = t->pairBlock;
     g.frontBlock = t->frontBlock;
+    g.pairStage = t->pairStage;
 }
 
 void nwGetTuning(NWTuning *t) {
@@ -1931,6 +1940,7 @@ void nwGetTuning(NWTuning *t) {
     t->srdMembers = g.srdMembers;
     t->pairBlock = g.pairBlock;
     t->frontBlock = g.frontBlock;
+    t->pairStage = g.pairStage;
 }
 
 int nwFusedPlanInfo(uint numParticles, uint numCells, uint filamentSize, uint out[4]) {

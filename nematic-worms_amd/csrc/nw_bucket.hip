@@ -354,6 +354,58 @@ __device__ __forceinline__ void md_gather_staged(const CellsArgs &o, const uint3
 // Bucket b of one system (see CellsArgs for the bases of a replica), BLOCK
 // threads; a bucket of at most CAP members is staged in LDS, each thread
 // holding at most PER = CAP / BLOCK of them.
+// Ascending sort of N values in registers: a bitonic network, every
+// compare-exchange at a compile-time position (branch-free min / max).
+template <int N>
+__device__ __forceinline__ void sort_network(uint32_t (&v)[N]) {
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint32_t a = v[i], b = v[l];
+                    const uint32_t lo = min(a, b), hi = max(a, b);
+                    const bool up = (i & k) == 0;
+                    v[i] = up ? lo : hi;
+                    v[l] = up ? hi : lo;
+                }
+            }
+        }
+    }
+}
+
+// The ids arr[a .. z) of one cell in ascending order (the order of a stable
+// sort by cell).  Up to NET ids are loaded together into registers and sorted
+// by the network (padded with 0xffffffff, which no id reaches); a longer cell
+// takes an insertion sort in place.
+template <int NET>
+__device__ __forceinline__ void sort_cell_ids(uint32_t *arr, uint32_t a, uint32_t z) {
+    const uint32_t m = z - a;
+    if (m <= 1u) return;
+    if (NET > 1 && m <= uint32_t(NET)) {
+        uint32_t v[NET > 1 ? NET : 2];
+#pragma unroll
+        for (int k = 0; k < NET; ++k) v[k] = uint32_t(k) < m ? arr[a + k] : 0xffffffffu;
+        sort_network<NET>(reinterpret_cast<uint32_t (&)[NET]>(v));
+#pragma unroll
+        for (int k = 0; k < NET; ++k)
+            if (uint32_t(k) < m) arr[a + k] = v[k];
+        return;
+    }
+    for (uint32_t u = a + 1; u < z; ++u) {
+        const uint32_t x = arr[u];
+        uint32_t w = u;
+        while (w > a && arr[w - 1] > x) {
+            arr[w] = arr[w - 1];
+            --w;
+        }
+        arr[w] = x;
+    }
+}
+
 template <bool SRD, int BLOCK, int CAP = int(kBucketCap)>
 __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const BucketPlan &bp, uint32_t b,
                                                   unsigned char *smem) {
@@ -451,15 +503,9 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
     uint32_t *arr = staged ? lid : out;
     for (uint32_t c = tid; c < nc; c += bd) {
         const uint32_t a = cnt[c], z = cnt[c + 1];
-        for (uint32_t u = a + 1; u < z; ++u) {
-            const uint32_t x = arr[u];
-            uint32_t w = u;
-            while (w > a && arr[w - 1] > x) {
-                arr[w] = arr[w - 1];
-                --w;
-            }
-            arr[w] = x;
-        }
+        // SRD cells hold ~6 particles (a network of 16 takes all but the
+        // rare crowded ones); MD cells hold ~1 bead
+        sort_cell_ids<SRD ? 16 : 1>(arr, a, z);
         o.off[cell0 + c] = o.slotBase + sb + a;
     }
     if (tid == 0 && cell0 + nc == bp.cells) o.off[bp.cells] = o.slotBase + bp.n;

@@ -59,6 +59,7 @@ struct Tuning {
     uint32_t srdMembers = 0;    // SRD sort second level
     uint32_t pairBlock = 0;     // sorted beads per pair-kernel workgroup (128, 256, 512)
     uint32_t frontBlock = 0;    // threads per front workgroup (64 .. 1024)
+    uint32_t pairStage = 0;     // beads staged per row run, on average: 2 pairBlock, else (0) pairBlock + 128
 };
 Tuning &tuning();
 // the plan of the fused steps' cell sort (srd: the SRD collision's) with the
@@ -98,8 +99,11 @@ struct FrontLane {
     bool writeTan;
 };
 
-// integrate + reversal + Langevin + bonded + tangent + MD cell
+// integrate + reversal + Langevin + bonded + tangent + MD cell (a.tangent
+// NULL: no tangents stored)
 hipError_t launch_md_front(const MdFrontArgs &a, const KP &kp, hipStream_t st);
+// the tangents the front stores, from the positions (worms of 1..64 beads)
+hipError_t launch_filament_tangents(const float4 *pos, float4 *tangent, uint32_t W, const KP &kp, hipStream_t st);
 // level 1 over precomputed cells
 hipError_t launch_hash_tiles(const uint32_t *hash, const BucketPlan &bp, BucketScratch &s, hipStream_t st);
 // stable sort of (keys, vals) by key, every key < bound (the per-call

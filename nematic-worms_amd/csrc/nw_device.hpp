@@ -32,30 +32,34 @@ __device__ __forceinline__ float length(f3 v) { return sqrtf(dot(v, v)); }
 __device__ __forceinline__ float4 f4(f3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
 
 // --------------------------------------------------------------------------
-// Kernel-argument parameter bundle.  Travels by value (272 bytes of kernarg),
-// so a launch never depends on a prior host->device upload.
+// Kernel-argument parameter bundle.  Travels by value (kernarg), so a launch
+// never depends on a prior host->device upload.
 // --------------------------------------------------------------------------
 struct KP {
     SimParams p;
     SolventParams s;
     NWExtParams e;
-    uint32_t fdivMagic;   // ceil(2^32 / filamentSize): q = mulhi(i, m)
-    uint32_t fdivLimit;   // mulhi quotient is exact for i < fdivLimit (0: always divide)
-    uint32_t gxMagic, gxLimit;        // the same for gridSize.x (cell index -> row)
-    uint32_t layerMagic, layerLimit;  // and for gridSize.x * gridSize.y (-> layer)
+    uint32_t fdivMagic;   // ceil(2^32 / filamentSize) (0: filamentSize 1)
+    uint32_t gxMagic;     // the same for gridSize.x (cell index -> row)
+    uint32_t layerMagic;  // and for gridSize.x * gridSize.y (-> layer)
+    uint32_t pad;
 };
 
-// x / d by the multiplier of a KP (exact below limit; the hardware sequence
-// otherwise)
-__device__ __forceinline__ uint32_t div_magic(uint32_t x, uint32_t magic, uint32_t limit, uint32_t d) {
-    return x < limit ? __umulhi(x, magic) : x / d;
+// x / d from m = ceil(2^32 / d), d >= 2.  m d = 2^32 + e with 0 <= e < d,
+// so x m / 2^32 = x / d + x e / (d 2^32) < x / d + 1 for x < 2^32: mulhi(x, m)
+// is floor(x / d) or one more, and one compare takes the excess off (exact
+// for x + d < 2^32; bead and cell ids stay below 2^31).  m = 0 stands for
+// d = 1.  No branch: four VALU operations.
+__device__ __forceinline__ uint32_t div_exact(uint32_t x, uint32_t m, uint32_t d) {
+    const uint32_t q = __umulhi(x, m);
+    return m == 0u ? x : q - (q * d > x ? 1u : 0u);
 }
 
 // coordinates of cell c of the MD grid
 __device__ __forceinline__ void cell_xyz(const KP &kp, uint32_t c, int &gx, int &gy, int &gz) {
     const uint32_t Gx = kp.p.gridSize.x, layer = kp.p.gridSize.x * kp.p.gridSize.y;
-    const uint32_t z = div_magic(c, kp.layerMagic, kp.layerLimit, layer), rem = c - z * layer;
-    const uint32_t y = div_magic(rem, kp.gxMagic, kp.gxLimit, Gx);
+    const uint32_t z = div_exact(c, kp.layerMagic, layer), rem = c - z * layer;
+    const uint32_t y = div_exact(rem, kp.gxMagic, Gx);
     gx = int(rem - y * Gx);
     gy = int(y);
     gz = int(z);
@@ -64,11 +68,7 @@ __device__ __forceinline__ void cell_xyz(const KP &kp, uint32_t c, int &gx, int 
 // i / filamentSize and i % filamentSize for bead ids.
 __device__ __forceinline__ void worm_of(const KP &kp, uint32_t i, uint32_t &w, uint32_t &c) {
     const uint32_t n = kp.p.filamentSize;
-    if (i < kp.fdivLimit) {
-        w = __umulhi(i, kp.fdivMagic);
-    } else {
-        w = i / n;
-    }
+    w = div_exact(i, kp.fdivMagic, n);
     c = i - w * n;
 }
 

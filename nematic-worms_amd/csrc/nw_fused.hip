@@ -118,7 +118,7 @@ __device__ __forceinline__ uint32_t md_front_store(const KP &kp, const WaveWorm 
         force[m.bead] = f4(o.acc, 0.0f);
         forceOld[m.bead] = f4(o.fold, o.fow);
     }
-    if (o.writeTan) tangent[m.bead] = f4(o.tg, 0.0f);
+    if (o.writeTan && tangent) tangent[m.bead] = f4(o.tg, 0.0f);
     return md_hash_wrapped(p, cell_coord(o.r.x, p.origin.x, p.cellSize.x), cell_coord(o.r.y, p.origin.y, p.cellSize.y),
                            cell_coord(o.r.z, p.origin.z, p.cellSize.z));
 }
@@ -130,6 +130,18 @@ __global__ __launch_bounds__(BLOCK) void k_md_front(MdFrontArgs a, KP kp) {
                                       a.forceOld, a.dt, a.lgvStd, a.langevin, a.seed, a.offRev, a.offLgv);
     if (m.active)
         a.hash[m.bead] = md_front_store(kp, m, o, a.pos, a.vel, a.force, a.forceOld, a.tangent, a.swapForces != 0);
+}
+
+// The tangents of the worms' beads from their positions, as the front's
+// bonded_lane writes them (the same arithmetic on the same values: bit for
+// bit), for a caller whose fused steps skipped them.
+__global__ __launch_bounds__(kBlock) void k_filament_tangents(const float4 *__restrict__ pos,
+                                                              float4 *__restrict__ tangent, uint32_t W, KP kp) {
+    const WaveWorm m = wave_worm(kp.p.filamentSize, W);
+    const f3 r = m.active ? mk(pos[m.bead]) : mk(0.f, 0.f, 0.f);
+    f3 acc = mk(0.f, 0.f, 0.f), tg = mk(0.f, 0.f, 0.f);
+    const bool write = bonded_lane(kp, r, m.pp, acc, tg);
+    if (m.active && write) tangent[m.bead] = f4(tg, 0.0f);
 }
 
 // ===========================================================================
@@ -362,30 +374,105 @@ __global__ __launch_bounds__(kBlock) void k_collide_csr(float4 *__restrict__ for
 // cells of the sorted beads given)
 //
 // A workgroup owns BLOCK (128, 256 or 512; 256 unless tuned) consecutive
-// sorted beads whose cells span the hash interval [h0, h1].  Every neighbour cell of those beads that does not wrap
-// lies in one of three hash intervals [h0 - 1 + dy Gx, h1 + 1 + dy Gx],
-// dy = -1, 0, 1, i.e. in three contiguous runs of the sorted arrays, staged in
-// LDS with coalesced loads (position + worm id in one float4; velocity +
-// chain index, tangent when the terms read them).  A bead whose three row
-// runs all lie in staged runs walks them as one flattened candidate list, so
-// the lanes of a wavefront stay busy until the longest list ends rather than
-// the longest run of every row; any other bead (x or y wrap, an overlong
-// run) takes the global-memory walk.  Same order and arithmetic as
-// k_collide_csr.
+// sorted beads whose cells span the hash interval [h0, h1].  Every neighbour
+// cell of those beads that does not wrap lies in one of three hash intervals
+// [h0 - 1 + dy Gx, h1 + 1 + dy Gx], dy = -1, 0, 1, i.e. in three contiguous
+// runs of the sorted arrays, staged back to back in one LDS pool with
+// coalesced loads (position + worm id in one float4; velocity + chain index,
+// tangent when the terms read them), beside the two cells an edge bead needs
+// across the x wrap.  A bead whose stencil lies in the stage walks it as one
+// flattened candidate list, so the lanes of a wavefront stay busy until the
+// longest list ends rather than the longest run of every row; any other bead
+// (a y wrap, an overfull stage) takes the global-memory walk.  Same order and
+// arithmetic as k_collide_csr.
 // ===========================================================================
-// staged beads per row run of a workgroup of BLOCK sorted beads
+// average staged beads per row run of a workgroup of BLOCK sorted beads (the
+// pool holds three times as many; the batched kernel's stage; a single
+// system's is tuned, pairStage)
 template <int BLOCK>
 constexpr int run_cap() {
     return BLOCK + 128;
 }
 
-// LDS of one pair-kernel workgroup
+// beads staged per wrap cell (a cell at the other end of a row)
+constexpr int kWrapCap = 8;
+
+// LDS of one pair-kernel workgroup: the three row runs back to back in a pool
+// of 3 CAP records, the two wrap cells (kWrapCap records each), and one spare
+// record (the walk reads one record ahead of the one it evaluates)
 template <int MODE, bool TAN, int CAP>
 struct RowsLds {
-    float4 pos[3 * CAP];
-    float4 vel[MODE == 0 ? 3 * CAP : 1];
-    float4 tan[TAN ? 3 * CAP : 1];
+    static constexpr int kPool = 3 * CAP;
+    static constexpr int kWrap0 = kPool;  // first slot of the wrap cells
+    static constexpr int kSlots = kPool + 2 * kWrapCap + 1;
+    float4 pos[kSlots];
+    float4 vel[MODE == 0 ? kSlots : 1];
+    float4 tan[TAN ? kSlots : 1];
 };
+
+// One candidate of the LDS walk.  The WCA term alone is evaluated for every
+// candidate and masked afterwards (no branch): a skipped one (the bead's own worm, or at
+// or beyond the cutoff, or a non-finite position) adds +0, as the branching
+// form does, which leaves the accumulator unchanged -- it starts at +0, and
+// an FP32 sum is -0 only when both of its terms are.
+template <int MODE, bool TAN, bool UNIT>
+__device__ __forceinline__ void lds_pair(const KP &kp, const Owner<MODE, TAN> &o, float rc2, float wt, f3 &acc,
+                                         const float4 &P, const float4 &V, const float4 &T) {
+    if (MODE == 1 && !TAN) {
+        const f3 rel = min_image_pair(kp.p, o.r, mk(P));
+        const float rr = dot(rel, rel);
+        const bool hit = __float_as_uint(P.w) != o.fil && rr < rc2;
+        const f3 f = -wca_mag(kp.e, rr) * rel;
+        const uint32_t m = hit ? 0xffffffffu : 0u;
+        const f3 t = mk(__uint_as_float(__float_as_uint(f.x) & m), __uint_as_float(__float_as_uint(f.y) & m),
+                        __uint_as_float(__float_as_uint(f.z) & m));
+        acc += UNIT ? t : wt * t;
+    } else {
+        add_pair<MODE, TAN, UNIT>(kp, o, rc2, wt, acc, mk(P), __float_as_uint(P.w),
+                                  MODE == 0 ? __float_as_uint(V.w) : 0u, MODE == 0 ? mk(V) : mk(0.f, 0.f, 0.f),
+                                  TAN ? mk(T) : mk(0.f, 0.f, 0.f));
+    }
+}
+
+// The walk over a bead's candidate list: NP pieces of consecutive LDS slots
+// in stencil order, piece k = pieces[k STRIDE] of the given arrays (n slots
+// from q), the pieces of stencil row r of weight wr.  Flattened index t lies in piece k for pre[k] <= t <
+// pre[k + 1], at slot t + d[k]; the next record is read while this one is
+// evaluated (t = K gives the slot after the last piece: q + n of a staged
+// piece, at most the spare slot, or 0 after an empty one).
+template <int MODE, bool TAN, bool UNIT, int NP, int STRIDE, int CAP>
+__device__ __forceinline__ void walk_lds(const KP &kp, const Owner<MODE, TAN> &o, float rc2, f3 &acc,
+                                         const uint32_t (&q)[6], const uint32_t (&n)[6], float w0, float w1,
+                                         float w2, const RowsLds<MODE, TAN, CAP> &L) {
+    uint32_t pre[NP + 1], d[NP];
+    pre[0] = 0u;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        pre[k + 1] = pre[k] + n[k * STRIDE];
+        d[k] = q[k * STRIDE] - pre[k];  // mod 2^32
+    }
+    const uint32_t K = pre[NP];
+    uint32_t s = d[NP - 1];
+#pragma unroll
+    for (int k = NP - 2; k >= 0; --k) s = 0u < pre[k + 1] ? d[k] : s;
+    float4 P = L.pos[s];
+    float4 V = MODE == 0 ? L.vel[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 T = TAN ? L.tan[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (uint32_t t = 0; t < K; ++t) {
+        const float4 Pc = P, Vc = V, Tc = T;
+        // stencil row r holds pieces [r NP / 3, (r + 1) NP / 3)
+        const float wt = UNIT ? 1.0f : (t < pre[NP / 3] ? w0 : (t < pre[2 * NP / 3] ? w1 : w2));
+        const uint32_t u = t + 1u;
+        s = d[NP - 1];
+#pragma unroll
+        for (int k = NP - 2; k >= 0; --k) s = u < pre[k + 1] ? d[k] : s;
+        s += u;
+        P = L.pos[s];
+        if (MODE == 0) V = L.vel[s];
+        if (TAN) T = L.tan[s];
+        lds_pair<MODE, TAN, UNIT>(kp, o, rc2, wt, acc, Pc, Vc, Tc);
+    }
+}
 
 // The workgroup's sorted beads [i0, min(i0 + blockDim, iEnd)) of one system
 // whose beads and cells start at beadBase / cellBase in the shared arrays
@@ -398,6 +485,7 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
                                                   const uint32_t *__restrict__ offR,
                                                   const uint32_t *__restrict__ sCell,
                                                   RowsLds<MODE, TAN, CAP> &L) {
+    using Lds = RowsLds<MODE, TAN, CAP>;
     // records move as plain 4-vectors (register values)
     typedef float v4 __attribute__((ext_vector_type(4)));
     const v4 *pos4 = reinterpret_cast<const v4 *>(sPos), *vel4 = reinterpret_cast<const v4 *>(sVel),
@@ -417,25 +505,59 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
     const v4 Vown = MODE == 0 ? vel4[ic] : v4{0.f, 0.f, 0.f, 0.f};
     const v4 Town = TAN ? tan4[ic] : v4{0.f, 0.f, 0.f, 0.f};
 
-    // the three row runs (the same for the whole workgroup: scalar loads)
-    uint32_t rs[3], rl[3];
+    // The three row runs (the same for the whole workgroup: scalar loads),
+    // back to back in the pool: run r, sorted slots [rs[r], rs[r] + rl[r]),
+    // at LDS slots [rq[r], rq[r] + rl[r]); a run that would overflow the pool
+    // is not staged (rl = 0).
+    uint32_t rs[3], rl[3], rq[3];
+    // A bead in the last cell of a row (x = Gx - 1) also needs cell x = 0 of
+    // its three stencil rows, one in the first cell (x = 0) cell x = Gx - 1.
+    // Two of those three cells lie in the row runs, and the third as well
+    // except for the tile's first right-edge cell (cell 0 of the row above
+    // it) and its last left-edge cell (cell Gx - 1 of the row below it; rows
+    // wrap within their layer).  Those two cells are staged too: sorted slots
+    // [ws[m], ws[m] + wl[m]) at LDS slots kWrap0 + m kWrapCap (wl = 0: not
+    // staged).
+    uint32_t ws[2] = {0u, 0u}, wl[2] = {0u, 0u};
+    uint32_t used = 0u;
     {
         const uint32_t last = min(i0 + blockDim.x, iEnd) - 1u;
         const uint32_t h0 = sCell[i0] - cellBase, h1 = sCell[last] - cellBase;
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-            const int64_t lo = int64_t(h0) - 1 + int64_t(r - 1) * Gx;
-            const int64_t hi = int64_t(h1) + 2 + int64_t(r - 1) * Gx;  // exclusive
-            const uint32_t clo = uint32_t(lo < 0 ? 0 : (lo > int64_t(C) ? C : lo));
-            const uint32_t chi = uint32_t(hi < 0 ? 0 : (hi > int64_t(C) ? C : hi));
+            // the hash interval [h0 - 1 + (r - 1) Gx, h1 + 2 + (r - 1) Gx) within [0, C] (C < 2^31)
+            const int lo = int(h0) - 1 + (r - 1) * int(Gx), hi = int(h1) + 2 + (r - 1) * int(Gx);
+            const uint32_t clo = uint32_t(min(max(lo, 0), int(C))), chi = uint32_t(min(max(hi, 0), int(C)));
             const uint32_t s = offR[clo], e = offR[chi];
+            const uint32_t n = e - s;
             rs[r] = s;
-            rl[r] = (e > s && e - s <= uint32_t(CAP)) ? e - s : 0u;
+            rl[r] = used + n <= uint32_t(Lds::kPool) ? n : 0u;
+            rq[r] = used;
+            used += rl[r];
+        }
+        int x0, y0, z0, x1, y1, z1;
+        cell_xyz(kp, h0, x0, y0, z0);
+        cell_xyz(kp, h1, x1, y1, z1);
+        // the first cell x = Gx - 1 at or after h0 (in h0's row)
+        if (h0 + (Gx - 1u - uint32_t(x0)) <= h1) {
+            const uint32_t y = uint32_t(y0) == 0u ? Gy - 1u : uint32_t(y0) - 1u;
+            const uint32_t c = (uint32_t(z0) * Gy + y) * Gx;
+            const uint32_t s = offR[c], e = offR[c + 1u];
+            ws[0] = s;
+            wl[0] = e - s <= uint32_t(kWrapCap) ? e - s : 0u;
+        }
+        // the last cell x = 0 at or before h1 (in h1's row)
+        if (h1 - uint32_t(x1) >= h0) {
+            const uint32_t y = uint32_t(y1) + 1u == Gy ? 0u : uint32_t(y1) + 1u;
+            const uint32_t c = (uint32_t(z1) * Gy + y) * Gx + Gx - 1u;
+            const uint32_t s = offR[c], e = offR[c + 1u];
+            ws[1] = s;
+            wl[1] = e - s <= uint32_t(kWrapCap) ? e - s : 0u;
         }
     }
 
     // in flight together: the old force, the bounds of the bead's three rows
-    // and the records of the three staged runs
+    // and the staged records
     const v4 Fo = reinterpret_cast<const v4 *>(force)[me];
     int gx, gy, gzi;
     cell_xyz(kp, cOwn, gx, gy, gzi);
@@ -443,45 +565,57 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
     const AxisCells ay = axis_cells_in(gy, Gy, false);
     const float wz = kp.e.stencil == NW_STENCIL_2D ? 1.0f : axis_cells_in(gzi, p.gridSize.z, false).w[0];
     const bool inner = gx >= 1 && gx + 1 < int(Gx);
-    uint32_t ra[3] = {0u, 0u, 0u}, rb[3] = {0u, 0u, 0u};
+    // an inner bead's three stencil rows (cells gx - 1 .. gx + 1), read by
+    // every lane from in-bounds cells (an edge bead's are unused: its six
+    // pieces are loaded after the barrier, few beads, fewer registers held
+    // across it)
+    const bool edgeOk = !inner && ay.n == 3;
+    const uint32_t gxc = inner ? uint32_t(gx) : 1u;
+    uint32_t ra[3], rb[3];
 #pragma unroll
     for (int yi = 0; yi < 3; ++yi) {
-        if (yi < ay.n && inner) {
-            const uint32_t base = (gz * Gy + uint32_t(ay.c[yi])) * Gx;
-            ra[yi] = offR[base + gx - 1];
-            rb[yi] = offR[base + gx + 2];
-        }
+        const uint32_t base = (gz * Gy + uint32_t(yi < ay.n ? ay.c[yi] : ay.c[0])) * Gx;
+        ra[yi] = offR[base + gxc - 1u];
+        rb[yi] = offR[base + gxc + 2u];
     }
-    constexpr int RH = (CAP + BLOCK - 1) / BLOCK;  // staged records per thread and row
-    v4 SP[3][RH], SV[3][RH], ST[3][RH];
-    uint32_t SI[3][RH];
+    // The staged records: thread x takes pool slots x + h BLOCK below
+    // `used`, and in the last round threads kWT.. take the wrap cells' records
+    // (past the pool).  Loads of unused records read bead 0 (in bounds).
+    constexpr int RH = (Lds::kPool + 2 * kWrapCap + BLOCK - 1) / BLOCK;
+    constexpr int kWT = BLOCK - 2 * kWrapCap;
+    static_assert((RH - 1) * BLOCK + kWT >= Lds::kPool, "wrap records past the pool");
+    v4 SP[RH], SV[RH], ST[RH];
+    uint32_t SI[RH], SD[RH];
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        const uint32_t s = rs[r], l = rl[r];
-#pragma unroll
-        for (int h = 0; h < RH; ++h) {
-            // an empty run reads bead 0 (in bounds, unused)
-            const uint32_t j = l ? s + min(threadIdx.x + uint32_t(h) * BLOCK, l - 1u) : 0u;
-            SP[r][h] = pos4[j];
-            SI[r][h] = sIdx[j];
-            if (MODE == 0) SV[r][h] = vel4[j];
-            if (TAN) ST[r][h] = tan4[j];
+    for (int h = 0; h < RH; ++h) {
+        const uint32_t k = threadIdx.x + uint32_t(h) * BLOCK;
+        uint32_t j = k < rq[1] ? rs[0] + k : (k < rq[2] ? rs[1] + (k - rq[1]) : rs[2] + (k - rq[2]));
+        bool st = k < used;
+        uint32_t dst = k;
+        if (h == RH - 1) {
+            const uint32_t wk = threadIdx.x - uint32_t(kWT), c = wk % uint32_t(kWrapCap);
+            const bool wt = threadIdx.x >= uint32_t(kWT), m1 = wk >= uint32_t(kWrapCap);
+            const bool wv = wt && c < (m1 ? wl[1] : wl[0]);
+            j = wv ? (m1 ? ws[1] : ws[0]) + c : j;
+            st = wt ? wv : st;
+            dst = wt ? uint32_t(Lds::kWrap0) + wk : dst;
         }
+        j = st ? j : 0u;
+        SD[h] = st ? dst : ~0u;
+        SP[h] = pos4[j];
+        SI[h] = sIdx[j];
+        if (MODE == 0) SV[h] = vel4[j];
+        if (TAN) ST[h] = tan4[j];
     }
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        const uint32_t l = rl[r];
-#pragma unroll
-        for (int h = 0; h < RH; ++h) {
-            const uint32_t k = threadIdx.x + uint32_t(h) * BLOCK;
-            if (k < l) {
-                uint32_t fil, ch;
-                worm_of(kp, SI[r][h] - beadBase, fil, ch);
-                L.pos[r * CAP + k] = make_float4(SP[r][h].x, SP[r][h].y, SP[r][h].z, __uint_as_float(fil));
-                if (MODE == 0)
-                    L.vel[r * CAP + k] = make_float4(SV[r][h].x, SV[r][h].y, SV[r][h].z, __uint_as_float(ch));
-                if (TAN) L.tan[r * CAP + k] = make_float4(ST[r][h].x, ST[r][h].y, ST[r][h].z, ST[r][h].w);
-            }
+    for (int h = 0; h < RH; ++h) {
+        const uint32_t dst = SD[h];
+        if (dst != ~0u) {
+            uint32_t fil, ch;
+            worm_of(kp, SI[h] - beadBase, fil, ch);
+            L.pos[dst] = make_float4(SP[h].x, SP[h].y, SP[h].z, __uint_as_float(fil));
+            if (MODE == 0) L.vel[dst] = make_float4(SV[h].x, SV[h].y, SV[h].z, __uint_as_float(ch));
+            if (TAN) L.tan[dst] = make_float4(ST[h].x, ST[h].y, ST[h].z, ST[h].w);
         }
     }
     __syncthreads();
@@ -494,66 +628,104 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
     worm_of(kp, me - beadBase, o.fil, o.ch);
     const float rc2 = kp.e.wcaCutoff * kp.e.wcaCutoff;
 
-    // LDS start and length of every row's run; any row outside the stage
-    // sends the bead down the global walk
-    uint32_t q[3] = {0u, 0u, 0u}, Lr[3] = {0u, 0u, 0u};
-    float w[3] = {0.f, 0.f, 0.f};
-    bool fast = inner;
+    // The bead's candidate pieces in stencil order, two per stencil row: an
+    // inner bead's row is one piece (cells gx - 1 .. gx + 1), an edge bead's
+    // two (x = 0: cell Gx - 1, then cells 0, 1; x = Gx - 1: cells Gx - 2,
+    // Gx - 1, then cell 0).  A bead with a piece outside the stage takes the
+    // global walk.
+    uint32_t pq[6] = {0u, 0u, 0u, 0u, 0u, 0u}, pn[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+    float rw[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int yi = 0; yi < 3; ++yi) {
-        if (yi < ay.n && fast) {
-            const uint32_t a = ra[yi], b = rb[yi];
+    for (int yi = 0; yi < 3; ++yi)
+        if (yi < ay.n) rw[yi] = wz * ay.w[yi];
+    bool lds = false;
+    if (inner) {
+        // stencil row yi (no y wrap) lies in the hash interval of run yi
+        bool ok = true;
+#pragma unroll
+        for (int yi = 0; yi < 3; ++yi) {
+            if (yi < ay.n) {
+                const uint32_t a = ra[yi], b = rb[yi];
+                const uint32_t u = a - rs[yi], v = b - rs[yi];  // in the run: u <= v <= rl (mod 2^32)
+                ok = ok && (a == b || (u <= rl[yi] && v <= rl[yi]));
+                pq[2 * yi] = rq[yi] + u;
+                pn[2 * yi] = b - a;
+            }
+        }
+        lds = ok;
+    } else if (edgeOk) {
+        const bool left = gx == 0;
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const uint32_t base = (gz * Gy + uint32_t(ay.c[k / 2])) * Gx;
+            const uint32_t a = (k & 1) ? offR[base] : offR[left ? base + Gx - 1u : base + Gx - 2u];
+            const uint32_t b = (k & 1) ? offR[left ? base + 2u : base + 1u] : offR[base + Gx];
             bool hit = a == b;
+            uint32_t q = 0u;
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
-                if (!hit && rl[r] && a >= rs[r] && b <= rs[r] + rl[r]) {
-                    q[yi] = uint32_t(r) * CAP + (a - rs[r]);
-                    hit = true;
-                }
+                const uint32_t u = a - rs[r], v = b - rs[r];
+                const bool in = u <= rl[r] && v <= rl[r];
+                q = !hit && in ? rq[r] + u : q;
+                hit = hit || in;
             }
-            Lr[yi] = b - a;
-            w[yi] = wz * ay.w[yi];
-            fast = hit;
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                const bool in = a == ws[m] && b - a == wl[m];
+                q = !hit && in ? uint32_t(Lds::kWrap0) + uint32_t(m) * kWrapCap : q;
+                hit = hit || in;
+            }
+            pq[k] = q;
+            pn[k] = b - a;
+            ok = ok && hit;
         }
+        lds = ok;
     }
 
     f3 acc = mk(0.f, 0.f, 0.f);
-    if (fast) {
-        const uint32_t L01 = Lr[0] + Lr[1], K = L01 + Lr[2];
-        const uint32_t d0 = q[0], d1 = q[1] - Lr[0], d2 = q[2] - L01;  // slot = t + d (mod 2^32)
-        for (uint32_t t = 0; t < K; ++t) {
-            const bool in0 = t < Lr[0], in1 = t < L01;
-            const uint32_t s = t + (in0 ? d0 : (in1 ? d1 : d2));
-            const float wt = UNIT ? 1.0f : (in0 ? w[0] : (in1 ? w[1] : w[2]));
-            const float4 P = L.pos[s];
-            f3 v2 = mk(0.f, 0.f, 0.f);
-            uint32_t ch2 = 0u;
-            if (MODE == 0) {
-                const float4 V = L.vel[s];
-                v2 = mk(V);
-                ch2 = __float_as_uint(V.w);
-            }
-            add_pair<MODE, TAN, UNIT>(kp, o, rc2, wt, acc, mk(P), __float_as_uint(P.w), ch2, v2,
-                                      TAN ? mk(L.tan[s]) : mk(0.f, 0.f, 0.f));
-        }
+    // a wavefront with an edge bead on the LDS walk walks six pieces for all
+    // of its beads, any other three
+    if (__any(lds && !inner)) {
+        if (lds) walk_lds<MODE, TAN, UNIT, 6, 1>(kp, o, rc2, acc, pq, pn, rw[0], rw[1], rw[2], L);
     } else {
-        add_all_global<MODE, TAN>(kp, o, rc2, acc, gx, gy, int(gz), offR, sPos, sVel, sTan, sIdx, beadBase);
+        if (lds) walk_lds<MODE, TAN, UNIT, 3, 2>(kp, o, rc2, acc, pq, pn, rw[0], rw[1], rw[2], L);
     }
+    if (!lds) add_all_global<MODE, TAN>(kp, o, rc2, acc, gx, gy, int(gz), offR, sPos, sVel, sTan, sIdx, beadBase);
     force[me] = make_float4(Fo.x + acc.x, Fo.y + acc.y, Fo.z + acc.z, Fo.w + 0.0f);
 }
 
-template <int MODE, bool TAN, bool UNIT, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_collide_rows(float4 *__restrict__ force, const float4 *__restrict__ sPos,
+// The stage of a single system's pair workgroup: BIG pools six times its
+// beads, else 3 (BLOCK + 128) (a pool of three runs overflows in a handful of
+// tiles of a relaxed 800k system); a stage whose LDS would pass the 64 KiB a
+// workgroup's static LDS may take falls back to the smaller one.
+template <int MODE, bool TAN, int BLOCK, bool BIG>
+constexpr int pair_stage() {
+    constexpr int rec = 16 * (1 + (MODE == 0 ? 1 : 0) + (TAN ? 1 : 0));
+    constexpr int big = 2 * BLOCK;
+    return BIG && (3 * big + 2 * kWrapCap + 1) * rec <= 65536 ? big : run_cap<BLOCK>();
+}
+
+// waves per SIMD the pair kernels are compiled for: the WCA-only kernel
+// fits eight (64 VGPRs; its walk waits on LDS and needs the waves to hide
+// it), the larger records leave the compiler its own choice
+template <int MODE, bool TAN>
+constexpr int pair_waves() {
+    return MODE == 1 && !TAN ? 8 : 1;
+}
+
+template <int MODE, bool TAN, bool UNIT, int BLOCK, int CAP>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(pair_waves<MODE, TAN>()))) void k_collide_rows(float4 *__restrict__ force, const float4 *__restrict__ sPos,
                                                          const float4 *__restrict__ sVel,
                                                          const float4 *__restrict__ sTan,
                                                          const uint32_t *__restrict__ sIdx,
                                                          const uint32_t *__restrict__ off,
                                                          const uint32_t *__restrict__ sCell, uint32_t n, KP kp) {
-    __shared__ RowsLds<MODE, TAN, run_cap<BLOCK>()> L;
+    __shared__ RowsLds<MODE, TAN, CAP> L;
     // neighbouring tiles share row runs: a contiguous range of tiles per XCD
     // keeps their re-reads in that XCD's L2
     const uint32_t tile = xcd_contiguous(blockIdx.x, gridDim.x);
-    collide_rows_tile<MODE, TAN, UNIT, BLOCK>(kp, tile * blockDim.x, n, 0u, 0u, force, sPos, sVel, sTan, sIdx, off, sCell,
+    collide_rows_tile<MODE, TAN, UNIT, BLOCK, CAP>(kp, tile * blockDim.x, n, 0u, 0u, force, sPos, sVel, sTan, sIdx, off, sCell,
                                        L);
 }
 
@@ -590,7 +762,7 @@ __global__ __launch_bounds__(kBlock) void k_md_front_batch(BatchArrays a, const 
 }
 
 template <int MODE, bool TAN, bool UNIT>
-__global__ __launch_bounds__(kPairBlock) void k_collide_rows_batch(BatchArrays a, const BatchRep *__restrict__ reps,
+__global__ __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(pair_waves<MODE, TAN>()))) void k_collide_rows_batch(BatchArrays a, const BatchRep *__restrict__ reps,
                                                                uint32_t R, uint64_t *__restrict__ stepCtr) {
     __shared__ RowsLds<MODE, TAN, run_cap<kPairBlock>()> L;
     // the step's last kernel advances the step counter the next front reads
@@ -624,6 +796,15 @@ hipError_t launch_md_front(const MdFrontArgs &a, const KP &kp, hipStream_t st) {
     return hipGetLastError();
 }
 
+hipError_t launch_filament_tangents(const float4 *pos, float4 *tangent, uint32_t W, const KP &kp, hipStream_t st) {
+    const uint32_t n = kp.p.filamentSize;
+    if (W == 0 || n < 1 || n > 64) return hipSuccess;
+    const uint32_t wpw = 64u / n;
+    const uint64_t threads = ((uint64_t(W) + wpw - 1) / wpw) * 64u;
+    NW_LAUNCH(k_filament_tangents, dim3(grid_for(threads)), dim3(kBlock), 0, st, pos, tangent, W, kp);
+    return hipGetLastError();
+}
+
 hipError_t launch_collide_off(float4 *force, const float4 *sPos, const float4 *sVel, const float4 *sTan,
                               const uint32_t *sIdx, const uint32_t *off, const uint32_t *sCell, uint32_t n,
                               const KP &kp, hipStream_t st) {
@@ -637,13 +818,20 @@ hipError_t launch_collide_off(float4 *force, const float4 *sPos, const float4 *s
     // pair term, whose stage fits the static LDS of a workgroup)
     const uint32_t tb = tuning().pairBlock;
     const int pb = tb == 128u ? 128 : (tb == 512u && wca ? 512 : kPairBlock);
+    // the larger stage when pairStage asks for it
+    const bool big = tuning().pairStage > uint32_t(pb + 128);
     const dim3 g(grid_for(n, lds ? uint32_t(pb) : uint32_t(kBlock))), b(lds ? pb : kBlock);
 #define NW_COLLIDE(K, ...) NW_LAUNCH((K<__VA_ARGS__>), g, b, 0, st, force, sPos, sVel, sTan, sIdx, off, sCell, n, kp)
+#define NW_ROWS_B(M, T, U, B)                                                                            \
+    do {                                                                                                 \
+        if (big) NW_COLLIDE(k_collide_rows, M, T, U, B, (pair_stage<M, T, B, true>()));                  \
+        else NW_COLLIDE(k_collide_rows, M, T, U, B, (pair_stage<M, T, B, false>()));                     \
+    } while (0)
 #define NW_COLLIDE_ROWS(M, T, U)                                                                         \
     do {                                                                                                 \
-        if (pb == 128) NW_COLLIDE(k_collide_rows, M, T, U, 128);                                         \
-        else if (pb == 512) NW_COLLIDE(k_collide_rows, M, T, U, (M == 1 ? 512 : kPairBlock));            \
-        else NW_COLLIDE(k_collide_rows, M, T, U, kPairBlock);                                            \
+        if (pb == 128) NW_ROWS_B(M, T, U, 128);                                                          \
+        else if (pb == 512) NW_ROWS_B(M, T, U, (M == 1 ? 512 : kPairBlock));                             \
+        else NW_ROWS_B(M, T, U, kPairBlock);                                                             \
     } while (0)
     if (lds) {
         // every row weight is 1 for the one-layer stencil over >= 3 rows
@@ -658,6 +846,7 @@ hipError_t launch_collide_off(float4 *force, const float4 *sPos, const float4 *s
             else NW_COLLIDE_ROWS(0, true, false);
         }
 #undef NW_COLLIDE_ROWS
+#undef NW_ROWS_B
     } else {
         if (wca && !tan) NW_COLLIDE(k_collide_csr, 1, false);
         else if (wca) NW_COLLIDE(k_collide_csr, 1, true);

@@ -98,7 +98,7 @@ __device__ __forceinline__ AxisCells axis_cells(int g, uint32_t G, bool onlyCent
 // axis_cells of a coordinate already in [0, G): the neighbours wrap by
 // selects (the same cells as wrap_mod gives)
 __device__ __forceinline__ AxisCells axis_cells_in(int g, uint32_t G, bool onlyCentre) {
-    AxisCells a;
+    AxisCells a = {{g, g, g}, {0.f, 0.f, 0.f}, 0};  // unused entries: the centre cell, weight 0
     if (onlyCentre) {
         a.c[0] = g;
         a.w[0] = 1.0f;
@@ -153,18 +153,20 @@ __device__ __forceinline__ f3 extensile_force(const SimParams &p, f3 rA, f3 rB, 
     return mk(0.f, 0.f, 0.f);
 }
 
+// 24 eps (2 s^12/r^14 - s^6/r^8) at rr = r^2, with the hardware reciprocal
+// (1 ulp): the force is a product of powers of 1/r^2
+__device__ __forceinline__ float wca_mag(const NWExtParams &e, float rr) {
+    const float inv2 = __builtin_amdgcn_rcpf(rr);
+    const float s2 = e.wcaSigma * e.wcaSigma * inv2;
+    const float s6 = s2 * s2 * s2;
+    return 24.0f * e.wcaEpsilon * inv2 * s6 * (2.0f * s6 - 1.0f);
+}
+
 // WCA-LJ force on A from B: F_A = -24 eps (2 s^12/r^14 - s^6/r^8) (rB - rA), r < rc
 __device__ __forceinline__ f3 wca_force(const SimParams &p, const NWExtParams &e, f3 rA, f3 rB, float rc2) {
     const f3 rel = min_image_pair(p, rA, rB);
     const float rr = dot(rel, rel);
-    if (rr < rc2) {
-        // hardware reciprocal (1 ulp): the force is a product of powers of 1/r^2
-        const float inv2 = __builtin_amdgcn_rcpf(rr);
-        const float s2 = e.wcaSigma * e.wcaSigma * inv2;
-        const float s6 = s2 * s2 * s2;
-        const float fmag = 24.0f * e.wcaEpsilon * inv2 * s6 * (2.0f * s6 - 1.0f);
-        return -fmag * rel;
-    }
+    if (rr < rc2) return -wca_mag(e, rr) * rel;
     return mk(0.f, 0.f, 0.f);
 }
 

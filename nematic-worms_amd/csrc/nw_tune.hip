@@ -81,7 +81,8 @@ bool cache_lookup(const std::string &file, const std::string &dev, const std::st
         if (line.substr(0, a) != dev || line.substr(a + 1, b - a - 1) != size) continue;
         NWTuning c{};
         std::istringstream v(line.substr(b + 1));
-        if (v >> c.mdTileItems >> c.mdMembers >> c.srdTileItems >> c.srdMembers >> c.pairBlock >> c.frontBlock) {
+        if (v >> c.mdTileItems >> c.mdMembers >> c.srdTileItems >> c.srdMembers >> c.pairBlock >> c.frontBlock >>
+            c.pairStage) {
             t = c;
             found = true;
         }
@@ -94,7 +95,7 @@ void cache_store(const std::string &file, const std::string &dev, const std::str
     std::ofstream out(file, std::ios::app);
     if (!out) return;  // an unwritable cache only costs the next run its timing
     out << dev << '\t' << size << '\t' << t.mdTileItems << ' ' << t.mdMembers << ' ' << t.srdTileItems << ' '
-        << t.srdMembers << ' ' << t.pairBlock << ' ' << t.frontBlock << '\n';
+        << t.srdMembers << ' ' << t.pairBlock << ' ' << t.frontBlock << ' ' << t.pairStage << '\n';
 }
 
 double avg_ms(const char *fam) {
@@ -222,12 +223,15 @@ extern "C" int nwAutotuneFused(float *pos, float *vel, float *force, float *forc
         search(c, {"md_tiles", "md_sort"}, false);
     }
     {
+        // the pair workgroup and its stage (0: pairBlock + 128 per row)
         std::vector<NWTuning> c;
-        for (uint32_t pb : {128u, 512u}) {
-            NWTuning t = best;
-            t.pairBlock = pb;
-            c.push_back(t);
-        }
+        for (uint32_t pb : {128u, 256u, 512u})
+            for (uint32_t stage : {0u, 2u * pb}) {
+                NWTuning t = best;
+                t.pairBlock = pb;
+                t.pairStage = stage;
+                c.push_back(t);
+            }
         search(c, {"md_collide"}, false);
     }
     if (numSolvent) {

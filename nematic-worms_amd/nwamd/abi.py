@@ -75,7 +75,7 @@ NW_COLLIDE_AUTO, NW_COLLIDE_GLOBAL, NW_COLLIDE_LDS = 0, 1, 2
 class NWTuning(ctypes.Structure):
     """include/nw_abi.h NWTuning: launch shapes of the fused kernels (0: built-in)."""
     _fields_ = [("mdTileItems", c_uint), ("mdMembers", c_uint), ("srdTileItems", c_uint), ("srdMembers", c_uint),
-                ("pairBlock", c_uint), ("frontBlock", c_uint), ("reserved", c_uint * 10)]
+                ("pairBlock", c_uint), ("frontBlock", c_uint), ("pairStage", c_uint), ("reserved", c_uint * 9)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -148,6 +148,7 @@ PROTOTYPES = {
     "nwFusedMdStep": (None, [P, P, P, P, P, P, P, P, P, P, c_float, c_float, c_float, c_int, c_uint]),
     "nwFusedMdStepSwapForces": (None, [P, P, P, P, P, P, P, P, P, P, c_float, c_float, c_float, c_int, c_uint]),
     "nwFusedMdStepNoAlloc": (None, [P, P, P, P, P, P, P, P, P, P, c_float, c_float, c_float, c_int, c_uint]),
+    "nwFilamentTangents": (None, [P, P, c_uint]),
     "nwFusedSrdStep": (None, [P, P, P, P, P, c_float, c_uint, c_uint, c_int]),
     "nwFusedPlanFits": (c_int, [c_uint, c_uint, c_uint]),
     "nwFusedPlanInfo": (c_int, [c_uint, c_uint, c_uint, P]),

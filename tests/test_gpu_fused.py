@@ -245,6 +245,71 @@ def test_lds_and_global_pair_kernels_agree_bitwise(nwlib, mode, activity):
         assert np.array_equal(bits(x), bits(y))
 
 
+def banded_rows(p, rng, dense=6):
+    """Straight worms along x filling every row of a one-layer grid: one bead
+    per cell in even rows, `dense` per cell in odd rows; each row shifted by
+    a random amount, so worms cross the x faces, and every bead jittered."""
+    gx, gy = p.gridSize.x, p.gridSize.y
+    c, n = p.cellSize.x, p.filamentSize
+    o = np.array([p.origin.x, p.origin.y, p.origin.z])
+    rows = []
+    for y in range(gy):
+        per = dense if y % 2 else 1
+        k = np.arange(gx * per)
+        x = (k + 0.5 + 0.3 * (rng.random(len(k)) - 0.5)) * (c / per) + rng.random() * gx * c
+        x = np.mod(x, gx * c)
+        yy = (y + 0.5 + 0.4 * (rng.random(len(k)) - 0.5)) * c
+        rows.append(np.stack([x, yy, np.full(len(k), 0.5 * p.cellSize.z)], 1))
+    r = np.concatenate(rows) + o
+    assert len(r) % n == 0
+    pos = np.zeros((len(r), 4), np.float32)
+    pos[:, :3] = r
+    pos[:, 3] = p.particleRadius
+    vel = np.zeros_like(pos)
+    vel[:, :3] = 0.1 * (rng.random((len(r), 3)) - 0.5)
+    vel[:, 2] = 0.0
+    vel[:, 3] = 1.0
+    return pos, vel
+
+
+@pytest.mark.parametrize("pair_block", [0, 128, 512])
+def test_lds_pair_kernel_overfull_stage_bitwise(nwlib, pair_block):
+    """Rows of six beads per cell between rows of one: a workgroup on a
+    sparse row needs more neighbour-row beads than its LDS pool holds (three
+    runs of pairBlock + 128 on average), so some runs stay unstaged and their
+    beads take the global walk.  Forces, positions and velocities equal the
+    global kernel's bit for bit, for every pair workgroup size."""
+    from nwamd.abi import NWTuning
+    rng = np.random.default_rng([31, pair_block])
+    lib = nwlib
+    grid, n = (120, 40, 1), 10
+    W = (120 * 20 + 120 * 6 * 20) // n
+    p = sim_params(grid=grid, cell=1.1236, W=W, n=n, bondSpringL=0.8, activity=0.0)
+    pos, vel = banded_rows(p, rng)
+    tan = np.zeros_like(pos)
+    f = np.zeros_like(pos)
+    res = []
+    lib.nwSetTuning(ctypes.byref(NWTuning(pairBlock=pair_block)))
+    try:
+        for kern in (abi.NW_COLLIDE_GLOBAL, abi.NW_COLLIDE_LDS):
+            e = ext_params(pairPotential=abi.NW_PAIR_WCA, bondedMode=abi.NW_BONDED_CHAIN, driveForce=1.0,
+                           stencil=abi.NW_STENCIL_2D, collideKernel=kern)
+            lib.setParameters(ctypes.byref(p))
+            lib.nwSetExtParameters(ctypes.byref(e))
+            b = state(lib, p, pos, vel, tan, f, f)
+            lib.nwRngSetSeed(3, 0)
+            lib.nwFusedMdStep(b["pos"].p, b["vel"].p, b["f"].p, b["fo"].p, b["tan"].p, b["h"].p, b["i"].p,
+                              b["sp"].p, None, None, 1e-4, 1.0, 0.5, 1, p.numFilaments)
+            assert lib.nwGetLastError() == 0
+            res.append([down(b, k, p.numParticles) for k in ("pos", "vel", "f")])
+    finally:
+        lib.nwSetTuning(ctypes.byref(NWTuning()))
+    assert np.all(np.isfinite(res[0][2]))
+    assert np.abs(res[0][2][:, :2]).max() > 1.0  # the dense rows push hard
+    for x, y in zip(*res):
+        assert np.array_equal(bits(x), bits(y))
+
+
 @pytest.mark.parametrize("mode,lgv", [("wca", 0), ("wca", 1), ("ref", 1)])
 def test_swap_forces_step_matches_fused_step(nwlib, mode, lgv):
     """nwFusedMdStepSwapForces with the caller swapping force / forceOld after
@@ -345,3 +410,41 @@ def test_fused_srd_step_8192_particle_tiles(nwlib):
     assert np.array_equal(bits(pa), bits(pb))
     assert np.array_equal(bits(va), bits(vb))
     assert np.array_equal(bits(ca[:, :3]), bits(cb[:, :3]))
+
+
+@pytest.mark.parametrize("n", [2, 3, 10, 20])
+def test_tangentless_step_and_tangents_from_positions(nwlib, n):
+    """nwFusedMdStep with no tangent array (the WCA term reads none) gives
+    every other output of the step with one bit for bit, and
+    nwFilamentTangents on the new positions gives the tangents that step
+    stored, bit for bit (none for n < 3, no tail tangent for n = 3)."""
+    rng = np.random.default_rng(40 + n)
+    lib = nwlib
+    p = sim_params(grid=(48, 48, 1), cell=1.1236, W=180, n=n, bondSpringL=0.8, activity=0.0)
+    e = ext_params(pairPotential=abi.NW_PAIR_WCA, bondedMode=abi.NW_BONDED_CHAIN, driveForce=1.0,
+                   stencil=abi.NW_STENCIL_2D)
+    pos, vel, _ = random_worms(p, rng, twod=True)
+    N = p.numFilaments * n
+    f, fo = random_forces(rng, N, 0.5), random_forces(rng, N, 0.5)
+    f[:, 3] = fo[:, 3] = 0
+    sentinel = np.full((N, 4), 7.0, np.float32)
+    lib.setParameters(ctypes.byref(p))
+    lib.nwSetExtParameters(ctypes.byref(e))
+    outs = []
+    for with_tan in (True, False):
+        d = {k: dev(lib, a) for k, a in dict(pos=pos, vel=vel, f=f, fo=fo, tan=sentinel).items()}
+        h, i, sp = DeviceArray(lib, 4 * N), DeviceArray(lib, 4 * N), DeviceArray(lib, 16 * N)
+        lib.nwRngSetSeed(8, 80)
+        lib.nwFusedMdStep(d["pos"].p, d["vel"].p, d["f"].p, d["fo"].p, d["tan"].p if with_tan else None, h.p, i.p,
+                          sp.p, None, None, 2e-3, 2.0, 50.0, 1, p.numFilaments)
+        outs.append({k: d[k].download(np.uint32, (N, 4)) for k in ("pos", "vel", "f", "fo", "tan")})
+        outs[-1]["i"] = i.download(np.uint32, (N,))
+        if not with_tan:
+            assert np.array_equal(outs[-1]["tan"], sentinel.view(np.uint32))  # nothing stored
+            lib.nwFilamentTangents(d["tan"].p, d["pos"].p, p.numFilaments)
+            outs[-1]["tan"] = d["tan"].download(np.uint32, (N, 4))
+    a, b = outs
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    if n < 3:
+        assert np.array_equal(a["tan"], sentinel.view(np.uint32))

@@ -297,3 +297,28 @@ def test_reference_start_is_the_lattice(lib, n, grid):
     assert 0.35 * cfg.numFilaments < reversed_count < 0.65 * cfg.numFilaments
     assert np.all(np.abs(V[:, :3]) <= 0.1) and np.all(V[:, 3] == p.particleMass)
     s.close()
+
+
+def test_tangents_rebuilt_on_read(lib):
+    """The benchmark physics reads no tangents, so the host driver's fused
+    steps store none; reading the array rebuilds it from the positions:
+    unit(r[p+1] - r[p-1]) under the minimum image, one-sided at the ends."""
+    cfg = system.config_srd_800k()
+    cfg.numFilaments, cfg.filamentSize = 400, 20
+    cfg.gridSize = abi.UInt3(89, 89, 1)
+    cfg.boxSize = abi.Float3(100.0, 100.0, 100.0 / 89)
+    cfg.srdGridSize = abi.UInt3(100, 100, 1)
+    s = system.System(cfg)
+    system.apply_worm_physics(s, kappa=5.0, gamma=2.0, kT=0.1)
+    s.reset(system.NW_INIT_LATTICE_RANDOM, 0.1)
+    s.update(1e-3, 300)
+    P, T = s.get(system.POSITION)[:s.n, :3].astype(np.float64), s.get(system.TANGENT)[:, :3]
+    W, n = cfg.numFilaments, cfg.filamentSize
+    L = np.array([s.params.boxSize.x, s.params.boxSize.y, s.params.boxSize.z])
+    r = P.reshape(W, n, 3)
+    hi = np.concatenate([r[:, 1:], r[:, -1:]], axis=1)
+    lo = np.concatenate([r[:, :1], r[:, :-1]], axis=1)
+    d = periodic_diff(hi, lo, L)
+    t = (d / np.linalg.norm(d, axis=-1, keepdims=True)).reshape(-1, 3)
+    assert np.abs(T - t).max() < 1e-5
+    s.close()

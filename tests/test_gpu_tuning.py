@@ -82,7 +82,8 @@ def candidates():
         for mem in (256, 512, 2048):
             yield f"md {items}/{mem}", NWTuning(mdTileItems=items, mdMembers=mem)
     for pb in (128, 256, 512):
-        yield f"pairBlock={pb}", NWTuning(pairBlock=pb)
+        for stage in (0, 2 * pb):
+            yield f"pairBlock={pb} pairStage={stage}", NWTuning(pairBlock=pb, pairStage=stage)
     for items in (4096, 8192, 16384):
         for mem in (768, 2048):
             yield f"srd {items}/{mem}", NWTuning(srdTileItems=items, srdMembers=mem)
