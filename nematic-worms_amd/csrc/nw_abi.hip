@@ -1544,7 +1544,7 @@ static bool batch_srd_collision(State &s, uint32_t c, float4 *P, float4 *V, uint
     BucketScratch sc = q.sort;
     if (offOut) sc.off = offOut;
     NW_HIP_R(launch_srd_hash_batch(P, hash, q.dev, q.R, b.N, NT, shift, s.stream), false);
-    NW_HIP_R(launch_hash_tiles(hash, bp, q.sort, s.stream), false);
+    NW_HIP_R(launch_hash_tiles(hash, bp, q.sort, s.stream, true), false);
     NW_HIP_R(launch_srd_cells_batch(bp, sc, ids, V, com, q.dev, q.R, c, s.stream), false);
     NW_HIP_R(launch_srd_rotate_batch(hash, com, P, V, q.dev, q.R, b.N, NT, dtSrd, q.cosA, q.sinA, q.rotBeads,
                                      s.stream),

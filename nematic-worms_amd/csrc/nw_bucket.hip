@@ -109,6 +109,12 @@ __host__ __device__ inline CellsLayout cells_layout(const BucketPlan &bp, bool s
 }
 
 
+// words of a staged level-1 tile's LDS before its pair stage: the bucket
+// counts (B1), the scan's wave sums, the tile's bucket range (2), aligned to 4
+__host__ __device__ constexpr uint32_t staged_stage_words(uint32_t B1, uint32_t TB) {
+    return (B1 + TB / 64 + 2 + 3) & ~3u;
+}
+
 // ---------------------------------------------------------------------------
 // level 1 as a kernel of its own: the cell of every particle (SRD: from the
 // position, written to hash; otherwise read from hash), tile histogram, P
@@ -117,7 +123,13 @@ __host__ __device__ inline CellsLayout cells_layout(const BucketPlan &bp, bool s
 template <bool FROM_POS>
 __global__ __launch_bounds__(kTileBlock) void k_tiles(const float4 *__restrict__ pos, uint32_t *__restrict__ hash,
                                                       uint32_t *__restrict__ pairs, uint32_t *__restrict__ P,
+                                                      uint32_t *__restrict__ range,
                                                       BucketPlan bp, KP kp) {
+    // the whole row is written: the full bucket range
+    if (range && threadIdx.x == 0) {
+        range[2u * blockIdx.x] = 0u;
+        range[2u * blockIdx.x + 1u] = bp.buckets - 1u;
+    }
     extern __shared__ uint32_t lh[];
     uint32_t *wsum = lh + bp.buckets + 1;
     const uint32_t B1 = bp.buckets + 1;
@@ -160,12 +172,17 @@ __global__ __launch_bounds__(kTileBlock) void k_tiles(const float4 *__restrict__
 template <bool FROM_POS, int ITEMS, int TB = kTileBlock>
 __device__ __forceinline__ void tiles_staged_body(const float4 *__restrict__ pos, uint32_t *__restrict__ hash,
                                                   uint32_t *__restrict__ pairs, uint32_t *__restrict__ P,
-                                                  const BucketPlan &bp, const KP &kp, uint32_t t, uint32_t cellBase,
-                                                  uint32_t *lh) {
+                                                  uint32_t *__restrict__ range, const BucketPlan &bp, const KP &kp,
+                                                  uint32_t t, uint32_t cellBase, uint32_t *lh) {
     const uint32_t B1 = bp.buckets + 1;
     uint32_t *wsum = lh + B1;
-    uint32_t *stage = lh + ((B1 + TB / 64 + 3) & ~3u);
+    uint32_t *brng = wsum + TB / 64;  // the tile's first and last bucket
+    uint32_t *stage = lh + staged_stage_words(B1, TB);
     for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) lh[b] = 0u;
+    if (threadIdx.x == 0) {
+        brng[0] = 0xffffffffu;
+        brng[1] = 0u;
+    }
     __syncthreads();
     const uint32_t i0 = t * bp.tileItems;
     const uint32_t i1 = min(i0 + bp.tileItems, bp.n);
@@ -186,10 +203,37 @@ __device__ __forceinline__ void tiles_staged_body(const float4 *__restrict__ pos
             rk[k] = atomicAdd(&lh[h[k] >> bp.shift], 1u);
         }
     }
+    if (range) {
+        uint32_t lo = 0xffffffffu, hi = 0u;
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            if (i0 + uint32_t(k) * TB + threadIdx.x < i1) {
+                lo = min(lo, h[k] >> bp.shift);
+                hi = max(hi, h[k] >> bp.shift);
+            }
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            lo = min(lo, uint32_t(__shfl_xor(int(lo), d, 64)));
+            hi = max(hi, uint32_t(__shfl_xor(int(hi), d, 64)));
+        }
+        if ((threadIdx.x & 63u) == 0u) {
+            atomicMin(&brng[0], lo);
+            atomicMax(&brng[1], hi);
+        }
+    }
     __syncthreads();
-    block_excl_scan(lh, bp.buckets, wsum);
+    // With a range: the buckets outside [b0, b1] hold no member, so the
+    // prefix is 0 below b0 and the tile's count above b1 -- only the entries
+    // b0 .. b1 + 1 are scanned and written (the cells pass reads no other).
+    const uint32_t b0 = range ? brng[0] : 0u, b1 = range ? brng[1] : bp.buckets - 1u;
+    block_excl_scan(lh + b0, b1 - b0 + 1u, wsum);
     uint32_t *row = P + uint64_t(t) * B1;
-    for (uint32_t b = threadIdx.x; b < B1; b += blockDim.x) row[b] = lh[b];
+    for (uint32_t b = b0 + threadIdx.x; b <= b1 + 1u; b += blockDim.x) row[b] = lh[b];
+    if (range && threadIdx.x == 0) {
+        range[2u * t] = b0;
+        range[2u * t + 1u] = b1;
+    }
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         const uint32_t i = i0 + uint32_t(k) * TB + threadIdx.x;
@@ -204,9 +248,9 @@ template <bool FROM_POS, int ITEMS, int TB = kTileBlock>
 __global__ __launch_bounds__(TB) void k_tiles_staged(const float4 *__restrict__ pos,
                                                              uint32_t *__restrict__ hash,
                                                              uint32_t *__restrict__ pairs, uint32_t *__restrict__ P,
-                                                             BucketPlan bp, KP kp) {
+                                                             uint32_t *__restrict__ range, BucketPlan bp, KP kp) {
     extern __shared__ __align__(16) uint32_t lh[];
-    tiles_staged_body<FROM_POS, ITEMS, TB>(pos, hash, pairs, P, bp, kp, blockIdx.x, 0u, lh);
+    tiles_staged_body<FROM_POS, ITEMS, TB>(pos, hash, pairs, P, range, bp, kp, blockIdx.x, 0u, lh);
 }
 
 // the replica that owns item v of a launch over all replicas (offsets
@@ -246,7 +290,7 @@ __global__ __launch_bounds__(kBatchTileBlock) void k_tiles_batch(const uint32_t 
     const SortRep &r = reps[sort_rep_of(reps, R, blockIdx.x, true)];
     const BucketPlan bp = plan_of(r);
     tiles_staged_body<false, kBatchTileItems / kBatchTileBlock, kBatchTileBlock>(
-        nullptr, const_cast<uint32_t *>(hash) + r.beadOffset, pairs + r.beadOffset, P + r.matOff, bp, KP{},
+        nullptr, const_cast<uint32_t *>(hash) + r.beadOffset, pairs + r.beadOffset, P + r.matOff, nullptr, bp, KP{},
         blockIdx.x - r.tileOff, r.cellOffset, lh);
 }
 
@@ -256,6 +300,8 @@ __global__ __launch_bounds__(kBatchTileBlock) void k_tiles_batch(const uint32_t 
 struct CellsArgs {
     const uint32_t *pairs;
     const uint32_t *P;
+    // each tile's first and last bucket (NULL: the rows of P are whole)
+    const uint32_t *tileRange;
     uint32_t *ids, *off, *cellOf;
     // added to the cells written to cellOf and to the slots written to off
     // (a replica's first cell and first bead; zero for a single system)
@@ -354,23 +400,27 @@ __device__ __forceinline__ void md_gather_staged(const CellsArgs &o, const uint3
 // Bucket b of one system (see CellsArgs for the bases of a replica), BLOCK
 // threads; a bucket of at most CAP members is staged in LDS, each thread
 // holding at most PER = CAP / BLOCK of them.
-// Ascending sort of N values in registers: a bitonic network, every
-// compare-exchange at a compile-time position (branch-free min / max).
+// Ascending sort of N values in registers (N a power of two): Batcher's
+// odd-even merge network (63 compare-exchanges for 16 against the bitonic
+// network's 80), every compare-exchange at a compile-time position
+// (branch-free min / max).
 template <int N>
 __device__ __forceinline__ void sort_network(uint32_t (&v)[N]) {
 #pragma unroll
-    for (int k = 2; k <= N; k <<= 1) {
+    for (int p = 1; p < N; p <<= 1) {
 #pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int k = p; k >= 1; k >>= 1) {
 #pragma unroll
-            for (int i = 0; i < N; ++i) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const uint32_t a = v[i], b = v[l];
-                    const uint32_t lo = min(a, b), hi = max(a, b);
-                    const bool up = (i & k) == 0;
-                    v[i] = up ? lo : hi;
-                    v[l] = up ? hi : lo;
+            for (int j = k % p; j + k < N; j += 2 * k) {
+#pragma unroll
+                for (int i = 0; i < k; ++i) {
+                    const int lo = i + j, hi = i + j + k;
+                    // pairs inside one block of 2p (the merge of two sorted halves)
+                    if (hi < N && lo / (2 * p) == hi / (2 * p)) {
+                        const uint32_t a = v[lo], b = v[hi];
+                        v[lo] = min(a, b);
+                        v[hi] = max(a, b);
+                    }
                 }
             }
         }
@@ -423,13 +473,26 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
     const uint32_t B1 = bp.buckets + 1;
     const uint32_t tid = threadIdx.x, bd = blockDim.x;
 
-    // segments of this bucket in every tile; first sorted slot
+    // segments of this bucket in every tile; first sorted slot.  With tile
+    // ranges, only the tiles whose members reach this bucket have their row
+    // read: a tile wholly below it contributes all its members to the slot,
+    // one wholly above none.
     uint32_t part = 0;
     for (uint32_t t = tid; t < bp.tiles; t += bd) {
-        const uint32_t *row = o.P + uint64_t(t) * B1 + b;
-        const uint32_t lo = row[0], hi = row[1];
+        uint32_t lo = 0u, len = 0u;
+        bool read = true;
+        if (o.tileRange) {
+            const uint32_t r0 = o.tileRange[2u * t], r1 = o.tileRange[2u * t + 1u];
+            read = b >= r0 && b <= r1;
+            if (b > r1) lo = min(bp.tileItems, bp.n - t * bp.tileItems);
+        }
+        if (read) {
+            const uint32_t *row = o.P + uint64_t(t) * B1 + b;
+            lo = row[0];
+            len = row[1] - lo;
+        }
         segBase[t] = lo;
-        segStart[t] = hi - lo;
+        segStart[t] = len;
         part += lo;
     }
     for (int d = 32; d >= 1; d >>= 1) part += __shfl_xor(part, d, 64);
@@ -499,6 +562,21 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
         }
     }
     __syncthreads();
+    // SRD, one system: the cells' rotation signs, one Philox block per four
+    // cells (uniform_at's draws) computed once -- by the first wave for a
+    // bucket of up to 256 cells -- into cur, free after the placement
+    if (SRD && !o.srdReps) {
+        const uint32_t b0 = cell0 >> 2, nb = ((cell0 + nc - 1u) >> 2) - b0 + 1u;
+        for (uint32_t t = tid; t < nb; t += bd) {
+            const u4 r = philox_block(o.seed, o.offset + b0 + t);
+            const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t c = ((b0 + t) << 2) + uint32_t(k);
+                if (c >= cell0 && c < cell0 + nc) cur[c - cell0] = u01(w[k]) > 0.5f ? 1u : 0u;
+            }
+        }
+    }
     // every cell in ascending id order; CSR offsets
     uint32_t *arr = staged ? lid : out;
     for (uint32_t c = tid; c < nc; c += bd) {
@@ -541,6 +619,9 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
     // one thread per cell, sums in sorted order: the arithmetic of
     // isokineticThermostatSRD and cellCenterOfMomentumSRD
     // (particles_kernel_impl.cuh:559-637)
+    // The velocities are read kSumChunk at a time with their loads in flight
+    // together (a cell holds ~6 particles), and summed in sorted order.
+    constexpr uint32_t kSumChunk = 8;
     for (uint32_t c = tid; c < nc; c += bd) {
         const uint32_t a = cnt[c], z = cnt[c + 1];
         f3 mom = mk(0.f, 0.f, 0.f);
@@ -548,32 +629,53 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
         if (a != z) {
             if (o.iso) {
                 float E = 0.0f;
-                for (uint32_t u = a; u < z; ++u) {
-                    const float4 V = o.srdVel[staged ? lid[u] : out[u]];
-                    const f3 v = mk(V);
-                    E += 0.5f * V.w * dot(v, v);
+                for (uint32_t u0 = a; u0 < z; u0 += kSumChunk) {
+                    float4 V[kSumChunk];
+#pragma unroll
+                    for (uint32_t k = 0; k < kSumChunk; ++k) {
+                        const uint32_t u = min(u0 + k, z - 1u);  // past the end: the last one again, unused
+                        V[k] = o.srdVel[staged ? lid[u] : out[u]];
+                    }
+#pragma unroll
+                    for (uint32_t k = 0; k < kSumChunk; ++k) {
+                        if (u0 + k < z) {
+                            const f3 v = mk(V[k]);
+                            E += 0.5f * V[k].w * dot(v, v);
+                        }
+                    }
                 }
                 lambda = sqrtf(o.kbT / E);
             }
             float M = 0.0f;
-            for (uint32_t u = a; u < z; ++u) {
-                float4 V = o.srdVel[staged ? lid[u] : out[u]];
-                if (o.iso) V = f4(lambda * mk(V), V.w);
-                mom += V.w * mk(V);
-                M += V.w;
+            for (uint32_t u0 = a; u0 < z; u0 += kSumChunk) {
+                float4 V[kSumChunk];
+#pragma unroll
+                for (uint32_t k = 0; k < kSumChunk; ++k) {
+                    const uint32_t u = min(u0 + k, z - 1u);
+                    V[k] = o.srdVel[staged ? lid[u] : out[u]];
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < kSumChunk; ++k) {
+                    if (u0 + k < z) {
+                        float4 W = V[k];
+                        if (o.iso) W = f4(lambda * mk(W), W.w);
+                        mom += W.w * mk(W);
+                        M += W.w;
+                    }
+                }
             }
             if (M > 0.0f) mom = mom * (1.0f / M);
         }
         const uint32_t cell = cell0 + c;
-        uint64_t seed = o.seed, offset = o.offset;
-        uint32_t e = cell;
+        bool up;
         if (o.srdReps) {
             const SrdRep &q = o.srdReps[srd_rep_of_cell(o.srdReps, o.srdR, cell)];
-            seed = q.seed;
-            offset = q.rngBase + uint64_t(o.collision) * q.cellBlocks;
-            e = cell - q.cellOffset;
+            const uint64_t offset = q.rngBase + uint64_t(o.collision) * q.cellBlocks;
+            up = uniform_at(q.seed, offset, cell - q.cellOffset) > 0.5f;
+        } else {
+            up = cur[c] != 0u;
         }
-        const float sgn = uniform_at(seed, offset, e) > 0.5f ? 1.0f : -1.0f;
+        const float sgn = up ? 1.0f : -1.0f;
         o.com[cell] = f4(mom, sgn);
         if (o.iso) o.lam[cell] = lambda;
     }
@@ -754,6 +856,7 @@ hipError_t ensure_bucket_scratch(BucketScratch &s, const BucketPlan &bp) {
     }
     if ((e = regrow(s.pairs, s.cap_pairs, uint64_t(bp.tiles) * bp.tileItems, 4)) != hipSuccess) return e;
     if ((e = regrow(s.P, s.cap_mat, uint64_t(bp.tiles) * (bp.buckets + 1), 4)) != hipSuccess) return e;
+    if ((e = regrow(s.tileRange, s.cap_range, 2ull * bp.tiles, 4)) != hipSuccess) return e;
     if (uint64_t(bp.cells) + 2 > s.cap_cells) {
         const uint64_t c = uint64_t(bp.cells) + bp.cells / 8 + 1024;
         if (s.off) (void)hipFree(s.off);
@@ -782,42 +885,45 @@ static hipError_t grant_lds(const void *fn, size_t lds) {
 
 template <bool FROM_POS, int ITEMS, int TB = kTileBlock>
 static hipError_t launch_staged(const float4 *pos, uint32_t *hash, const BucketPlan &bp, BucketScratch &s,
-                                const KP &kp, hipStream_t st) {
+                                const KP &kp, uint32_t *range, hipStream_t st) {
     static_assert(ITEMS * TB == 1024 || ITEMS * TB == 2048 || ITEMS * TB == 4096 || ITEMS * TB == 8192 ||
                       ITEMS * TB == 16384,
                   "tile size");
-    const size_t lds = (((size_t(bp.buckets) + 1 + TB / 64 + 3) & ~size_t(3)) * sizeof(uint32_t)) +
+    const size_t lds = size_t(staged_stage_words(bp.buckets + 1, TB)) * sizeof(uint32_t) +
                        size_t(bp.tileItems) * sizeof(uint32_t);
     const void *fn = reinterpret_cast<const void *>(&k_tiles_staged<FROM_POS, ITEMS, TB>);
     hipError_t e = grant_lds(fn, lds);
     if (e != hipSuccess) return e;
     NW_LAUNCH((k_tiles_staged<FROM_POS, ITEMS, TB>), dim3(bp.tiles), dim3(TB), lds, st, pos, hash, s.pairs, s.P,
-              bp, kp);
+              range, bp, kp);
     return hipGetLastError();
 }
 
 // staged kernel when the tile is 4096 or 16384 particles and its pairs and
 // histogram fit the 160 KiB of LDS; otherwise the two-pass kernel
+// ranges: record every tile's bucket range and write only that part of its
+// P row (the cells pass must then read them: o.tileRange)
 static hipError_t launch_tiles(bool fromPos, const float4 *pos, uint32_t *hash, const BucketPlan &bp,
-                               BucketScratch &s, const KP &kp, hipStream_t st) {
+                               BucketScratch &s, const KP &kp, bool ranges, hipStream_t st) {
     if (bp.n == 0) return hipSuccess;
-    const size_t stagedLds = (size_t(bp.buckets) + 1 + kTileBlock / 64 + 3) * 4 + size_t(bp.tileItems) * 4;
+    uint32_t *range = ranges ? s.tileRange : nullptr;
+    const size_t stagedLds = size_t(staged_stage_words(bp.buckets + 1, kTileBlock)) * 4 + size_t(bp.tileItems) * 4;
     if (stagedLds <= 160 * 1024) {
         // small tiles (more workgroups for a chip of 256 CUs) as 256 threads
-        if (!fromPos && bp.tileItems == 1024) return launch_staged<false, 4, 256>(pos, hash, bp, s, kp, st);
-        if (!fromPos && bp.tileItems == 2048) return launch_staged<false, 8, 256>(pos, hash, bp, s, kp, st);
+        if (!fromPos && bp.tileItems == 1024) return launch_staged<false, 4, 256>(pos, hash, bp, s, kp, range, st);
+        if (!fromPos && bp.tileItems == 2048) return launch_staged<false, 8, 256>(pos, hash, bp, s, kp, range, st);
         if (bp.tileItems == 4 * kTileBlock)
-            return fromPos ? launch_staged<true, 4>(pos, hash, bp, s, kp, st)
-                           : launch_staged<false, 4>(pos, hash, bp, s, kp, st);
+            return fromPos ? launch_staged<true, 4>(pos, hash, bp, s, kp, range, st)
+                           : launch_staged<false, 4>(pos, hash, bp, s, kp, range, st);
         // the SRD tiles of 8192 as 512 threads x 16: three workgroups per CU
         // by LDS and registers against two of 1024 threads (SRD workload:
         // 708 tiles in one round, 35 -> 33 us)
         if (bp.tileItems == 8 * kTileBlock)
-            return fromPos ? launch_staged<true, 16, 512>(pos, hash, bp, s, kp, st)
-                           : launch_staged<false, 8>(pos, hash, bp, s, kp, st);
+            return fromPos ? launch_staged<true, 16, 512>(pos, hash, bp, s, kp, range, st)
+                           : launch_staged<false, 8>(pos, hash, bp, s, kp, range, st);
         if (bp.tileItems == 16 * kTileBlock)
-            return fromPos ? launch_staged<true, 16>(pos, hash, bp, s, kp, st)
-                           : launch_staged<false, 16>(pos, hash, bp, s, kp, st);
+            return fromPos ? launch_staged<true, 16>(pos, hash, bp, s, kp, range, st)
+                           : launch_staged<false, 16>(pos, hash, bp, s, kp, range, st);
     }
     const size_t lds = (size_t(bp.buckets) + 1 + kTileBlock / 64) * sizeof(uint32_t);
     const void *fn = fromPos ? reinterpret_cast<const void *>(&k_tiles<true>)
@@ -825,27 +931,29 @@ static hipError_t launch_tiles(bool fromPos, const float4 *pos, uint32_t *hash, 
     const hipError_t e = grant_lds(fn, lds);
     if (e != hipSuccess) return e;
     if (fromPos)
-        NW_LAUNCH(k_tiles<true>, dim3(bp.tiles), dim3(kTileBlock), lds, st, pos, hash, s.pairs, s.P, bp, kp);
+        NW_LAUNCH(k_tiles<true>, dim3(bp.tiles), dim3(kTileBlock), lds, st, pos, hash, s.pairs, s.P, range,
+                  bp, kp);
     else
-        NW_LAUNCH(k_tiles<false>, dim3(bp.tiles), dim3(kTileBlock), lds, st, pos, hash, s.pairs, s.P, bp,
-                           kp);
+        NW_LAUNCH(k_tiles<false>, dim3(bp.tiles), dim3(kTileBlock), lds, st, pos, hash, s.pairs, s.P, range,
+                  bp, kp);
     return hipGetLastError();
 }
 
 hipError_t launch_srd_tiles(const float4 *pos, uint32_t *hash, const BucketPlan &bp, BucketScratch &s,
                             const KP &kp, hipStream_t st) {
-    return launch_tiles(true, pos, hash, bp, s, kp, st);
+    return launch_tiles(true, pos, hash, bp, s, kp, true, st);
 }
 
-hipError_t launch_hash_tiles(const uint32_t *hash, const BucketPlan &bp, BucketScratch &s, hipStream_t st) {
-    return launch_tiles(false, nullptr, const_cast<uint32_t *>(hash), bp, s, KP{}, st);
+hipError_t launch_hash_tiles(const uint32_t *hash, const BucketPlan &bp, BucketScratch &s, hipStream_t st,
+                             bool ranges) {
+    return launch_tiles(false, nullptr, const_cast<uint32_t *>(hash), bp, s, KP{}, ranges, st);
 }
 
 // ---------------------------------------------------------------------------
 // the batched sort
 // ---------------------------------------------------------------------------
 static size_t batch_tiles_lds(uint32_t maxBuckets) {
-    return ((size_t(maxBuckets) + 1 + kBatchTileBlock / 64 + 3) & ~size_t(3)) * sizeof(uint32_t) +
+    return size_t(staged_stage_words(maxBuckets + 1, kBatchTileBlock)) * sizeof(uint32_t) +
            size_t(kBatchTileItems) * sizeof(uint32_t);
 }
 
@@ -1050,6 +1158,7 @@ hipError_t launch_srd_cells(const BucketPlan &bp, BucketScratch &s, uint32_t *id
     CellsArgs o{};
     o.pairs = s.pairs;
     o.P = s.P;
+    o.tileRange = s.tileRange;
     o.ids = ids;
     o.off = s.off;
     o.cellOf = s.sortedCell;
@@ -1072,6 +1181,7 @@ hipError_t launch_srd_cells_batch(const BucketPlan &bp, BucketScratch &s, uint32
     CellsArgs o{};
     o.pairs = s.pairs;
     o.P = s.P;
+    o.tileRange = s.tileRange;
     o.ids = ids;
     o.off = s.off;
     o.cellOf = s.sortedCell;

@@ -70,11 +70,12 @@ struct BucketScratch {
     uint32_t *hash = nullptr;        // cell per particle, particle order
     uint32_t *pairs = nullptr;       // packed (cell in bucket, index in tile), per tile grouped by bucket
     uint32_t *P = nullptr;           // tiles x (buckets + 1)
+    uint32_t *tileRange = nullptr;   // per tile the first and last bucket of its members (NULL: full rows)
     uint32_t *sortedCell = nullptr;  // cell of each sorted slot
     uint32_t *off = nullptr;         // CSR cell offsets (cells + 1)
     float *lam = nullptr;            // isokinetic factor per SRD cell
     uint32_t *aux = nullptr;         // per particle: the values of a key-value sort
-    uint64_t cap_n = 0, cap_pairs = 0, cap_mat = 0, cap_cells = 0;
+    uint64_t cap_n = 0, cap_pairs = 0, cap_mat = 0, cap_cells = 0, cap_range = 0;
 };
 
 hipError_t ensure_bucket_scratch(BucketScratch &s, const BucketPlan &bp);
@@ -105,7 +106,10 @@ hipError_t launch_md_front(const MdFrontArgs &a, const KP &kp, hipStream_t st);
 // the tangents the front stores, from the positions (worms of 1..64 beads)
 hipError_t launch_filament_tangents(const float4 *pos, float4 *tangent, uint32_t W, const KP &kp, hipStream_t st);
 // level 1 over precomputed cells
-hipError_t launch_hash_tiles(const uint32_t *hash, const BucketPlan &bp, BucketScratch &s, hipStream_t st);
+// ranges: also record every tile's bucket range (a cells pass reading
+// o.tileRange must follow; the SRD cells launchers do)
+hipError_t launch_hash_tiles(const uint32_t *hash, const BucketPlan &bp, BucketScratch &s, hipStream_t st,
+                             bool ranges = false);
 // stable sort of (keys, vals) by key, every key < bound (the per-call
 // sortParticles after calcHash): the two-level bucket sort of the positions,
 // then keys from the slots' cells and vals gathered by position; a key >=
