@@ -97,7 +97,10 @@ def dispatched_kernel(family, w):
     if family == "md_front":
         return f"k_md_front<{tun.get('frontBlock') or 256}>"
     if family == "md_collide":
-        return f"k_collide_rows<1, false, true, {tun.get('pairBlock') or 256}>"
+        # pair_stage: pairBlock + 128 beads per row run, or 2 pairBlock when pairStage asks for more
+        pb = tun.get('pairBlock') or 256
+        stage = 2 * pb if (tun.get('pairStage') or 0) > pb + 128 else pb + 128
+        return f"k_collide_rows<1, false, true, {pb}, {stage}>"
     if family == "md_tiles":
         items = w["md_plan"][2]
         shapes = {1024: "4, 256", 2048: "8, 256", 4096: "4, 1024", 8192: "8, 1024", 16384: "16, 1024"}

@@ -10,8 +10,15 @@
       substring of its kernel name; it reads B bytes and writes B bytes);
       prints markdown and, with --json PATH, writes {kernel: bytes} there
 
-The counters are per dispatch in KiB.  FETCH_SIZE counts L2 fills from the
-fabric (HBM and the MALL), WRITE_SIZE L2 write-backs.
+  python -m nwamd.profsum issue <SQ counter csv> [--cus 256] [--ghz 2.4] [--min-calls 5]
+      instructions per wavefront (SQ_INSTS_VALU / SALU / LDS / VMEM / SMEM
+      over SQ_WAVES) and the time the dispatch's VALU and SALU instructions
+      take to issue at one per cycle per CU (a CU issues at most one VALU
+      wave-instruction per cycle over its four SIMDs, and its one scalar unit
+      one SALU instruction per cycle): the instruction-issue floor of a kernel
+
+The traffic counters are per dispatch in KiB.  FETCH_SIZE counts L2 fills
+from the fabric (HBM and the MALL), WRITE_SIZE L2 write-backs.
 """
 import argparse
 import csv
@@ -88,6 +95,31 @@ def traffic(fetch_csv, write_csv, copy, copy_bytes):
     return "\n".join(lines), res, dict(fetch_scale=f_scale, write_scale=w_scale, copy_kernel=ck)
 
 
+def issue(path, cus, ghz, min_calls):
+    per = defaultdict(lambda: defaultdict(list))
+    for r in csv.DictReader(open(path)):
+        per[kernel_key(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    lines = ["| kernel | calls | waves | VALU / wave | SALU / wave | LDS / wave | VMEM / wave | SMEM / wave | "
+             "VALU issue us | SALU issue us |", "|---|" + "---:|" * 9]
+    rows = []
+    for k, c in per.items():
+        n = len(c.get("SQ_WAVES", []))
+        if n < min_calls or "SQ_INSTS_VALU" not in c:
+            continue
+        m = {name: sum(v) / len(v) for name, v in c.items()}
+        w = m["SQ_WAVES"]
+        if w <= 0:
+            continue
+        rate = cus * ghz * 1e3  # instructions per microsecond over the chip
+        rows.append((m["SQ_INSTS_VALU"] / rate, k, n, w, m))
+    for t, k, n, w, m in sorted(rows, reverse=True):
+        lines.append(f"| `{k}` | {n} | {w:.0f} | {m['SQ_INSTS_VALU'] / w:.0f} | {m['SQ_INSTS_SALU'] / w:.0f} | "
+                     f"{m.get('SQ_INSTS_LDS', 0) / w:.1f} | {m.get('SQ_INSTS_VMEM', 0) / w:.1f} | "
+                     f"{m.get('SQ_INSTS_SMEM', 0) / w:.1f} | {t:.2f} | "
+                     f"{m['SQ_INSTS_SALU'] / (cus * ghz * 1e3):.2f} |")
+    return "\n".join(lines)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m nwamd.profsum")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -102,9 +134,16 @@ def main(argv=None):
     p.add_argument("--copy", required=True)
     p.add_argument("--copy-bytes", type=float, required=True)
     p.add_argument("--json")
+    q = sub.add_parser("issue")
+    q.add_argument("csv")
+    q.add_argument("--cus", type=int, default=256)
+    q.add_argument("--ghz", type=float, default=2.4)
+    q.add_argument("--min-calls", type=int, default=5)
     a = ap.parse_args(argv)
     if a.cmd == "trace":
         print(trace(a.csv, a.last, a.by_grid, a.fold_below))
+    elif a.cmd == "issue":
+        print(issue(a.csv, a.cus, a.ghz, a.min_calls))
     else:
         text, res, cal = traffic(a.fetch_csv, a.write_csv, a.copy, a.copy_bytes)
         print(text)
