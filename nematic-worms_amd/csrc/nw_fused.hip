@@ -41,7 +41,8 @@ __device__ __forceinline__ void bead_cell(const KP &kp, const uint32_t *__restri
 __device__ __forceinline__ FrontLane md_front_lane(const KP &kp, const WaveWorm &m, const float4 *__restrict__ pos,
                                                    const float4 *__restrict__ vel, const float4 *__restrict__ force,
                                                    const float4 *__restrict__ forceOld, float dt, float lgvStd,
-                                                   int langevin, uint64_t seed, uint64_t offRev, uint64_t offLgv) {
+                                                   int langevin, uint64_t seed, uint64_t offRev, uint64_t offLgv,
+                                                   bool wantTan) {
     const SimParams &p = kp.p;
     const uint32_t n = p.filamentSize;
     const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -90,8 +91,8 @@ __device__ __forceinline__ FrontLane md_front_lane(const KP &kp, const WaveWorm 
         acc = mk(0.0f + f.x, 0.0f + f.y, 0.0f + f.z);
     }
 
-    // bonded forces and tangents on the new positions
-    o.writeTan = bonded_lane(kp, r, m.pp, acc, o.tg);
+    // bonded forces and (wantTan) tangents on the new positions
+    o.writeTan = bonded_lane(kp, r, m.pp, acc, o.tg, wantTan);
     o.r = r;
     o.v = v;
     o.acc = acc;
@@ -127,7 +128,8 @@ template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_md_front(MdFrontArgs a, KP kp) {
     const WaveWorm m = wave_worm(kp.p.filamentSize, a.W);
     const FrontLane o = md_front_lane(kp, m, a.posIn ? a.posIn : a.pos, a.velIn ? a.velIn : a.vel, a.force,
-                                      a.forceOld, a.dt, a.lgvStd, a.langevin, a.seed, a.offRev, a.offLgv);
+                                      a.forceOld, a.dt, a.lgvStd, a.langevin, a.seed, a.offRev, a.offLgv,
+                                      a.tangent != nullptr);
     if (m.active)
         a.hash[m.bead] = md_front_store(kp, m, o, a.pos, a.vel, a.force, a.forceOld, a.tangent, a.swapForces != 0);
 }
@@ -753,10 +755,10 @@ __global__ __launch_bounds__(kBlock) void k_md_front_batch(BatchArrays a, const 
     float4 *pos = a.pos + rep.beadOffset, *vel = a.vel + rep.beadOffset;
     float4 *force = a.force + rep.beadOffset, *forceOld = a.forceOld + rep.beadOffset;
     const FrontLane o = md_front_lane(kp, m, pos, vel, force, forceOld, dt, rep.lgvStd, rep.langevin, rep.seed,
-                                      offRev, offLgv);
+                                      offRev, offLgv, a.tangent != nullptr);
     if (m.active) {
-        const uint32_t h =
-            md_front_store(kp, m, o, pos, vel, force, forceOld, a.tangent + rep.beadOffset, a.swapForces != 0);
+        const uint32_t h = md_front_store(kp, m, o, pos, vel, force, forceOld,
+                                          a.tangent ? a.tangent + rep.beadOffset : nullptr, a.swapForces != 0);
         a.hash[rep.beadOffset + m.bead] = rep.cellOffset + h;
     }
 }

@@ -223,7 +223,9 @@ __device__ __forceinline__ WaveWorm wave_worm(uint32_t n, uint32_t W) {
 // other beads arrive by shuffles.  Every lane of the wave must call this.
 // Returns whether the tangent tg is written (the reference writes none for
 // n < 3 and no tail tangent for n == 3).
-__device__ __forceinline__ bool bonded_lane(const KP &kp, f3 r, uint32_t pp, f3 &acc, f3 &tg) {
+// wantTan = false: the tangent is neither computed nor reported (a caller
+// that stores none)
+__device__ __forceinline__ bool bonded_lane(const KP &kp, f3 r, uint32_t pp, f3 &acc, f3 &tg, bool wantTan = true) {
     const SimParams &p = kp.p;
     const uint32_t n = p.filamentSize;
     const f3 rn1 = shfl_down3(r, 1);
@@ -268,7 +270,7 @@ __device__ __forceinline__ bool bonded_lane(const KP &kp, f3 r, uint32_t pp, f3 
             acc += drive;
         }
     }
-    if (n < 3) return false;
+    if (n < 3 || !wantTan) return false;
     if (pp == 0u) tg = tangent_of(p, r, rn1);
     else if (pp + 1u < n) tg = tangent_of(p, rp1, rn1);
     else {
