@@ -108,7 +108,7 @@ __device__ __forceinline__ f3 min_image_pair(const SimParams &p, f3 a, f3 b) {
 
 // unit(b - a) under the minimum image (particles_kernel_impl.cuh:303-309)
 __device__ __forceinline__ f3 tangent_of(const SimParams &p, f3 a, f3 b) {
-    const f3 rel = min_image(p, a, b);
+    const f3 rel = min_image_pair(p, a, b);
     const float d = length(rel);
     return rel / d;
 }

@@ -235,8 +235,10 @@ __device__ __forceinline__ bool bonded_lane(const KP &kp, f3 r, uint32_t pp, f3 
     f3 fi = mk(0.f, 0.f, 0.f), fj = fi, fk = fi;
     const bool hasTriple = n >= 3 && pp + 2u < n;
     if (hasTriple) {
-        const f3 r_ij = min_image(p, r, rn1);
-        const f3 r_jk = min_image(p, rn1, rn2);
+        // min_image_pair: min_image's value, the wrap tests only when some
+        // lane's chain spans half the box
+        const f3 r_ij = min_image_pair(p, r, rn1);
+        const f3 r_jk = min_image_pair(p, rn1, rn2);
         const float d_ij = length(r_ij);
         const float d_jk = length(r_jk);
         const float d_ij_jk = dot(r_ij, r_jk);
@@ -253,7 +255,7 @@ __device__ __forceinline__ bool bonded_lane(const KP &kp, f3 r, uint32_t pp, f3 
     const bool chain = kp.e.bondedMode == NW_BONDED_CHAIN;
     f3 fb = mk(0.f, 0.f, 0.f), drive = fb;
     if (chain && pp + 1u < n) {
-        const f3 d = min_image(p, r, rn1);
+        const f3 d = min_image_pair(p, r, rn1);
         const float len = length(d);
         fb = (p.bondSpringK * (len - p.bondSpringL) / len) * d;
         drive = (kp.e.driveForce / len) * d;
