@@ -456,10 +456,50 @@ __device__ __forceinline__ void sort_cell_ids(uint32_t *arr, uint32_t a, uint32_
     }
 }
 
+// The tile of each of a staged bucket's m members (members are in tile order,
+// tile t's segment at [segStart[t], segStart[t + 1])): a mark at the first
+// slot of every nonempty segment, then a block-wide running maximum over the
+// BLOCK * PER slots, PER consecutive slots per thread.  A segment can hold
+// every member (a tile of particles in cell order), so no thread fills one
+// segment alone.  Entries at m and above are left undefined.
+template <int BLOCK, int PER>
+__device__ __forceinline__ void fill_tile_of(uint16_t *tileOf, const uint32_t *segStart, uint32_t tiles, uint32_t m,
+                                             uint32_t *wsum) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
+    for (uint32_t k = tid; k < m; k += BLOCK) tileOf[k] = 0;
+    __syncthreads();
+    for (uint32_t t = tid; t < tiles; t += BLOCK) {
+        const uint32_t s = segStart[t];
+        if (segStart[t + 1] != s) tileOf[s] = uint16_t(t);
+    }
+    __syncthreads();
+    uint32_t v[PER];
+    uint32_t run = 0;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        run = max(run, uint32_t(tileOf[tid * PER + j]));
+        v[j] = run;
+    }
+    uint32_t incl = run;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(incl, d, 64);
+        if (lane >= uint32_t(d)) incl = max(incl, y);
+    }
+    if (lane == 63u) wsum[wid] = incl;
+    __syncthreads();
+    uint32_t pre = __shfl_up(incl, 1, 64);
+    if (lane == 0u) pre = 0u;
+    for (uint32_t w = 0; w < wid; ++w) pre = max(pre, wsum[w]);
+#pragma unroll
+    for (int j = 0; j < PER; ++j) tileOf[tid * PER + j] = uint16_t(max(v[j], pre));
+    __syncthreads();
+}
+
 template <bool SRD, int BLOCK, int CAP = int(kBucketCap)>
 __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const BucketPlan &bp, uint32_t b,
                                                   unsigned char *smem) {
     constexpr int PER = CAP / BLOCK;
+    static_assert(PER * BLOCK == CAP, "the staged members are PER per thread");
     const CellsLayout L = cells_layout(bp, SRD, CAP);
     uint32_t *wsum = reinterpret_cast<uint32_t *>(smem + L.wsum);
     uint32_t *cnt = reinterpret_cast<uint32_t *>(smem + L.cnt);
@@ -514,9 +554,7 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
     uint2 q[PER];
     if (staged && m) {
         // the tile of every member, one tile's segment per thread at a time
-        for (uint32_t t = tid; t < bp.tiles; t += bd)
-            for (uint32_t k = segStart[t]; k < segStart[t + 1]; ++k) tileOf[k] = uint16_t(t);
-        __syncthreads();
+        fill_tile_of<BLOCK, PER>(tileOf, segStart, bp.tiles, m, wsum);
         // every load of the thread's members in flight before the first use
         // (indices past the last member load the last member again, unused)
 #pragma unroll
@@ -619,25 +657,36 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
     // one thread per cell, sums in sorted order: the arithmetic of
     // isokineticThermostatSRD and cellCenterOfMomentumSRD
     // (particles_kernel_impl.cuh:559-637)
-    // The velocities are read kSumChunk at a time with their loads in flight
-    // together (a cell holds ~6 particles), and summed in sorted order.
-    constexpr uint32_t kSumChunk = 8;
+    // The first kSumHead velocities of a cell are loaded together and kept in
+    // registers from the energy pass to the momentum pass (a cell holds ~6
+    // particles); the rest, kSumTail at a time, are read in each pass.
+    constexpr uint32_t kSumHead = 8, kSumTail = 4;
     for (uint32_t c = tid; c < nc; c += bd) {
         const uint32_t a = cnt[c], z = cnt[c + 1];
         f3 mom = mk(0.f, 0.f, 0.f);
         float lambda = 1.0f;
         if (a != z) {
+            float4 H[kSumHead];
+#pragma unroll
+            for (uint32_t k = 0; k < kSumHead; ++k) {
+                const uint32_t u = min(a + k, z - 1u);  // past the end: the last one again, unused
+                H[k] = o.srdVel[arr[u]];
+            }
             if (o.iso) {
                 float E = 0.0f;
-                for (uint32_t u0 = a; u0 < z; u0 += kSumChunk) {
-                    float4 V[kSumChunk];
 #pragma unroll
-                    for (uint32_t k = 0; k < kSumChunk; ++k) {
-                        const uint32_t u = min(u0 + k, z - 1u);  // past the end: the last one again, unused
-                        V[k] = o.srdVel[staged ? lid[u] : out[u]];
+                for (uint32_t k = 0; k < kSumHead; ++k) {
+                    if (a + k < z) {
+                        const f3 v = mk(H[k]);
+                        E += 0.5f * H[k].w * dot(v, v);
                     }
+                }
+                for (uint32_t u0 = a + kSumHead; u0 < z; u0 += kSumTail) {
+                    float4 V[kSumTail];
 #pragma unroll
-                    for (uint32_t k = 0; k < kSumChunk; ++k) {
+                    for (uint32_t k = 0; k < kSumTail; ++k) V[k] = o.srdVel[arr[min(u0 + k, z - 1u)]];
+#pragma unroll
+                    for (uint32_t k = 0; k < kSumTail; ++k) {
                         if (u0 + k < z) {
                             const f3 v = mk(V[k]);
                             E += 0.5f * V[k].w * dot(v, v);
@@ -647,15 +696,21 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
                 lambda = sqrtf(o.kbT / E);
             }
             float M = 0.0f;
-            for (uint32_t u0 = a; u0 < z; u0 += kSumChunk) {
-                float4 V[kSumChunk];
 #pragma unroll
-                for (uint32_t k = 0; k < kSumChunk; ++k) {
-                    const uint32_t u = min(u0 + k, z - 1u);
-                    V[k] = o.srdVel[staged ? lid[u] : out[u]];
+            for (uint32_t k = 0; k < kSumHead; ++k) {
+                if (a + k < z) {
+                    float4 W = H[k];
+                    if (o.iso) W = f4(lambda * mk(W), W.w);
+                    mom += W.w * mk(W);
+                    M += W.w;
                 }
+            }
+            for (uint32_t u0 = a + kSumHead; u0 < z; u0 += kSumTail) {
+                float4 V[kSumTail];
 #pragma unroll
-                for (uint32_t k = 0; k < kSumChunk; ++k) {
+                for (uint32_t k = 0; k < kSumTail; ++k) V[k] = o.srdVel[arr[min(u0 + k, z - 1u)]];
+#pragma unroll
+                for (uint32_t k = 0; k < kSumTail; ++k) {
                     if (u0 + k < z) {
                         float4 W = V[k];
                         if (o.iso) W = f4(lambda * mk(W), W.w);
