@@ -456,6 +456,79 @@ __device__ __forceinline__ void sort_cell_ids(uint32_t *arr, uint32_t a, uint32_
     }
 }
 
+// The mean velocity of the ids[a .. z) of one SRD cell (a < z), summed in
+// that order, and with the isokinetic thermostat on its factor lambda, which
+// scales the velocities before the mean (isokineticThermostatSRD and
+// cellCenterOfMomentumSRD, particles_kernel_impl.cuh:559-637).  The first
+// kSumHead ids are read, then their velocities loaded together and kept in
+// registers for both sums (a cell holds ~6 particles); the rest are read
+// kSumTail at a time in each sum.  Slots past z load the last particle
+// again, unused.
+__device__ __forceinline__ void srd_cell_sums(const CellsArgs &o, const uint32_t *ids, uint32_t a, uint32_t z,
+                                              f3 &mom, float &lambda) {
+    constexpr uint32_t kSumHead = 8, kSumTail = 4;
+    uint32_t I[kSumHead];
+#pragma unroll
+    for (uint32_t k = 0; k < kSumHead; ++k) I[k] = ids[min(a + k, z - 1u)];
+    float4 H[kSumHead];
+#pragma unroll
+    for (uint32_t k = 0; k < kSumHead; ++k) H[k] = o.srdVel[I[k]];
+    if (o.iso) {
+        float E = 0.0f;
+#pragma unroll
+        for (uint32_t k = 0; k < kSumHead; ++k) {
+            if (a + k < z) {
+                const f3 v = mk(H[k]);
+                E += 0.5f * H[k].w * dot(v, v);
+            }
+        }
+        for (uint32_t u0 = a + kSumHead; u0 < z; u0 += kSumTail) {
+            uint32_t J[kSumTail];
+#pragma unroll
+            for (uint32_t k = 0; k < kSumTail; ++k) J[k] = ids[min(u0 + k, z - 1u)];
+            float4 V[kSumTail];
+#pragma unroll
+            for (uint32_t k = 0; k < kSumTail; ++k) V[k] = o.srdVel[J[k]];
+#pragma unroll
+            for (uint32_t k = 0; k < kSumTail; ++k) {
+                if (u0 + k < z) {
+                    const f3 v = mk(V[k]);
+                    E += 0.5f * V[k].w * dot(v, v);
+                }
+            }
+        }
+        lambda = sqrtf(o.kbT / E);
+    }
+    float M = 0.0f;
+#pragma unroll
+    for (uint32_t k = 0; k < kSumHead; ++k) {
+        if (a + k < z) {
+            float4 W = H[k];
+            if (o.iso) W = f4(lambda * mk(W), W.w);
+            mom += W.w * mk(W);
+            M += W.w;
+        }
+    }
+    for (uint32_t u0 = a + kSumHead; u0 < z; u0 += kSumTail) {
+        uint32_t J[kSumTail];
+#pragma unroll
+        for (uint32_t k = 0; k < kSumTail; ++k) J[k] = ids[min(u0 + k, z - 1u)];
+        float4 V[kSumTail];
+#pragma unroll
+        for (uint32_t k = 0; k < kSumTail; ++k) V[k] = o.srdVel[J[k]];
+#pragma unroll
+        for (uint32_t k = 0; k < kSumTail; ++k) {
+            if (u0 + k < z) {
+                float4 W = V[k];
+                if (o.iso) W = f4(lambda * mk(W), W.w);
+                mom += W.w * mk(W);
+                M += W.w;
+            }
+        }
+    }
+    if (M > 0.0f) mom = mom * (1.0f / M);
+}
+
 // The tile of each of a staged bucket's m members (members are in tile order,
 // tile t's segment at [segStart[t], segStart[t + 1])): a mark at the first
 // slot of every nonempty segment, then a block-wide running maximum over the
@@ -615,13 +688,14 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
             }
         }
     }
-    // every cell in ascending id order; CSR offsets
-    uint32_t *arr = staged ? lid : out;
+    // every cell in ascending id order; CSR offsets.  SRD cells hold ~6
+    // particles (a network of 16 takes all but the rare crowded ones); MD
+    // cells hold ~1 bead.  The two storages take separate calls so that the
+    // staged ids are read as LDS, not through a generic pointer.
     for (uint32_t c = tid; c < nc; c += bd) {
         const uint32_t a = cnt[c], z = cnt[c + 1];
-        // SRD cells hold ~6 particles (a network of 16 takes all but the
-        // rare crowded ones); MD cells hold ~1 bead
-        sort_cell_ids<SRD ? 16 : 1>(arr, a, z);
+        if (staged) sort_cell_ids<SRD ? 16 : 1>(lid, a, z);
+        else sort_cell_ids<SRD ? 16 : 1>(out, a, z);
         o.off[cell0 + c] = o.slotBase + sb + a;
     }
     if (tid == 0 && cell0 + nc == bp.cells) o.off[bp.cells] = o.slotBase + bp.n;
@@ -637,7 +711,7 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
             return;
         }
         for (uint32_t k = tid; k < m; k += bd) {
-            const uint32_t id = arr[k];
+            const uint32_t id = out[k];
             o.cellOf[sb + k] = o.cellBase + cell0 + seg_of(cnt, nc, k);
             if (o.sPos) {
                 o.sPos[sb + k] = o.pos[id];
@@ -648,78 +722,24 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
         return;
     }
 
-    // SRD: the sorted ids out (the cell sums below read the velocities from
-    // memory by id: with the solvent in cell order those reads are near
-    // sequential, and without an LDS stage of them more buckets fit per CU)
+    // SRD: the sorted ids out.  The cell sums below read the velocities from
+    // memory by id (with the solvent in cell order those reads are near
+    // sequential, and without an LDS stage of them more buckets fit per CU).
+    // No barrier after the store: the sums read the staged ids from LDS, and
+    // an unstaged cell's ids from memory only by the thread that sorted them.
     if (staged && m)
         for (uint32_t k = tid; k < m; k += bd) out[k] = lid[k];
-    __syncthreads();
     // one thread per cell, sums in sorted order: the arithmetic of
     // isokineticThermostatSRD and cellCenterOfMomentumSRD
     // (particles_kernel_impl.cuh:559-637)
-    // The first kSumHead velocities of a cell are loaded together and kept in
-    // registers from the energy pass to the momentum pass (a cell holds ~6
-    // particles); the rest, kSumTail at a time, are read in each pass.
-    constexpr uint32_t kSumHead = 8, kSumTail = 4;
     for (uint32_t c = tid; c < nc; c += bd) {
         const uint32_t a = cnt[c], z = cnt[c + 1];
         f3 mom = mk(0.f, 0.f, 0.f);
         float lambda = 1.0f;
         if (a != z) {
-            float4 H[kSumHead];
-#pragma unroll
-            for (uint32_t k = 0; k < kSumHead; ++k) {
-                const uint32_t u = min(a + k, z - 1u);  // past the end: the last one again, unused
-                H[k] = o.srdVel[arr[u]];
-            }
-            if (o.iso) {
-                float E = 0.0f;
-#pragma unroll
-                for (uint32_t k = 0; k < kSumHead; ++k) {
-                    if (a + k < z) {
-                        const f3 v = mk(H[k]);
-                        E += 0.5f * H[k].w * dot(v, v);
-                    }
-                }
-                for (uint32_t u0 = a + kSumHead; u0 < z; u0 += kSumTail) {
-                    float4 V[kSumTail];
-#pragma unroll
-                    for (uint32_t k = 0; k < kSumTail; ++k) V[k] = o.srdVel[arr[min(u0 + k, z - 1u)]];
-#pragma unroll
-                    for (uint32_t k = 0; k < kSumTail; ++k) {
-                        if (u0 + k < z) {
-                            const f3 v = mk(V[k]);
-                            E += 0.5f * V[k].w * dot(v, v);
-                        }
-                    }
-                }
-                lambda = sqrtf(o.kbT / E);
-            }
-            float M = 0.0f;
-#pragma unroll
-            for (uint32_t k = 0; k < kSumHead; ++k) {
-                if (a + k < z) {
-                    float4 W = H[k];
-                    if (o.iso) W = f4(lambda * mk(W), W.w);
-                    mom += W.w * mk(W);
-                    M += W.w;
-                }
-            }
-            for (uint32_t u0 = a + kSumHead; u0 < z; u0 += kSumTail) {
-                float4 V[kSumTail];
-#pragma unroll
-                for (uint32_t k = 0; k < kSumTail; ++k) V[k] = o.srdVel[arr[min(u0 + k, z - 1u)]];
-#pragma unroll
-                for (uint32_t k = 0; k < kSumTail; ++k) {
-                    if (u0 + k < z) {
-                        float4 W = V[k];
-                        if (o.iso) W = f4(lambda * mk(W), W.w);
-                        mom += W.w * mk(W);
-                        M += W.w;
-                    }
-                }
-            }
-            if (M > 0.0f) mom = mom * (1.0f / M);
+            // one instance for both storages, through a generic pointer (two
+            // instances specialised to LDS and global memory measured slower)
+            srd_cell_sums(o, staged ? lid : out, a, z, mom, lambda);
         }
         const uint32_t cell = cell0 + c;
         bool up;
