@@ -212,11 +212,10 @@ __device__ __forceinline__ void tiles_staged_body(const float4 *__restrict__ pos
                 hi = max(hi, h[k] >> bp.shift);
             }
         }
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            lo = min(lo, uint32_t(__shfl_xor(int(lo), d, 64)));
-            hi = max(hi, uint32_t(__shfl_xor(int(hi), d, 64)));
-        }
+        // the wave's least and greatest (the least as the complement of the
+        // greatest complement)
+        lo = ~wave_last_lane(wave_scan_max(~lo));
+        hi = wave_last_lane(wave_scan_max(hi));
         if ((threadIdx.x & 63u) == 0u) {
             atomicMin(&brng[0], lo);
             atomicMax(&brng[1], hi);
@@ -553,11 +552,7 @@ __device__ __forceinline__ void fill_tile_of(uint16_t *tileOf, const uint32_t *s
         run = max(run, uint32_t(tileOf[tid * PER + j]));
         v[j] = run;
     }
-    uint32_t incl = run;
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(incl, d, 64);
-        if (lane >= uint32_t(d)) incl = max(incl, y);
-    }
+    const uint32_t incl = wave_scan_max(run);
     if (lane == 63u) wsum[wid] = incl;
     __syncthreads();
     uint32_t pre = __shfl_up(incl, 1, 64);
@@ -608,8 +603,8 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
         segStart[t] = len;
         part += lo;
     }
-    for (int d = 32; d >= 1; d >>= 1) part += __shfl_xor(part, d, 64);
-    if ((tid & 63u) == 0u) wsum[BLOCK / 64 + (tid >> 6)] = part;  // upper half of wsum
+    part = wave_scan_add(part);
+    if ((tid & 63u) == 63u) wsum[BLOCK / 64 + (tid >> 6)] = part;  // upper half of wsum
     __syncthreads();
     uint32_t sb = 0;
     for (uint32_t w = 0; w < BLOCK / 64; ++w) sb += wsum[BLOCK / 64 + w];

@@ -276,11 +276,7 @@ __device__ __forceinline__ void block_excl_scan(uint32_t *a, uint32_t m, uint32_
     for (uint32_t k = 0; k < per; ++k)
         if (b0 + k < m) s += a[b0 + k];
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
-    uint32_t incl = s;
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t t = __shfl_up(incl, d, 64);
-        if (lane >= uint32_t(d)) incl += t;
-    }
+    const uint32_t incl = wave_scan_add(s);
     if (lane == 63u) wsum[wid] = incl;
     __syncthreads();
     uint32_t run = incl - s;

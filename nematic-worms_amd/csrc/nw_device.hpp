@@ -55,6 +55,36 @@ __device__ __forceinline__ uint32_t div_exact(uint32_t x, uint32_t m, uint32_t d
     return m == 0u ? x : q - (q * d > x ? 1u : 0u);
 }
 
+// ---------------------------------------------------------------------------
+// Wave-wide scans over the 64 lanes of a wavefront, every lane active, in
+// DPP lane moves (no LDS round trip): Hillis-Steele steps of 1, 2, 4 and 8
+// lanes inside each row of 16 (row_shr), then the row totals carried forward
+// by the broadcasts of lane 15 into rows 1 and 3 (row_bcast:15) and of lane
+// 31 into rows 2 and 3 (row_bcast:31).  A lane with no source lane, or in a
+// row the broadcast skips, reads 0: the identity of + and of max on uint32.
+// ---------------------------------------------------------------------------
+template <bool MAX>
+__device__ __forceinline__ uint32_t dpp_step(uint32_t v, uint32_t w) {
+    return MAX ? (v > w ? v : w) : v + w;
+}
+template <bool MAX>
+__device__ __forceinline__ uint32_t wave_incl_scan_dpp(uint32_t v) {
+    v = dpp_step<MAX>(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x111, 0xf, 0xf, false)));  // row_shr:1
+    v = dpp_step<MAX>(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x112, 0xf, 0xf, false)));  // row_shr:2
+    v = dpp_step<MAX>(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x114, 0xf, 0xf, false)));  // row_shr:4
+    v = dpp_step<MAX>(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x118, 0xf, 0xf, false)));  // row_shr:8
+    v = dpp_step<MAX>(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x142, 0xa, 0xf, false)));  // row_bcast:15
+    v = dpp_step<MAX>(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x143, 0xc, 0xf, false)));  // row_bcast:31
+    return v;
+}
+// inclusive prefix sum / running maximum of v over the lanes
+__device__ __forceinline__ uint32_t wave_scan_add(uint32_t v) { return wave_incl_scan_dpp<false>(v); }
+__device__ __forceinline__ uint32_t wave_scan_max(uint32_t v) { return wave_incl_scan_dpp<true>(v); }
+// lane 63's value (of an inclusive scan: the wave's total), wave-uniform
+__device__ __forceinline__ uint32_t wave_last_lane(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_readlane(int(v), 63));
+}
+
 // coordinates of cell c of the MD grid
 __device__ __forceinline__ void cell_xyz(const KP &kp, uint32_t c, int &gx, int &gy, int &gz) {
     const uint32_t Gx = kp.p.gridSize.x, layer = kp.p.gridSize.x * kp.p.gridSize.y;

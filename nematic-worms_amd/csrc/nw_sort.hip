@@ -18,15 +18,7 @@ namespace nw {
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;  // 2048
 
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-    const uint32_t lane = threadIdx.x & 63u;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t t = __shfl_up(v, d, 64);
-        if (lane >= uint32_t(d)) v += t;
-    }
-    return v;
-}
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) { return wave_scan_add(v); }
 
 // block-wide exclusive scan of one value per thread; returns the block total
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t &total) {
