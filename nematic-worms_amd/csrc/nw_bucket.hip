@@ -390,7 +390,8 @@ __device__ __forceinline__ void md_gather_group(const CellsArgs &o, const uint32
 template <bool POS, bool VEL, bool TAN, int BLOCK, int CAP>
 __device__ __forceinline__ void md_gather_staged(const CellsArgs &o, const uint32_t *lid, const uint16_t *lcell,
                                                  uint32_t sb, uint32_t cell0, uint32_t m) {
-    constexpr int PER = CAP / BLOCK, G = PER < 4 ? PER : 4;
+    // groups of four members, two when all three records move (register room)
+    constexpr int PER = CAP / BLOCK, GM = VEL && TAN ? 2 : 4, G = PER < GM ? PER : GM;
 #pragma unroll
     for (int g = 0; g < PER; g += G)
         if (m > uint32_t(g) * BLOCK) md_gather_group<POS, VEL, TAN, BLOCK, G>(o, lid, lcell, sb, cell0, m, g);
@@ -528,23 +529,16 @@ __device__ __forceinline__ void srd_cell_sums(const CellsArgs &o, const uint32_t
     if (M > 0.0f) mom = mom * (1.0f / M);
 }
 
-// The tile of each of a staged bucket's m members (members are in tile order,
-// tile t's segment at [segStart[t], segStart[t + 1])): a mark at the first
-// slot of every nonempty segment, then a block-wide running maximum over the
-// BLOCK * PER slots, PER consecutive slots per thread.  A segment can hold
-// every member (a tile of particles in cell order), so no thread fills one
-// segment alone.  Entries at m and above are left undefined.
+// The tile of each of a staged bucket's members (members are in tile order):
+// tileOf holds, cleared elsewhere, the tile index at the first slot of every
+// nonempty segment (written by the segment scan); a block-wide running
+// maximum over the BLOCK * PER slots, PER consecutive slots per thread, fills
+// the rest.  A segment can hold every member (a tile of particles in cell
+// order), so no thread fills one segment alone.  Entries at the member count
+// and above are left undefined.
 template <int BLOCK, int PER>
-__device__ __forceinline__ void fill_tile_of(uint16_t *tileOf, const uint32_t *segStart, uint32_t tiles, uint32_t m,
-                                             uint32_t *wsum) {
+__device__ __forceinline__ void fill_tile_of(uint16_t *tileOf, uint32_t *wsum) {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
-    for (uint32_t k = tid; k < m; k += BLOCK) tileOf[k] = 0;
-    __syncthreads();
-    for (uint32_t t = tid; t < tiles; t += BLOCK) {
-        const uint32_t s = segStart[t];
-        if (segStart[t + 1] != s) tileOf[s] = uint16_t(t);
-    }
-    __syncthreads();
     uint32_t v[PER];
     uint32_t run = 0;
 #pragma unroll
@@ -580,6 +574,12 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
 
     const uint32_t B1 = bp.buckets + 1;
     const uint32_t tid = threadIdx.x, bd = blockDim.x;
+    const uint32_t cell0 = b << bp.shift;
+    const uint32_t nc = min(1u << bp.shift, bp.cells - cell0);
+    // cleared here, under the barriers of the segment pass: the cell counts
+    // and the member tile marks (tileOf, CAP entries over lid)
+    for (uint32_t c = tid; c < nc; c += bd) cnt[c] = 0u;
+    for (uint32_t k = tid; k < uint32_t(CAP) / 2u; k += bd) reinterpret_cast<uint32_t *>(tileOf)[k] = 0u;
 
     // segments of this bucket in every tile; first sorted slot.  With tile
     // ranges, only the tiles whose members reach this bucket have their row
@@ -608,12 +608,8 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
     __syncthreads();
     uint32_t sb = 0;
     for (uint32_t w = 0; w < BLOCK / 64; ++w) sb += wsum[BLOCK / 64 + w];
-    block_excl_scan(segStart, bp.tiles, wsum);
+    block_excl_scan<false, true>(segStart, bp.tiles, wsum, nullptr, tileOf, uint32_t(CAP));
     const uint32_t m = segStart[bp.tiles];
-    const uint32_t cell0 = b << bp.shift;
-    const uint32_t nc = min(1u << bp.shift, bp.cells - cell0);
-    for (uint32_t c = tid; c < nc; c += bd) cnt[c] = 0u;
-    __syncthreads();
 
     const bool staged = m <= uint32_t(CAP);
     const uint64_t T = bp.tileItems;
@@ -621,8 +617,8 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
     // threads from the count to the placement (member k = tid + j * BLOCK)
     uint2 q[PER];
     if (staged && m) {
-        // the tile of every member, one tile's segment per thread at a time
-        fill_tile_of<BLOCK, PER>(tileOf, segStart, bp.tiles, m, wsum);
+        // the tile of every member
+        fill_tile_of<BLOCK, PER>(tileOf, wsum);
         // every load of the thread's members in flight before the first use
         // (indices past the last member load the last member again, unused)
 #pragma unroll
@@ -643,9 +639,7 @@ __device__ __forceinline__ void bucket_cells_body(const CellsArgs &o, const Buck
         }
     }
     __syncthreads();
-    block_excl_scan(cnt, nc, wsum);
-    for (uint32_t c = tid; c < nc; c += bd) cur[c] = cnt[c];
-    __syncthreads();
+    block_excl_scan<true>(cnt, nc, wsum, cur);
 
     uint32_t *out = o.ids + sb;
     if (staged) {

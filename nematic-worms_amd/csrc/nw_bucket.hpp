@@ -267,8 +267,12 @@ hipError_t launch_collide_batch(const BatchArrays &a, const BatchRep *reps, uint
 // block-level helpers shared by the level-1 kernels (whole workgroup calls)
 // ---------------------------------------------------------------------------
 // in-place exclusive scan of a[0..m) over the workgroup, a[m] = total;
-// wsum holds blockDim/64 words; ends with a barrier
-__device__ __forceinline__ void block_excl_scan(uint32_t *a, uint32_t m, uint32_t *wsum) {
+// wsum holds blockDim/64 words; ends with a barrier.  COPY: copy[i] also
+// receives the prefix of i.  MARK: marks[prefix] = i for every nonzero a[i]
+// whose prefix is below markCap (the first slot of each nonempty segment).
+template <bool COPY = false, bool MARK = false>
+__device__ __forceinline__ void block_excl_scan(uint32_t *a, uint32_t m, uint32_t *wsum, uint32_t *copy = nullptr,
+                                                uint16_t *marks = nullptr, uint32_t markCap = 0u) {
     const uint32_t T = blockDim.x;
     const uint32_t per = (m + T - 1) / T;
     const uint32_t b0 = threadIdx.x * per;
@@ -285,6 +289,8 @@ __device__ __forceinline__ void block_excl_scan(uint32_t *a, uint32_t m, uint32_
         if (b0 + k < m) {
             const uint32_t v = a[b0 + k];
             a[b0 + k] = run;
+            if (COPY) copy[b0 + k] = run;
+            if (MARK && v != 0u && run < markCap) marks[run] = uint16_t(b0 + k);
             run += v;
         }
     }
