@@ -208,13 +208,18 @@ __device__ __forceinline__ uint32_t srd_hash_at(const SimParams &p, const Solven
     const f3 r = position_relative_to_box(p, shifted);
     int gx = floor_div(r.x, s.cellSize.x);
     int gy = floor_div(r.y, s.cellSize.y);
-    int gz = floor_div(r.z, s.cellSize.z);
     if (p.boundaryX == PERIODIC) gx = wrap_mod(gx, s.gridSize.x);
     else gx = min(max(gx, 0), int(s.gridSize.x) - 1);
     if (p.boundaryY == PERIODIC) gy = wrap_mod(gy, s.gridSize.y);
     else gy = min(max(gy, 0), int(s.gridSize.y) - 1);
-    if (p.boundaryZ == PERIODIC) gz = wrap_mod(gz, s.gridSize.z);
-    else gz = min(max(gz, 0), int(s.gridSize.z) - 1);
+    // a grid of one layer puts every z in layer 0 (the wrap and the clamp
+    // both give 0 for any coordinate), so its z is not computed
+    int gz = 0;
+    if (s.gridSize.z > 1u) {
+        gz = floor_div(r.z, s.cellSize.z);
+        if (p.boundaryZ == PERIODIC) gz = wrap_mod(gz, s.gridSize.z);
+        else gz = min(max(gz, 0), int(s.gridSize.z) - 1);
+    }
     return (uint32_t(gz) * s.gridSize.y) * s.gridSize.x + uint32_t(gy) * s.gridSize.x + uint32_t(gx);
 }
 

@@ -120,8 +120,9 @@ __device__ __forceinline__ uint32_t md_front_store(const KP &kp, const WaveWorm 
         forceOld[m.bead] = f4(o.fold, o.fow);
     }
     if (o.writeTan && tangent) tangent[m.bead] = f4(o.tg, 0.0f);
+    // one layer: the wrap puts every z in layer 0, so z is not computed
     return md_hash_wrapped(p, cell_coord(o.r.x, p.origin.x, p.cellSize.x), cell_coord(o.r.y, p.origin.y, p.cellSize.y),
-                           cell_coord(o.r.z, p.origin.z, p.cellSize.z));
+                           p.gridSize.z > 1u ? cell_coord(o.r.z, p.origin.z, p.cellSize.z) : 0);
 }
 
 template <int BLOCK>
