@@ -135,6 +135,39 @@ def test_sort_bounded_bit_exact(nwlib, n, bound):
     assert nwlib.nwGetDeviceFlags(1) & 1 == 0
 
 
+@pytest.mark.parametrize("order", ["sorted", "nearly_sorted", "reversed"])
+def test_sort_particles_cell_ordered_input_bit_exact(nwlib, order):
+    """calcHash + sortParticles (the two-level bucket sort) on particles
+    already (or nearly) in cell order, as the solvent kept in cell order
+    hands them over: a first-level tile then holds every member of the
+    buckets it reaches in one segment (the case the second level's member
+    table is filled for in parallel); reversed, the segments come in the
+    opposite tile order."""
+    rng = np.random.default_rng(11)
+    p = sim_params(grid=(256, 256, 1), cell=1.0, W=25_000, n=12)
+    n = p.numParticles
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, 0] = rng.uniform(p.origin.x, p.origin.x + p.boxSize.x, n)
+    pos[:, 1] = rng.uniform(p.origin.y, p.origin.y + p.boxSize.y, n)
+    pos[:, 2] = p.origin.z + 0.5 * p.boxSize.z
+    h0, _ = orc.calc_hash(p, None, pos)
+    idx = np.argsort(h0, kind="stable")
+    pos = np.ascontiguousarray(pos[idx[::-1] if order == "reversed" else idx])
+    if order == "nearly_sorted":
+        k = rng.choice(n, n // 100, replace=False)
+        pos[k, 0] = rng.uniform(p.origin.x, p.origin.x + p.boxSize.x, k.size)
+    oh, oi = orc.calc_hash(p, None, pos)
+    if order == "sorted":
+        assert np.all(np.diff(oh.astype(np.int64)) >= 0)
+    set_params(nwlib, p)
+    dpos = dev(nwlib, pos)
+    dh, di = DeviceArray(nwlib, 4 * n), DeviceArray(nwlib, 4 * n)
+    nwlib.calcHash(dh.p, di.p, dpos.p, n, False)
+    nwlib.sortParticles(dh.p, di.p, n)
+    ok, ov = orc.sort_pairs(oh, oi)
+    assert np.array_equal(downu(dh, n), ok) and np.array_equal(downu(di, n), ov)
+
+
 def test_sort_keys_above_bound_still_sorted(nwlib):
     rng = np.random.default_rng(5)
     n = 5000
