@@ -1018,8 +1018,19 @@ void nwBatchSetReplicas(const NWReplicaParams *reps, uint numReplicas) {
     if (b.dev) (void)hipFree(b.dev);
     b.dev = nullptr;
     if (numReplicas) {
-        NW_HIP(hipMalloc(&b.dev, sizeof(BatchRep) * numReplicas));
-        NW_HIP(hipMemcpy(b.dev, b.reps.data(), sizeof(BatchRep) * numReplicas, hipMemcpyHostToDevice));
+        // the records, then their wavefront and workgroup offsets packed (the
+        // kernels' replica lookups bisect those)
+        static_assert(sizeof(BatchRep) % alignof(uint32_t) == 0, "offsets follow the records");
+        const size_t recBytes = sizeof(BatchRep) * numReplicas;
+        std::vector<unsigned char> table(recBytes + 2 * sizeof(uint32_t) * numReplicas);
+        memcpy(table.data(), b.reps.data(), recBytes);
+        uint32_t *off = reinterpret_cast<uint32_t *>(table.data() + recBytes);
+        for (uint r = 0; r < numReplicas; ++r) {
+            off[r] = b.reps[r].waveOffset;
+            off[numReplicas + r] = b.reps[r].blockOffset;
+        }
+        NW_HIP(hipMalloc(&b.dev, table.size()));
+        NW_HIP(hipMemcpy(b.dev, table.data(), table.size(), hipMemcpyHostToDevice));
     }
     if (!b.stepCtr) NW_HIP(hipMalloc(&b.stepCtr, sizeof(uint64_t)));
     NW_HIP(hipMemset(b.stepCtr, 0, sizeof(uint64_t)));

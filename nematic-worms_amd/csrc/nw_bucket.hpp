@@ -165,26 +165,24 @@ struct BatchRep {
     uint32_t numBeads;     // numFilaments x filamentSize
 };
 
-// the replica of a front wavefront / pair workgroup of a batched launch
-// (offsets ascending; bisection over the descriptor table)
-__device__ __forceinline__ uint32_t replica_of_wave(const BatchRep *__restrict__ reps, uint32_t R, uint32_t wave) {
+// the replica of a front wavefront / pair workgroup of a batched launch.
+// The device table holds the R records, then their wavefront offsets, then
+// their workgroup offsets, packed (ascending): the bisection reads 4 bytes per
+// replica that stay in the scalar cache rather than one record's line per step.
+__device__ __forceinline__ uint32_t replica_of_offset(const uint32_t *__restrict__ off, uint32_t R, uint32_t v) {
     uint32_t lo = 0, hi = R;
     while (hi - lo > 1u) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (reps[mid].waveOffset <= wave) lo = mid;
+        if (off[mid] <= v) lo = mid;
         else hi = mid;
     }
     return lo;
 }
-
+__device__ __forceinline__ uint32_t replica_of_wave(const BatchRep *__restrict__ reps, uint32_t R, uint32_t wave) {
+    return replica_of_offset(reinterpret_cast<const uint32_t *>(reps + R), R, wave);
+}
 __device__ __forceinline__ uint32_t replica_of_block(const BatchRep *__restrict__ reps, uint32_t R, uint32_t blk) {
-    uint32_t lo = 0, hi = R;
-    while (hi - lo > 1u) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (reps[mid].blockOffset <= blk) lo = mid;
-        else hi = mid;
-    }
-    return lo;
+    return replica_of_offset(reinterpret_cast<const uint32_t *>(reps + R) + R, R, blk);
 }
 
 // The batched sort: every replica sorted against its own plan (tiles of
