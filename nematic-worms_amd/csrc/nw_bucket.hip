@@ -187,6 +187,7 @@ __device__ __forceinline__ void tiles_staged_body(const float4 *__restrict__ pos
     const uint32_t i0 = t * bp.tileItems;
     const uint32_t i1 = min(i0 + bp.tileItems, bp.n);
     uint32_t h[ITEMS], rk[ITEMS];
+    const SrdGrid g = srd_grid(kp.s, mk(kp.s.randOffset.x, kp.s.randOffset.y, kp.s.randOffset.z));
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         const uint32_t i = i0 + uint32_t(k) * TB + threadIdx.x;
@@ -194,7 +195,7 @@ __device__ __forceinline__ void tiles_staged_body(const float4 *__restrict__ pos
         rk[k] = 0u;
         if (i < i1) {
             if (FROM_POS) {
-                h[k] = srd_hash(kp.p, kp.s, mk(pos[i]));
+                h[k] = srd_hash_g(kp.p, kp.s, g, mk(pos[i]));
                 hash[i] = h[k];
             } else {
                 // clamped: a key outside the grid sorts as its last cell

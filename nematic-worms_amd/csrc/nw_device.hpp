@@ -158,17 +158,20 @@ __device__ __forceinline__ int wrap_mod(int g, uint32_t G) {
     return m < 0 ? m + Gi : m;
 }
 
-// floor(d / c) of the correctly rounded quotient.  d * rcp(c) is within
-// 2^-22 |t| of d / c (hardware reciprocal 1 ulp, one rounding each for the
-// product and the quotient), so farther than 2^-19 |t| from an integer both
-// have the same floor; closer (and for |t| below 2^-100, NaN or infinity)
-// the quotient is formed exactly.
-__device__ __forceinline__ int floor_div(float d, float c) {
-    const float t = d * __builtin_amdgcn_rcpf(c);
-    const float f = floorf(t);
-    if (fminf(t - f, f + 1.0f - t) > fabsf(t) * 0x1p-19f + 0x1p-100f) return int(f);
+// floor(d / c) of the correctly rounded quotient, rc the hardware reciprocal
+// of c.  t = d * rc is within 2^-22 |t| of d / c (reciprocal 1 ulp, one
+// rounding each for the product and the quotient), so farther than
+// 2^-19 |t| from an integer both have the same floor; closer (and for |t|
+// below 2^-100, NaN or infinity, where the distance is NaN or 0) the
+// quotient is formed exactly.  The distance to the nearest integer,
+// |t - rint(t)|, is exact (t and rint(t) within a factor of two, or t
+// small).
+__device__ __forceinline__ int floor_div_rc(float d, float c, float rc) {
+    const float t = d * rc;
+    if (fabsf(t - rintf(t)) > fmaf(fabsf(t), 0x1p-19f, 0x1p-100f)) return int(floorf(t));
     return int(floorf(d / c));
 }
+__device__ __forceinline__ int floor_div(float d, float c) { return floor_div_rc(d, c, __builtin_amdgcn_rcpf(c)); }
 
 __device__ __forceinline__ int cell_coord(float p, float o, float c) {
     return floor_div(p - o, c);
@@ -181,33 +184,42 @@ __device__ __forceinline__ uint32_t md_hash_wrapped(const SimParams &p, int gx, 
     return (z * p.gridSize.y) * p.gridSize.x + y * p.gridSize.x + x;
 }
 
-// positionRelativeToBox (particles_kernel_impl.cuh:208-228)
-__device__ __forceinline__ f3 position_relative_to_box(const SimParams &p, f3 q) {
-    f3 r = mk(q.x - p.origin.x, q.y - p.origin.y, q.z - p.origin.z);
-    if (p.boundaryX == PERIODIC) {
-        if (r.x >= p.boxSize.x) r.x -= p.boxSize.x;
-        else if (r.x < 0.0f) r.x += p.boxSize.x;
-    }
-    if (p.boundaryY == PERIODIC) {
-        if (r.y >= p.boxSize.y) r.y -= p.boxSize.y;
-        else if (r.y < 0.0f) r.y += p.boxSize.y;
-    }
-    if (p.boundaryZ == PERIODIC) {
-        if (r.z >= p.boxSize.z) r.z -= p.boxSize.z;
-        else if (r.z < 0.0f) r.z += p.boxSize.z;
+// positionRelativeToBox (particles_kernel_impl.cuh:208-228), one axis: the
+// coordinate relative to the origin o, moved by one box length L into
+// [0, L) on a periodic axis
+__device__ __forceinline__ float relative_to_box(float q, float o, float L, bool periodic) {
+    float r = q - o;
+    if (periodic) {
+        if (r >= L) r -= L;
+        else if (r < 0.0f) r += L;
     }
     return r;
+}
+__device__ __forceinline__ f3 position_relative_to_box(const SimParams &p, f3 q) {
+    return mk(relative_to_box(q.x, p.origin.x, p.boxSize.x, p.boundaryX == PERIODIC),
+              relative_to_box(q.y, p.origin.y, p.boxSize.y, p.boundaryY == PERIODIC),
+              relative_to_box(q.z, p.origin.z, p.boxSize.z, p.boundaryZ == PERIODIC));
 }
 
 // SRD cell of a position (calcGridPosSRD :241-249 + calcGridHashSRD :261-276).
 // Periodic axes wrap; a non-periodic axis is clamped into [0, G) (the
 // reference leaves it unwrapped, which can index past the cell table).
-// the grid shifted by `shift` (the collision's randOffset)
-__device__ __forceinline__ uint32_t srd_hash_at(const SimParams &p, const SolventParams &s, f3 shift, f3 q) {
-    const f3 shifted = q + shift;
-    const f3 r = position_relative_to_box(p, shifted);
-    int gx = floor_div(r.x, s.cellSize.x);
-    int gy = floor_div(r.y, s.cellSize.y);
+// The grid is shifted by the collision's randOffset; SrdGrid holds that
+// shift and the cell sizes' reciprocals, formed once for many positions.
+struct SrdGrid {
+    f3 shift;
+    float rcx, rcy, rcz;
+};
+__device__ __forceinline__ SrdGrid srd_grid(const SolventParams &s, f3 shift) {
+    return {shift, __builtin_amdgcn_rcpf(s.cellSize.x), __builtin_amdgcn_rcpf(s.cellSize.y),
+            __builtin_amdgcn_rcpf(s.cellSize.z)};
+}
+__device__ __forceinline__ uint32_t srd_hash_g(const SimParams &p, const SolventParams &s, const SrdGrid &g, f3 q) {
+    const f3 shifted = q + g.shift;
+    const float rx = relative_to_box(shifted.x, p.origin.x, p.boxSize.x, p.boundaryX == PERIODIC);
+    const float ry = relative_to_box(shifted.y, p.origin.y, p.boxSize.y, p.boundaryY == PERIODIC);
+    int gx = floor_div_rc(rx, s.cellSize.x, g.rcx);
+    int gy = floor_div_rc(ry, s.cellSize.y, g.rcy);
     if (p.boundaryX == PERIODIC) gx = wrap_mod(gx, s.gridSize.x);
     else gx = min(max(gx, 0), int(s.gridSize.x) - 1);
     if (p.boundaryY == PERIODIC) gy = wrap_mod(gy, s.gridSize.y);
@@ -216,11 +228,15 @@ __device__ __forceinline__ uint32_t srd_hash_at(const SimParams &p, const Solven
     // both give 0 for any coordinate), so its z is not computed
     int gz = 0;
     if (s.gridSize.z > 1u) {
-        gz = floor_div(r.z, s.cellSize.z);
+        const float rz = relative_to_box(shifted.z, p.origin.z, p.boxSize.z, p.boundaryZ == PERIODIC);
+        gz = floor_div_rc(rz, s.cellSize.z, g.rcz);
         if (p.boundaryZ == PERIODIC) gz = wrap_mod(gz, s.gridSize.z);
         else gz = min(max(gz, 0), int(s.gridSize.z) - 1);
     }
     return (uint32_t(gz) * s.gridSize.y) * s.gridSize.x + uint32_t(gy) * s.gridSize.x + uint32_t(gx);
+}
+__device__ __forceinline__ uint32_t srd_hash_at(const SimParams &p, const SolventParams &s, f3 shift, f3 q) {
+    return srd_hash_g(p, s, srd_grid(s, shift), q);
 }
 
 __device__ __forceinline__ uint32_t srd_hash(const SimParams &p, const SolventParams &s, f3 q) {
