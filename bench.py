@@ -485,7 +485,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(replicas shard across GPUs without communication; see DESIGN.md)", file=sys.stderr)
+              "(a batch of replicas would shard across GPUs by replica, without communication)",
+              file=sys.stderr)
         sys.exit(2)
 
     from nwamd import abi
