@@ -49,3 +49,47 @@ def test_mulhi_alone_is_not_exact_for_the_benchmark_layer():
     q = (x * np.uint64(magic(d))) >> np.uint64(32)
     assert np.any(q != x // np.uint64(d))
     np.testing.assert_array_equal(div_exact(x, d), x // np.uint64(d))
+
+
+# ---------------------------------------------------------------------------
+# floor_div_rc (nw_device.hpp): floor(d / c) of the correctly rounded quotient
+# from t = d * rc, rc the hardware reciprocal of c (within 1 ulp).  The fast
+# path is taken when t is farther than fma(|t|, 2^-19, 2^-100) from the
+# nearest integer; the claim is that floor(t) then equals floor(RN(d / c)).
+# Checked with rc = RN(1/c) and its float neighbours (the 1-ulp reciprocal),
+# on random operands and on operands at and next to multiples of c (cell faces).
+# ---------------------------------------------------------------------------
+def _fast_path_floor_ok(d, c):
+    d = d.astype(np.float32)
+    c = np.float32(c)
+    q = np.floor(d / c)  # float32 division: correctly rounded
+    r0 = np.float32(1.0) / c
+    for rc in (np.nextafter(r0, np.float32(0)), r0, np.nextafter(r0, np.float32(np.inf))):
+        t = d * rc  # float32 product: correctly rounded
+        dist = np.abs(t - np.rint(t))
+        bound = (np.abs(t).astype(np.float64) * 2.0 ** -19 + 2.0 ** -100).astype(np.float32)
+        fast = dist > bound
+        assert np.array_equal(np.floor(t)[fast], q[fast]), (c, rc)
+        # the fast path is the common one away from the faces
+        yield fast.mean()
+
+
+@pytest.mark.parametrize("c", [1.0, 1.1235955, 1.1236, 0.7, 1.5625, 3.0, 1e-3, 1234.5])
+def test_floor_div_fast_path_random(c):
+    rng = np.random.default_rng(7)
+    d = np.concatenate([rng.uniform(-2000 * c, 2000 * c, 200_000), rng.uniform(-4 * c, 4 * c, 50_000)])
+    fracs = list(_fast_path_floor_ok(d, c))
+    assert min(fracs) > 0.99
+
+
+@pytest.mark.parametrize("c", [1.1235955, 1.0, 0.7, 1.5625])
+def test_floor_div_fast_path_at_cell_faces(c):
+    k = np.arange(-1000, 1001, dtype=np.float32)
+    g = (k * np.float32(c)).astype(np.float32)
+    d = [g]
+    for _ in range(3):
+        d.append(np.nextafter(d[-1], np.float32(np.inf)))
+    e = [g]
+    for _ in range(3):
+        e.append(np.nextafter(e[-1], np.float32(-np.inf)))
+    list(_fast_path_floor_ok(np.concatenate(d + e[1:]), c))
