@@ -147,7 +147,10 @@ hipError_t launch_collide_off(float4 *force, const float4 *sPos, const float4 *s
                               const KP &kp, hipStream_t st);
 
 // ---------------------------------------------------------------------------
-// batched replicas (the fused MD step over many independent systems)
+// batched replicas (the fused MD step over many independent systems).  A
+// device table of R records is followed by the R records' waveOffset values,
+// then their blockOffset values, packed (replica_of_wave / replica_of_block
+// read those); nwBatchSetReplicas builds it so.
 // ---------------------------------------------------------------------------
 struct BatchRep {
     KP kp;                 // the replica's parameters (box, grid, worms, force constants)
