@@ -348,7 +348,7 @@ __device__ __forceinline__ uint32_t seg_of(const uint32_t *segStart, uint32_t ti
 
 
 // MD level 2, staged bucket of m members: ids, cells and the gathered
-// records of the sorted slots, in groups of four members per thread with the
+// records of the sorted slots, in groups of G members per thread with the
 // gathers of a group in flight together; past the last member a thread
 // handles member m - 1 again (the same values to the same addresses)
 template <bool POS, bool VEL, bool TAN, int BLOCK, int G>
@@ -391,8 +391,9 @@ __device__ __forceinline__ void md_gather_group(const CellsArgs &o, const uint32
 template <bool POS, bool VEL, bool TAN, int BLOCK, int CAP>
 __device__ __forceinline__ void md_gather_staged(const CellsArgs &o, const uint32_t *lid, const uint16_t *lcell,
                                                  uint32_t sb, uint32_t cell0, uint32_t m) {
-    // groups of four members, two when all three records move (register room)
-    constexpr int PER = CAP / BLOCK, GM = VEL && TAN ? 2 : 4, G = PER < GM ? PER : GM;
+    // groups of two members, a group's gathers in flight together (groups of
+    // four measured about 0.4 us slower per launch at 800k beads)
+    constexpr int PER = CAP / BLOCK, G = PER < 2 ? PER : 2;
 #pragma unroll
     for (int g = 0; g < PER; g += G)
         if (m > uint32_t(g) * BLOCK) md_gather_group<POS, VEL, TAN, BLOCK, G>(o, lid, lcell, sb, cell0, m, g);
