@@ -559,9 +559,9 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
         }
     }
 
-    // in flight together: the old force, the bounds of the bead's three rows
-    // and the staged records
-    const v4 Fo = reinterpret_cast<const v4 *>(force)[me];
+    // in flight together: the bounds of the bead's three rows, the staged
+    // records, then the old force (issued last: loads retire in issue order,
+    // so the waits for the records before the barrier leave it in flight)
     int gx, gy, gzi;
     cell_xyz(kp, cOwn, gx, gy, gzi);
     const uint32_t gz = uint32_t(gzi);
@@ -577,7 +577,10 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
     uint32_t ra[3], rb[3];
 #pragma unroll
     for (int yi = 0; yi < 3; ++yi) {
-        const uint32_t base = (gz * Gy + uint32_t(yi < ay.n ? ay.c[yi] : ay.c[0])) * Gx;
+        // rows past ay.n hold the centre row (in bounds, unused): every row
+        // is loaded, none copied from another's load (a copy would wait for
+        // that load, and for every load issued before it)
+        const uint32_t base = (gz * Gy + uint32_t(ay.c[yi])) * Gx;
         ra[yi] = offR[base + gxc - 1u];
         rb[yi] = offR[base + gxc + 2u];
     }
@@ -609,6 +612,17 @@ __device__ __forceinline__ void collide_rows_tile(const KP &kp, uint32_t i0, uin
         SI[h] = sIdx[j];
         if (MODE == 0) SV[h] = vel4[j];
         if (TAN) ST[h] = tan4[j];
+    }
+    const v4 Fo = reinterpret_cast<const v4 *>(force)[me];
+    // The records are stored below under a branch per record; these empty
+    // statements use them here, so their loads stay above, issued together
+    // (the compiler would otherwise move a record's loads into its branch,
+    // to issue there after every other load and wait for them all)
+#pragma unroll
+    for (int h = 0; h < RH; ++h) {
+        asm volatile("" ::"v"(SP[h]), "v"(SI[h]));
+        if constexpr (MODE == 0) asm volatile("" ::"v"(SV[h]));
+        if constexpr (TAN) asm volatile("" ::"v"(ST[h]));
     }
 #pragma unroll
     for (int h = 0; h < RH; ++h) {
