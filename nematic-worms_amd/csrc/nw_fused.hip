@@ -45,11 +45,21 @@ __device__ __forceinline__ FrontLane md_front_lane(const KP &kp, const WaveWorm 
                                                    bool wantTan) {
     const SimParams &p = kp.p;
     const uint32_t n = p.filamentSize;
+    // Every lane loads its four records as 16-byte vectors (a lane past the
+    // wave's worms reads bead 0, in bounds) and takes zeros by select.  A
+    // load under the lane's condition was split into one 4-byte load per
+    // component, each behind its own branch; the empty statements keep the
+    // four loads unconditional, here.
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const uint32_t b = m.active ? m.bead : 0u;
+    const v4 Pl = reinterpret_cast<const v4 *>(pos)[b], Vl = reinterpret_cast<const v4 *>(vel)[b];
+    const v4 Fl = reinterpret_cast<const v4 *>(force)[b], FOl = reinterpret_cast<const v4 *>(forceOld)[b];
+    asm volatile("" ::"v"(Pl), "v"(Vl), "v"(Fl), "v"(FOl));
     const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 P = m.active ? pos[m.bead] : zero4;
-    const float4 V = m.active ? vel[m.bead] : zero4;
-    const float4 F = m.active ? force[m.bead] : zero4;
-    const float4 FO = m.active ? forceOld[m.bead] : zero4;
+    const float4 P = m.active ? make_float4(Pl.x, Pl.y, Pl.z, Pl.w) : zero4;
+    const float4 V = m.active ? make_float4(Vl.x, Vl.y, Vl.z, Vl.w) : zero4;
+    const float4 F = m.active ? make_float4(Fl.x, Fl.y, Fl.z, Fl.w) : zero4;
+    const float4 FO = m.active ? make_float4(FOl.x, FOl.y, FOl.z, FOl.w) : zero4;
 
     // velocity Verlet (filament_integrator, particles_kernel_impl.cuh:71-142)
     FrontLane o;
