@@ -109,9 +109,8 @@ __device__ __forceinline__ void adj_step_lanes(const AdjStepArgs &a, const KP &k
                                                uint32_t idBase, double (&acc)[kAdjParams]) {
     const SimParams &p = kp.p;
     const uint32_t n = p.filamentSize;
-    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    const f3 rv = mk(m.active ? a.pos[m.bead] : zero4);
-    const f3 lam = mk(m.active ? a.lF[m.bead] : zero4);
+    const f3 rv = mk(load4_or_zero(a.pos, m.bead, m.active));
+    const f3 lam = mk(load4_or_zero(a.lF, m.bead, m.active));
     const d3 r = {rv, lam};
     const uint32_t pp = m.pp;
 
@@ -165,7 +164,7 @@ __device__ __forceinline__ void adj_step_lanes(const AdjStepArgs &a, const KP &k
         if (pp + 1u < n) acc[kAdjDrive] = double(dot(lam, that));
         if (p.bondBendingK != 0.0f) acc[kAdjKappa] = double(dot(lam, bend.v)) / double(p.bondBendingK);
         if (a.langevin) {
-            const f3 v = mk(a.vel[m.bead]);
+            const f3 v = mk(load4(a.vel, m.bead));
             const u4 b = philox_block(a.seed, a.offLgv + m.bead);
             float z0, z1, z2, z3;
             box_muller(b.x, b.y, z0, z1);
@@ -263,10 +262,10 @@ __device__ __forceinline__ void adj_step_lanes(const AdjStepArgs &a, const KP &k
         (void)me;
 
         // adjoint of the step for this bead
-        const f3 Rt = mk(a.lr[m.bead]) + R + Rp;
+        const f3 Rt = mk(load4(a.lr, m.bead)) + R + Rp;
         // the drag -gamma v' is part of F' only with the Langevin thermostat on
-        const f3 V = mk(a.lv[m.bead]) - (a.langevin ? a.gamma : 0.0f) * lam + a.dt * Rt;
-        const f3 lG = mk(a.lG[m.bead]);
+        const f3 V = mk(load4(a.lv, m.bead)) - (a.langevin ? a.gamma : 0.0f) * lam + a.dt * Rt;
+        const f3 lG = mk(load4(a.lG, m.bead));
         a.lr[m.bead] = f4(Rt, 0.0f);
         a.lv[m.bead] = f4(V, 0.0f);
         a.lFout[m.bead] = f4(lG + (0.5f * a.dt * a.dt) * Rt + (0.5f * a.dt) * V, 0.0f);
@@ -353,7 +352,7 @@ __global__ __launch_bounds__(kBlock) void k_bond_order_grad(const float4 *__rest
     const SimParams &p = kp.p;
     const uint32_t n = p.filamentSize;
     const WaveWorm m = wave_worm(n, W);
-    const f3 r = mk(m.active ? pos[m.bead] : make_float4(0.f, 0.f, 0.f, 0.f));
+    const f3 r = mk(load4_or_zero(pos, m.bead, m.active));
     const f3 rn = shfl_down3(r, 1);
     f3 g = mk(0.f, 0.f, 0.f);
     if (m.active && m.pp + 1u < n) {
@@ -440,9 +439,8 @@ __global__ __launch_bounds__(kBlock) void k_polar_speed_grad(const float4 *__res
     const SimParams &p = kp.p;
     const uint32_t n = p.filamentSize;
     const WaveWorm m = wave_worm(n, W);
-    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    const f3 r = mk(m.active ? pos[m.bead] : zero4);
-    const f3 v = mk(m.active ? vel[m.bead] : zero4);
+    const f3 r = mk(load4_or_zero(pos, m.bead, m.active));
+    const f3 v = mk(load4_or_zero(vel, m.bead, m.active));
     const f3 rn = shfl_down3(r, 1), rp = shfl_up3(r, 1);
     f3 g = mk(0.f, 0.f, 0.f), t = g;
     if (m.active && n >= 2u) {

@@ -859,7 +859,7 @@ __global__ __launch_bounds__(kBlock) void k_srd_hash_batch(const float4 *__restr
     const uint32_t r = srd_rep_of_particle(reps, R, i, i < Nb);
     const SrdRep &q = reps[r];
     const f3 sh = mk(shift[3 * r], shift[3 * r + 1], shift[3 * r + 2]);
-    hash[i] = q.cellOffset + srd_hash_at(q.kp.p, q.kp.s, sh, mk(pos[i]));
+    hash[i] = q.cellOffset + srd_hash_at(q.kp.p, q.kp.s, sh, mk(load4(pos, i)));
 }
 
 __global__ __launch_bounds__(kBlock) void k_srd_rotate_batch(const uint32_t *__restrict__ hash,
@@ -871,7 +871,7 @@ __global__ __launch_bounds__(kBlock) void k_srd_rotate_batch(const uint32_t *__r
     if (i >= NT) return;
     const bool solvent = i >= Nb;
     if (!solvent && !rotBeads) return;
-    const float4 V = vel[i], C = com[hash[i]];
+    const float4 V = load4(vel, i), C = load4(com, hash[i]);
     // the arithmetic of k_srd_rotate
     f3 v = mk(V.x, V.y, V.z);
     const f3 u = mk(C.x, C.y, C.z);
@@ -881,7 +881,7 @@ __global__ __launch_bounds__(kBlock) void k_srd_rotate_batch(const uint32_t *__r
     v = u + rot;
     if (solvent) {
         const SimParams &p = reps[srd_rep_of_particle(reps, R, i, false)].kp.p;
-        const float4 P = pos[i];
+        const float4 P = load4(pos, i);
         f3 r = mk(P.x, P.y, P.z);
         const f3 dr = dt * v;
         r += dr;

@@ -278,6 +278,25 @@ __device__ __forceinline__ float u01(uint32_t x) {
     return float(x >> 8) * 5.9604644775390625e-8f + 5.9604644775390625e-8f;
 }
 
+// Record i of a as one 16-byte load (read through a float4 whose w goes
+// unused, the compiler may issue three or four 4-byte loads instead)
+__device__ __forceinline__ float4 load4(const float4 *__restrict__ a, uint32_t i) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 x = reinterpret_cast<const v4 *>(a)[i];
+    return make_float4(x.x, x.y, x.z, x.w);
+}
+
+// Record i of a, or zeros where !use, as one 16-byte load (a lane with !use
+// reads record 0: a must hold at least one) and a select.  Written as
+// `use ? a[i] : 0` the compiler splits the record into four 4-byte loads, each
+// behind its own branch; here it keeps one load (which it may still guard by
+// `use`: a caller that needs the loads unconditional pins them itself).
+__device__ __forceinline__ float4 load4_or_zero(const float4 *__restrict__ a, uint32_t i, bool use) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 x = reinterpret_cast<const v4 *>(a)[use ? i : 0u];
+    return use ? make_float4(x.x, x.y, x.z, x.w) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 // Box-Muller: two normals from two uniforms in (0, 1].
 __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float &z0, float &z1) {
     const float u1 = u01(a), u2 = u01(b);
