@@ -411,13 +411,12 @@ constexpr int run_cap() {
 constexpr int kWrapCap = 8;
 
 // LDS of one pair-kernel workgroup: the three row runs back to back in a pool
-// of 3 CAP records, the two wrap cells (kWrapCap records each), and one spare
-// record (the walk reads one record ahead of the one it evaluates)
+// of 3 CAP records, then the two wrap cells (kWrapCap records each)
 template <int MODE, bool TAN, int CAP>
 struct RowsLds {
     static constexpr int kPool = 3 * CAP;
     static constexpr int kWrap0 = kPool;  // first slot of the wrap cells
-    static constexpr int kSlots = kPool + 2 * kWrapCap + 1;
+    static constexpr int kSlots = kPool + 2 * kWrapCap;
     float4 pos[kSlots];
     float4 vel[MODE == 0 ? kSlots : 1];
     float4 tan[TAN ? kSlots : 1];
@@ -450,9 +449,9 @@ __device__ __forceinline__ void lds_pair(const KP &kp, const Owner<MODE, TAN> &o
 // The walk over a bead's candidate list: NP pieces of consecutive LDS slots
 // in stencil order, piece k = pieces[k STRIDE] of the given arrays (n slots
 // from q), the pieces of stencil row r of weight wr.  Flattened index t lies in piece k for pre[k] <= t <
-// pre[k + 1], at slot t + d[k]; the next record is read while this one is
-// evaluated (t = K gives the slot after the last piece: q + n of a staged
-// piece, at most the spare slot, or 0 after an empty one).
+// pre[k + 1], at slot t + d[k].  Each record is read from LDS where it is
+// evaluated: reading the next one ahead kept a second copy of the records
+// live and moved it every step, slower than the wait the other waves cover.
 template <int MODE, bool TAN, bool UNIT, int NP, int STRIDE, int CAP>
 __device__ __forceinline__ void walk_lds(const KP &kp, const Owner<MODE, TAN> &o, float rc2, f3 &acc,
                                          const uint32_t (&q)[6], const uint32_t (&n)[6], float w0, float w1,
@@ -465,24 +464,16 @@ __device__ __forceinline__ void walk_lds(const KP &kp, const Owner<MODE, TAN> &o
         d[k] = q[k * STRIDE] - pre[k];  // mod 2^32
     }
     const uint32_t K = pre[NP];
-    uint32_t s = d[NP - 1];
-#pragma unroll
-    for (int k = NP - 2; k >= 0; --k) s = 0u < pre[k + 1] ? d[k] : s;
-    float4 P = L.pos[s];
-    float4 V = MODE == 0 ? L.vel[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 T = TAN ? L.tan[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     for (uint32_t t = 0; t < K; ++t) {
-        const float4 Pc = P, Vc = V, Tc = T;
         // stencil row r holds pieces [r NP / 3, (r + 1) NP / 3)
         const float wt = UNIT ? 1.0f : (t < pre[NP / 3] ? w0 : (t < pre[2 * NP / 3] ? w1 : w2));
-        const uint32_t u = t + 1u;
-        s = d[NP - 1];
+        uint32_t s = d[NP - 1];
 #pragma unroll
-        for (int k = NP - 2; k >= 0; --k) s = u < pre[k + 1] ? d[k] : s;
-        s += u;
-        P = L.pos[s];
-        if (MODE == 0) V = L.vel[s];
-        if (TAN) T = L.tan[s];
+        for (int k = NP - 2; k >= 0; --k) s = t < pre[k + 1] ? d[k] : s;
+        s += t;
+        const float4 Pc = L.pos[s];
+        const float4 Vc = MODE == 0 ? L.vel[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 Tc = TAN ? L.tan[s] : make_float4(0.f, 0.f, 0.f, 0.f);
         lds_pair<MODE, TAN, UNIT>(kp, o, rc2, wt, acc, Pc, Vc, Tc);
     }
 }
@@ -730,7 +721,7 @@ template <int MODE, bool TAN, int BLOCK, bool BIG>
 constexpr int pair_stage() {
     constexpr int rec = 16 * (1 + (MODE == 0 ? 1 : 0) + (TAN ? 1 : 0));
     constexpr int big = 2 * BLOCK;
-    return BIG && (3 * big + 2 * kWrapCap + 1) * rec <= 65536 ? big : run_cap<BLOCK>();
+    return BIG && (3 * big + 2 * kWrapCap) * rec <= 65536 ? big : run_cap<BLOCK>();
 }
 
 // waves per SIMD the pair kernels are compiled for: the WCA-only kernel
