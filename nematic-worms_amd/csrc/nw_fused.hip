@@ -54,6 +54,19 @@ __device__ __forceinline__ FrontLane md_front_lane(const KP &kp, const WaveWorm 
     const uint32_t b = m.active ? m.bead : 0u;
     const v4 Pl = reinterpret_cast<const v4 *>(pos)[b], Vl = reinterpret_cast<const v4 *>(vel)[b];
     const v4 Fl = reinterpret_cast<const v4 *>(force)[b], FOl = reinterpret_cast<const v4 *>(forceOld)[b];
+    // the random draws depend on the bead alone: formed while the loads are
+    // in flight
+    const bool rev = p.reverseProbability > 0.0f && m.active &&
+                     !(uniform_at(seed, offRev, m.worm) > p.reverseProbability);
+    f3 xi = mk(0.f, 0.f, 0.f);
+    if (langevin) {
+        const u4 rb = philox_block(seed, offLgv + m.bead);
+        float z0, z1, z2, z3;
+        box_muller(rb.x, rb.y, z0, z1);
+        box_muller(rb.z, rb.w, z2, z3);
+        (void)z3;
+        xi = mk(z0 * lgvStd + 0.0f, z1 * lgvStd + 0.0f, z2 * lgvStd + 0.0f);
+    }
     asm volatile("" ::"v"(Pl), "v"(Vl), "v"(Fl), "v"(FOl));
     const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 P = m.active ? make_float4(Pl.x, Pl.y, Pl.z, Pl.w) : zero4;
@@ -76,7 +89,6 @@ __device__ __forceinline__ FrontLane md_front_lane(const KP &kp, const WaveWorm 
     // polarity reversal (reverseFilamentsKernel :820-857): bead p <-> n-1-p
     // of the worm, taken from the mirror lane; the force is zero here
     if (p.reverseProbability > 0.0f) {
-        const bool rev = m.active && !(uniform_at(seed, offRev, m.worm) > p.reverseProbability);
         const int src = rev ? int(m.wl * n + (n - 1u - m.pp)) : int(m.lane);
         r = mk(__shfl(r.x, src, 64), __shfl(r.y, src, 64), __shfl(r.z, src, 64));
         v = mk(__shfl(v.x, src, 64), __shfl(v.y, src, 64), __shfl(v.z, src, 64));
@@ -89,12 +101,6 @@ __device__ __forceinline__ FrontLane md_front_lane(const KP &kp, const WaveWorm 
     // Langevin (langevinKernel :859-874), the force starts from zero
     f3 acc = mk(0.f, 0.f, 0.f);
     if (langevin) {
-        const u4 b = philox_block(seed, offLgv + m.bead);
-        float z0, z1, z2, z3;
-        box_muller(b.x, b.y, z0, z1);
-        box_muller(b.z, b.w, z2, z3);
-        (void)z3;
-        const f3 xi = mk(z0 * lgvStd + 0.0f, z1 * lgvStd + 0.0f, z2 * lgvStd + 0.0f);
         f3 f = mk(0.f, 0.f, 0.f);
         f -= p.gamma * v;
         f += xi;
